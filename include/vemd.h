@@ -1,0 +1,219 @@
+/*
+ * vemd.h -- C ABI of libvemd, an MI355X-native matrix-free mixed virtual
+ * element Darcy solver (lowest and first order, curved polygonal meshes).
+ *
+ * The reference exposes this path only through C++ classes; each entry point
+ * below names the reference interface it replaces (paths relative to the
+ * reference root; CPU/ = vem/CurvedVEM_Darcy/, GPU/ = vem/CurvedVEM_Darcy_gpu/).
+ *
+ * Conventions:
+ *   - every function returns a vemd_status; on failure vemd_last_error()
+ *     gives a message for the calling thread.  No C++ exception crosses the
+ *     ABI (the reference's rule at GPU/src/krylov.cu:8-29);
+ *   - all buffers are caller-owned; host pointers unless a name ends in _dev;
+ *   - vectors of unknowns use the reference's global DOF order
+ *     (CPU/solver/DarcySolver.cpp:182-227): edge moments degree-major
+ *     (r * num_edges + edge), then cell-internal flux dofs, then q pressure
+ *     coefficients per cell, then (essential-flux condition only) the
+ *     mean-zero multiplier;
+ *   - a solver object is used by one thread at a time.
+ */
+#ifndef VEMD_H
+#define VEMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    VEMD_OK = 0,
+    VEMD_ERR_ARG = 1,           /* invalid argument */
+    VEMD_ERR_IO = 2,            /* file missing or malformed */
+    VEMD_ERR_MESH = 3,          /* degenerate or unsupported mesh */
+    VEMD_ERR_SINGULAR = 4,      /* singular element matrix (first bad cell in the message) */
+    VEMD_ERR_NO_DEVICE = 5,     /* no HIP device: the solve path has no CPU fallback */
+    VEMD_ERR_DEVICE = 6,        /* HIP runtime error */
+    VEMD_ERR_NOT_CONVERGED = 7, /* Krylov iteration limit reached (info holds the residual) */
+    VEMD_ERR_STATE = 8,         /* call out of order (e.g. solve before upload) */
+    VEMD_ERR_INTERNAL = 9
+} vemd_status;
+
+typedef struct vemd_mesh_s* vemd_mesh;
+typedef struct vemd_solver_s* vemd_solver;
+
+/* Message of the last failure on this thread (never NULL). */
+const char* vemd_last_error(void);
+
+/* ---------------------------------------------------------------- meshes */
+
+enum { VEMD_MESH_MSH = 0, VEMD_MESH_VTK = 1 };
+
+/* Reads a Gmsh 4.1 (.msh) or legacy VTK polygon mesh with the reference's
+ * conventions (CCW cells, quad centroid = bounding-box centre, others = vertex
+ * average, diameter = max vertex distance, edges numbered by sorted node pair).
+ * Replaces vem::StraightMeshReader::read_mesh / read_mesh_vtk
+ * (CPU/mesh/straight_mesh.cpp:10-41,64-90). */
+vemd_status vemd_mesh_read(const char* path, int format, vemd_mesh* out);
+
+/* Builds a mesh from arrays: cell c has nodes cell_nodes[cell_off[c] ..
+ * cell_off[c+1]).  Optional circular boundary arcs (centre at the origin):
+ * arc_nodes holds (n0, n1) per arc, arc_radius its radius; the straight edge
+ * n0-n1 is the chord.  Replaces the same reader's post-processing chain
+ * (CPU/mesh/straight_mesh.cpp:504-660) for meshes made in memory. */
+vemd_status vemd_mesh_from_arrays(int num_nodes, const double* xy, int num_cells, const int* cell_off,
+                                  const int* cell_nodes, int num_arcs, const int* arc_nodes,
+                                  const double* arc_radius, vemd_mesh* out);
+
+enum { VEMD_DOMAIN_DISK = 0, VEMD_DOMAIN_ANNULUS = 1 };
+
+typedef struct {
+    int domain;          /* VEMD_DOMAIN_DISK (r <= r_outer) or VEMD_DOMAIN_ANNULUS */
+    double r_inner;      /* annulus inner radius */
+    double r_outer;
+    int num_cells;       /* number of Voronoi sites */
+    int lloyd_iters;     /* Lloyd relaxation steps */
+    uint64_t seed;
+    double min_edge_frac;/* edges shorter than this fraction of the local spacing are collapsed */
+} vemd_mesh_opts;
+
+/* Seeded Lloyd-relaxed Voronoi mesh of a disk or annulus whose boundary edges
+ * carry exact circular arcs.  No reference counterpart (the reference's Lloyd
+ * generator is a Python script on the unit square, vem/gmesh_py/get_gmesh.py:15-76). */
+vemd_status vemd_mesh_generate(const vemd_mesh_opts* opts, vemd_mesh* out);
+
+/* counts: [num_nodes, num_cells, num_edges, total cell-node entries, num_arcs] */
+vemd_status vemd_mesh_counts(vemd_mesh m, int64_t* counts);
+
+/* Copies the mesh out (any pointer may be NULL).  xy: 2*num_nodes; cell_off:
+ * num_cells+1; cell_nodes and cell_edges: total entries; edge_nodes: 2*num_edges;
+ * edge_occ: num_edges; centroid/area/diameter: num_cells each; arc_nodes:
+ * 2*num_arcs; arc_radius: num_arcs. */
+vemd_status vemd_mesh_export(vemd_mesh m, double* xy, int* cell_off, int* cell_nodes, int* cell_edges,
+                             int* edge_nodes, int* edge_occ, double* centroid_x, double* centroid_y,
+                             double* area, double* diameter, int* arc_nodes, double* arc_radius);
+
+/* Writes the mesh as a legacy VTK polygon file readable by vemd_mesh_read and
+ * by the reference's read_mesh_vtk. */
+vemd_status vemd_mesh_write_vtk(vemd_mesh m, const char* path);
+
+void vemd_mesh_destroy(vemd_mesh m);
+
+/* ---------------------------------------------------------------- problem */
+
+enum { VEMD_MAP_SIN = 0, VEMD_MAP_ARCS = 1 };
+enum { VEMD_SOL_SINCOS = 0, VEMD_SOL_SINSIN = 1 };
+enum { VEMD_BC_FLUX_MEANZERO = 0, VEMD_BC_DIRICHLET_P = 1 };
+
+typedef struct {
+    int k;              /* polynomial degree: 0 or 1 on the device (0..3 in setup) */
+    int nq2d;           /* triangle rule points: 1, 3, 4, 7, 9, 12 */
+    int nq1d;           /* Gauss-Legendre points on edges: 1..10 */
+    int map_kind;       /* VEMD_MAP_SIN: global x + eps sin(2 pi eta), y + eps sin(2 pi xi)
+                           (reference examples 1 and 2); VEMD_MAP_ARCS: per-cell maps that
+                           push boundary chords onto their arcs */
+    double eps;         /* VEMD_MAP_SIN amplitude */
+    int solution;       /* manufactured pressure: VEMD_SOL_SINCOS (reference) or SINSIN */
+    int bc;             /* VEMD_BC_FLUX_MEANZERO (reference: essential normal flux and a
+                           mean-zero pressure multiplier) or VEMD_BC_DIRICHLET_P */
+    const double* theta;/* per-cell log-permeability (K = exp(theta) I), or NULL for K = I */
+} vemd_problem_desc;
+
+/* Builds every element operator in FP64 on the host (projector, stabilisation,
+ * divergence, right-hand side, boundary data).  Replaces the constructor
+ * DarcySolver(reader, problem, nq2d, nq1d, k) and its per-element assembly
+ * (CPU/solver/DarcySolver.h:21-25, CPU/solver/DarcySolver.cpp:402-500). */
+vemd_status vemd_solver_create(vemd_mesh m, const vemd_problem_desc* desc, vemd_solver* out);
+
+/* sizes: [total flux dofs, total pressure dofs, total unknowns (incl. multiplier),
+ *         internal device vector length, number of colours, max valence].
+ * Replaces HdivMatrix::getTotalDofFlux / getTotalDofConc (CPU/solver/HdivMatrix.h:69-71). */
+vemd_status vemd_solver_sizes(vemd_solver s, int64_t* sizes);
+
+/* Uploads the valence-grouped, colour-ordered operator store (FP64 and FP32)
+ * to HBM of the given device.  No reference counterpart as a separate call
+ * (GPU/src/hdiv_gpu.cu:139-192 does it inside the constructor). */
+vemd_status vemd_solver_upload(vemd_solver s, int device);
+
+/* Replaces the per-cell log-permeability; updates the cell scales, the
+ * preconditioner and (essential-flux condition) the lifted right-hand side.
+ * Replaces MixedGpu::update (GPU/include/equation_gpu.cuh:76-84). */
+vemd_status vemd_set_theta(vemd_solver s, const double* theta);
+
+enum { VEMD_FP64 = 0, VEMD_MIXED = 1, VEMD_FP32 = 2 };
+
+typedef struct {
+    int precision;      /* VEMD_FP64, or VEMD_MIXED (FP32 operator and preconditioner inside
+                           FP64 iterative refinement) */
+    double rtol;        /* target true relative residual ||b - K x||_2 / ||b||_2 */
+    int max_iters;      /* total Krylov iteration limit */
+    double inner_rtol;  /* VEMD_MIXED: inner FP32 MINRES tolerance per refinement step */
+    int check_every;    /* iterations between host convergence checks */
+    int use_graph;      /* capture the iteration chunk in a hipGraph */
+} vemd_solve_opts;
+
+typedef struct {
+    int iterations;     /* total Krylov iterations */
+    int refinements;    /* outer FP64 refinement steps (VEMD_MIXED) */
+    double rel_residual;/* final true relative residual (FP64) */
+    double seconds;     /* solve wall time, device-resident data */
+    double apply_ms;    /* mean operator apply time of the last solve */
+} vemd_solve_info;
+
+/* Solves with preconditioned MINRES on the symmetric form of the saddle-point
+ * system (pressure rows negated).  rhs: optional right-hand side in reference
+ * order (NULL = the problem's own); solution: reference order, boundary values
+ * included.  Replaces DarcySolver::solve + getSolution (CPU/solver/DarcySolver.cpp:
+ * 665-799) and Krylov::solve (GPU/src/krylov.cu:73-175). */
+vemd_status vemd_solve(vemd_solver s, const vemd_solve_opts* opts, const double* rhs, double* solution,
+                       vemd_solve_info* info);
+
+/* y = K x for the reference-form operator (rows as in CPU/solver/DarcySolver.cpp:
+ * 445-476 with direction signs; essential boundary rows are identity).
+ * precision VEMD_FP64 or VEMD_FP32.  Replaces MixedGpu::apply
+ * (GPU/src/equation_gpu.cu:580-598). */
+vemd_status vemd_apply(vemd_solver s, int precision, const double* x, double* y);
+
+/* Device-resident apply in the internal (colour-ordered) layout, on the given
+ * HIP stream (NULL = default); x_dev and y_dev must not alias.  Used by
+ * benchmarks; vector length is sizes[3], element type double or float. */
+vemd_status vemd_apply_device(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* stream);
+
+/* The problem's right-hand side in reference order (ntot values), after the
+ * essential-boundary lift. */
+vemd_status vemd_rhs(vemd_solver s, double* rhs);
+
+/* L2 errors of a solution against the manufactured one: [pressure, flux] in the
+ * computational measure (the reference's metric, CPU/solver/DarcySolver.cpp:
+ * 942-1131) and [pressure, flux] in the physical measure.  Host FP64. */
+vemd_status vemd_errors(vemd_solver s, const double* solution, int nq2d, double* out4);
+
+/* Adjoint gradient of a loss L(x) with respect to theta: solves K lambda =
+ * dL/dx (dldx in reference order) and returns dL/dtheta_E =
+ * lambda_E^T (exp(-theta_E) A_E) x_E per cell.  No reference counterpart. */
+vemd_status vemd_adjoint_theta(vemd_solver s, const vemd_solve_opts* opts, const double* solution,
+                               const double* dldx, double* grad_theta, vemd_solve_info* info);
+
+/* FP64 element operator of one cell (host setup output): n local flux dofs;
+ * A (n x n, K = I part, direction signs applied to rows and columns; the cell
+ * operator is exp(-theta_E) A), W (q x n, signed columns; row r is int m_r div
+ * phi_i), L (q, int m_r det J), f (q, pressure source), sign (n), gdof (n,
+ * reference global dof of each local flux dof).  Any output may be NULL.
+ * Replaces HdivMatrix::getMatrixW and DarcySolver::getMatrixK_ac/K_as,
+ * getVecLagrange, getLocalRhs (CPU/solver/HdivMatrix.h:36-55,
+ * CPU/solver/DarcySolver.h:54-73). */
+vemd_status vemd_element_matrices(vemd_solver s, int cell, int* n, double* A, double* W, double* L, double* f,
+                                  double* sign, int64_t* gdof);
+
+void vemd_solver_destroy(vemd_solver s);
+
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int vemd_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VEMD_H */

@@ -1,0 +1,107 @@
+"""C-ABI boundary (CPU): the library loads, exports every declared symbol, the
+host FP64 setup equals the oracle, and the solve path fails loudly without a
+HIP device."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "vemd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(vemd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_every_declared_symbol(vemd):
+    import ctypes
+    L = ctypes.CDLL(os.path.join(ROOT, "wsl-code_amd", "libvemd.so"))
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(vemd._lib.SIGNATURES), set(names) ^ set(vemd._lib.SIGNATURES)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4])
+def test_mesh_reader_matches_oracle(vemd, oracle, n):
+    path = golden("square_%dx%d.msh" % (n, n))
+    a = vemd.Mesh.read(path).arrays()
+    o = oracle.Mesh.read(path)
+    assert np.array_equal(a["cell_off"], o.off) and np.array_equal(a["cell_nodes"], o.nodes)
+    assert np.array_equal(a["cell_edges"], o.cell_edges) and np.array_equal(a["edge_nodes"], o.edge_nodes)
+    assert np.array_equal(a["edge_occ"], o.edge_occ)
+    for key in ("cx", "cy", "area", "diam"):
+        assert np.array_equal(a[key], getattr(o, key))
+
+
+def test_vtk_roundtrip(vemd, tmp_path):
+    m = vemd.Mesh.read(golden("square_2x2.msh"))
+    p = tmp_path / "m.vtk"
+    m.write_vtk(p)
+    a, b = m.arrays(), vemd.Mesh.read(p, vtk=True).arrays()
+    for key in ("cell_nodes", "edge_nodes", "cx", "diam"):
+        assert np.array_equal(a[key], b[key])
+
+
+CASES = [(1, 0.05, 0), (0, 0.05, 0), (1, 0.0, 0), (1, 0.05, 1), (2, 0.05, 0), (3, 0.05, 0)]
+
+
+@pytest.mark.parametrize("k,eps,bc", CASES)
+def test_setup_matches_oracle(vemd, oracle, k, eps, bc):
+    """Element operators, direction signs, dof map, lifted rhs and the L2 error
+    functionals of the product's FP64 setup against the oracle."""
+    path = golden("square_2x2.msh")
+    nq = 12 if k == 3 else 9
+    s = vemd.Solver(vemd.Mesh.read(path), k=k, nq2d=nq, eps=eps, bc=bc)
+    op = oracle.Problem(oracle.Mesh.read(path), k=k, nq2=nq, eps=eps, bc=bc)
+    for c in range(16):
+        e = s.element(c)
+        n = e["n"]
+        d = op.element(c, 13)[:n]
+        Ao = op.element(c, 7, (n, n)) * np.outer(d, d)
+        Wo = op.element(c, 2).reshape(-1, n) * d
+        assert np.abs(e["A"] - Ao).max() <= 1e-13 * np.abs(Ao).max()
+        assert np.abs(e["W"] - Wo).max() <= 1e-13
+        assert np.allclose(e["L"], op.element(c, 8), rtol=1e-13, atol=1e-16)
+        assert np.allclose(e["f"], op.element(c, 9), rtol=1e-12, atol=1e-14)
+        assert np.array_equal(e["sign"], d) and np.array_equal(e["gdof"], op.element(c, 12)[:n].astype(int))
+    A, rr = op.csr(True)
+    fi, ub = op.free()
+    b = s.rhs()
+    assert np.abs(b[fi] - rr).max() <= 1e-13 * np.abs(rr).max()
+    mask = np.ones(len(b), bool)
+    mask[fi] = False
+    assert np.allclose(b[mask], ub[mask], rtol=1e-14, atol=1e-15)
+    x = op.solve_direct()
+    eo, ep = op.errors(nq), s.errors(x, nq)
+    for key in eo:
+        assert abs(ep[key] - eo[key]) <= 1e-13 * eo[key]
+
+
+def test_errors_reported_not_raised(vemd):
+    with pytest.raises(vemd.VemdError) as ei:
+        vemd.Mesh.read("/nonexistent/mesh.msh")
+    assert ei.value.status == "ERR_IO"
+    m = vemd.Mesh.read(golden("square_1x1.msh"))
+    with pytest.raises(vemd.VemdError) as ei:
+        vemd.Solver(m, k=5)
+    assert ei.value.status == "ERR_ARG"
+    with pytest.raises(vemd.VemdError) as ei:
+        vemd.Solver(m, k=3, nq2d=9)
+    assert ei.value.status == "ERR_ARG"
+
+
+def test_solve_without_device_fails_loudly(vemd):
+    if vemd.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    s = vemd.Solver(vemd.Mesh.read(golden("square_1x1.msh")))
+    with pytest.raises(vemd.VemdError) as ei:
+        s.upload(0)
+    assert ei.value.status == "ERR_NO_DEVICE"
+    with pytest.raises(vemd.VemdError) as ei:
+        s.solve()
+    assert ei.value.status == "ERR_STATE"

@@ -1,0 +1,312 @@
+// extern "C" entry points of libvemd (declared in include/vemd.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <thread>
+
+#include "solver.h"
+
+namespace vemd {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+int host_threads() {
+    static int n = [] {
+        const char* e = std::getenv("VEMD_THREADS");
+        int v = e ? std::atoi(e) : (int)std::thread::hardware_concurrency();
+        return std::max(1, std::min(v, 16));
+    }();
+    return n;
+}
+
+Solver::~Solver() {
+    try {
+        dev_free(*this);
+    } catch (...) {
+    }
+}
+
+void Solver::to_internal(const double* ref, double* in, bool scale_rows) const {
+    std::fill(in, in + layout.nvec, 0.0);
+    for (int64_t g = 0; g < setup->ntot; ++g) {
+        int64_t i = layout.int_of_ref[g];
+        in[i] = scale_rows ? ref[g] * rowscale[i] : ref[g];
+    }
+}
+
+void Solver::to_reference(const double* in, double* ref, bool unscale_rows) const {
+    for (int64_t g = 0; g < setup->ntot; ++g) {
+        int64_t i = layout.int_of_ref[g];
+        ref[g] = unscale_rows ? in[i] / rowscale[i] : in[i];
+    }
+}
+
+void Solver::refresh_theta_dependent() {
+    pack_scale(*setup, layout, scale);
+    build_minv(*mesh, *setup, layout, minv);
+    build_row_scale(*mesh, *setup, layout, rowscale);
+    build_rhs_ref(*mesh, *setup, rhs_ref);
+}
+
+// L2 errors of a reference-order solution (computational and physical measure)
+// against the manufactured solution, following CPU/solver/DarcySolver.cpp:942-1131.
+void errors_host(const Solver& s, const double* sol, int nq, double* out);
+
+}  // namespace vemd
+
+using namespace vemd;
+
+struct vemd_mesh_s {
+    std::shared_ptr<Mesh> m;
+};
+struct vemd_solver_s {
+    std::unique_ptr<Solver> s;
+};
+
+#define API_BEGIN try {
+#define API_END                                  \
+    return VEMD_OK;                              \
+    }                                            \
+    catch (const Error& e) {                     \
+        set_last_error(e.what());                \
+        return (vemd_status)e.code;              \
+    }                                            \
+    catch (const std::bad_alloc&) {              \
+        set_last_error("out of host memory");    \
+        return VEMD_ERR_INTERNAL;                \
+    }                                            \
+    catch (const std::exception& e) {            \
+        set_last_error(e.what());                \
+        return VEMD_ERR_INTERNAL;                \
+    }
+
+#define REQUIRE(c, msg) \
+    if (!(c)) throw Error(VEMD_ERR_ARG, msg)
+
+extern "C" {
+
+const char* vemd_last_error(void) { return g_last_error.c_str(); }
+
+int vemd_device_count(void) { return device_count(); }
+
+vemd_status vemd_mesh_read(const char* path, int format, vemd_mesh* out) {
+    API_BEGIN
+    REQUIRE(path && out, "null argument");
+    REQUIRE(format == VEMD_MESH_MSH || format == VEMD_MESH_VTK, "unknown mesh format");
+    Mesh* m = format == VEMD_MESH_MSH ? mesh_read_msh(path) : mesh_read_vtk(path);
+    *out = new vemd_mesh_s{std::shared_ptr<Mesh>(m)};
+    API_END
+}
+
+vemd_status vemd_mesh_from_arrays(int num_nodes, const double* xy, int num_cells, const int* cell_off,
+                                  const int* cell_nodes, int num_arcs, const int* arc_nodes, const double* arc_radius,
+                                  vemd_mesh* out) {
+    API_BEGIN
+    REQUIRE(out, "null argument");
+    REQUIRE(num_arcs == 0 || (arc_nodes && arc_radius), "arc arrays missing");
+    Mesh* m = mesh_from_arrays(num_nodes, xy, num_cells, cell_off, cell_nodes, num_arcs, arc_nodes, arc_radius);
+    *out = new vemd_mesh_s{std::shared_ptr<Mesh>(m)};
+    API_END
+}
+
+vemd_status vemd_mesh_generate(const vemd_mesh_opts* opts, vemd_mesh* out) {
+    API_BEGIN
+    REQUIRE(opts && out, "null argument");
+    Mesh* m = mesh_generate(*opts);
+    *out = new vemd_mesh_s{std::shared_ptr<Mesh>(m)};
+    API_END
+}
+
+vemd_status vemd_mesh_counts(vemd_mesh mh, int64_t* c) {
+    API_BEGIN
+    REQUIRE(mh && c, "null argument");
+    const Mesh& m = *mh->m;
+    c[0] = m.nn;
+    c[1] = m.nc;
+    c[2] = m.ne;
+    c[3] = (int64_t)m.nodes.size();
+    c[4] = (int64_t)m.arcs.size();
+    API_END
+}
+
+vemd_status vemd_mesh_export(vemd_mesh mh, double* xy, int* cell_off, int* cell_nodes, int* cell_edges,
+                             int* edge_nodes, int* edge_occ, double* cx, double* cy, double* area, double* diam,
+                             int* arc_nodes, double* arc_radius) {
+    API_BEGIN
+    REQUIRE(mh, "null mesh");
+    const Mesh& m = *mh->m;
+    if (xy) std::copy(m.xy.begin(), m.xy.end(), xy);
+    if (cell_off) std::copy(m.off.begin(), m.off.end(), cell_off);
+    if (cell_nodes) std::copy(m.nodes.begin(), m.nodes.end(), cell_nodes);
+    if (cell_edges) std::copy(m.cell_edges.begin(), m.cell_edges.end(), cell_edges);
+    if (edge_nodes) std::copy(m.edge_nodes.begin(), m.edge_nodes.end(), edge_nodes);
+    if (edge_occ) std::copy(m.edge_occ.begin(), m.edge_occ.end(), edge_occ);
+    if (cx) std::copy(m.cx.begin(), m.cx.end(), cx);
+    if (cy) std::copy(m.cy.begin(), m.cy.end(), cy);
+    if (area) std::copy(m.area.begin(), m.area.end(), area);
+    if (diam) std::copy(m.diam.begin(), m.diam.end(), diam);
+    if (arc_nodes || arc_radius) {
+        std::vector<std::pair<uint64_t, double>> arcs(m.arcs.begin(), m.arcs.end());
+        std::sort(arcs.begin(), arcs.end());
+        for (size_t i = 0; i < arcs.size(); ++i) {
+            if (arc_nodes) {
+                arc_nodes[2 * i] = (int)(arcs[i].first >> 32);
+                arc_nodes[2 * i + 1] = (int)(arcs[i].first & 0xffffffffu);
+            }
+            if (arc_radius) arc_radius[i] = arcs[i].second;
+        }
+    }
+    API_END
+}
+
+vemd_status vemd_mesh_write_vtk(vemd_mesh mh, const char* path) {
+    API_BEGIN
+    REQUIRE(mh && path, "null argument");
+    const Mesh& m = *mh->m;
+    FILE* f = std::fopen(path, "w");
+    if (!f) throw Error(VEMD_ERR_IO, std::string("cannot write ") + path);
+    std::fprintf(f, "# vtk DataFile Version 3.0\nvemd mesh\nASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS %d double\n", m.nn);
+    for (int i = 0; i < m.nn; ++i) std::fprintf(f, "%.17g %.17g 0\n", m.xy[2 * i], m.xy[2 * i + 1]);
+    std::fprintf(f, "CELLS %d %d\n", m.nc, (int)(m.nc + m.nodes.size()));
+    for (int c = 0; c < m.nc; ++c) {
+        std::fprintf(f, "%d", m.npc(c));
+        for (int j = m.off[c]; j < m.off[c + 1]; ++j) std::fprintf(f, " %d", m.nodes[j]);
+        std::fprintf(f, "\n");
+    }
+    std::fprintf(f, "CELL_TYPES %d\n", m.nc);
+    for (int c = 0; c < m.nc; ++c) std::fprintf(f, "7\n");
+    std::fclose(f);
+    API_END
+}
+
+void vemd_mesh_destroy(vemd_mesh m) { delete m; }
+
+vemd_status vemd_solver_create(vemd_mesh mh, const vemd_problem_desc* d, vemd_solver* out) {
+    API_BEGIN
+    REQUIRE(mh && d && out, "null argument");
+    std::unique_ptr<Solver> s(new Solver());
+    s->mesh = mh->m;
+    s->setup.reset(setup_build(*s->mesh, *d));
+    if (d->k <= 1) {
+        s->layout = layout_build(*s->mesh, *s->setup);
+        s->refresh_theta_dependent();
+    } else {
+        build_rhs_ref(*s->mesh, *s->setup, s->rhs_ref);
+    }
+    *out = new vemd_solver_s{std::move(s)};
+    API_END
+}
+
+vemd_status vemd_solver_sizes(vemd_solver sh, int64_t* z) {
+    API_BEGIN
+    REQUIRE(sh && z, "null argument");
+    const Solver& s = *sh->s;
+    z[0] = s.setup->nflux_total;
+    z[1] = s.setup->npres_total;
+    z[2] = s.setup->ntot;
+    z[3] = s.layout.nvec;
+    z[4] = s.layout.ncolours;
+    z[5] = s.layout.max_ne;
+    API_END
+}
+
+vemd_status vemd_solver_upload(vemd_solver sh, int device) {
+    API_BEGIN
+    REQUIRE(sh, "null solver");
+    if (sh->s->setup->sh.k > 1) throw Error(VEMD_ERR_ARG, "the device path supports k = 0 and k = 1");
+    dev_upload(*sh->s, device);
+    API_END
+}
+
+vemd_status vemd_set_theta(vemd_solver sh, const double* theta) {
+    API_BEGIN
+    REQUIRE(sh && theta, "null argument");
+    Solver& s = *sh->s;
+    for (int c = 0; c < s.mesh->nc; ++c)
+        if (!std::isfinite(theta[c])) throw Error(VEMD_ERR_ARG, "theta must be finite");
+    s.setup->theta.assign(theta, theta + s.mesh->nc);
+    if (s.setup->sh.k <= 1) {
+        s.refresh_theta_dependent();
+        dev_refresh(s);
+    }
+    API_END
+}
+
+vemd_status vemd_solve(vemd_solver sh, const vemd_solve_opts* o, const double* rhs, double* sol, vemd_solve_info* info) {
+    API_BEGIN
+    REQUIRE(sh && o, "null argument");
+    REQUIRE(o->rtol > 0 && o->rtol < 1, "rtol must be in (0, 1)");
+    REQUIRE(o->precision == VEMD_FP64 || o->precision == VEMD_MIXED, "precision must be VEMD_FP64 or VEMD_MIXED");
+    dev_solve(*sh->s, *o, rhs, sol, info);
+    API_END
+}
+
+vemd_status vemd_apply(vemd_solver sh, int precision, const double* x, double* y) {
+    API_BEGIN
+    REQUIRE(sh && x && y && x != y, "null or aliased argument");
+    REQUIRE(precision == VEMD_FP64 || precision == VEMD_FP32, "precision must be VEMD_FP64 or VEMD_FP32");
+    dev_apply_ref(*sh->s, precision, x, y);
+    API_END
+}
+
+vemd_status vemd_apply_device(vemd_solver sh, int precision, const void* x, void* y, void* stream) {
+    API_BEGIN
+    REQUIRE(sh && x && y && x != y, "null or aliased argument");
+    REQUIRE(precision == VEMD_FP64 || precision == VEMD_FP32, "precision must be VEMD_FP64 or VEMD_FP32");
+    dev_apply_int(*sh->s, precision, x, y, stream);
+    API_END
+}
+
+vemd_status vemd_rhs(vemd_solver sh, double* rhs) {
+    API_BEGIN
+    REQUIRE(sh && rhs, "null argument");
+    std::copy(sh->s->rhs_ref.begin(), sh->s->rhs_ref.end(), rhs);
+    API_END
+}
+
+vemd_status vemd_errors(vemd_solver sh, const double* sol, int nq, double* out) {
+    API_BEGIN
+    REQUIRE(sh && sol && out, "null argument");
+    errors_host(*sh->s, sol, nq, out);
+    API_END
+}
+
+vemd_status vemd_adjoint_theta(vemd_solver sh, const vemd_solve_opts* o, const double* sol, const double* dldx,
+                               double* grad, vemd_solve_info* info) {
+    API_BEGIN
+    REQUIRE(sh && o && sol && dldx && grad, "null argument");
+    if (sh->s->setup->desc.bc != VEMD_BC_DIRICHLET_P)
+        throw Error(VEMD_ERR_ARG, "the adjoint gradient is defined for the Dirichlet-pressure condition");
+    dev_adjoint(*sh->s, *o, sol, dldx, grad, info);
+    API_END
+}
+
+vemd_status vemd_element_matrices(vemd_solver sh, int cell, int* n_out, double* A, double* W, double* L,
+                                  double* f, double* sign, int64_t* gdof) {
+    API_BEGIN
+    REQUIRE(sh && n_out, "null argument");
+    const Solver& s = *sh->s;
+    REQUIRE(cell >= 0 && cell < s.mesh->nc, "cell out of range");
+    const Setup& S = *s.setup;
+    int n = S.sh.nflux(s.mesh->npc(cell)), q = S.sh.q;
+    *n_out = n;
+    const double* Ap = &S.A[S.ofs.a_off[cell]];
+    if (A)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) A[i * n + j] = Ap[i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i];
+    if (W) std::copy(&S.W[S.ofs.w_off[cell]], &S.W[S.ofs.w_off[cell]] + (size_t)q * n, W);
+    if (L) std::copy(&S.L[(size_t)cell * q], &S.L[(size_t)cell * q] + q, L);
+    if (f) std::copy(&S.f[(size_t)cell * q], &S.f[(size_t)cell * q] + q, f);
+    if (sign) std::copy(&S.sgn[S.ofs.v_off[cell]], &S.sgn[S.ofs.v_off[cell]] + n, sign);
+    if (gdof) std::copy(&S.gdof[S.ofs.v_off[cell]], &S.gdof[S.ofs.v_off[cell]] + n, gdof);
+    API_END
+}
+
+void vemd_solver_destroy(vemd_solver s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,975 @@
+// Device path of libvemd for gfx950: HBM operator store, coloured matrix-free
+// apply, fused preconditioned MINRES with on-device scalar recurrences, and
+// FP64 iterative refinement around an FP32 inner solve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <utility>
+
+#include "solver.h"
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw ::vemd::Error(VEMD_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+namespace vemd {
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct VT;
+template <>
+struct VT<double> {
+    using V = d2v;
+    static constexpr int W = 2;
+};
+template <>
+struct VT<float> {
+    using V = f4v;
+    static constexpr int W = 4;
+};
+
+template <typename F, int... I>
+__device__ __forceinline__ void static_for(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+__host__ __device__ constexpr int tri_row(int t) {
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    return i;
+}
+
+// Krylov scalar state kept in device memory: the host never reads it inside
+// the iteration loop except for the periodic convergence check.
+struct MState {
+    double beta1, beta, oldb, alfa, dbar, epsln, phibar, cs, sn, rho;
+    double c_az, c_r1, c_r2;
+    double u_v, u_w1, u_w2, u_x;
+    double tol;
+    int itn, maxit, done, pending;
+};
+
+template <typename T>
+struct ApplyArgs {
+    const GroupDesc* groups;
+    const int* tile_group;
+    const T* A;
+    const T* W;
+    const T* L;
+    const T* scale;
+    const uint32_t* idx;
+    const T* x;          // input when st == nullptr
+    const T* xs[2];      // inputs selected by the Krylov iteration parity when st != nullptr
+    T* y;
+    int64_t cbase, cpad, lam_idx;
+    int tile_base, ntiles, per_xcd;
+    double* pdot;        // per-tile partial of x.Kx (nullptr: not needed)
+    double* plam;        // per-tile partial of the multiplier row
+    const MState* st;
+};
+
+// One cell per lane: gather signed-folded local dofs, y_E = K_E x_E, scatter
+// edge dofs by colour (first writer stores, later colours accumulate).
+template <typename T, int K, int NE>
+__device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __restrict__ x, const GroupDesc& g, int lane,
+                                          double& dot, double& lamp) {
+    using V = typename VT<T>::V;
+    constexpr int VW = VT<T>::W;
+    constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE, N = NED + NI;
+    constexpr int NA = N * (N + 1) / 2, NAV = (NA + VW - 1) / VW, NW = Q * N, NWV = (NW + VW - 1) / VW;
+    const int64_t npad = g.npad;
+    const int64_t gpos = g.gpos0 + lane;
+    uint32_t id[NED];
+    T u[N], p[Q], acc[N], yp[Q];
+#pragma unroll
+    for (int j = 0; j < NED; ++j) id[j] = a.idx[g.i_off + j * npad + lane];
+#pragma unroll
+    for (int j = 0; j < NED; ++j) u[j] = x[id[j] & 0x7fffffffu];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) u[NED + j] = x[a.cbase + j * a.cpad + gpos];
+#pragma unroll
+    for (int r = 0; r < Q; ++r) p[r] = x[a.cbase + (NI + r) * a.cpad + gpos];
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = T(0);
+    const V* Av = reinterpret_cast<const V*>(a.A + g.a_off) + lane;
+    static_for(
+        [&](auto vv) {
+            constexpr int v = decltype(vv)::value;
+            V av = __builtin_nontemporal_load(Av + v * npad);
+            static_for(
+                [&](auto ll) {
+                    constexpr int l = decltype(ll)::value;
+                    constexpr int t = v * VW + l;
+                    if constexpr (t < NA) {
+                        constexpr int i = tri_row(t);
+                        constexpr int j = t - i * (i + 1) / 2;
+                        acc[i] += av[l] * u[j];
+                        if constexpr (i != j) acc[j] += av[l] * u[i];
+                    }
+                },
+                std::make_integer_sequence<int, VW>{});
+        },
+        std::make_integer_sequence<int, NAV>{});
+    const T s = a.scale[gpos];
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] *= s;
+#pragma unroll
+    for (int r = 0; r < Q; ++r) yp[r] = T(0);
+    const V* Wv = reinterpret_cast<const V*>(a.W + g.w_off) + lane;
+    static_for(
+        [&](auto vv) {
+            constexpr int v = decltype(vv)::value;
+            V wv = __builtin_nontemporal_load(Wv + v * npad);
+            static_for(
+                [&](auto ll) {
+                    constexpr int l = decltype(ll)::value;
+                    constexpr int t = v * VW + l;
+                    if constexpr (t < NW) {
+                        constexpr int r = t / N, i = t % N;
+                        acc[i] -= wv[l] * p[r];
+                        yp[r] -= wv[l] * u[i];
+                    }
+                },
+                std::make_integer_sequence<int, VW>{});
+        },
+        std::make_integer_sequence<int, NWV>{});
+    if (a.lam_idx >= 0) {
+        const T xl = x[a.lam_idx];
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            T l = a.L[g.l_off + r * npad + lane];
+            yp[r] -= l * xl;
+            lamp -= (double)l * (double)p[r];
+        }
+    }
+    if (a.pdot) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) dot += (double)u[i] * (double)acc[i];
+#pragma unroll
+        for (int r = 0; r < Q; ++r) dot += (double)p[r] * (double)yp[r];
+    }
+    T* __restrict__ y = a.y;
+#pragma unroll
+    for (int j = 0; j < NED; ++j) {
+        uint32_t k = id[j] & 0x7fffffffu;
+        if (id[j] & 0x80000000u)
+            y[k] += acc[j];
+        else
+            y[k] = acc[j];
+    }
+#pragma unroll
+    for (int j = 0; j < NI; ++j) y[a.cbase + j * a.cpad + gpos] = acc[NED + j];
+#pragma unroll
+    for (int r = 0; r < Q; ++r) y[a.cbase + (NI + r) * a.cpad + gpos] = yp[r];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(64) void k_apply(ApplyArgs<T> a) {
+    if (a.st && a.st->done) return;
+    const int b = blockIdx.x;
+    const int t = (b & 7) * a.per_xcd + (b >> 3);  // consecutive tiles stay on one XCD (one L2)
+    if (t >= a.ntiles) return;
+    const int tile = a.tile_base + t;
+    const GroupDesc g = a.groups[a.tile_group[tile]];
+    const int lane = (tile - g.tile0) * 64 + threadIdx.x;
+    const T* x = a.st ? a.xs[a.st->itn & 1] : a.x;
+    double dot = 0.0, lamp = 0.0;
+    switch (g.ne) {
+        case 3: cell_body<T, K, 3>(a, x, g, lane, dot, lamp); break;
+        case 4: cell_body<T, K, 4>(a, x, g, lane, dot, lamp); break;
+        case 5: cell_body<T, K, 5>(a, x, g, lane, dot, lamp); break;
+        case 6: cell_body<T, K, 6>(a, x, g, lane, dot, lamp); break;
+        case 7: cell_body<T, K, 7>(a, x, g, lane, dot, lamp); break;
+        case 8: cell_body<T, K, 8>(a, x, g, lane, dot, lamp); break;
+        case 9: cell_body<T, K, 9>(a, x, g, lane, dot, lamp); break;
+        case 10: cell_body<T, K, 10>(a, x, g, lane, dot, lamp); break;
+        default: break;
+    }
+    if (a.pdot || a.lam_idx >= 0) {
+        dot = wave_sum(dot);
+        lamp = wave_sum(lamp);
+        if (threadIdx.x == 0) {
+            if (a.pdot) a.pdot[tile] = dot;
+            if (a.plam) a.plam[tile] = lamp;
+        }
+    }
+}
+
+// Sums n partials in a fixed order with one block of 256 threads.
+__device__ double block_sum_fixed(const double* p, int n) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += p[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Multiplier row of a plain apply: y_lambda = sum of per-tile partials.
+template <typename T>
+__global__ void k_lam_row(const double* plam, int ntiles, T* y, int64_t lam_idx) {
+    double s = block_sum_fixed(plam, ntiles);
+    if (threadIdx.x == 0) y[lam_idx] = (T)s;
+}
+
+// After the apply of iteration itn: alfa and the Lanczos coefficients.
+template <typename T>
+__global__ void k_finA(MState* st, const double* pdot, const double* plam, int ntiles, T* Az, const T* Z0,
+                       const T* Z1, int64_t lam_idx) {
+    if (st->done) return;
+    double zaz = block_sum_fixed(pdot, ntiles);
+    double yl = lam_idx >= 0 ? block_sum_fixed(plam, ntiles) : 0.0;
+    if (threadIdx.x == 0) {
+        if (lam_idx >= 0) {
+            const T* z = (st->itn & 1) ? Z1 : Z0;
+            Az[lam_idx] = (T)yl;
+            zaz += (double)z[lam_idx] * yl;
+        }
+        double beta = st->beta;
+        double alfa = zaz / (beta * beta) - (st->itn > 0 ? st->rho / st->oldb : 0.0);
+        st->alfa = alfa;
+        st->c_az = 1.0 / beta;
+        st->c_r1 = st->itn > 0 ? beta / st->oldb : 0.0;
+        st->c_r2 = alfa / beta;
+    }
+}
+
+template <typename T>
+struct UpdArgs {
+    T* R[2];
+    T* Z[2];
+    T* WB[2];
+    T* x;
+    const T* Az;
+    const T* minv;
+    int64_t n;
+    double* p1;
+    double* p2;
+    const MState* st;
+};
+
+// Lanczos step, preconditioner, and the solution update of the previous
+// iteration (delayed by one so that it needs no extra pass), in one sweep.
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
+    __shared__ double s1[256], s2[256];
+    const MState* st = a.st;
+    if (st->done) return;
+    const int it = st->itn;
+    T* __restrict__ r2 = a.R[it & 1];
+    T* __restrict__ r1 = a.R[(it + 1) & 1];
+    T* __restrict__ zo = a.Z[(it + 1) & 1];
+    T* __restrict__ wn = a.WB[(it + 1) & 1];
+    const T* __restrict__ wp = a.WB[it & 1];
+    const T caz = (T)st->c_az, cr1 = (T)st->c_r1, cr2 = (T)st->c_r2;
+    const bool pend = st->pending;
+    const T uv = (T)st->u_v, uw1 = (T)st->u_w1, uw2 = (T)st->u_w2, ux = (T)st->u_x;
+    double d1 = 0.0, d2 = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
+        T r2i = r2[i];
+        T y = caz * a.Az[i] - cr1 * r1[i] - cr2 * r2i;
+        T zn = a.minv[i] * y;
+        if (pend) {
+            T w = uv * zo[i] - uw1 * wn[i] - uw2 * wp[i];
+            wn[i] = w;
+            a.x[i] += ux * w;
+        }
+        r1[i] = y;
+        zo[i] = zn;
+        d1 += (double)y * (double)zn;
+        d2 += (double)zn * (double)r2i;
+    }
+    s1[threadIdx.x] = d1;
+    s2[threadIdx.x] = d2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.p1[blockIdx.x] = s1[0];
+        a.p2[blockIdx.x] = s2[0];
+    }
+}
+
+// Givens recurrence of MINRES (Paige and Saunders) and the stopping test.
+__global__ void k_finB(MState* st, const double* p1, const double* p2, int nb) {
+    if (st->done) return;
+    double b2 = block_sum_fixed(p1, nb);
+    double rho = block_sum_fixed(p2, nb);
+    if (threadIdx.x == 0) {
+        double beta_new = std::sqrt(b2 > 0.0 ? b2 : 0.0);
+        st->rho = rho;
+        st->oldb = st->beta;
+        st->beta = beta_new;
+        double oldeps = st->epsln;
+        double delta = st->cs * st->dbar + st->sn * st->alfa;
+        double gbar = st->sn * st->dbar - st->cs * st->alfa;
+        st->epsln = st->sn * beta_new;
+        st->dbar = -st->cs * beta_new;
+        double gamma = std::sqrt(gbar * gbar + beta_new * beta_new);
+        if (gamma < 1e-300) gamma = 1e-300;
+        st->cs = gbar / gamma;
+        st->sn = beta_new / gamma;
+        double phi = st->cs * st->phibar;
+        st->phibar = st->sn * st->phibar;
+        st->u_v = 1.0 / (st->oldb * gamma);
+        st->u_w1 = oldeps / gamma;
+        st->u_w2 = delta / gamma;
+        st->u_x = phi;
+        st->pending = 1;
+        st->itn += 1;
+        if (st->phibar <= st->tol || st->itn >= st->maxit || !(beta_new > 0.0)) st->done = 1;
+    }
+}
+
+// r2 = b, r1 = 0, z = M^{-1} b, w = 0, x = 0 and partials of b.z.
+template <typename T>
+__global__ __launch_bounds__(256) void k_init(UpdArgs<T> a, const T* b) {
+    __shared__ double s1[256];
+    double d = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
+        T bi = b[i];
+        T z = a.minv[i] * bi;
+        a.R[0][i] = bi;
+        a.R[1][i] = T(0);
+        a.Z[0][i] = z;
+        a.Z[1][i] = T(0);
+        a.WB[0][i] = T(0);
+        a.WB[1][i] = T(0);
+        a.x[i] = T(0);
+        d += (double)bi * (double)z;
+    }
+    s1[threadIdx.x] = d;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.p1[blockIdx.x] = s1[0];
+}
+
+__global__ void k_finInit(MState* st, const double* p1, int nb, double rtol, int maxit) {
+    double b2 = block_sum_fixed(p1, nb);
+    if (threadIdx.x == 0) {
+        double beta1 = std::sqrt(b2 > 0.0 ? b2 : 0.0);
+        MState s;
+        std::memset(&s, 0, sizeof s);
+        s.beta1 = beta1;
+        s.beta = beta1;
+        s.phibar = beta1;
+        s.cs = -1.0;
+        s.tol = rtol * beta1;
+        s.maxit = maxit;
+        s.done = (beta1 > 0.0 && maxit > 0) ? 0 : 1;
+        *st = s;
+    }
+}
+
+// Pending solution update of the last iteration.
+template <typename T>
+__global__ __launch_bounds__(256) void k_flush(UpdArgs<T> a) {
+    const MState* st = a.st;
+    if (!st->pending) return;
+    const int m = st->itn - 1;
+    const T* z = a.Z[m & 1];
+    const T* wpp = a.WB[m & 1];
+    const T* wp = a.WB[(m + 1) & 1];
+    const T uv = (T)st->u_v, uw1 = (T)st->u_w1, uw2 = (T)st->u_w2, ux = (T)st->u_x;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
+        T w = uv * z[i] - uw1 * wpp[i] - uw2 * wp[i];
+        a.x[i] += ux * w;
+    }
+}
+
+// r = b - y with partial sums of |r|^2 (FP64).
+__global__ __launch_bounds__(256) void k_residual(const double* b, const double* y, double* r, int64_t n, double* p) {
+    __shared__ double s1[256];
+    double d = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        double v = b[i] - y[i];
+        r[i] = v;
+        d += v * v;
+    }
+    s1[threadIdx.x] = d;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) p[blockIdx.x] = s1[0];
+}
+
+__global__ void k_sum(const double* p, int n, double* out) {
+    double s = block_sum_fixed(p, n);
+    if (threadIdx.x == 0) *out = s;
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void k_cast(const S* a, D* b, int64_t n) {
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) b[i] = (D)a[i];
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_add(double* x, const S* d, int64_t n) {
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) x[i] += (double)d[i];
+}
+
+// Adjoint contraction per cell: g_E = lambda_E^T (exp(-theta_E) A_E) x_E.
+template <typename T, int K, int NE>
+__device__ __forceinline__ double cell_quad(const ApplyArgs<T>& a, const T* lam, const T* x, const GroupDesc& g,
+                                            int lane) {
+    using V = typename VT<T>::V;
+    constexpr int VW = VT<T>::W;
+    constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE, N = NED + NI;
+    constexpr int NA = N * (N + 1) / 2, NAV = (NA + VW - 1) / VW;
+    const int64_t npad = g.npad, gpos = g.gpos0 + lane;
+    T u[N], l[N];
+#pragma unroll
+    for (int j = 0; j < NED; ++j) {
+        uint32_t k = a.idx[g.i_off + j * npad + lane] & 0x7fffffffu;
+        u[j] = x[k];
+        l[j] = lam[k];
+    }
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        u[NED + j] = x[a.cbase + j * a.cpad + gpos];
+        l[NED + j] = lam[a.cbase + j * a.cpad + gpos];
+    }
+    double s = 0.0;
+    const V* Av = reinterpret_cast<const V*>(a.A + g.a_off) + lane;
+    static_for(
+        [&](auto vv) {
+            constexpr int v = decltype(vv)::value;
+            V av = Av[v * npad];
+            static_for(
+                [&](auto ll) {
+                    constexpr int li = decltype(ll)::value;
+                    constexpr int t = v * VW + li;
+                    if constexpr (t < NA) {
+                        constexpr int i = tri_row(t);
+                        constexpr int j = t - i * (i + 1) / 2;
+                        double av_ = (double)av[li];
+                        s += av_ * (double)l[i] * (double)u[j];
+                        if constexpr (i != j) s += av_ * (double)l[j] * (double)u[i];
+                    }
+                },
+                std::make_integer_sequence<int, VW>{});
+        },
+        std::make_integer_sequence<int, NAV>{});
+    return s * (double)a.scale[gpos];
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(64) void k_adjoint(ApplyArgs<T> a, const T* lam, const T* x, const int* cell_of_gpos,
+                                              double* grad) {
+    const int tile = blockIdx.x;
+    const GroupDesc g = a.groups[a.tile_group[tile]];
+    const int lane = (tile - g.tile0) * 64 + threadIdx.x;
+    double v = 0.0;
+    switch (g.ne) {
+        case 3: v = cell_quad<T, K, 3>(a, lam, x, g, lane); break;
+        case 4: v = cell_quad<T, K, 4>(a, lam, x, g, lane); break;
+        case 5: v = cell_quad<T, K, 5>(a, lam, x, g, lane); break;
+        case 6: v = cell_quad<T, K, 6>(a, lam, x, g, lane); break;
+        case 7: v = cell_quad<T, K, 7>(a, lam, x, g, lane); break;
+        case 8: v = cell_quad<T, K, 8>(a, lam, x, g, lane); break;
+        case 9: v = cell_quad<T, K, 9>(a, lam, x, g, lane); break;
+        case 10: v = cell_quad<T, K, 10>(a, lam, x, g, lane); break;
+        default: break;
+    }
+    int c = cell_of_gpos[g.gpos0 + lane];
+    if (c >= 0) grad[c] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side of the device path.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct DevOps {
+    T* A = nullptr;
+    T* W = nullptr;
+    T* L = nullptr;
+    T* scale = nullptr;
+    T* minv = nullptr;
+    GroupDesc* groups = nullptr;
+};
+
+template <typename T>
+struct DevKrylov {
+    T* R[2] = {nullptr, nullptr};
+    T* Z[2] = {nullptr, nullptr};
+    T* WB[2] = {nullptr, nullptr};
+    T* x = nullptr;
+    T* Az = nullptr;
+    T* b = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+};
+
+struct DeviceState {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevOps<double> o64;
+    DevOps<float> o32;
+    uint32_t* idx = nullptr;
+    int* tile_group = nullptr;
+    int* cell_of_gpos = nullptr;
+    int ntiles = 0;
+    double* pdot = nullptr;
+    double* plam = nullptr;
+    double* p1 = nullptr;
+    double* p2 = nullptr;
+    int nblk = 0;
+    MState* st = nullptr;
+    MState* st_host = nullptr;  // pinned
+    double* dsum = nullptr;
+    double* hsum = nullptr;     // pinned
+    DevKrylov<double> k64;
+    DevKrylov<float> k32;
+    double* x64 = nullptr;  // outer solution
+    double* r64 = nullptr;  // outer residual
+    double* y64 = nullptr;
+    double* b64 = nullptr;
+    float* r32 = nullptr;
+    std::vector<void*> allocs;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+static void* dalloc(DeviceState& d, size_t bytes) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+    d.allocs.push_back(p);
+    return p;
+}
+
+template <typename T>
+static T* dupload(DeviceState& d, const std::vector<T>& v) {
+    T* p = (T*)dalloc(d, v.size() * sizeof(T));
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+template <typename T>
+static void upload_ops(DeviceState& d, const Layout& L, DevOps<T>& o, const std::vector<T>& A, const std::vector<T>& W,
+                       const std::vector<T>& Lv, bool wide) {
+    o.A = dupload(d, A);
+    o.W = dupload(d, W);
+    o.L = dupload(d, Lv);
+    std::vector<GroupDesc> g = L.groups;
+    if (wide)
+        for (size_t i = 0; i < g.size(); ++i) {
+            g[i].a_off = L.a_off4[i];
+            g[i].w_off = L.w_off4[i];
+        }
+    o.groups = dupload(d, g);
+}
+
+template <typename T>
+static void alloc_krylov(DeviceState& d, DevKrylov<T>& k, int64_t n) {
+    for (int i = 0; i < 2; ++i) {
+        k.R[i] = (T*)dalloc(d, n * sizeof(T));
+        k.Z[i] = (T*)dalloc(d, n * sizeof(T));
+        k.WB[i] = (T*)dalloc(d, n * sizeof(T));
+        HIPCHK(hipMemset(k.R[i], 0, n * sizeof(T)));
+        HIPCHK(hipMemset(k.Z[i], 0, n * sizeof(T)));
+        HIPCHK(hipMemset(k.WB[i], 0, n * sizeof(T)));
+    }
+    k.x = (T*)dalloc(d, n * sizeof(T));
+    k.Az = (T*)dalloc(d, n * sizeof(T));
+    k.b = (T*)dalloc(d, n * sizeof(T));
+    HIPCHK(hipMemset(k.x, 0, n * sizeof(T)));
+    HIPCHK(hipMemset(k.Az, 0, n * sizeof(T)));
+    HIPCHK(hipMemset(k.b, 0, n * sizeof(T)));
+}
+
+static void upload_theta(Solver& s) {
+    DeviceState& d = *s.dev;
+    std::vector<float> f(s.scale.begin(), s.scale.end());
+    HIPCHK(hipMemcpy(d.o64.scale, s.scale.data(), s.scale.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.o32.scale, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> m(s.minv.begin(), s.minv.end());
+    HIPCHK(hipMemcpy(d.o64.minv, s.minv.data(), s.minv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.o32.minv, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+void dev_upload(Solver& s, int device) {
+    if (device_count() <= device || device < 0)
+        throw Error(VEMD_ERR_NO_DEVICE, "no HIP device " + std::to_string(device) + " (the solve path has no CPU fallback)");
+    dev_free(s);
+    HIPCHK(hipSetDevice(device));
+    s.dev = new DeviceState();
+    DeviceState& d = *s.dev;
+    d.device = device;
+    s.device = device;
+    const Layout& L = s.layout;
+    HIPCHK(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    {
+        std::vector<double> A64, W64, L64;
+        std::vector<float> A32, W32, L32;
+        pack_store(*s.mesh, *s.setup, L, A64, W64, L64, A32, W32, L32);
+        upload_ops(d, L, d.o64, A64, W64, L64, false);
+        upload_ops(d, L, d.o32, A32, W32, L32, true);
+    }
+    d.idx = dupload(d, L.idx);
+    d.tile_group = dupload(d, L.tile_group);
+    d.cell_of_gpos = dupload(d, L.cell_of_gpos);
+    d.ntiles = (int)L.tile_group.size();
+    d.o64.scale = (double*)dalloc(d, L.cpad * sizeof(double));
+    d.o32.scale = (float*)dalloc(d, L.cpad * sizeof(float));
+    d.o64.minv = (double*)dalloc(d, L.nvec * sizeof(double));
+    d.o32.minv = (float*)dalloc(d, L.nvec * sizeof(float));
+    d.pdot = (double*)dalloc(d, d.ntiles * sizeof(double));
+    d.plam = (double*)dalloc(d, d.ntiles * sizeof(double));
+    d.nblk = (int)std::min<int64_t>(2048, (L.nvec + 255) / 256);
+    d.p1 = (double*)dalloc(d, d.nblk * sizeof(double));
+    d.p2 = (double*)dalloc(d, d.nblk * sizeof(double));
+    d.st = (MState*)dalloc(d, sizeof(MState));
+    HIPCHK(hipMemset(d.st, 0, sizeof(MState)));
+    HIPCHK(hipHostMalloc((void**)&d.st_host, sizeof(MState), hipHostMallocDefault));
+    d.dsum = (double*)dalloc(d, sizeof(double));
+    HIPCHK(hipHostMalloc((void**)&d.hsum, sizeof(double), hipHostMallocDefault));
+    alloc_krylov(d, d.k64, L.nvec);
+    alloc_krylov(d, d.k32, L.nvec);
+    d.x64 = (double*)dalloc(d, L.nvec * sizeof(double));
+    d.r64 = (double*)dalloc(d, L.nvec * sizeof(double));
+    d.y64 = (double*)dalloc(d, L.nvec * sizeof(double));
+    d.b64 = (double*)dalloc(d, L.nvec * sizeof(double));
+    d.r32 = (float*)dalloc(d, L.nvec * sizeof(float));
+    for (double* p : {d.x64, d.r64, d.y64, d.b64}) HIPCHK(hipMemset(p, 0, L.nvec * sizeof(double)));
+    HIPCHK(hipMemset(d.r32, 0, L.nvec * sizeof(float)));
+    HIPCHK(hipEventCreate(&d.e0));
+    HIPCHK(hipEventCreate(&d.e1));
+    upload_theta(s);
+}
+
+void dev_refresh(Solver& s) {
+    if (!s.dev) return;
+    HIPCHK(hipSetDevice(s.dev->device));
+    upload_theta(s);
+}
+
+template <typename T>
+static void destroy_graph(DevKrylov<T>& k) {
+    if (k.graph) (void)hipGraphExecDestroy(k.graph);
+    k.graph = nullptr;
+}
+
+void dev_free(Solver& s) {
+    if (!s.dev) return;
+    DeviceState& d = *s.dev;
+    (void)hipSetDevice(d.device);
+    destroy_graph(d.k64);
+    destroy_graph(d.k32);
+    for (void* p : d.allocs) (void)hipFree(p);
+    if (d.st_host) (void)hipHostFree(d.st_host);
+    if (d.hsum) (void)hipHostFree(d.hsum);
+    if (d.e0) (void)hipEventDestroy(d.e0);
+    if (d.e1) (void)hipEventDestroy(d.e1);
+    if (d.stream) (void)hipStreamDestroy(d.stream);
+    delete s.dev;
+    s.dev = nullptr;
+}
+
+template <typename T>
+static const DevOps<T>& ops(DeviceState& d);
+template <>
+const DevOps<double>& ops<double>(DeviceState& d) {
+    return d.o64;
+}
+template <>
+const DevOps<float>& ops<float>(DeviceState& d) {
+    return d.o32;
+}
+
+template <typename T>
+static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, const MState* st, const T* x0, const T* x1, bool dot) {
+    DeviceState& d = *s.dev;
+    const DevOps<T>& o = ops<T>(d);
+    const Layout& L = s.layout;
+    ApplyArgs<T> a;
+    a.groups = o.groups;
+    a.tile_group = d.tile_group;
+    a.A = o.A;
+    a.W = o.W;
+    a.L = o.L;
+    a.scale = o.scale;
+    a.idx = d.idx;
+    a.x = x;
+    a.xs[0] = x0;
+    a.xs[1] = x1;
+    a.y = y;
+    a.cbase = L.cbase;
+    a.cpad = L.cpad;
+    a.lam_idx = L.lam ? L.lam_idx : -1;
+    a.pdot = dot ? d.pdot : nullptr;
+    a.plam = d.plam;
+    a.st = st;
+    a.tile_base = 0;
+    a.ntiles = 0;
+    a.per_xcd = 0;
+    return a;
+}
+
+// Launches the colour sweeps of one apply on the solver stream.
+template <typename T>
+static void launch_apply(Solver& s, ApplyArgs<T> a) {
+    const Layout& L = s.layout;
+    hipStream_t st = s.dev->stream;
+    for (int c = 0; c < L.ncolours; ++c) {
+        a.tile_base = L.colour_tile0[c];
+        a.ntiles = L.colour_tile0[c + 1] - L.colour_tile0[c];
+        if (a.ntiles <= 0) continue;
+        a.per_xcd = (a.ntiles + 7) / 8;
+        dim3 grid(8 * a.per_xcd), block(64);
+        if (L.k == 0)
+            hipLaunchKernelGGL((k_apply<T, 0>), grid, block, 0, st, a);
+        else
+            hipLaunchKernelGGL((k_apply<T, 1>), grid, block, 0, st, a);
+    }
+}
+
+template <typename T>
+static void apply_plain(Solver& s, const T* x, T* y) {
+    ApplyArgs<T> a = make_args<T>(s, x, y, nullptr, nullptr, nullptr, false);
+    launch_apply(s, a);
+    if (s.layout.lam)
+        hipLaunchKernelGGL((k_lam_row<T>), dim3(1), dim3(256), 0, s.dev->stream, s.dev->plam, s.dev->ntiles, y,
+                           s.layout.lam_idx);
+    HIPCHK(hipGetLastError());
+}
+
+void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* stream) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    HIPCHK(hipSetDevice(s.dev->device));
+    hipStream_t saved = s.dev->stream;
+    if (stream) s.dev->stream = (hipStream_t)stream;
+    try {
+        if (precision == VEMD_FP64)
+            apply_plain<double>(s, (const double*)x, (double*)y);
+        else
+            apply_plain<float>(s, (const float*)x, (float*)y);
+    } catch (...) {
+        s.dev->stream = saved;
+        throw;
+    }
+    s.dev->stream = saved;
+}
+
+void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    DeviceState& d = *s.dev;
+    HIPCHK(hipSetDevice(d.device));
+    const Layout& L = s.layout;
+    std::vector<double> xi(L.nvec, 0.0), yi(L.nvec);
+    s.to_internal(x_ref, xi.data(), false);
+    if (precision == VEMD_FP64) {
+        HIPCHK(hipMemcpyAsync(d.x64, xi.data(), L.nvec * sizeof(double), hipMemcpyHostToDevice, d.stream));
+        apply_plain<double>(s, d.x64, d.y64);
+        HIPCHK(hipMemcpyAsync(yi.data(), d.y64, L.nvec * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    } else {
+        std::vector<float> xf(xi.begin(), xi.end()), yf(L.nvec);
+        HIPCHK(hipMemcpyAsync(d.r32, xf.data(), L.nvec * sizeof(float), hipMemcpyHostToDevice, d.stream));
+        apply_plain<float>(s, d.r32, d.k32.Az);
+        HIPCHK(hipMemcpyAsync(yf.data(), d.k32.Az, L.nvec * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));
+        for (int64_t i = 0; i < L.nvec; ++i) yi[i] = yf[i];
+    }
+    HIPCHK(hipStreamSynchronize(d.stream));
+    s.to_reference(yi.data(), y_ref, true);
+}
+
+// One MINRES iteration: colour sweeps, alfa, Lanczos/precondition/update, Givens.
+template <typename T>
+static void enqueue_iteration(Solver& s, DevKrylov<T>& k) {
+    DeviceState& d = *s.dev;
+    const Layout& L = s.layout;
+    ApplyArgs<T> a = make_args<T>(s, nullptr, k.Az, d.st, k.Z[0], k.Z[1], true);
+    launch_apply(s, a);
+    hipLaunchKernelGGL((k_finA<T>), dim3(1), dim3(256), 0, d.stream, d.st, d.pdot, d.plam, d.ntiles, k.Az, k.Z[0],
+                       k.Z[1], L.lam ? L.lam_idx : (int64_t)-1);
+    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x, k.Az, ops<T>(d).minv, L.nvec, d.p1, d.p2, d.st};
+    hipLaunchKernelGGL((k_update<T>), dim3(d.nblk), dim3(256), 0, d.stream, u);
+    hipLaunchKernelGGL(k_finB, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.p2, d.nblk);
+}
+
+// Preconditioned MINRES on K x = b (internal order, precision T): b in k.b,
+// solution in k.x.  Stops when the M^{-1}-norm residual estimate falls to
+// rtol times its initial value.  Returns the iteration count.
+template <typename T>
+static int minres(Solver& s, DevKrylov<T>& k, double rtol, int maxit, int check_every, bool use_graph) {
+    DeviceState& d = *s.dev;
+    const Layout& L = s.layout;
+    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x, k.Az, ops<T>(d).minv, L.nvec, d.p1, d.p2, d.st};
+    check_every = std::max(1, check_every);
+    if (use_graph && k.graph_iters != check_every) {
+        HIPCHK(hipStreamSynchronize(d.stream));
+        destroy_graph(k);
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k);
+        HIPCHK(hipStreamEndCapture(d.stream, &g));
+        HIPCHK(hipGraphInstantiate(&k.graph, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        k.graph_iters = check_every;
+    }
+    hipLaunchKernelGGL((k_init<T>), dim3(d.nblk), dim3(256), 0, d.stream, u, (const T*)k.b);
+    hipLaunchKernelGGL(k_finInit, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.nblk, rtol, maxit);
+    HIPCHK(hipGetLastError());
+    while (true) {
+        if (use_graph)
+            HIPCHK(hipGraphLaunch(k.graph, d.stream));
+        else
+            for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k);
+        HIPCHK(hipMemcpyAsync(d.st_host, d.st, sizeof(MState), hipMemcpyDeviceToHost, d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));
+        if (d.st_host->done) break;
+    }
+    hipLaunchKernelGGL((k_flush<T>), dim3(d.nblk), dim3(256), 0, d.stream, u);
+    HIPCHK(hipGetLastError());
+    return d.st_host->itn;
+}
+
+// r64 = b64 - K x64 and returns ||r64||_2.
+static double residual64(Solver& s) {
+    DeviceState& d = *s.dev;
+    apply_plain<double>(s, d.x64, d.y64);
+    hipLaunchKernelGGL(k_residual, dim3(d.nblk), dim3(256), 0, d.stream, d.b64, d.y64, d.r64, s.layout.nvec, d.p1);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, d.stream, d.p1, d.nblk, d.dsum);
+    HIPCHK(hipMemcpyAsync(d.hsum, d.dsum, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    return std::sqrt(*d.hsum);
+}
+
+// Solve with b64 already on the device; solution in x64.
+static void solve_device(Solver& s, const vemd_solve_opts& o, vemd_solve_info* info) {
+    DeviceState& d = *s.dev;
+    const int64_t n = s.layout.nvec;
+    HIPCHK(hipMemsetAsync(d.x64, 0, n * sizeof(double), d.stream));
+    double bnorm = residual64(s);  // r = b
+    double rel = bnorm > 0 ? 1.0 : 0.0;
+    int iters = 0, refin = 0;
+    int maxit = o.max_iters > 0 ? o.max_iters : 1000000;
+    bool graph = o.use_graph != 0;
+    int chk = o.check_every > 0 ? o.check_every : 32;
+    while (rel > o.rtol && iters < maxit && refin < 100) {
+        if (o.precision == VEMD_FP64) {
+            HIPCHK(hipMemcpyAsync(d.k64.b, d.r64, n * sizeof(double), hipMemcpyDeviceToDevice, d.stream));
+            double target = std::min(0.5, 0.5 * o.rtol / rel);
+            iters += minres<double>(s, d.k64, target, maxit - iters, chk, graph);
+            hipLaunchKernelGGL((k_add<double>), dim3(d.nblk), dim3(256), 0, d.stream, d.x64, d.k64.x, n);
+        } else {
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(d.nblk), dim3(256), 0, d.stream, d.r64, d.k32.b, n);
+            double target = std::max(o.inner_rtol > 0 ? o.inner_rtol : 1e-4, 0.5 * o.rtol / rel);
+            iters += minres<float>(s, d.k32, target, maxit - iters, chk, graph);
+            hipLaunchKernelGGL((k_add<float>), dim3(d.nblk), dim3(256), 0, d.stream, d.x64, d.k32.x, n);
+        }
+        ++refin;
+        rel = residual64(s) / bnorm;
+    }
+    if (info) {
+        info->iterations = iters;
+        info->refinements = refin;
+        info->rel_residual = rel;
+    }
+    if (rel > o.rtol)
+        throw Error(VEMD_ERR_NOT_CONVERGED, "not converged: relative residual " + std::to_string(rel) + " after " +
+                                                std::to_string(iters) + " iterations");
+}
+
+void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, double* sol_ref, vemd_solve_info* info) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    DeviceState& d = *s.dev;
+    HIPCHK(hipSetDevice(d.device));
+    const Layout& L = s.layout;
+    std::vector<double> bi(L.nvec, 0.0);
+    s.to_internal(rhs_ref ? rhs_ref : s.rhs_ref.data(), bi.data(), true);
+    HIPCHK(hipMemcpyAsync(d.b64, bi.data(), L.nvec * sizeof(double), hipMemcpyHostToDevice, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    auto t0 = std::chrono::steady_clock::now();
+    vemd_solve_info tmp{};
+    std::string err;
+    int code = 0;
+    try {
+        solve_device(s, o, &tmp);
+    } catch (const Error& e) {
+        if (e.code != VEMD_ERR_NOT_CONVERGED) throw;
+        err = e.what();
+        code = e.code;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    tmp.seconds = std::chrono::duration<double>(t1 - t0).count();
+    HIPCHK(hipMemcpyAsync(bi.data(), d.x64, L.nvec * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    if (sol_ref) s.to_reference(bi.data(), sol_ref, false);
+    if (info) *info = tmp;
+    if (code) throw Error(code, err);
+}
+
+void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, const double* dldx_ref, double* grad,
+                 vemd_solve_info* info) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    DeviceState& d = *s.dev;
+    HIPCHK(hipSetDevice(d.device));
+    const Layout& L = s.layout;
+    // adjoint rhs in internal form: the internal operator is S K~ with S the
+    // row scaling, symmetric; K~^T lambda~ = g  <=>  (S K~) (S^{-1}... ) -- we
+    // solve K_int mu = g_int with g_int = g (no row scaling: K_int symmetric,
+    // lambda~ = S mu).
+    std::vector<double> gi(L.nvec, 0.0), xi(L.nvec, 0.0);
+    s.to_internal(dldx_ref, gi.data(), false);
+    s.to_internal(sol_ref, xi.data(), false);
+    HIPCHK(hipMemcpyAsync(d.b64, gi.data(), L.nvec * sizeof(double), hipMemcpyHostToDevice, d.stream));
+    vemd_solve_info tmp{};
+    solve_device(s, o, &tmp);
+    // grad_E = -mu^T (dK_int/dtheta_E) x = mu_E^T (exp(-theta_E) A_E) x_E
+    HIPCHK(hipMemcpyAsync(d.y64, xi.data(), L.nvec * sizeof(double), hipMemcpyHostToDevice, d.stream));
+    double* g = (double*)dalloc(d, s.mesh->nc * sizeof(double));
+    ApplyArgs<double> a = make_args<double>(s, nullptr, nullptr, nullptr, nullptr, nullptr, false);
+    if (L.k == 0)
+        hipLaunchKernelGGL((k_adjoint<double, 0>), dim3(d.ntiles), dim3(64), 0, d.stream, a, d.x64, d.y64,
+                           d.cell_of_gpos, g);
+    else
+        hipLaunchKernelGGL((k_adjoint<double, 1>), dim3(d.ntiles), dim3(64), 0, d.stream, a, d.x64, d.y64,
+                           d.cell_of_gpos, g);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(grad, g, s.mesh->nc * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    HIPCHK(hipFree(g));
+    d.allocs.pop_back();
+    if (info) *info = tmp;
+}
+
+}  // namespace vemd

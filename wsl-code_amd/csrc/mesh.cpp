@@ -1,0 +1,287 @@
+// Mesh readers and the reference's mesh conventions
+// (CPU/mesh/straight_mesh.cpp:92-214 VTK, :215-451 Gmsh, :462-482 CCW,
+// :504-576 cell properties, :584-660 edge numbering).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+
+#include "common.h"
+
+namespace vemd {
+
+static void make_ccw(std::vector<int>& v, const std::vector<double>& xy) {
+    double a = 0.0;
+    int n = (int)v.size();
+    for (int i = 0; i < n; ++i) {
+        int j = (i + 1) % n;
+        a += xy[2 * v[i]] * xy[2 * v[j] + 1] - xy[2 * v[j]] * xy[2 * v[i] + 1];
+    }
+    if (a < 0) std::reverse(v.begin(), v.end());
+}
+
+void mesh_finalize(Mesh& m) {
+    m.nc = (int)m.off.size() - 1;
+    m.cx.assign(m.nc, 0.0);
+    m.cy.assign(m.nc, 0.0);
+    m.area.assign(m.nc, 0.0);
+    m.diam.assign(m.nc, 0.0);
+    parallel_for(m.nc, [&](int64_t b, int64_t e) {
+        std::vector<double> x, y;
+        for (int64_t c = b; c < e; ++c) {
+            int s = m.off[c], n = m.npc((int)c);
+            x.resize(n);
+            y.resize(n);
+            for (int i = 0; i < n; ++i) {
+                x[i] = m.xy[2 * m.nodes[s + i]];
+                y[i] = m.xy[2 * m.nodes[s + i] + 1];
+            }
+            if (n == 4) {  // quads: bounding-box centre and area
+                double x0 = *std::min_element(x.begin(), x.end()), x1 = *std::max_element(x.begin(), x.end());
+                double y0 = *std::min_element(y.begin(), y.end()), y1 = *std::max_element(y.begin(), y.end());
+                m.cx[c] = (x0 + x1) / 2.0;
+                m.cy[c] = (y0 + y1) / 2.0;
+                m.area[c] = (x1 - x0) * (y1 - y0);
+            } else {  // vertex average and shoelace area
+                double sx = 0, sy = 0, a = 0;
+                for (int i = 0; i < n; ++i) {
+                    sx += x[i];
+                    sy += y[i];
+                    int j = (i + 1) % n;
+                    a += x[i] * y[j] - x[j] * y[i];
+                }
+                m.cx[c] = sx / n;
+                m.cy[c] = sy / n;
+                m.area[c] = std::fabs(a) / 2.0;
+            }
+            double md = 0.0;
+            for (int i = 0; i < n; ++i)
+                for (int j = i + 1; j < n; ++j) md = std::max(md, std::hypot(x[i] - x[j], y[i] - y[j]));
+            if (!(md > 0.0)) throw Error(VEMD_ERR_MESH, "cell " + std::to_string(c) + " is degenerate");
+            m.diam[c] = md;
+        }
+    });
+    // Edges: unique sorted (min,max) node pairs, numbered in sorted order.
+    size_t nt = m.nodes.size();
+    std::vector<std::pair<uint64_t, int>> all(nt);
+    for (int c = 0; c < m.nc; ++c) {
+        int s = m.off[c], n = m.npc(c);
+        for (int j = 0; j < n; ++j) all[s + j] = {Mesh::key(m.nodes[s + j], m.nodes[s + (j + 1) % n]), s + j};
+    }
+    std::sort(all.begin(), all.end());
+    m.cell_edges.assign(nt, -1);
+    m.edge_nodes.clear();
+    m.edge_occ.clear();
+    int ne = 0;
+    for (size_t i = 0; i < nt; ++i) {
+        if (i == 0 || all[i].first != all[i - 1].first) {
+            m.edge_nodes.push_back((int)(all[i].first >> 32));
+            m.edge_nodes.push_back((int)(all[i].first & 0xffffffffu));
+            m.edge_occ.push_back(0);
+            ++ne;
+        }
+        m.cell_edges[all[i].second] = ne - 1;
+        m.edge_occ[ne - 1]++;
+    }
+    m.ne = ne;
+    m.edge_cells.assign(2 * (size_t)ne, -1);
+    for (int c = 0; c < m.nc; ++c)
+        for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
+            int e = m.cell_edges[j];
+            if (m.edge_occ[e] > 2) throw Error(VEMD_ERR_MESH, "edge shared by more than two cells");
+            if (m.edge_cells[2 * e] < 0)
+                m.edge_cells[2 * e] = c;
+            else
+                m.edge_cells[2 * e + 1] = c;
+        }
+}
+
+static void push_cell(Mesh& m, std::vector<int> v) {
+    make_ccw(v, m.xy);
+    m.nodes.insert(m.nodes.end(), v.begin(), v.end());
+    m.off.push_back((int)m.nodes.size());
+}
+
+Mesh* mesh_read_msh(const char* path) {
+    std::ifstream f(path);
+    if (!f) throw Error(VEMD_ERR_IO, std::string("cannot open ") + path);
+    Mesh* m = new Mesh();
+    std::string line;
+    bool found = false;
+    while (std::getline(f, line))
+        if (line.find("$Nodes") != std::string::npos) {
+            found = true;
+            break;
+        }
+    if (!found) {
+        delete m;
+        throw Error(VEMD_ERR_IO, "$Nodes section not found");
+    }
+    int nblocks = 0, nn = 0;
+    std::getline(f, line);
+    std::istringstream(line) >> nblocks >> nn;
+    if (nn <= 0) {
+        delete m;
+        throw Error(VEMD_ERR_IO, "invalid number of nodes");
+    }
+    m->nn = nn;
+    m->xy.assign(2 * (size_t)nn, 0.0);
+    for (int b = 0; b < nblocks; ++b) {
+        std::getline(f, line);
+        int dim, tag, par, cnt = 0;
+        std::istringstream(line) >> dim >> tag >> par >> cnt;
+        std::vector<int> tags;
+        while ((int)tags.size() < cnt && std::getline(f, line)) {
+            std::istringstream ts(line);
+            int t;
+            while ((int)tags.size() < cnt && ts >> t) tags.push_back(t);
+        }
+        for (int i = 0; i < cnt; ++i) {
+            std::getline(f, line);
+            std::istringstream cs(line);
+            double x, y, z;
+            if (!(cs >> x >> y >> z)) continue;
+            int id = tags[i] - 1;
+            if (id >= 0 && id < nn) {
+                m->xy[2 * id] = x;
+                m->xy[2 * id + 1] = y;
+            }
+        }
+    }
+    found = false;
+    while (std::getline(f, line))
+        if (line.find("$Elements") != std::string::npos) {
+            found = true;
+            break;
+        }
+    if (!found) {
+        delete m;
+        throw Error(VEMD_ERR_IO, "$Elements section not found");
+    }
+    std::getline(f, line);
+    std::istringstream(line) >> nblocks;
+    m->off.push_back(0);
+    for (int b = 0; b < nblocks; ++b) {
+        std::getline(f, line);
+        int dim, tag, type, cnt = 0;
+        std::istringstream(line) >> dim >> tag >> type >> cnt;
+        // Gmsh element types 2/3/11/12 are read as 3/4/5/6-vertex polygons
+        // (the reference's convention, CPU/mesh/straight_mesh.cpp:442-450)
+        int nv = type == 2 ? 3 : type == 3 ? 4 : type == 11 ? 5 : type == 12 ? 6 : -1;
+        for (int i = 0; i < cnt; ++i) {
+            std::getline(f, line);
+            if (nv < 3) continue;
+            std::istringstream es(line);
+            int etag;
+            es >> etag;
+            std::vector<int> v(nv);
+            for (int j = 0; j < nv; ++j) {
+                es >> v[j];
+                v[j] -= 1;
+                if (v[j] < 0 || v[j] >= nn) {
+                    delete m;
+                    throw Error(VEMD_ERR_IO, "element references a missing node");
+                }
+            }
+            push_cell(*m, v);
+        }
+    }
+    if (m->off.size() < 2) {
+        delete m;
+        throw Error(VEMD_ERR_IO, "no polygon elements");
+    }
+    mesh_finalize(*m);
+    return m;
+}
+
+Mesh* mesh_read_vtk(const char* path) {
+    std::ifstream f(path);
+    if (!f) throw Error(VEMD_ERR_IO, std::string("cannot open ") + path);
+    auto seek = [&](const char* k) {
+        std::string t;
+        while (f >> t)
+            if (t == k) return true;
+        return false;
+    };
+    Mesh* m = new Mesh();
+    std::string dt;
+    if (!seek("POINTS") || !(f >> m->nn >> dt) || m->nn <= 0) {
+        delete m;
+        throw Error(VEMD_ERR_IO, "invalid POINTS section");
+    }
+    m->xy.assign(2 * (size_t)m->nn, 0.0);
+    for (int i = 0; i < m->nn; ++i) {
+        double z;
+        if (!(f >> m->xy[2 * i] >> m->xy[2 * i + 1] >> z)) {
+            delete m;
+            throw Error(VEMD_ERR_IO, "truncated POINTS section");
+        }
+    }
+    int nc = 0, tot = 0;
+    if (!seek("CELLS") || !(f >> nc >> tot) || nc <= 0) {
+        delete m;
+        throw Error(VEMD_ERR_IO, "invalid CELLS section");
+    }
+    m->off.push_back(0);
+    for (int c = 0; c < nc; ++c) {
+        int nv = 0;
+        f >> nv;
+        if (nv < 3) {
+            delete m;
+            throw Error(VEMD_ERR_IO, "cell with fewer than 3 vertices");
+        }
+        std::vector<int> v(nv);
+        for (int i = 0; i < nv; ++i) {
+            f >> v[i];
+            if (v[i] < 0 || v[i] >= m->nn) {
+                delete m;
+                throw Error(VEMD_ERR_IO, "cell references a missing node");
+            }
+        }
+        push_cell(*m, v);
+    }
+    int nt = 0;
+    if (!seek("CELL_TYPES") || !(f >> nt) || nt != nc) {
+        delete m;
+        throw Error(VEMD_ERR_IO, "CELL_TYPES missing or count mismatch");
+    }
+    for (int c = 0; c < nc; ++c) {
+        int t = 0;
+        f >> t;
+        if (t != 5 && t != 7 && t != 9) {
+            delete m;
+            throw Error(VEMD_ERR_IO, "unsupported VTK cell type");
+        }
+    }
+    mesh_finalize(*m);
+    return m;
+}
+
+Mesh* mesh_from_arrays(int nn, const double* xy, int nc, const int* off, const int* nodes, int narc,
+                       const int* arc_nodes, const double* arc_r) {
+    if (nn <= 0 || nc <= 0 || !xy || !off || !nodes) throw Error(VEMD_ERR_ARG, "empty mesh arrays");
+    Mesh* m = new Mesh();
+    m->nn = nn;
+    m->xy.assign(xy, xy + 2 * (size_t)nn);
+    m->off.push_back(0);
+    for (int c = 0; c < nc; ++c) {
+        int n = off[c + 1] - off[c];
+        if (n < 3) {
+            delete m;
+            throw Error(VEMD_ERR_MESH, "cell with fewer than 3 vertices");
+        }
+        std::vector<int> v(nodes + off[c], nodes + off[c + 1]);
+        for (int a : v)
+            if (a < 0 || a >= nn) {
+                delete m;
+                throw Error(VEMD_ERR_MESH, "cell references a missing node");
+            }
+        push_cell(*m, v);
+    }
+    mesh_finalize(*m);
+    for (int a = 0; a < narc; ++a) m->arcs[Mesh::key(arc_nodes[2 * a], arc_nodes[2 * a + 1])] = arc_r[a];
+    return m;
+}
+
+}  // namespace vemd

@@ -1,0 +1,41 @@
+// Solver object shared by the C ABI and the device code.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "store.h"
+
+namespace vemd {
+
+struct DeviceState;
+
+struct Solver {
+    std::shared_ptr<Mesh> mesh;
+    std::unique_ptr<Setup> setup;
+    Layout layout;
+    std::vector<double> scale;     // exp(-theta) per store position
+    std::vector<double> minv;      // preconditioner inverse, internal order
+    std::vector<double> rowscale;  // internal = rowscale * reference-form rows
+    std::vector<double> rhs_ref;   // reference-form right-hand side
+    DeviceState* dev = nullptr;
+    int device = -1;
+    double last_apply_ms = 0.0;
+    ~Solver();
+    // host-side conversions between reference order and internal order
+    void to_internal(const double* ref, double* in, bool scale_rows) const;
+    void to_reference(const double* in, double* ref, bool unscale_rows) const;
+    void refresh_theta_dependent();
+};
+
+int device_count();
+void dev_upload(Solver& s, int device);
+void dev_refresh(Solver& s);  // cell scales and preconditioner after a theta change
+void dev_free(Solver& s);
+void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref);
+void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* stream);
+void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, double* sol_ref, vemd_solve_info* info);
+void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, const double* dldx_ref, double* grad,
+                 vemd_solve_info* info);
+
+}  // namespace vemd

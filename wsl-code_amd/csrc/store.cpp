@@ -1,0 +1,325 @@
+// Cell ordering, colouring and packing of the HBM operator store.
+#include "store.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace vemd {
+
+// Hilbert index of (x, y) on a 2^16 x 2^16 grid.
+static uint64_t hilbert(uint32_t x, uint32_t y) {
+    uint64_t d = 0;
+    for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
+        uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
+        d += (uint64_t)s * s * ((3 * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - x;
+                y = s - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+static std::vector<uint64_t> hilbert_keys(const std::vector<double>& px, const std::vector<double>& py) {
+    double x0 = *std::min_element(px.begin(), px.end()), x1 = *std::max_element(px.begin(), px.end());
+    double y0 = *std::min_element(py.begin(), py.end()), y1 = *std::max_element(py.begin(), py.end());
+    double sc = 65535.0 / std::max(std::max(x1 - x0, y1 - y0), 1e-300);
+    std::vector<uint64_t> k(px.size());
+    for (size_t i = 0; i < px.size(); ++i)
+        k[i] = hilbert((uint32_t)((px[i] - x0) * sc), (uint32_t)((py[i] - y0) * sc));
+    return k;
+}
+
+Layout layout_build(const Mesh& m, const Setup& S) {
+    Layout L;
+    const int nc = m.nc;
+    L.k = S.sh.k;
+    L.q = S.sh.q;
+    L.ni = S.sh.ni;
+    L.lam = S.desc.bc == VEMD_BC_FLUX_MEANZERO;
+    L.nedge = m.ne;
+    L.ncell = nc;
+    for (int c = 0; c < nc; ++c) {
+        L.max_ne = std::max(L.max_ne, m.npc(c));
+        if (m.npc(c) > MAX_NE)
+            throw Error(VEMD_ERR_MESH, "cell " + std::to_string(c) + " has more than 10 edges (device limit)");
+    }
+    if (L.k > 1) throw Error(VEMD_ERR_ARG, "the device path supports k = 0 and k = 1");
+    // spatial order of cells
+    std::vector<uint64_t> hk = hilbert_keys(m.cx, m.cy);
+    std::vector<int> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return hk[a] < hk[b] || (hk[a] == hk[b] && a < b); });
+    // greedy colouring over edge neighbours in spatial order
+    std::vector<int> colour(nc, -1);
+    int ncol = 0;
+    for (int c : order) {
+        uint32_t used = 0;
+        for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
+            int e = m.cell_edges[j];
+            for (int t = 0; t < 2; ++t) {
+                int o = m.edge_cells[2 * e + t];
+                if (o >= 0 && o != c && colour[o] >= 0) used |= 1u << colour[o];
+            }
+        }
+        int col = 0;
+        while (used & (1u << col)) ++col;
+        colour[c] = col;
+        ncol = std::max(ncol, col + 1);
+    }
+    L.ncolours = ncol;
+    // store order: colour, valence, spatial
+    std::vector<int> rank(nc);
+    for (int i = 0; i < nc; ++i) rank[order[i]] = i;
+    std::vector<int> so(nc);
+    std::iota(so.begin(), so.end(), 0);
+    std::sort(so.begin(), so.end(), [&](int a, int b) {
+        if (colour[a] != colour[b]) return colour[a] < colour[b];
+        if (m.npc(a) != m.npc(b)) return m.npc(a) < m.npc(b);
+        return rank[a] < rank[b];
+    });
+    // groups with padding to whole tiles
+    int gpos = 0, tile = 0;
+    L.colour_tile0.assign(ncol + 1, 0);
+    L.cell_of_gpos.clear();
+    L.gpos_of_cell.assign(nc, -1);
+    for (size_t i = 0; i < so.size();) {
+        size_t j = i;
+        while (j < so.size() && colour[so[j]] == colour[so[i]] && m.npc(so[j]) == m.npc(so[i])) ++j;
+        GroupDesc g{};
+        g.colour = colour[so[i]];
+        g.ne = m.npc(so[i]);
+        g.n = S.sh.nflux(g.ne);
+        int cnt = (int)(j - i);
+        g.npad = (cnt + TILE - 1) / TILE * TILE;
+        g.gpos0 = gpos;
+        g.tile0 = tile;
+        for (int t = 0; t < g.npad / TILE; ++t) L.tile_group.push_back((int)L.groups.size());
+        for (int a = 0; a < g.npad; ++a) {
+            int c = a < cnt ? so[i + a] : -1;
+            L.cell_of_gpos.push_back(c);
+            if (c >= 0) L.gpos_of_cell[c] = gpos + a;
+        }
+        gpos += g.npad;
+        tile += g.npad / TILE;
+        L.groups.push_back(g);
+        i = j;
+    }
+    for (auto& g : L.groups) L.colour_tile0[g.colour + 1] = g.tile0 + g.npad / TILE;
+    for (int c = 1; c <= ncol; ++c) L.colour_tile0[c] = std::max(L.colour_tile0[c], L.colour_tile0[c - 1]);
+    L.cpad = gpos;
+    // edges numbered by the store position of the first cell that owns them
+    std::vector<int> efirst(m.ne, INT32_MAX);
+    for (int c = 0; c < nc; ++c)
+        for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
+            int e = m.cell_edges[j];
+            efirst[e] = std::min(efirst[e], rank[c]);
+        }
+    std::vector<int> eo(m.ne);
+    std::iota(eo.begin(), eo.end(), 0);
+    std::sort(eo.begin(), eo.end(), [&](int a, int b) { return efirst[a] < efirst[b] || (efirst[a] == efirst[b] && a < b); });
+    L.edge_int.assign(m.ne, 0);
+    for (int i = 0; i < m.ne; ++i) L.edge_int[eo[i]] = i;
+    // internal vector layout
+    int kk = L.k + 1;
+    L.cbase = (int64_t)kk * m.ne;
+    int nslot = L.ni + L.q;
+    L.lam_idx = L.lam ? L.cbase + (int64_t)nslot * L.cpad : -1;
+    L.trash_idx = L.cbase + (int64_t)nslot * L.cpad + (L.lam ? 1 : 0);
+    L.nvec = L.trash_idx + 1;
+    // reference -> internal map
+    L.int_of_ref.assign(S.ntot, -1);
+    for (int e = 0; e < m.ne; ++e)
+        for (int r = 0; r < kk; ++r) L.int_of_ref[(int64_t)r * m.ne + e] = (int64_t)kk * L.edge_int[e] + r;
+    for (int c = 0; c < nc; ++c) {
+        int64_t gp = L.gpos_of_cell[c];
+        for (int i = 0; i < L.ni; ++i) L.int_of_ref[S.nflux_total - (int64_t)nc * L.ni + (int64_t)c * L.ni + i] = L.cbase + i * L.cpad + gp;
+        for (int r = 0; r < L.q; ++r) L.int_of_ref[S.nflux_total + (int64_t)c * L.q + r] = L.cbase + (L.ni + r) * L.cpad + gp;
+    }
+    if (L.lam) L.int_of_ref[S.ntot - 1] = L.lam_idx;
+    // packed edge-dof indices with the accumulate flag for the second writer
+    std::vector<int> first_colour(m.ne, INT32_MAX);
+    for (int c = 0; c < nc; ++c)
+        for (int j = m.off[c]; j < m.off[c + 1]; ++j) first_colour[m.cell_edges[j]] = std::min(first_colour[m.cell_edges[j]], colour[c]);
+    int64_t ioff = 0, aoff2 = 0, woff2 = 0, aoff4 = 0, woff4 = 0, loff = 0;
+    L.a_off4.resize(L.groups.size());
+    L.w_off4.resize(L.groups.size());
+    for (size_t gi = 0; gi < L.groups.size(); ++gi) {
+        GroupDesc& g = L.groups[gi];
+        int ned = kk * g.ne, na = g.n * (g.n + 1) / 2, nw = L.q * g.n;
+        g.i_off = ioff;
+        g.a_off = aoff2;
+        g.w_off = woff2;
+        g.l_off = loff;
+        L.a_off4[gi] = aoff4;
+        L.w_off4[gi] = woff4;
+        ioff += (int64_t)ned * g.npad;
+        aoff2 += (int64_t)((na + 1) / 2 * 2) * g.npad;
+        woff2 += (int64_t)((nw + 1) / 2 * 2) * g.npad;
+        aoff4 += (int64_t)((na + 3) / 4 * 4) * g.npad;
+        woff4 += (int64_t)((nw + 3) / 4 * 4) * g.npad;
+        loff += (int64_t)L.q * g.npad;
+    }
+    L.a_len2 = aoff2;
+    L.w_len2 = woff2;
+    L.a_len4 = aoff4;
+    L.w_len4 = woff4;
+    L.l_len = loff;
+    L.idx.assign(ioff, (uint32_t)L.trash_idx);
+    if (L.nvec >= (int64_t)ACC) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit dof indices");
+    for (auto& g : L.groups) {
+        int ned = kk * g.ne;
+        for (int a = 0; a < g.npad; ++a) {
+            int c = L.cell_of_gpos[g.gpos0 + a];
+            if (c < 0) continue;
+            for (int r = 0; r < kk; ++r)
+                for (int j = 0; j < g.ne; ++j) {
+                    int e = m.cell_edges[m.off[c] + j];
+                    uint32_t id = (uint32_t)(kk * L.edge_int[e] + r);
+                    if (colour[c] != first_colour[e]) id |= ACC;
+                    L.idx[g.i_off + (int64_t)(r * g.ne + j) * g.npad + a] = id;
+                }
+            (void)ned;
+        }
+    }
+    return L;
+}
+
+void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& A64, std::vector<double>& W64,
+                std::vector<double>& L64, std::vector<float>& A32, std::vector<float>& W32, std::vector<float>& L32) {
+    A64.assign(L.a_len2, 0.0);
+    W64.assign(L.w_len2, 0.0);
+    A32.assign(L.a_len4, 0.0f);
+    W32.assign(L.w_len4, 0.0f);
+    L64.assign(L.l_len, 0.0);
+    L32.assign(L.l_len, 0.0f);
+    const int q = L.q;
+    parallel_for((int64_t)L.groups.size(), [&](int64_t gb, int64_t ge) {
+        std::vector<double> a;
+        for (int64_t gi = gb; gi < ge; ++gi) {
+            const GroupDesc& g = L.groups[gi];
+            int n = g.n, na = n * (n + 1) / 2;
+            for (int lane = 0; lane < g.npad; ++lane) {
+                int c = L.cell_of_gpos[g.gpos0 + lane];
+                if (c < 0) continue;
+                const double* Ac = &S.A[S.ofs.a_off[c]];
+                const double* Wc = &S.W[S.ofs.w_off[c]];
+                const char* isb = &S.isb[S.ofs.v_off[c]];
+                a.assign(Ac, Ac + na);
+                // decouple essential boundary dofs: zero row/column, unit diagonal
+                for (int i = 0; i < n; ++i)
+                    for (int j = 0; j <= i; ++j)
+                        if (isb[i] || isb[j]) a[i * (i + 1) / 2 + j] = (i == j) ? 1.0 : 0.0;
+                for (int t = 0; t < na; ++t) {
+                    A64[g.a_off + ((int64_t)(t / 2) * g.npad + lane) * 2 + t % 2] = a[t];
+                    A32[L.a_off4[gi] + ((int64_t)(t / 4) * g.npad + lane) * 4 + t % 4] = (float)a[t];
+                }
+                for (int t = 0; t < q * n; ++t) {
+                    double w = isb[t % n] ? 0.0 : Wc[t];
+                    W64[g.w_off + ((int64_t)(t / 2) * g.npad + lane) * 2 + t % 2] = w;
+                    W32[L.w_off4[gi] + ((int64_t)(t / 4) * g.npad + lane) * 4 + t % 4] = (float)w;
+                }
+                for (int r = 0; r < q; ++r) {
+                    L64[g.l_off + (int64_t)r * g.npad + lane] = S.L[(size_t)c * q + r];
+                    L32[g.l_off + (int64_t)r * g.npad + lane] = (float)S.L[(size_t)c * q + r];
+                }
+            }
+        }
+    });
+    (void)m;
+}
+
+void pack_scale(const Setup& S, const Layout& L, std::vector<double>& s) {
+    s.assign(L.cpad, 0.0);
+    for (int64_t gp = 0; gp < L.cpad; ++gp) {
+        int c = L.cell_of_gpos[gp];
+        if (c >= 0) s[gp] = std::exp(-S.theta[c]);
+    }
+}
+
+void build_minv(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& minv) {
+    std::vector<double> d(L.nvec, 0.0);
+    const int q = L.q;
+    // flux: assembled diagonal
+    for (int c = 0; c < m.nc; ++c) {
+        int n = S.sh.nflux(m.npc(c));
+        double sc = std::exp(-S.theta[c]);
+        const double* A = &S.A[S.ofs.a_off[c]];
+        for (int i = 0; i < n; ++i) {
+            int64_t v = S.ofs.v_off[c] + i;
+            double aii = S.isb[v] ? 1.0 : A[i * (i + 1) / 2 + i];
+            d[L.int_of_ref[S.gdof[v]]] += sc * aii;
+        }
+    }
+    // pressure: diag of W diag(A)^{-1} W^T per cell; multiplier: sum L^2 / S
+    double lam = 0.0;
+    for (int c = 0; c < m.nc; ++c) {
+        int n = S.sh.nflux(m.npc(c));
+        const double* W = &S.W[S.ofs.w_off[c]];
+        for (int r = 0; r < q; ++r) {
+            double sr = 0.0;
+            for (int i = 0; i < n; ++i) {
+                int64_t v = S.ofs.v_off[c] + i;
+                if (S.isb[v]) continue;
+                double w = W[r * n + i];
+                sr += w * w / d[L.int_of_ref[S.gdof[v]]];
+            }
+            if (!(sr > 0.0)) sr = 1.0;
+            d[L.int_of_ref[S.nflux_total + (int64_t)c * q + r]] = sr;
+            double l = S.L[(size_t)c * q + r];
+            lam += l * l / sr;
+        }
+    }
+    if (L.lam) d[L.lam_idx] = lam > 0 ? lam : 1.0;
+    minv.assign(L.nvec, 1.0);
+    for (int64_t i = 0; i < L.nvec; ++i)
+        if (d[i] > 0.0) minv[i] = 1.0 / d[i];
+}
+
+void build_rhs_ref(const Mesh& m, const Setup& S, std::vector<double>& b) {
+    b.assign(S.ntot, 0.0);
+    const int q = S.sh.q;
+    for (int c = 0; c < m.nc; ++c) {
+        int n = S.sh.nflux(m.npc(c));
+        double sc = std::exp(-S.theta[c]);
+        const double* A = &S.A[S.ofs.a_off[c]];
+        const double* W = &S.W[S.ofs.w_off[c]];
+        int64_t v0 = S.ofs.v_off[c];
+        for (int i = 0; i < n; ++i) {
+            int64_t g = S.gdof[v0 + i];
+            if (S.isb[v0 + i]) {
+                b[g] = S.ub[v0 + i];
+                continue;
+            }
+            double lift = 0.0;
+            for (int j = 0; j < n; ++j)
+                if (S.isb[v0 + j]) lift += A[i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i] * S.ub[v0 + j];
+            b[g] += S.ru[v0 + i] - sc * lift;
+        }
+        for (int r = 0; r < q; ++r) {
+            double lift = 0.0;
+            for (int j = 0; j < n; ++j)
+                if (S.isb[v0 + j]) lift += W[r * n + j] * S.ub[v0 + j];
+            b[S.nflux_total + (int64_t)c * q + r] += S.f[(size_t)c * q + r] - lift;
+        }
+    }
+}
+
+void build_row_scale(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& rs) {
+    rs.assign(L.nvec, 1.0);
+    for (int c = 0; c < m.nc; ++c) {
+        int n = S.sh.nflux(m.npc(c));
+        for (int i = 0; i < n; ++i) {
+            int64_t v = S.ofs.v_off[c] + i;
+            if (S.isb[v]) rs[L.int_of_ref[S.gdof[v]]] = std::exp(-S.theta[c]);
+        }
+        for (int r = 0; r < L.q; ++r) rs[L.int_of_ref[S.nflux_total + (int64_t)c * L.q + r]] = -1.0;
+    }
+    if (L.lam) rs[L.lam_idx] = -1.0;
+}
+
+}  // namespace vemd

@@ -1,0 +1,68 @@
+// Host-side description of the HBM operator store: cell order, colouring,
+// valence groups, internal dof numbering and the packed per-precision arrays.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+
+namespace vemd {
+
+constexpr int TILE = 64;       // cells per tile: one wavefront, one lane per cell
+constexpr int MAX_NE = 10;     // largest polygon the device kernels are built for
+constexpr uint32_t ACC = 0x80000000u;  // index flag: accumulate (a lower colour wrote first)
+
+// POD shared with the device code.
+struct GroupDesc {
+    int colour, ne, n, npad;  // valence, local flux dofs, cells incl. padding
+    int gpos0, tile0;         // first store position, first global tile
+    int64_t a_off, w_off, l_off, i_off;  // element offsets of A, W, L and index blocks
+};
+
+struct Layout {
+    int k = 1, q = 0, ni = 0;
+    int ncolours = 0, max_ne = 0;
+    bool lam = false;
+    int64_t nedge = 0, ncell = 0, cpad = 0;  // edges, real cells, padded cell slots
+    int64_t cbase = 0;        // first cell-owned slot
+    int64_t lam_idx = -1, trash_idx = 0, nvec = 0;
+    std::vector<GroupDesc> groups;
+    std::vector<int> tile_group;        // group of each global tile
+    std::vector<int> colour_tile0;      // first tile of each colour (+ end)
+    std::vector<int> cell_of_gpos;      // mesh cell at each store position (-1 padding)
+    std::vector<int> gpos_of_cell;
+    std::vector<int> edge_int;          // internal number of each mesh edge
+    std::vector<int64_t> int_of_ref;    // reference dof -> internal slot
+    std::vector<uint32_t> idx;          // packed edge-dof indices (SoA per group) with ACC flag
+    int64_t a_len2 = 0, w_len2 = 0, a_len4 = 0, w_len4 = 0, l_len = 0;  // packed sizes (elements)
+    std::vector<int64_t> a_off4, w_off4;  // FP32 offsets per group (vector width 4)
+};
+
+// Builds ordering, colouring and numbering from the mesh and setup.
+Layout layout_build(const Mesh& m, const Setup& S);
+
+// Packs A (packed symmetric, boundary dofs decoupled), W and L into the
+// interleaved layouts: vector width 2 for FP64, 4 for FP32.
+void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& A64, std::vector<double>& W64,
+                std::vector<double>& L64, std::vector<float>& A32, std::vector<float>& W32, std::vector<float>& L32);
+
+// Per-store-position cell scale exp(-theta) (padding 0).
+void pack_scale(const Setup& S, const Layout& L, std::vector<double>& s);
+
+// Inverse of the block-diagonal preconditioner (internal order):
+// flux: assembled diag of exp(-theta) A; pressure: sum_i W_ri^2 / d_i;
+// multiplier: sum L_r^2 / S_r (the reference's choice, GPU/src/equation_gpu.cu:282-315).
+void build_minv(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& minv);
+
+// Right-hand side in reference order and reference form: flux rows the natural
+// boundary term minus the lift, pressure rows the source minus the lift,
+// essential boundary rows their values, multiplier row 0.
+void build_rhs_ref(const Mesh& m, const Setup& S, std::vector<double>& b);
+
+// Diagonal scaling between the reference-form system (identity essential rows)
+// and the internal symmetric one: pressure and multiplier rows -1, essential
+// boundary rows exp(-theta_E), other flux rows 1.  Internal order.
+void build_row_scale(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& rs);
+
+}  // namespace vemd

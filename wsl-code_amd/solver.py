@@ -1,0 +1,187 @@
+"""Solver objects: the low-level Solver over the C ABI and the reference-style
+DarcyProblem / DarcySolver pair."""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import (BC_DIRICHLET_P, BC_FLUX_MEANZERO, FP32, FP64, MAP_ARCS, MAP_SIN, MIXED, ProblemDesc, SOL_SINCOS,
+                   SOL_SINSIN, SolveInfo, SolveOptsC, VemdError, c_dp, c_ip, c_lp, c_vp, check, lib)
+
+
+def _d(a):
+    return a.ctypes.data_as(c_dp)
+
+
+@dataclass
+class SolveOptions:
+    precision: int = FP64
+    rtol: float = 1e-8
+    max_iters: int = 1000000
+    inner_rtol: float = 1e-4
+    check_every: int = 32
+    use_graph: int = 1
+
+    def c(self):
+        return SolveOptsC(self.precision, self.rtol, self.max_iters, self.inner_rtol, self.check_every,
+                          self.use_graph)
+
+
+class Solver:
+    """Element setup (host FP64) and device solve for one mesh and problem."""
+
+    def __init__(self, mesh, k=1, nq2d=9, nq1d=9, map_kind=MAP_SIN, eps=0.05, solution=SOL_SINCOS,
+                 bc=BC_FLUX_MEANZERO, theta=None, device=None):
+        self.mesh = mesh
+        self._theta = None if theta is None else np.ascontiguousarray(theta, np.float64)
+        d = ProblemDesc(k, nq2d, nq1d, map_kind, eps, solution, bc,
+                        None if self._theta is None else _d(self._theta))
+        self.h = c_vp()
+        check(lib().vemd_solver_create(mesh.h, ctypes.byref(d), ctypes.byref(self.h)))
+        s = np.zeros(6, np.int64)
+        check(lib().vemd_solver_sizes(self.h, s.ctypes.data_as(c_lp)))
+        self.nflux, self.npres, self.ntot, self.nvec, self.ncolours, self.max_ne = (int(v) for v in s)
+        self.k, self.bc = k, bc
+        self.uploaded = False
+        if device is not None:
+            self.upload(device)
+
+    def upload(self, device=0):
+        check(lib().vemd_solver_upload(self.h, int(device)))
+        self.uploaded = True
+
+    def set_theta(self, theta):
+        self._theta = np.ascontiguousarray(theta, np.float64)
+        check(lib().vemd_set_theta(self.h, _d(self._theta)))
+
+    def rhs(self):
+        b = np.zeros(self.ntot)
+        check(lib().vemd_rhs(self.h, _d(b)))
+        return b
+
+    def apply(self, x, precision=FP64):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.ntot)
+        check(lib().vemd_apply(self.h, precision, _d(x), _d(y)))
+        return y
+
+    def solve(self, opts=None, rhs=None):
+        opts = opts or SolveOptions()
+        x = np.zeros(self.ntot)
+        info = SolveInfo()
+        rb = None
+        if rhs is not None:
+            rhs = np.ascontiguousarray(rhs, np.float64)
+            rb = _d(rhs)
+        oc = opts.c()
+        check(lib().vemd_solve(self.h, ctypes.byref(oc), rb, _d(x), ctypes.byref(info)))
+        return x, dict(iterations=info.iterations, refinements=info.refinements, rel_residual=info.rel_residual,
+                       seconds=info.seconds)
+
+    def adjoint_theta(self, x, dldx, opts=None):
+        opts = opts or SolveOptions()
+        x = np.ascontiguousarray(x, np.float64)
+        g = np.ascontiguousarray(dldx, np.float64)
+        grad = np.zeros(self.mesh.num_cells)
+        info = SolveInfo()
+        oc = opts.c()
+        check(lib().vemd_adjoint_theta(self.h, ctypes.byref(oc), _d(x), _d(g), _d(grad), ctypes.byref(info)))
+        return grad, dict(iterations=info.iterations, rel_residual=info.rel_residual)
+
+    def errors(self, x, nq=9):
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(4)
+        check(lib().vemd_errors(self.h, _d(x), nq, _d(out)))
+        return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3])
+
+    def element(self, cell):
+        """FP64 element operator of one cell: dict with A (n x n), W (q x n), L, f, sign, gdof."""
+        q = (self.k + 1) * (self.k + 2) // 2
+        n = ctypes.c_int()
+        A = np.zeros(64 * 64)
+        W = np.zeros(q * 64)
+        L = np.zeros(q)
+        f = np.zeros(q)
+        sg = np.zeros(64)
+        gd = np.zeros(64, np.int64)
+        check(lib().vemd_element_matrices(self.h, int(cell), ctypes.byref(n), _d(A), _d(W), _d(L), _d(f), _d(sg),
+                                          gd.ctypes.data_as(c_lp)))
+        n = n.value
+        return dict(n=n, A=A[: n * n].reshape(n, n), W=W[: q * n].reshape(q, n), L=L, f=f, sign=sg[:n],
+                    gdof=gd[:n])
+
+    def __del__(self):
+        try:
+            lib().vemd_solver_destroy(self.h)
+        except Exception:
+            pass
+
+
+class DarcyProblem:
+    """Mirror of Darcy::DarcyProblem (CPU/examples/darcy_problem.h:185-229):
+    example 1 = sinusoidal curved map eps = 0.05, example 2 = eps = 0, both with
+    p = sin(pi x) cos(pi y), K = I, essential normal flux + mean-zero pressure.
+    Extensions: map_kind=MAP_ARCS (per-cell arc maps), solution=SOL_SINSIN,
+    bc=BC_DIRICHLET_P and a per-cell log-permeability theta."""
+
+    def __init__(self, map_kind=MAP_SIN, eps=0.05, solution=SOL_SINCOS, bc=BC_FLUX_MEANZERO, theta=None):
+        self.problem_index = 0
+        self.map_kind, self.eps, self.solution, self.bc, self.theta = map_kind, eps, solution, bc, theta
+
+    def set_problem_index(self, index):
+        self.problem_index = index
+
+    def init_problem(self):
+        if self.problem_index == 1:
+            self.map_kind, self.eps = MAP_SIN, 0.05
+        elif self.problem_index == 2:
+            self.map_kind, self.eps = MAP_SIN, 0.0
+        elif self.problem_index != 0:
+            return False
+        return True
+
+
+class DarcySolver:
+    """Mirror of DarcySolver (CPU/solver/DarcySolver.h:17-186) on the device path."""
+
+    def __init__(self, mesh_reader, problem, gauss_point_num=9, gauss_point_num_1d=9, k=1, device=0):
+        mesh = getattr(mesh_reader, "mesh", mesh_reader)
+        self.solver = Solver(mesh, k=k, nq2d=gauss_point_num, nq1d=gauss_point_num_1d, map_kind=problem.map_kind,
+                             eps=problem.eps, solution=problem.solution, bc=problem.bc, theta=problem.theta)
+        self.device = device
+        self.solution = None
+        self.info = None
+        self.opts = SolveOptions(precision=FP64, rtol=1e-10)
+
+    def getTotalDofFlux(self):
+        return self.solver.nflux
+
+    def getTotalDofConc(self):
+        return self.solver.npres
+
+    def getTotalDof(self):
+        return self.solver.nflux + self.solver.npres
+
+    def solve(self):
+        try:
+            if not self.solver.uploaded:
+                self.solver.upload(self.device)
+            self.solution, self.info = self.solver.solve(self.opts)
+        except VemdError as e:
+            if e.status in ("ERR_NO_DEVICE",):
+                raise
+            return False
+        return True
+
+    def getSolution(self):
+        return self.solution
+
+    def computeL2ErrorPressure(self, gauss_point_num=9):
+        return self.solver.errors(self.solution, gauss_point_num)["p_comp"]
+
+    def computeL2ErrorFlux(self, gauss_point_num=9):
+        return self.solver.errors(self.solution, gauss_point_num)["u_comp"]
+
+
+__all__ = ["Solver", "SolveOptions", "DarcyProblem", "DarcySolver", "FP32", "FP64", "MIXED", "MAP_ARCS", "MAP_SIN",
+           "SOL_SINCOS", "SOL_SINSIN", "BC_FLUX_MEANZERO", "BC_DIRICHLET_P", "c_ip"]
