@@ -494,9 +494,12 @@ static Mesh* read_vtk(const char* path) {
 }
 
 // ---------------------------------------------------------------------------
-// Ear clipping.  CPU/mesh/polygon_triangulator.cpp:50-145 (absolute tolerances
-// 1e-6 on the area test and 1e-8 on the convexity cross product, kept as in the
-// reference; the oracle is used only on meshes where they are harmless).
+// Ear clipping.  CPU/mesh/polygon_triangulator.cpp:50-145.  The reference tests
+// the area sum against an absolute 1e-6 and the convexity cross product against
+// an absolute 1e-8; both thresholds are scaled here by the squared cell
+// diameter (h^2 = 1 reproduces the reference).  On the reference's meshes the
+// decisions are identical (pinned against the reference's own triangulator);
+// on fine Voronoi meshes the absolute form rejects every ear.
 // ---------------------------------------------------------------------------
 
 struct Tri {
@@ -504,25 +507,25 @@ struct Tri {
     double v[6];
 };
 
-static bool in_tri(const std::vector<double>& p, int pi, int a, int b, int c) {
+static bool in_tri(const std::vector<double>& p, int pi, int a, int b, int c, double h2) {
     double px = p[2 * pi], py = p[2 * pi + 1], ax = p[2 * a], ay = p[2 * a + 1];
     double bx = p[2 * b], by = p[2 * b + 1], cx = p[2 * c], cy = p[2 * c + 1];
     double at = 0.5 * std::fabs((bx - ax) * (cy - ay) - (cx - ax) * (by - ay));
     double a1 = 0.5 * std::fabs((ax - px) * (by - py) - (bx - px) * (ay - py));
     double a2 = 0.5 * std::fabs((bx - px) * (cy - py) - (cx - px) * (by - py));
     double a3 = 0.5 * std::fabs((cx - px) * (ay - py) - (ax - px) * (cy - py));
-    return std::fabs((a1 + a2 + a3) - at) < 1e-6;
+    return std::fabs((a1 + a2 + a3) - at) < 1e-6 * h2;
 }
 
-static bool is_ear(int i, const std::vector<double>& p, int n) {
+static bool is_ear(int i, const std::vector<double>& p, int n, double h2) {
     int pr = (i - 1 + n) % n, nx = (i + 1) % n;
     double xi = p[2 * i], yi = p[2 * i + 1];
     double xp = p[2 * pr], yp = p[2 * pr + 1], xn = p[2 * nx], yn = p[2 * nx + 1];
     double cr = (xn - xi) * (yp - yi) - (xp - xi) * (yn - yi);
-    if (cr <= 1e-8) return false;
+    if (cr <= 1e-8 * h2) return false;
     for (int j = 0; j < n; ++j) {
         if (j == i || j == pr || j == nx) continue;
-        if (in_tri(p, j, pr, i, nx)) return false;
+        if (in_tri(p, j, pr, i, nx, h2)) return false;
     }
     return true;
 }
@@ -546,11 +549,12 @@ static std::vector<Tri> ear_clip(const Mesh& m, int c, double* total_area) {
         std::memcpy(t.v, v, sizeof v);
         out.push_back(t);
     };
+    double h2 = m.diam[c] * m.diam[c];
     int rem = n;
     while (rem > 3) {
         bool found = false;
         for (int i = 0; i < rem; ++i) {
-            if (is_ear(i, p, rem)) {
+            if (is_ear(i, p, rem, h2)) {
                 push((i - 1 + rem) % rem, i, (i + 1) % rem);
                 p.erase(p.begin() + 2 * i, p.begin() + 2 * (i + 1));
                 idx.erase(idx.begin() + i);
@@ -604,42 +608,76 @@ static Geo sin_map(double eps, double xi, double eta) {
     return g;
 }
 
-// EXTENSION: blended chord-to-arc map on one sub-triangle.  For a chord A->B of
-// a circle of radius R centred at the origin, lying in triangle (A,B,C):
-//   F(xi) = xi + (1 - l_C) * delta(s),  s = l_B / (l_A + l_B),
-//   delta(s) = R (cos th(s), sin th(s)) - (A + s (B - A)),  th linear in s.
-// F is the identity on AC and BC and maps AB onto the minor arc.
+// EXTENSION: per-cell chord-to-arc map.  For a boundary chord A->B (local
+// vertices ia, ib of the cell) of a circle of radius R centred at the origin:
+//   F(xi) = xi + S^2 delta(s),  S = l_A + l_B,  s = l_B / S,
+//   delta(s) = R (cos th(s), sin th(s)) - (A + s (B - A)),  th linear in s,
+// with l the Wachspress coordinates of the (convex) cell.  On AB, S = 1 and
+// s is the chord parameter, so F maps the chord onto the arc; on every other
+// edge F is the identity; F is smooth inside the cell (a map that is only
+// piecewise smooth, e.g. per sub-triangle, costs half an order of flux accuracy).
 struct Chord {
     double ax, ay, bx, by, R;
+    int ia, ib;
 };
 
-static void arc_term(const Chord& ch, const double* cv, double xi, double eta, double* F, double* J) {
-    // barycentric coordinates of (xi,eta) in triangle (A, B, C)
-    double ax = ch.ax, ay = ch.ay, bx = ch.bx, by = ch.by, cx = cv[0], cy = cv[1];
-    double d = (bx - ax) * (cy - ay) - (cx - ax) * (by - ay);
-    // l_B and l_C as affine functions
-    double lB = ((xi - ax) * (cy - ay) - (cx - ax) * (eta - ay)) / d;
-    double lC = ((bx - ax) * (eta - ay) - (xi - ax) * (by - ay)) / d;
-    double lA = 1.0 - lB - lC;
-    double gB[2] = {(cy - ay) / d, -(cx - ax) / d};
-    double gC[2] = {-(by - ay) / d, (bx - ax) / d};
-    double gA[2] = {-gB[0] - gC[0], -gB[1] - gC[1]};
-    double sAB = lA + lB;
-    double s = lB / sAB;
-    double tA = std::atan2(ay, ax), tB = std::atan2(by, bx);
+// Wachspress coordinates and gradients in product form (no division by the
+// edge areas, so edge points are fine): l_i = A_i P_i / D.
+static void wachspress(const std::vector<double>& vx, const std::vector<double>& vy, double x, double y, double* lam,
+                       double* gl) {
+    int n = (int)vx.size();
+    double a[64], ga[64][2], Ac[64], P[64], gP[64][2];
+    for (int i = 0; i < n; ++i) {
+        int j = (i + 1) % n;
+        a[i] = 0.5 * ((vx[i] - x) * (vy[j] - y) - (vy[i] - y) * (vx[j] - x));
+        ga[i][0] = 0.5 * (vy[i] - vy[j]);
+        ga[i][1] = 0.5 * (vx[j] - vx[i]);
+    }
+    double D = 0.0, gD[2] = {0.0, 0.0};
+    for (int i = 0; i < n; ++i) {
+        int p = (i - 1 + n) % n, q = (i + 1) % n;
+        Ac[i] = 0.5 * ((vx[i] - vx[p]) * (vy[q] - vy[p]) - (vy[i] - vy[p]) * (vx[q] - vx[p]));
+        P[i] = 1.0;
+        gP[i][0] = gP[i][1] = 0.0;
+        for (int m = 0; m < n; ++m) {
+            if (m == p || m == i) continue;
+            // product rule: d(P * a_m) = dP a_m + P da_m
+            gP[i][0] = gP[i][0] * a[m] + P[i] * ga[m][0];
+            gP[i][1] = gP[i][1] * a[m] + P[i] * ga[m][1];
+            P[i] *= a[m];
+        }
+        D += Ac[i] * P[i];
+        gD[0] += Ac[i] * gP[i][0];
+        gD[1] += Ac[i] * gP[i][1];
+    }
+    for (int i = 0; i < n; ++i) {
+        lam[i] = Ac[i] * P[i] / D;
+        gl[2 * i] = (Ac[i] * gP[i][0] - lam[i] * gD[0]) / D;
+        gl[2 * i + 1] = (Ac[i] * gP[i][1] - lam[i] * gD[1]) / D;
+    }
+}
+
+static void arc_term(const Chord& ch, const double* lam, const double* gl, double* F, double* J) {
+    double lA = lam[ch.ia], lB = lam[ch.ib];
+    double S = lA + lB;
+    if (S <= 1e-300) return;
+    const double* gA = &gl[2 * ch.ia];
+    const double* gB = &gl[2 * ch.ib];
+    double s = lB / S;
+    double tA = std::atan2(ch.ay, ch.ax), tB = std::atan2(ch.by, ch.bx);
     double dt = tB - tA;
     while (dt > M_PI) dt -= 2 * M_PI;
     while (dt <= -M_PI) dt += 2 * M_PI;
     double th = tA + s * dt;
-    double del[2] = {ch.R * std::cos(th) - (ax + s * (bx - ax)), ch.R * std::sin(th) - (ay + s * (by - ay))};
-    double ddel[2] = {-ch.R * dt * std::sin(th) - (bx - ax), ch.R * dt * std::cos(th) - (by - ay)};
-    // (1 - l_C) grad s = (l_A grad l_B - l_B grad l_A) / (l_A + l_B)
-    double gs[2] = {(lA * gB[0] - lB * gA[0]) / sAB, (lA * gB[1] - lB * gA[1]) / sAB};
-    double w = 1.0 - lC;
-    F[0] += w * del[0];
-    F[1] += w * del[1];
+    double del[2] = {ch.R * std::cos(th) - (ch.ax + s * (ch.bx - ch.ax)), ch.R * std::sin(th) - (ch.ay + s * (ch.by - ch.ay))};
+    double ddel[2] = {-ch.R * dt * std::sin(th) - (ch.bx - ch.ax), ch.R * dt * std::cos(th) - (ch.by - ch.ay)};
+    // grad (S^2 delta(s)) = 2 S delta grad S + delta'(s) (l_A grad l_B - l_B grad l_A)
+    double gS[2] = {gA[0] + gB[0], gA[1] + gB[1]};
+    double w[2] = {lA * gB[0] - lB * gA[0], lA * gB[1] - lB * gA[1]};
+    F[0] += S * S * del[0];
+    F[1] += S * S * del[1];
     for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) J[2 * i + j] += del[i] * (-gC[j]) + ddel[i] * gs[j];
+        for (int j = 0; j < 2; ++j) J[2 * i + j] += 2.0 * S * del[i] * gS[j] + ddel[i] * w[j];
 }
 
 struct Ctx;
@@ -756,7 +794,7 @@ struct Ctx {
                     int a0 = m->nodes[s + j], a1 = m->nodes[s + (j + 1) % n];
                     auto it = m->arcs.find({std::min(a0, a1), std::max(a0, a1)});
                     if (it == m->arcs.end()) continue;
-                    chords[c].push_back({m->xy[2 * a0], m->xy[2 * a0 + 1], m->xy[2 * a1], m->xy[2 * a1 + 1], it->second});
+                    chords[c].push_back({m->xy[2 * a0], m->xy[2 * a0 + 1], m->xy[2 * a1], m->xy[2 * a1 + 1], it->second, j, (j + 1) % n});
                 }
             }
         }
@@ -766,20 +804,20 @@ struct Ctx {
 
 static Geo eval_map(const Ctx& X, int cell, const Tri* tri, double xi, double eta) {
     if (X.pb.map_kind == MAP_SIN) return sin_map(X.pb.eps, xi, eta);
+    (void)tri;
     Geo g;
     double F[2] = {xi, eta}, J[4] = {1.0, 0.0, 0.0, 1.0};
-    for (const Chord& ch : X.chords[cell]) {
-        // the sub-triangle that holds both chord end points carries the blend
-        int hit = 0, apex = -1;
-        for (int i = 0; i < 3; ++i) {
-            double vx = tri->v[2 * i], vy = tri->v[2 * i + 1];
-            if ((vx == ch.ax && vy == ch.ay) || (vx == ch.bx && vy == ch.by))
-                ++hit;
-            else
-                apex = i;
+    if (!X.chords[cell].empty()) {
+        const Mesh& m = *X.m;
+        int n = m.npc[cell], s0 = m.off[cell];
+        std::vector<double> vx(n), vy(n);
+        for (int i = 0; i < n; ++i) {
+            vx[i] = m.xy[2 * m.nodes[s0 + i]];
+            vy[i] = m.xy[2 * m.nodes[s0 + i] + 1];
         }
-        if (hit != 2) continue;
-        arc_term(ch, &tri->v[2 * apex], xi, eta, F, J);
+        double lam[64], gl[128];
+        wachspress(vx, vy, xi, eta, lam, gl);
+        for (const Chord& ch : X.chords[cell]) arc_term(ch, lam, gl, F, J);
     }
     g.x = F[0];
     g.y = F[1];

@@ -257,43 +257,78 @@ static void triangulate(const Mesh& m, int c, std::vector<Tri>& out, double& tot
 
 struct Chord {
     double ax, ay, bx, by, R;
+    int ia, ib;  // local vertex indices of the chord end points
 };
 
 struct Geo {
     double X, Y, j00, j01, j10, j11, det;
 };
 
-// Chord-to-arc blend on the sub-triangle (A, B, C) holding chord AB:
-// F = xi + (1 - l_C) delta(s), s = l_B / (l_A + l_B), delta(s) = arc(s) - chord(s).
-static void chord_blend(const Chord& ch, double cx, double cy, double x, double y, double* F, double* J) {
-    double d = (ch.bx - ch.ax) * (cy - ch.ay) - (cx - ch.ax) * (ch.by - ch.ay);
-    double lB = ((x - ch.ax) * (cy - ch.ay) - (cx - ch.ax) * (y - ch.ay)) / d;
-    double lC = ((ch.bx - ch.ax) * (y - ch.ay) - (x - ch.ax) * (ch.by - ch.ay)) / d;
-    double lA = 1 - lB - lC;
-    double gB0 = (cy - ch.ay) / d, gB1 = -(cx - ch.ax) / d;
-    double gC0 = -(ch.by - ch.ay) / d, gC1 = (ch.bx - ch.ax) / d;
-    double gA0 = -gB0 - gC0, gA1 = -gB1 - gC1;
-    double sab = lA + lB, s = lB / sab;
+// Wachspress coordinates of a convex polygon and their gradients, in the
+// product form l_i = A_i P_i / D (P_i the product of the edge areas not
+// adjacent to vertex i), which stays finite on the polygon's edges.
+static void wachspress(const double* vx, const double* vy, int n, double x, double y, double* lam, double* gl) {
+    double a[16], ga[16][2], P[16], gP[16][2], Ac[16];
+    for (int i = 0; i < n; ++i) {
+        int j = (i + 1) % n;
+        a[i] = 0.5 * ((vx[i] - x) * (vy[j] - y) - (vy[i] - y) * (vx[j] - x));
+        ga[i][0] = 0.5 * (vy[i] - vy[j]);
+        ga[i][1] = 0.5 * (vx[j] - vx[i]);
+    }
+    double D = 0, gD0 = 0, gD1 = 0;
+    for (int i = 0; i < n; ++i) {
+        int p = (i - 1 + n) % n, q = (i + 1) % n;
+        Ac[i] = 0.5 * ((vx[i] - vx[p]) * (vy[q] - vy[p]) - (vy[i] - vy[p]) * (vx[q] - vx[p]));
+        P[i] = 1.0;
+        gP[i][0] = gP[i][1] = 0.0;
+        for (int m = 0; m < n; ++m) {
+            if (m == p || m == i) continue;
+            gP[i][0] = gP[i][0] * a[m] + P[i] * ga[m][0];
+            gP[i][1] = gP[i][1] * a[m] + P[i] * ga[m][1];
+            P[i] *= a[m];
+        }
+        D += Ac[i] * P[i];
+        gD0 += Ac[i] * gP[i][0];
+        gD1 += Ac[i] * gP[i][1];
+    }
+    for (int i = 0; i < n; ++i) {
+        lam[i] = Ac[i] * P[i] / D;
+        gl[2 * i] = (Ac[i] * gP[i][0] - lam[i] * gD0) / D;
+        gl[2 * i + 1] = (Ac[i] * gP[i][1] - lam[i] * gD1) / D;
+    }
+}
+
+// Chord-to-arc map of one boundary cell: F = xi + sum over its chords of
+// S^2 delta(s), S = l_A + l_B, s = l_B / S, delta(s) = arc(s) - chord(s), with
+// Wachspress coordinates l.  F is smooth inside the cell, maps each chord onto
+// its arc and is the identity on the straight edges.
+static void chord_blend(const Chord& ch, const double* lam, const double* gl, double* F, double* J) {
+    double lA = lam[ch.ia], lB = lam[ch.ib], S = lA + lB;
+    if (S <= 1e-300) return;
+    const double *gA = &gl[2 * ch.ia], *gB = &gl[2 * ch.ib];
+    double s = lB / S;
     double tA = std::atan2(ch.ay, ch.ax), dt = std::atan2(ch.by, ch.bx) - tA;
     if (dt > M_PI) dt -= 2 * M_PI;
     if (dt <= -M_PI) dt += 2 * M_PI;
     double th = tA + s * dt, ct = std::cos(th), st = std::sin(th);
     double del0 = ch.R * ct - (ch.ax + s * (ch.bx - ch.ax)), del1 = ch.R * st - (ch.ay + s * (ch.by - ch.ay));
     double dd0 = -ch.R * dt * st - (ch.bx - ch.ax), dd1 = ch.R * dt * ct - (ch.by - ch.ay);
-    double gs0 = (lA * gB0 - lB * gA0) / sab, gs1 = (lA * gB1 - lB * gA1) / sab;
-    double w = 1 - lC;
-    F[0] += w * del0;
-    F[1] += w * del1;
-    J[0] += -del0 * gC0 + dd0 * gs0;
-    J[1] += -del0 * gC1 + dd0 * gs1;
-    J[2] += -del1 * gC0 + dd1 * gs0;
-    J[3] += -del1 * gC1 + dd1 * gs1;
+    double gS0 = gA[0] + gB[0], gS1 = gA[1] + gB[1];
+    double w0 = lA * gB[0] - lB * gA[0], w1 = lA * gB[1] - lB * gA[1];
+    F[0] += S * S * del0;
+    F[1] += S * S * del1;
+    J[0] += 2 * S * del0 * gS0 + dd0 * w0;
+    J[1] += 2 * S * del0 * gS1 + dd0 * w1;
+    J[2] += 2 * S * del1 * gS0 + dd1 * w0;
+    J[3] += 2 * S * del1 * gS1 + dd1 * w1;
 }
 
 struct CellGeom {
     std::vector<Chord> chords;
-    // which chords a triangle carries (bitmask) and their apex
+    double vx[16], vy[16];
+    int n = 0;
     Geo eval(const vemd_problem_desc& d, const Tri& t, double x, double y) const {
+        (void)t;
         Geo g;
         if (d.map_kind == VEMD_MAP_SIN) {
             double e = d.eps;
@@ -305,17 +340,10 @@ struct CellGeom {
             g.j11 = 1.0;
         } else {
             double F[2] = {x, y}, J[4] = {1, 0, 0, 1};
-            for (const Chord& ch : chords) {
-                int hit = 0, apex = -1;
-                for (int i = 0; i < 3; ++i) {
-                    bool isA = t.v[2 * i] == ch.ax && t.v[2 * i + 1] == ch.ay;
-                    bool isB = t.v[2 * i] == ch.bx && t.v[2 * i + 1] == ch.by;
-                    if (isA || isB)
-                        ++hit;
-                    else
-                        apex = i;
-                }
-                if (hit == 2) chord_blend(ch, t.v[2 * apex], t.v[2 * apex + 1], x, y, F, J);
+            if (!chords.empty()) {
+                double lam[16], gl[32];
+                wachspress(vx, vy, n, x, y, lam, gl);
+                for (const Chord& ch : chords) chord_blend(ch, lam, gl, F, J);
             }
             g.X = F[0];
             g.Y = F[1];
@@ -328,6 +356,34 @@ struct CellGeom {
         return g;
     }
 };
+
+static CellGeom cell_geom(const Mesh& m, const vemd_problem_desc& d, int c) {
+    CellGeom cg;
+    int ne = m.npc(c), s0 = m.off[c];
+    if (d.map_kind != VEMD_MAP_ARCS || m.arcs.empty()) return cg;
+    for (int j = 0; j < ne; ++j) {
+        int a0 = m.nodes[s0 + j], a1 = m.nodes[s0 + (j + 1) % ne];
+        auto it = m.arcs.find(Mesh::key(a0, a1));
+        if (it != m.arcs.end())
+            cg.chords.push_back({m.xy[2 * a0], m.xy[2 * a0 + 1], m.xy[2 * a1], m.xy[2 * a1 + 1], it->second, j, (j + 1) % ne});
+    }
+    if (!cg.chords.empty()) {
+        if (ne > 16) throw Error(VEMD_ERR_MESH, "curved cell with more than 16 edges");
+        cg.n = ne;
+        for (int j = 0; j < ne; ++j) {
+            cg.vx[j] = m.xy[2 * m.nodes[s0 + j]];
+            cg.vy[j] = m.xy[2 * m.nodes[s0 + j] + 1];
+        }
+        // the Wachspress blend needs a strictly convex cell
+        for (int j = 0; j < ne; ++j) {
+            int p = (j - 1 + ne) % ne, q = (j + 1) % ne;
+            double cr = (cg.vx[j] - cg.vx[p]) * (cg.vy[q] - cg.vy[p]) - (cg.vy[j] - cg.vy[p]) * (cg.vx[q] - cg.vx[p]);
+            if (!(cr > 0.0))
+                throw Error(VEMD_ERR_MESH, "curved boundary cell " + std::to_string(c) + " is not strictly convex");
+        }
+    }
+    return cg;
+}
 
 static void exact(int sol, double x, double y, double& p, double& ux, double& uy, double& g) {
     double sx = std::sin(M_PI * x), cx = std::cos(M_PI * x), sy = std::sin(M_PI * y), cy = std::cos(M_PI * y);
@@ -372,14 +428,7 @@ static void build_cell(const Mesh& m, const vemd_problem_desc& d, const Shape& S
     const int nm = (k + 2) * (k + 3) / 2;  // monomials up to degree k+1
     const double xc = m.cx[c], yc = m.cy[c], h = m.diam[c];
     const int s0 = m.off[c];
-    CellGeom cg;
-    if (d.map_kind == VEMD_MAP_ARCS && !m.arcs.empty())
-        for (int j = 0; j < ne; ++j) {
-            int a0 = m.nodes[s0 + j], a1 = m.nodes[s0 + (j + 1) % ne];
-            auto it = m.arcs.find(Mesh::key(a0, a1));
-            if (it != m.arcs.end())
-                cg.chords.push_back({m.xy[2 * a0], m.xy[2 * a0 + 1], m.xy[2 * a1], m.xy[2 * a1 + 1], it->second});
-        }
+    CellGeom cg = cell_geom(m, d, c);
     double tri_area;
     triangulate(m, c, w.tris, tri_area);
     // volume points
@@ -404,6 +453,8 @@ static void build_cell(const Mesh& m, const vemd_problem_desc& d, const Shape& S
     w.mono.resize((size_t)np * nm);
     for (int l = 0; l < np; ++l) {
         w.geo[l] = cg.eval(d, w.tris[w.pt[l]], w.px[l], w.py[l]);
+        if (!(w.geo[l].det > 0.0))
+            throw Error(VEMD_ERR_MESH, "non-positive map Jacobian in cell " + std::to_string(c));
         double xm = (w.px[l] - xc) / h, ym = (w.py[l] - yc) / h;
         for (int a = 0; a < p; ++a) hdiv_eval(S, a, xm, ym, h, w.gx[l * p + a], w.gy[l * p + a]);
         for (int r = 0; r < nm; ++r) {
@@ -741,15 +792,7 @@ void errors_host(const Solver& s, const double* sol, int nq, double* out) {
                 for (int i = 0; i < n; ++i) v += P[a * n + i] * u[i];
                 coef[a] = v;
             }
-            CellGeom cg;
-            int s0 = m.off[c];
-            if (d.map_kind == VEMD_MAP_ARCS && !m.arcs.empty())
-                for (int j = 0; j < ne; ++j) {
-                    int a0 = m.nodes[s0 + j], a1 = m.nodes[s0 + (j + 1) % ne];
-                    auto it = m.arcs.find(Mesh::key(a0, a1));
-                    if (it != m.arcs.end())
-                        cg.chords.push_back({m.xy[2 * a0], m.xy[2 * a0 + 1], m.xy[2 * a1], m.xy[2 * a1 + 1], it->second});
-                }
+            CellGeom cg = cell_geom(m, d, (int)c);
             double ta;
             triangulate(m, (int)c, tris, ta);
             double xc = m.cx[c], yc = m.cy[c], h = m.diam[c];
