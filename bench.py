@@ -1,0 +1,225 @@
+"""Benchmark: time-to-solve of the mixed-VEM Darcy system on one MI355X.
+
+Workload (the 1M-cell annulus configuration): seeded Lloyd-relaxed Voronoi mesh
+of the annulus 0.5 <= r <= 1 with exact boundary arcs, k = 1, log-normal
+permeability K = exp(theta) I with theta a squared-exponential Gaussian random
+field (sigma 1, correlation length 0.05) sampled on cell centroids, manufactured
+data p = sin(pi x) sin(pi y) with Dirichlet pressure.  One step = one
+mixed-precision solve (FP32 operator and preconditioner inside FP64 iterative
+refinement) to a true relative residual of 1e-8, with the right-hand side
+resident in HBM.  Also reported: the FP64 solve of the same system, the
+operator-apply bandwidth against HBM peak, and the CPU restatement of the
+reference algorithm on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells C]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def load_pkg():
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    return ge._load_package()
+
+
+def algorithmic_bytes(valences, k, ntot, w):
+    """Minimal bytes of one operator apply: per cell the packed symmetric local
+    matrix, W, one scale and n_E int32 flux indices; x read once and y written
+    once."""
+    q = (k + 1) * (k + 2) // 2
+    ni = (q - 1) + (q - (k + 1))
+    total = 0
+    for ne, cnt in enumerate(valences):
+        if cnt == 0:
+            continue
+        n = (k + 1) * ne + ni
+        total += cnt * (w * (n * (n + 1) // 2 + q * n + 1) + 4 * n)
+    return total + w * 2 * ntot
+
+
+def build(pkg, cells, seed):
+    from wsl_code_amd.fields import grf_theta
+    t0 = time.time()
+    mesh = pkg.generate_mesh(cells, domain=pkg.DOMAIN_ANNULUS, r_inner=0.5, r_outer=1.0, lloyd_iters=20, seed=seed,
+                             min_edge_frac=0.05)
+    a = mesh.arrays()
+    t1 = time.time()
+    theta = grf_theta(a["cx"], a["cy"], sigma=1.0, ell=0.05, seed=seed + 1)
+    t2 = time.time()
+    s = pkg.Solver(mesh, k=1, nq2d=9, nq1d=9, map_kind=pkg.MAP_ARCS, solution=pkg.SOL_SINSIN, bc=pkg.BC_DIRICHLET_P,
+                   theta=theta)
+    t3 = time.time()
+    log("mesh %d cells %.1fs, theta %.1fs, FP64 element setup %.1fs; unknowns %d, colours %d"
+        % (mesh.num_cells, t1 - t0, t2 - t1, t3 - t2, s.ntot, s.ncolours))
+    return mesh, a, theta, s
+
+
+def time_applies(pkg, s, torch, precision, reps):
+    dt = torch.float64 if precision == pkg.FP64 else torch.float32
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(s.nvec, dtype=dt, device="cuda", generator=g)
+    y = torch.empty_like(x)
+    st = torch.cuda.current_stream()
+    L = pkg.lib()
+
+    def one():
+        rc = L.vemd_apply_device(s.h, precision, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                 ctypes.c_void_p(st.cuda_stream))
+        if rc:
+            raise RuntimeError(L.vemd_last_error().decode())
+
+    for _ in range(3):
+        one()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        one()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps  # ms per apply (all colour sweeps)
+
+
+def cpu_baseline(pkg, cells, seed):
+    """The oracle (C++ restatement of the reference's CPU algorithm: element
+    assembly, CSR, GMRES(100) + ILU(0), rtol 1e-10, one thread) on a bounded
+    sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from wsl_code_amd.fields import grf_theta
+    mesh = pkg.generate_mesh(cells, domain=pkg.DOMAIN_ANNULUS, r_inner=0.5, lloyd_iters=20, seed=seed,
+                             min_edge_frac=0.05)
+    a = mesh.arrays()
+    theta = grf_theta(a["cx"], a["cy"], sigma=1.0, ell=0.05, seed=seed + 1)
+    t0 = time.time()
+    om = O.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+    op = O.Problem(om, k=1, nq2=9, nq1=9, map_kind=O.MAP_ARCS, sol=O.SOL_SINSIN, bc=O.BC_DIRICHLET_P, theta=theta)
+    op.assemble()
+    _, its, rel = op.solve_gmres_ilu(rtol=1e-10, restart=100)
+    t1 = time.time()
+    return t1 - t0, its, rel
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1000000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--apply-reps", type=int, default=50)
+    ap.add_argument("--mode", choices=["solve", "apply"], default="solve",
+                    help="apply: only the operator-apply timing (for profiling runs)")
+    ap.add_argument("--cpu-cells", type=int, default=8192)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; a figure at %d GPUs is not produced (one process per GPU "
+              "would need the halo exchange described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    pkg = load_pkg()
+    assert torch.cuda.is_available() and pkg.device_count() > 0, "no HIP device"
+    mesh, arr, theta, s = build(pkg, args.cells, args.seed)
+    t0 = time.time()
+    s.upload(0)
+    log("upload %.1fs" % (time.time() - t0))
+    valences = np.bincount(np.diff(arr["cell_off"]))
+    b64 = algorithmic_bytes(valences, 1, s.ntot, 8)
+    b32 = algorithmic_bytes(valences, 1, s.ntot, 4)
+    ms64 = time_applies(pkg, s, torch, pkg.FP64, args.apply_reps)
+    ms32 = time_applies(pkg, s, torch, pkg.FP32, args.apply_reps)
+    gbs64, gbs32 = b64 / ms64 / 1e6, b32 / ms32 / 1e6
+    log("apply FP64 %.3f ms (%.0f GB/s, %.1f%% of HBM peak), FP32 %.3f ms (%.0f GB/s)"
+        % (ms64, gbs64, 100 * gbs64 / HBM_PEAK_GBS, ms32, gbs32))
+    roofline = {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_apply<double,1> (all colour sweeps of one FP64 apply)",
+                "bytes_per_apply": int(b64), "ms_per_apply": round(ms64, 4),
+                "fp32": {"achieved": round(gbs32, 1), "frac": round(gbs32 / HBM_PEAK_GBS, 4),
+                         "bytes_per_apply": int(b32), "ms_per_apply": round(ms32, 4)}}
+    if args.mode == "apply":
+        print(json.dumps({"mode": "apply", "roofline": roofline}))
+        return
+    L = pkg.lib()
+    xm = torch.empty(s.nvec, dtype=torch.float64, device="cuda")
+    x64 = torch.empty_like(xm)
+
+    def solve(prec, out):
+        o = pkg.SolveOptions(precision=prec, rtol=1e-8, inner_rtol=1e-4, check_every=64).c()
+        info = pkg._lib.SolveInfo()
+        rc = L.vemd_solve_device(s.h, ctypes.byref(o), None, ctypes.c_void_p(out.data_ptr()), ctypes.byref(info))
+        if rc:
+            raise RuntimeError(L.vemd_last_error().decode())
+        return dict(iterations=info.iterations, refinements=info.refinements, rel_residual=info.rel_residual,
+                    seconds=info.seconds)
+
+    for _ in range(args.warmup):
+        i = solve(pkg.MIXED, xm)
+        log("warmup mixed: %s" % i)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    infos = [solve(pkg.MIXED, xm) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_mixed = (time.time() - t0) / args.steps
+    log("mixed: %.3f s per solve, %s" % (t_mixed, infos[-1]))
+    torch.cuda.synchronize()
+    t0 = time.time()
+    i64 = solve(pkg.FP64, x64)
+    torch.cuda.synchronize()
+    t_fp64 = time.time() - t0
+    log("fp64: %.3f s, %s" % (t_fp64, i64))
+    # accuracy of the mixed solution against the FP64 one
+    xr_m, xr_64 = np.zeros(s.ntot), np.zeros(s.ntot)
+    hm, h64 = xm.cpu().numpy(), x64.cpu().numpy()
+    L.vemd_to_reference(s.h, hm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        xr_m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0)
+    L.vemd_to_reference(s.h, h64.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        xr_64.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0)
+    nf = s.nflux
+    flux_rel = float(np.linalg.norm(xr_m[:nf] - xr_64[:nf]) / np.linalg.norm(xr_64[:nf]))
+    pres_rel = float(np.linalg.norm(xr_m[nf:] - xr_64[nf:]) / np.linalg.norm(xr_64[nf:]))
+    em, e64 = s.errors(xr_m), s.errors(xr_64)
+    cpu = None
+    if not args.no_cpu_baseline:
+        ct, cits, crel = cpu_baseline(pkg, args.cpu_cells, args.seed)
+        cpu = {"value": round(ct, 3), "unit": "s per solve of the sample", "cores": 1, "kind": "port",
+               "sample": "%d-cell annulus of the same workload: oracle element assembly + CSR + GMRES(100)/ILU(0) "
+                         "to rtol 1e-10 (%d iterations), 1 thread of %d" % (args.cpu_cells, cits, os.cpu_count())}
+        log("cpu baseline: %.2f s (%d its)" % (ct, cits))
+    out = {
+        "metric": "time-to-solve at 1e-8 rel. residual, mixed vs FP64 GPU; operator-apply GB/s",
+        "value": round(t_mixed, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000 * t_mixed, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 operator+preconditioner, fp64 refinement/residuals/inner products",
+        "data": "synthetic: seeded Voronoi annulus, GRF log-permeability, manufactured sin*sin data",
+        "config": {"workload": "annulus 0.5<=r<=1, %d-cell Lloyd Voronoi (20 iters), k=1, log-normal K "
+                               "(sigma 1, ell 0.05), Dirichlet p" % mesh.num_cells,
+                   "cells": mesh.num_cells, "unknowns": s.ntot, "colours": s.ncolours, "parallelism": "1 GPU"},
+        "fp64_seconds": round(t_fp64, 4), "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 3),
+        "mixed": infos[-1], "fp64": i64,
+        "mixed_vs_fp64": {"flux_rel_l2": flux_rel, "pressure_rel_l2": pres_rel,
+                          "flux_err_diff": abs(em["u_phys"] - e64["u_phys"]),
+                          "pressure_err_diff": abs(em["p_phys"] - e64["p_phys"])},
+        "apply_GBps": {"fp64": round(gbs64, 1), "fp32": round(gbs32, 1)},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

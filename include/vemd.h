@@ -170,6 +170,21 @@ typedef struct {
 vemd_status vemd_solve(vemd_solver s, const vemd_solve_opts* opts, const double* rhs, double* solution,
                        vemd_solve_info* info);
 
+/* Device-resident solve in the internal layout: b_dev holds the symmetric-form
+ * right-hand side (NULL = the problem's own, kept in HBM since upload) and
+ * x_dev receives the solution (NULL = keep it on the device only); both are
+ * vectors of sizes[3] doubles.  info->seconds covers the whole call.  Used by
+ * benchmarks so that no host transfer is timed. */
+vemd_status vemd_solve_device(vemd_solver s, const vemd_solve_opts* opts, const void* b_dev, void* x_dev,
+                              vemd_solve_info* info);
+
+/* Host conversions between the reference DOF order and the internal device
+ * layout; scale_rows applies the row scaling from the reference-form system to
+ * the internal symmetric one (pressure and multiplier rows negated, essential
+ * boundary rows times exp(-theta_E)) and unscale_rows undoes it. */
+vemd_status vemd_to_internal(vemd_solver s, const double* ref, double* internal, int scale_rows);
+vemd_status vemd_to_reference(vemd_solver s, const double* internal, double* ref, int unscale_rows);
+
 /* y = K x for the reference-form operator (rows as in CPU/solver/DarcySolver.cpp:
  * 445-476 with direction signs; essential boundary rows are identity).
  * precision VEMD_FP64 or VEMD_FP32.  Replaces MixedGpu::apply

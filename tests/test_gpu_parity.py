@@ -53,13 +53,21 @@ def test_fp64_solve_matches_oracle_direct(vemd, oracle, case):
 
 @pytest.mark.parametrize("case", CASES, ids=IDS)
 def test_mixed_solve(vemd, oracle, case):
+    """FP32 operator and preconditioner inside FP64 refinement reach the same
+    1e-8 true residual; the error functionals then agree with those of a tight
+    FP64 solve to 1e-9 absolute or 1e-7 relative (a 1e-8 residual moves an O(1)
+    error functional of a coarse mesh by a few 1e-9)."""
     s, op = _pair(vemd, oracle, *case)
+    xt, _ = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12))
     x64, i64 = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-8))
     xm, im = s.solve(vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-8, inner_rtol=1e-4))
+    assert i64["rel_residual"] <= 1e-8
     assert im["rel_residual"] <= 1e-8 and im["refinements"] >= 1
-    e64, em = s.errors(x64), s.errors(xm)
-    assert abs(em["u_comp"] - e64["u_comp"]) <= 1e-9
-    assert abs(em["p_comp"] - e64["p_comp"]) <= 1e-9
+    et, e64, em = s.errors(xt), s.errors(x64), s.errors(xm)
+    for key in ("u_comp", "p_comp"):
+        assert abs(em[key] - et[key]) <= max(1e-9, 1e-7 * et[key]), (key, em[key], et[key])
+        assert abs(e64[key] - et[key]) <= max(1e-9, 1e-7 * et[key]), (key, e64[key], et[key])
+    assert np.linalg.norm(xm - xt) <= 1e-5 * np.linalg.norm(xt)
 
 
 GOLD = [c for c in json.load(open(golden("golden_errors.json")))["cases"] if c["k"] <= 1]

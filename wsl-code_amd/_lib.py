@@ -65,6 +65,9 @@ SIGNATURES = {
     "vemd_solver_upload": [c_vp, ctypes.c_int],
     "vemd_set_theta": [c_vp, c_dp],
     "vemd_solve": [c_vp, ctypes.POINTER(SolveOptsC), c_dp, c_dp, ctypes.POINTER(SolveInfo)],
+    "vemd_solve_device": [c_vp, ctypes.POINTER(SolveOptsC), c_vp, c_vp, ctypes.POINTER(SolveInfo)],
+    "vemd_to_internal": [c_vp, c_dp, c_dp, ctypes.c_int],
+    "vemd_to_reference": [c_vp, c_dp, c_dp, ctypes.c_int],
     "vemd_apply": [c_vp, ctypes.c_int, c_dp, c_dp],
     "vemd_apply_device": [c_vp, ctypes.c_int, c_vp, c_vp, c_vp],
     "vemd_rhs": [c_vp, c_dp],

@@ -246,6 +246,30 @@ vemd_status vemd_solve(vemd_solver sh, const vemd_solve_opts* o, const double* r
     API_END
 }
 
+vemd_status vemd_solve_device(vemd_solver sh, const vemd_solve_opts* o, const void* b_dev, void* x_dev,
+                              vemd_solve_info* info) {
+    API_BEGIN
+    REQUIRE(sh && o, "null argument");
+    REQUIRE(o->rtol > 0 && o->rtol < 1, "rtol must be in (0, 1)");
+    REQUIRE(o->precision == VEMD_FP64 || o->precision == VEMD_MIXED, "precision must be VEMD_FP64 or VEMD_MIXED");
+    dev_solve_int(*sh->s, *o, b_dev, x_dev, info);
+    API_END
+}
+
+vemd_status vemd_to_internal(vemd_solver sh, const double* ref, double* internal, int scale_rows) {
+    API_BEGIN
+    REQUIRE(sh && ref && internal, "null argument");
+    sh->s->to_internal(ref, internal, scale_rows != 0);
+    API_END
+}
+
+vemd_status vemd_to_reference(vemd_solver sh, const double* internal, double* ref, int unscale_rows) {
+    API_BEGIN
+    REQUIRE(sh && ref && internal, "null argument");
+    sh->s->to_reference(internal, ref, unscale_rows != 0);
+    API_END
+}
+
 vemd_status vemd_apply(vemd_solver sh, int precision, const double* x, double* y) {
     API_BEGIN
     REQUIRE(sh && x && y && x != y, "null or aliased argument");

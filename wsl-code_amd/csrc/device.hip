@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -557,6 +559,7 @@ struct DeviceState {
     double* r64 = nullptr;  // outer residual
     double* y64 = nullptr;
     double* b64 = nullptr;
+    double* bprob = nullptr;  // the problem's own right-hand side, internal order
     float* r32 = nullptr;
     std::vector<void*> allocs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -623,6 +626,9 @@ static void upload_theta(Solver& s) {
     std::vector<float> m(s.minv.begin(), s.minv.end());
     HIPCHK(hipMemcpy(d.o64.minv, s.minv.data(), s.minv.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.o32.minv, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<double> bi(s.layout.nvec, 0.0);
+    s.to_internal(s.rhs_ref.data(), bi.data(), true);
+    HIPCHK(hipMemcpy(d.bprob, bi.data(), bi.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 void dev_upload(Solver& s, int device) {
@@ -668,7 +674,8 @@ void dev_upload(Solver& s, int device) {
     d.y64 = (double*)dalloc(d, L.nvec * sizeof(double));
     d.b64 = (double*)dalloc(d, L.nvec * sizeof(double));
     d.r32 = (float*)dalloc(d, L.nvec * sizeof(float));
-    for (double* p : {d.x64, d.r64, d.y64, d.b64}) HIPCHK(hipMemset(p, 0, L.nvec * sizeof(double)));
+    d.bprob = (double*)dalloc(d, L.nvec * sizeof(double));
+    for (double* p : {d.x64, d.r64, d.y64, d.b64, d.bprob}) HIPCHK(hipMemset(p, 0, L.nvec * sizeof(double)));
     HIPCHK(hipMemset(d.r32, 0, L.nvec * sizeof(float)));
     HIPCHK(hipEventCreate(&d.e0));
     HIPCHK(hipEventCreate(&d.e1));
@@ -848,6 +855,8 @@ static int minres(Solver& s, DevKrylov<T>& k, double rtol, int maxit, int check_
     hipLaunchKernelGGL((k_init<T>), dim3(d.nblk), dim3(256), 0, d.stream, u, (const T*)k.b);
     hipLaunchKernelGGL(k_finInit, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.nblk, rtol, maxit);
     HIPCHK(hipGetLastError());
+    static const bool verbose = std::getenv("VEMD_VERBOSE") != nullptr;
+    auto last = std::chrono::steady_clock::now();
     while (true) {
         if (use_graph)
             HIPCHK(hipGraphLaunch(k.graph, d.stream));
@@ -856,6 +865,11 @@ static int minres(Solver& s, DevKrylov<T>& k, double rtol, int maxit, int check_
         HIPCHK(hipMemcpyAsync(d.st_host, d.st, sizeof(MState), hipMemcpyDeviceToHost, d.stream));
         HIPCHK(hipStreamSynchronize(d.stream));
         if (d.st_host->done) break;
+        if (verbose && std::chrono::steady_clock::now() - last > std::chrono::seconds(10)) {
+            last = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "[vemd] minres %s: %d iterations, residual estimate %.3e of %.3e\n",
+                         sizeof(T) == 8 ? "fp64" : "fp32", d.st_host->itn, d.st_host->phibar / d.st_host->beta1, rtol);
+        }
     }
     hipLaunchKernelGGL((k_flush<T>), dim3(d.nblk), dim3(256), 0, d.stream, u);
     HIPCHK(hipGetLastError());
@@ -934,6 +948,31 @@ void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, doubl
     HIPCHK(hipMemcpyAsync(bi.data(), d.x64, L.nvec * sizeof(double), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
     if (sol_ref) s.to_reference(bi.data(), sol_ref, false);
+    if (info) *info = tmp;
+    if (code) throw Error(code, err);
+}
+
+void dev_solve_int(Solver& s, const vemd_solve_opts& o, const void* b_dev, void* x_dev, vemd_solve_info* info) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    DeviceState& d = *s.dev;
+    HIPCHK(hipSetDevice(d.device));
+    const int64_t n = s.layout.nvec;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(d.b64, b_dev ? b_dev : d.bprob, n * sizeof(double), hipMemcpyDeviceToDevice, d.stream));
+    vemd_solve_info tmp{};
+    std::string err;
+    int code = 0;
+    try {
+        solve_device(s, o, &tmp);
+    } catch (const Error& e) {
+        if (e.code != VEMD_ERR_NOT_CONVERGED) throw;
+        err = e.what();
+        code = e.code;
+    }
+    if (x_dev) HIPCHK(hipMemcpyAsync(x_dev, d.x64, n * sizeof(double), hipMemcpyDeviceToDevice, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    tmp.seconds = std::chrono::duration<double>(t1 - t0).count();
     if (info) *info = tmp;
     if (code) throw Error(code, err);
 }

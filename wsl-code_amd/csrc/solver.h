@@ -35,6 +35,7 @@ void dev_free(Solver& s);
 void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref);
 void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* stream);
 void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, double* sol_ref, vemd_solve_info* info);
+void dev_solve_int(Solver& s, const vemd_solve_opts& o, const void* b_dev, void* x_dev, vemd_solve_info* info);
 void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, const double* dldx_ref, double* grad,
                  vemd_solve_info* info);
 
