@@ -153,7 +153,13 @@ def main():
                 "fp32": {"achieved": round(gbs32, 1), "frac": round(gbs32 / HBM_PEAK_GBS, 4),
                          "bytes_per_apply": int(b32), "ms_per_apply": round(ms32, 4)}}
     if args.mode == "apply":
-        print(json.dumps({"mode": "apply", "roofline": roofline}))
+        # calibration for hardware byte counters: a plain copy of known size
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "apply", "roofline": roofline, "calibration_copy_bytes": 2 * src.numel() * 8}))
         return
     L = pkg.lib()
     xm = torch.empty(s.nvec, dtype=torch.float64, device="cuda")

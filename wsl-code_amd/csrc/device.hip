@@ -781,8 +781,9 @@ static void apply_plain(Solver& s, const T* x, T* y) {
 void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* stream) {
     if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
     HIPCHK(hipSetDevice(s.dev->device));
+    // the caller's stream; NULL is the null (default) stream, as documented
     hipStream_t saved = s.dev->stream;
-    if (stream) s.dev->stream = (hipStream_t)stream;
+    s.dev->stream = (hipStream_t)stream;
     try {
         if (precision == VEMD_FP64)
             apply_plain<double>(s, (const double*)x, (double*)y);
