@@ -73,12 +73,52 @@ struct ApplyArgs {
     const T* x;          // input when st == nullptr
     const T* xs[2];      // inputs selected by the Krylov iteration parity when st != nullptr
     T* y;
-    int64_t cbase, cpad, lam_idx;
+    int64_t cbase, cpad, lam_idx, nvec;
     int tile_base, ntiles, per_xcd;
     double* pdot;        // per-tile partial of x.Kx (nullptr: not needed)
     double* plam;        // per-tile partial of the multiplier row
     const MState* st;
 };
+
+// Buffer-resource access (raw buffer loads/stores): a scalar descriptor per
+// array and one 32-bit per-lane byte offset, so streaming the operator costs no
+// per-load 64-bit address registers; out-of-range accesses are dropped.
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* p, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)(bytes < 0x7fffffff ? bytes : 0x7fffffff),
+                                             0x00020000);
+}
+
+template <typename V>
+__device__ __forceinline__ V bload_vec(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ double bload1(Rsrc r, uint32_t voff, uint32_t soff, double*) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ float bload1(Rsrc r, uint32_t voff, uint32_t soff, float*) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ uint32_t bload_u32(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore1(double v, Rsrc r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore1(float v, Rsrc r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+
+// Marks register values as used and redefined here (an empty asm statement),
+// so the arithmetic that produces them cannot sink below this point.
+template <typename T, int M>
+__device__ __forceinline__ void pin(T (&v)[M]) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) asm volatile("" : "+v"(v[i]));
+}
 
 // One cell per lane: gather signed-folded local dofs, y_E = K_E x_E, scatter
 // edge dofs by colour (first writer stores, later colours accumulate).
@@ -89,67 +129,105 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
     constexpr int VW = VT<T>::W;
     constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE, N = NED + NI;
     constexpr int NA = N * (N + 1) / 2, NAV = (NA + VW - 1) / VW, NW = Q * N, NWV = (NW + VW - 1) / VW;
-    const int64_t npad = g.npad;
-    const int64_t gpos = g.gpos0 + lane;
+    constexpr uint32_t TS = sizeof(T);
+    const uint32_t npad = (uint32_t)g.npad;
+    const uint32_t gpos = (uint32_t)(g.gpos0 + lane);
+    const Rsrc rx = make_rsrc(x, a.nvec * (int64_t)TS);
+    const Rsrc ry = make_rsrc(a.y, a.nvec * (int64_t)TS);
+    const Rsrc ri = make_rsrc(a.idx + g.i_off, (int64_t)NED * npad * 4);
+    const Rsrc rA = make_rsrc(a.A + g.a_off, (int64_t)NAV * npad * 16);
+    const Rsrc rW = make_rsrc(a.W + g.w_off, (int64_t)NWV * npad * 16);
+    const uint32_t lv = (uint32_t)lane * 16u;  // per-lane offset into the interleaved operator blocks
     uint32_t id[NED];
     T u[N], p[Q], acc[N], yp[Q];
 #pragma unroll
-    for (int j = 0; j < NED; ++j) id[j] = a.idx[g.i_off + j * npad + lane];
+    for (int j = 0; j < NED; ++j) id[j] = bload_u32(ri, (uint32_t)lane * 4u, (uint32_t)j * npad * 4u);
 #pragma unroll
-    for (int j = 0; j < NED; ++j) u[j] = x[id[j] & 0x7fffffffu];
+    for (int j = 0; j < NED; ++j) u[j] = bload1(rx, (id[j] & 0x7fffffffu) * TS, 0u, (T*)nullptr);
 #pragma unroll
-    for (int j = 0; j < NI; ++j) u[NED + j] = x[a.cbase + j * a.cpad + gpos];
+    for (int j = 0; j < NI; ++j) u[NED + j] = bload1(rx, gpos * TS, (uint32_t)((a.cbase + j * a.cpad) * TS), (T*)nullptr);
 #pragma unroll
-    for (int r = 0; r < Q; ++r) p[r] = x[a.cbase + (NI + r) * a.cpad + gpos];
+    for (int r = 0; r < Q; ++r) p[r] = bload1(rx, gpos * TS, (uint32_t)((a.cbase + (NI + r) * a.cpad) * TS), (T*)nullptr);
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] = T(0);
-    const V* Av = reinterpret_cast<const V*>(a.A + g.a_off) + lane;
+    // The packed matrix streams through in chunks of CH vector loads; the
+    // scheduling barrier keeps the compiler from hoisting every load of the
+    // cell, which would need more registers than a wave has.
+    constexpr int CH = 8;
     static_for(
-        [&](auto vv) {
-            constexpr int v = decltype(vv)::value;
-            V av = __builtin_nontemporal_load(Av + v * npad);
+        [&](auto cc) {
+            constexpr int c0 = decltype(cc)::value * CH;
+            V av[CH];
             static_for(
-                [&](auto ll) {
-                    constexpr int l = decltype(ll)::value;
-                    constexpr int t = v * VW + l;
-                    if constexpr (t < NA) {
-                        constexpr int i = tri_row(t);
-                        constexpr int j = t - i * (i + 1) / 2;
-                        acc[i] += av[l] * u[j];
-                        if constexpr (i != j) acc[j] += av[l] * u[i];
-                    }
+                [&](auto vv) {
+                    constexpr int v = c0 + decltype(vv)::value;
+                    if constexpr (v < NAV) av[decltype(vv)::value] = bload_vec<V>(rA, lv, (uint32_t)v * npad * 16u);
                 },
-                std::make_integer_sequence<int, VW>{});
+                std::make_integer_sequence<int, CH>{});
+            static_for(
+                [&](auto vv) {
+                    constexpr int v = c0 + decltype(vv)::value;
+                    static_for(
+                        [&](auto ll) {
+                            constexpr int l = decltype(ll)::value;
+                            constexpr int t = v * VW + l;
+                            if constexpr (v < NAV && t < NA) {
+                                constexpr int i = tri_row(t);
+                                constexpr int j = t - i * (i + 1) / 2;
+                                const T aij = av[decltype(vv)::value][l];
+                                acc[i] += aij * u[j];
+                                if constexpr (i != j) acc[j] += aij * u[i];
+                            }
+                        },
+                        std::make_integer_sequence<int, VW>{});
+                },
+                std::make_integer_sequence<int, CH>{});
+            pin(acc);
+            __builtin_amdgcn_sched_barrier(0);
         },
-        std::make_integer_sequence<int, NAV>{});
+        std::make_integer_sequence<int, (NAV + CH - 1) / CH>{});
     const T s = a.scale[gpos];
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] *= s;
 #pragma unroll
     for (int r = 0; r < Q; ++r) yp[r] = T(0);
-    const V* Wv = reinterpret_cast<const V*>(a.W + g.w_off) + lane;
     static_for(
-        [&](auto vv) {
-            constexpr int v = decltype(vv)::value;
-            V wv = __builtin_nontemporal_load(Wv + v * npad);
+        [&](auto cc) {
+            constexpr int c0 = decltype(cc)::value * CH;
+            V wv[CH];
             static_for(
-                [&](auto ll) {
-                    constexpr int l = decltype(ll)::value;
-                    constexpr int t = v * VW + l;
-                    if constexpr (t < NW) {
-                        constexpr int r = t / N, i = t % N;
-                        acc[i] -= wv[l] * p[r];
-                        yp[r] -= wv[l] * u[i];
-                    }
+                [&](auto vv) {
+                    constexpr int v = c0 + decltype(vv)::value;
+                    if constexpr (v < NWV) wv[decltype(vv)::value] = bload_vec<V>(rW, lv, (uint32_t)v * npad * 16u);
                 },
-                std::make_integer_sequence<int, VW>{});
+                std::make_integer_sequence<int, CH>{});
+            static_for(
+                [&](auto vv) {
+                    constexpr int v = c0 + decltype(vv)::value;
+                    static_for(
+                        [&](auto ll) {
+                            constexpr int l = decltype(ll)::value;
+                            constexpr int t = v * VW + l;
+                            if constexpr (v < NWV && t < NW) {
+                                constexpr int r = t / N, i = t % N;
+                                const T w = wv[decltype(vv)::value][l];
+                                acc[i] -= w * p[r];
+                                yp[r] -= w * u[i];
+                            }
+                        },
+                        std::make_integer_sequence<int, VW>{});
+                },
+                std::make_integer_sequence<int, CH>{});
+            pin(acc);
+            pin(yp);
+            __builtin_amdgcn_sched_barrier(0);
         },
-        std::make_integer_sequence<int, NWV>{});
+        std::make_integer_sequence<int, (NWV + CH - 1) / CH>{});
     if (a.lam_idx >= 0) {
         const T xl = x[a.lam_idx];
 #pragma unroll
         for (int r = 0; r < Q; ++r) {
-            T l = a.L[g.l_off + r * npad + lane];
+            T l = a.L[g.l_off + r * (int64_t)npad + lane];
             yp[r] -= l * xl;
             lamp -= (double)l * (double)p[r];
         }
@@ -160,19 +238,18 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
 #pragma unroll
         for (int r = 0; r < Q; ++r) dot += (double)p[r] * (double)yp[r];
     }
-    T* __restrict__ y = a.y;
 #pragma unroll
     for (int j = 0; j < NED; ++j) {
-        uint32_t k = id[j] & 0x7fffffffu;
+        const uint32_t off = (id[j] & 0x7fffffffu) * TS;
         if (id[j] & 0x80000000u)
-            y[k] += acc[j];
+            bstore1(bload1(ry, off, 0u, (T*)nullptr) + acc[j], ry, off, 0u);
         else
-            y[k] = acc[j];
+            bstore1(acc[j], ry, off, 0u);
     }
 #pragma unroll
-    for (int j = 0; j < NI; ++j) y[a.cbase + j * a.cpad + gpos] = acc[NED + j];
+    for (int j = 0; j < NI; ++j) bstore1(acc[NED + j], ry, gpos * TS, (uint32_t)((a.cbase + j * a.cpad) * TS));
 #pragma unroll
-    for (int r = 0; r < Q; ++r) y[a.cbase + (NI + r) * a.cpad + gpos] = yp[r];
+    for (int r = 0; r < Q; ++r) bstore1(yp[r], ry, gpos * TS, (uint32_t)((a.cbase + (NI + r) * a.cpad) * TS));
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -181,28 +258,19 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-template <typename T, int K>
-__global__ __launch_bounds__(64) void k_apply(ApplyArgs<T> a) {
+// One launch per (colour, valence) group: the kernel is specialised for the
+// group's polygon size so that its registers fit that size only.
+template <typename T, int K, int NE>
+__global__ __launch_bounds__(64) void k_apply(ApplyArgs<T> a, GroupDesc g) {
     if (a.st && a.st->done) return;
     const int b = blockIdx.x;
     const int t = (b & 7) * a.per_xcd + (b >> 3);  // consecutive tiles stay on one XCD (one L2)
     if (t >= a.ntiles) return;
-    const int tile = a.tile_base + t;
-    const GroupDesc g = a.groups[a.tile_group[tile]];
-    const int lane = (tile - g.tile0) * 64 + threadIdx.x;
+    const int tile = g.tile0 + t;
+    const int lane = t * 64 + threadIdx.x;
     const T* x = a.st ? a.xs[a.st->itn & 1] : a.x;
     double dot = 0.0, lamp = 0.0;
-    switch (g.ne) {
-        case 3: cell_body<T, K, 3>(a, x, g, lane, dot, lamp); break;
-        case 4: cell_body<T, K, 4>(a, x, g, lane, dot, lamp); break;
-        case 5: cell_body<T, K, 5>(a, x, g, lane, dot, lamp); break;
-        case 6: cell_body<T, K, 6>(a, x, g, lane, dot, lamp); break;
-        case 7: cell_body<T, K, 7>(a, x, g, lane, dot, lamp); break;
-        case 8: cell_body<T, K, 8>(a, x, g, lane, dot, lamp); break;
-        case 9: cell_body<T, K, 9>(a, x, g, lane, dot, lamp); break;
-        case 10: cell_body<T, K, 10>(a, x, g, lane, dot, lamp); break;
-        default: break;
-    }
+    cell_body<T, K, NE>(a, x, g, lane, dot, lamp);
     if (a.pdot || a.lam_idx >= 0) {
         dot = wave_sum(dot);
         lamp = wave_sum(lamp);
@@ -741,6 +809,7 @@ static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, const MState* st, con
     a.cbase = L.cbase;
     a.cpad = L.cpad;
     a.lam_idx = L.lam ? L.lam_idx : -1;
+    a.nvec = L.nvec;
     a.pdot = dot ? d.pdot : nullptr;
     a.plam = d.plam;
     a.st = st;
@@ -750,21 +819,48 @@ static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, const MState* st, con
     return a;
 }
 
-// Launches the colour sweeps of one apply on the solver stream.
+template <typename T>
+static const std::vector<GroupDesc>& ops_groups(const Layout& L);
+template <>
+const std::vector<GroupDesc>& ops_groups<double>(const Layout& L) {
+    return L.groups;
+}
+template <>
+const std::vector<GroupDesc>& ops_groups<float>(const Layout& L) {
+    return L.groups32;
+}
+
+template <typename T, int K>
+static void launch_group(ApplyArgs<T> a, const GroupDesc& g, hipStream_t st) {
+    dim3 grid(8 * a.per_xcd), block(64);
+    switch (g.ne) {
+        case 3: hipLaunchKernelGGL((k_apply<T, K, 3>), grid, block, 0, st, a, g); break;
+        case 4: hipLaunchKernelGGL((k_apply<T, K, 4>), grid, block, 0, st, a, g); break;
+        case 5: hipLaunchKernelGGL((k_apply<T, K, 5>), grid, block, 0, st, a, g); break;
+        case 6: hipLaunchKernelGGL((k_apply<T, K, 6>), grid, block, 0, st, a, g); break;
+        case 7: hipLaunchKernelGGL((k_apply<T, K, 7>), grid, block, 0, st, a, g); break;
+        case 8: hipLaunchKernelGGL((k_apply<T, K, 8>), grid, block, 0, st, a, g); break;
+        case 9: hipLaunchKernelGGL((k_apply<T, K, 9>), grid, block, 0, st, a, g); break;
+        case 10: hipLaunchKernelGGL((k_apply<T, K, 10>), grid, block, 0, st, a, g); break;
+        default: throw Error(VEMD_ERR_INTERNAL, "polygon size outside the device kernels");
+    }
+}
+
+// Launches the colour sweeps of one apply on the solver stream: one launch per
+// (colour, valence) group; groups of one colour write disjoint dofs.
 template <typename T>
 static void launch_apply(Solver& s, ApplyArgs<T> a) {
     const Layout& L = s.layout;
     hipStream_t st = s.dev->stream;
-    for (int c = 0; c < L.ncolours; ++c) {
-        a.tile_base = L.colour_tile0[c];
-        a.ntiles = L.colour_tile0[c + 1] - L.colour_tile0[c];
-        if (a.ntiles <= 0) continue;
+    const std::vector<GroupDesc>& groups = ops_groups<T>(L);
+    for (const GroupDesc& g : groups) {
+        a.ntiles = g.npad / TILE;
+        a.tile_base = g.tile0;
         a.per_xcd = (a.ntiles + 7) / 8;
-        dim3 grid(8 * a.per_xcd), block(64);
         if (L.k == 0)
-            hipLaunchKernelGGL((k_apply<T, 0>), grid, block, 0, st, a);
+            launch_group<T, 0>(a, g, st);
         else
-            hipLaunchKernelGGL((k_apply<T, 1>), grid, block, 0, st, a);
+            launch_group<T, 1>(a, g, st);
     }
 }
 

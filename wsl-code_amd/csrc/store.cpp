@@ -34,6 +34,79 @@ static std::vector<uint64_t> hilbert_keys(const std::vector<double>& px, const s
     return k;
 }
 
+// Colours cells so that no two cells sharing an edge have the same colour:
+// DSATUR (most-constrained cell first, ties to the earlier cell in spatial
+// order), then a balancing pass that moves cells from the largest classes into
+// smaller ones where no neighbour forbids it.  Few, balanced colours mean few,
+// full launches of the apply.
+static std::vector<int> colour_cells(const Mesh& m, const std::vector<int>& order) {
+    const int nc = m.nc;
+    auto neighbours = [&](int c, auto&& f) {
+        for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
+            int e = m.cell_edges[j];
+            for (int t = 0; t < 2; ++t) {
+                int o = m.edge_cells[2 * e + t];
+                if (o >= 0 && o != c) f(o);
+            }
+        }
+    };
+    std::vector<int> rank(nc);
+    for (int i = 0; i < nc; ++i) rank[order[i]] = i;
+    std::vector<int> colour(nc, -1);
+    std::vector<uint32_t> mask(nc, 0);
+    // buckets by saturation; each bucket is a min-heap on spatial rank
+    std::vector<std::vector<std::pair<int, int>>> bucket(33);
+    auto push = [&](int c) {
+        int s = __builtin_popcount(mask[c]);
+        bucket[s].push_back({-rank[c], c});
+        std::push_heap(bucket[s].begin(), bucket[s].end());
+    };
+    for (int c = 0; c < nc; ++c) push(c);
+    int top = 0, done = 0;
+    while (done < nc) {
+        while (top > 0 && bucket[top].empty()) --top;
+        auto& b = bucket[top];
+        if (b.empty()) {
+            ++top;  // cannot happen while cells remain; keeps the loop total
+            if (top > 32) throw Error(VEMD_ERR_INTERNAL, "colouring failed");
+            continue;
+        }
+        std::pop_heap(b.begin(), b.end());
+        int c = b.back().second;
+        b.pop_back();
+        if (colour[c] >= 0 || __builtin_popcount(mask[c]) != top) continue;  // stale entry
+        int col = 0;
+        while (mask[c] & (1u << col)) ++col;
+        if (col >= 32) throw Error(VEMD_ERR_MESH, "cell graph needs more than 32 colours");
+        colour[c] = col;
+        ++done;
+        neighbours(c, [&](int o) {
+            if (colour[o] < 0 && !(mask[o] & (1u << col))) {
+                mask[o] |= 1u << col;
+                push(o);
+                top = std::max(top, __builtin_popcount(mask[o]));
+            }
+        });
+    }
+    int ncol = 1 + *std::max_element(colour.begin(), colour.end());
+    std::vector<int64_t> size(ncol, 0);
+    for (int c = 0; c < nc; ++c) size[colour[c]]++;
+    for (int pass = 0; pass < 4; ++pass)
+        for (int c : order) {
+            uint32_t used = 0;
+            neighbours(c, [&](int o) { used |= 1u << colour[o]; });
+            int best = colour[c];
+            for (int k = 0; k < ncol; ++k)
+                if (!(used & (1u << k)) && size[k] + 1 < size[best]) best = k;
+            if (best != colour[c]) {
+                size[colour[c]]--;
+                size[best]++;
+                colour[c] = best;
+            }
+        }
+    return colour;
+}
+
 Layout layout_build(const Mesh& m, const Setup& S) {
     Layout L;
     const int nc = m.nc;
@@ -54,23 +127,8 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     std::vector<int> order(nc);
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int a, int b) { return hk[a] < hk[b] || (hk[a] == hk[b] && a < b); });
-    // greedy colouring over edge neighbours in spatial order
-    std::vector<int> colour(nc, -1);
-    int ncol = 0;
-    for (int c : order) {
-        uint32_t used = 0;
-        for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
-            int e = m.cell_edges[j];
-            for (int t = 0; t < 2; ++t) {
-                int o = m.edge_cells[2 * e + t];
-                if (o >= 0 && o != c && colour[o] >= 0) used |= 1u << colour[o];
-            }
-        }
-        int col = 0;
-        while (used & (1u << col)) ++col;
-        colour[c] = col;
-        ncol = std::max(ncol, col + 1);
-    }
+    std::vector<int> colour = colour_cells(m, order);
+    int ncol = 1 + *std::max_element(colour.begin(), colour.end());
     L.ncolours = ncol;
     // store order: colour, valence, spatial
     std::vector<int> rank(nc);
@@ -163,6 +221,11 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         aoff4 += (int64_t)((na + 3) / 4 * 4) * g.npad;
         woff4 += (int64_t)((nw + 3) / 4 * 4) * g.npad;
         loff += (int64_t)L.q * g.npad;
+    }
+    L.groups32 = L.groups;
+    for (size_t gi = 0; gi < L.groups.size(); ++gi) {
+        L.groups32[gi].a_off = L.a_off4[gi];
+        L.groups32[gi].w_off = L.w_off4[gi];
     }
     L.a_len2 = aoff2;
     L.w_len2 = woff2;

@@ -27,7 +27,8 @@ struct Layout {
     int64_t nedge = 0, ncell = 0, cpad = 0;  // edges, real cells, padded cell slots
     int64_t cbase = 0;        // first cell-owned slot
     int64_t lam_idx = -1, trash_idx = 0, nvec = 0;
-    std::vector<GroupDesc> groups;
+    std::vector<GroupDesc> groups;      // offsets of the FP64 store (vector width 2)
+    std::vector<GroupDesc> groups32;    // the same groups with FP32 store offsets (width 4)
     std::vector<int> tile_group;        // group of each global tile
     std::vector<int> colour_tile0;      // first tile of each colour (+ end)
     std::vector<int> cell_of_gpos;      // mesh cell at each store position (-1 padding)
