@@ -128,11 +128,11 @@ typedef struct {
 vemd_status vemd_solver_create(vemd_mesh m, const vemd_problem_desc* desc, vemd_solver* out);
 
 /* sizes: [total flux dofs, total pressure dofs, total unknowns (incl. multiplier),
- *         internal device vector length, number of colours, max valence].
+ *         internal device vector length, number of valence groups, max valence].
  * Replaces HdivMatrix::getTotalDofFlux / getTotalDofConc (CPU/solver/HdivMatrix.h:69-71). */
 vemd_status vemd_solver_sizes(vemd_solver s, int64_t* sizes);
 
-/* Uploads the valence-grouped, colour-ordered operator store (FP64 and FP32)
+/* Uploads the valence-grouped, spatially ordered operator store (FP64 and FP32)
  * to HBM of the given device.  No reference counterpart as a separate call
  * (GPU/src/hdiv_gpu.cu:139-192 does it inside the constructor). */
 vemd_status vemd_solver_upload(vemd_solver s, int device);
@@ -143,6 +143,7 @@ vemd_status vemd_solver_upload(vemd_solver s, int device);
 vemd_status vemd_set_theta(vemd_solver s, const double* theta);
 
 enum { VEMD_FP64 = 0, VEMD_MIXED = 1, VEMD_FP32 = 2 };
+enum { VEMD_PC_AUTO = 0, VEMD_PC_JACOBI = 1, VEMD_PC_MULTIGRID = 2 };
 
 typedef struct {
     int precision;      /* VEMD_FP64, or VEMD_MIXED (FP32 operator and preconditioner inside
@@ -152,6 +153,12 @@ typedef struct {
     double inner_rtol;  /* VEMD_MIXED: inner FP32 MINRES tolerance per refinement step */
     int check_every;    /* iterations between host convergence checks */
     int use_graph;      /* capture the iteration chunk in a hipGraph */
+    int precond;        /* VEMD_PC_JACOBI: block-diagonal with the assembled flux diagonal and
+                           diagonal pressure/multiplier Schur approximations (the reference's
+                           choice, GPU/src/equation_gpu.cu:282-315); VEMD_PC_MULTIGRID: the same
+                           flux block and one smoothed-aggregation V-cycle on the pressure Schur
+                           approximation W diag(A)^-1 W^T (Dirichlet pressure problems);
+                           VEMD_PC_AUTO (0): multigrid where built, else Jacobi */
 } vemd_solve_opts;
 
 typedef struct {
@@ -159,7 +166,7 @@ typedef struct {
     int refinements;    /* outer FP64 refinement steps (VEMD_MIXED) */
     double rel_residual;/* final true relative residual (FP64) */
     double seconds;     /* solve wall time, device-resident data */
-    double apply_ms;    /* mean operator apply time of the last solve */
+    double iter_ms;     /* mean wall time per Krylov iteration (seconds / iterations, in ms) */
 } vemd_solve_info;
 
 /* Solves with preconditioned MINRES on the symmetric form of the saddle-point
@@ -191,10 +198,21 @@ vemd_status vemd_to_reference(vemd_solver s, const double* internal, double* ref
  * (GPU/src/equation_gpu.cu:580-598). */
 vemd_status vemd_apply(vemd_solver s, int precision, const double* x, double* y);
 
-/* Device-resident apply in the internal (colour-ordered) layout, on the given
+/* Device-resident apply in the internal (store-ordered) layout, on the given
  * HIP stream (NULL = default); x_dev and y_dev must not alias.  Used by
  * benchmarks; vector length is sizes[3], element type double or float. */
 vemd_status vemd_apply_device(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* stream);
+
+/* The same apply without its final edge sum, for consumers that fuse it: y_dev
+ * receives the first-side value of every edge dof and all cell-owned and
+ * multiplier rows, y1_dev (sizes[3] values, of which only the leading edge-dof
+ * block is used) the second-side values, so that K x = y + y1 on that block.
+ * Edge dofs with one cell have no second side and are never written in y1_dev:
+ * zero it once before the first call.  This is the form the solver's own
+ * iteration uses (a single kernel launch plus, with the multiplier, one
+ * reduction).  No reference counterpart. */
+vemd_status vemd_apply_device_split(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* y1_dev,
+                                    void* stream);
 
 /* The problem's right-hand side in reference order (ntot values), after the
  * essential-boundary lift. */

@@ -15,7 +15,8 @@ The solve runs only on a HIP device; without one it raises VemdError
 (VEMD_ERR_NO_DEVICE) -- there is no CPU fallback.
 """
 from ._lib import (VemdError, lib, device_count, BC_DIRICHLET_P, BC_FLUX_MEANZERO, MAP_ARCS, MAP_SIN,
-                   SOL_SINCOS, SOL_SINSIN, FP32, FP64, MIXED, DOMAIN_ANNULUS, DOMAIN_DISK)
+                   SOL_SINCOS, SOL_SINSIN, FP32, FP64, MIXED, DOMAIN_ANNULUS, DOMAIN_DISK, PC_AUTO, PC_JACOBI,
+                   PC_MULTIGRID)
 from .mesh import Mesh, StraightMeshReader, generate_mesh
 from .solver import DarcyProblem, DarcySolver, Solver, SolveOptions
 

@@ -9,6 +9,7 @@ MAP_SIN, MAP_ARCS = 0, 1
 SOL_SINCOS, SOL_SINSIN = 0, 1
 BC_FLUX_MEANZERO, BC_DIRICHLET_P = 0, 1
 FP64, MIXED, FP32 = 0, 1, 2
+PC_AUTO, PC_JACOBI, PC_MULTIGRID = 0, 1, 2
 DOMAIN_DISK, DOMAIN_ANNULUS = 0, 1
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_IO", 3: "ERR_MESH", 4: "ERR_SINGULAR", 5: "ERR_NO_DEVICE",
@@ -40,12 +41,13 @@ class ProblemDesc(ctypes.Structure):
 
 class SolveOptsC(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iters", ctypes.c_int),
-                ("inner_rtol", ctypes.c_double), ("check_every", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("inner_rtol", ctypes.c_double), ("check_every", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("precond", ctypes.c_int)]
 
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("refinements", ctypes.c_int), ("rel_residual", ctypes.c_double),
-                ("seconds", ctypes.c_double), ("apply_ms", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("iter_ms", ctypes.c_double)]
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPE)
@@ -70,6 +72,7 @@ SIGNATURES = {
     "vemd_to_reference": [c_vp, c_dp, c_dp, ctypes.c_int],
     "vemd_apply": [c_vp, ctypes.c_int, c_dp, c_dp],
     "vemd_apply_device": [c_vp, ctypes.c_int, c_vp, c_vp, c_vp],
+    "vemd_apply_device_split": [c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_vp],
     "vemd_rhs": [c_vp, c_dp],
     "vemd_errors": [c_vp, c_dp, ctypes.c_int, c_dp],
     "vemd_adjoint_theta": [c_vp, ctypes.POINTER(SolveOptsC), c_dp, c_dp, c_dp, ctypes.POINTER(SolveInfo)],
@@ -87,6 +90,15 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIBPATH):
             raise RuntimeError("libvemd.so is not built; run __graft_entry__.build()")
+        # torch wheels bundle their own HIP runtime under the same soname; the
+        # first one loaded serves the whole process, and torch's HIP fails on
+        # another build's runtime.  Load torch's first when it is installed so
+        # that device tensors and libvemd share one runtime.
+        if os.environ.get("VEMD_NO_TORCH") != "1":
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
         L = ctypes.CDLL(LIBPATH)
         for name, args in SIGNATURES.items():
             f = getattr(L, name)

@@ -1,0 +1,424 @@
+// Host construction of the smoothed-aggregation hierarchy (Vanek, Mandel and
+// Brezina, "Algebraic multigrid by smoothed aggregation for second and fourth
+// order elliptic problems", Computing 56, 1996): strength graph, greedy
+// aggregation in spatial order, tentative prolongator from the near-null
+// vector, one damped-Jacobi smoothing step, Galerkin coarse operators.
+#include "amg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace vemd {
+
+void csr_spmv(const Csr& A, const double* x, double* y) {
+    parallel_for(A.nrows, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+            double s = 0.0;
+            for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) s += A.v[p] * x[A.ci[p]];
+            y[i] = s;
+        }
+    });
+}
+
+Csr csr_transpose(const Csr& A) {
+    Csr T;
+    T.nrows = A.ncols;
+    T.ncols = A.nrows;
+    T.rp.assign(T.nrows + 1, 0);
+    for (int c : A.ci) T.rp[c + 1]++;
+    for (int64_t i = 0; i < T.nrows; ++i) T.rp[i + 1] += T.rp[i];
+    T.ci.resize(A.nnz());
+    T.v.resize(A.nnz());
+    std::vector<int64_t> pos(T.rp.begin(), T.rp.end() - 1);
+    for (int64_t i = 0; i < A.nrows; ++i)
+        for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+            int64_t q = pos[A.ci[p]]++;
+            T.ci[q] = (int)i;
+            T.v[q] = A.v[p];
+        }
+    return T;
+}
+
+// Gustavson's row-by-row product, two passes (structure, then values), rows
+// in parallel; columns of each row sorted.
+Csr csr_mult(const Csr& A, const Csr& B) {
+    Csr C;
+    C.nrows = A.nrows;
+    C.ncols = B.ncols;
+    C.rp.assign(C.nrows + 1, 0);
+    parallel_for(A.nrows, [&](int64_t b, int64_t e) {
+        std::vector<int64_t> mark(B.ncols, -1);
+        for (int64_t i = b; i < e; ++i) {
+            int64_t cnt = 0;
+            for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+                int k = A.ci[p];
+                for (int64_t t = B.rp[k]; t < B.rp[k + 1]; ++t)
+                    if (mark[B.ci[t]] != i) {
+                        mark[B.ci[t]] = i;
+                        ++cnt;
+                    }
+            }
+            C.rp[i + 1] = cnt;
+        }
+    });
+    for (int64_t i = 0; i < C.nrows; ++i) C.rp[i + 1] += C.rp[i];
+    C.ci.resize(C.rp[C.nrows]);
+    C.v.resize(C.rp[C.nrows]);
+    parallel_for(A.nrows, [&](int64_t b, int64_t e) {
+        std::vector<double> acc(B.ncols, 0.0);
+        std::vector<int64_t> mark(B.ncols, -1);
+        std::vector<int> cols;
+        for (int64_t i = b; i < e; ++i) {
+            cols.clear();
+            for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+                int k = A.ci[p];
+                double a = A.v[p];
+                for (int64_t t = B.rp[k]; t < B.rp[k + 1]; ++t) {
+                    int j = B.ci[t];
+                    if (mark[j] != i) {
+                        mark[j] = i;
+                        acc[j] = 0.0;
+                        cols.push_back(j);
+                    }
+                    acc[j] += a * B.v[t];
+                }
+            }
+            std::sort(cols.begin(), cols.end());
+            int64_t o = C.rp[i];
+            for (size_t t = 0; t < cols.size(); ++t) {
+                C.ci[o + t] = cols[t];
+                C.v[o + t] = acc[cols[t]];
+            }
+        }
+    });
+    return C;
+}
+
+Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::vector<double>& minv) {
+    const int q = L.q;
+    const int64_t cpad = L.cpad, n = (int64_t)q * cpad;
+    // neighbour cells (self included) of every store position, sorted by position
+    std::vector<std::vector<int>> nb(cpad);
+    parallel_for(cpad, [&](int64_t b, int64_t e) {
+        for (int64_t g = b; g < e; ++g) {
+            int c = L.cell_of_gpos[g];
+            std::vector<int>& v = nb[g];
+            v.push_back((int)g);
+            if (c < 0) continue;
+            for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
+                int ed = m.cell_edges[j];
+                for (int t = 0; t < 2; ++t) {
+                    int o = m.edge_cells[2 * ed + t];
+                    if (o >= 0 && o != c) v.push_back(L.gpos_of_cell[o]);
+                }
+            }
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+        }
+    });
+    Csr A;
+    A.nrows = A.ncols = n;
+    A.rp.assign(n + 1, 0);
+    for (int r = 0; r < q; ++r)
+        for (int64_t g = 0; g < cpad; ++g)
+            A.rp[r * cpad + g + 1] = L.cell_of_gpos[g] < 0 ? 1 : (int64_t)q * nb[g].size();
+    for (int64_t i = 0; i < n; ++i) A.rp[i + 1] += A.rp[i];
+    A.ci.resize(A.rp[n]);
+    A.v.assign(A.rp[n], 0.0);
+    parallel_for(cpad, [&](int64_t b, int64_t e) {
+        std::vector<double> blk;
+        for (int64_t g = b; g < e; ++g) {
+            int c = L.cell_of_gpos[g];
+            if (c < 0) {
+                for (int r = 0; r < q; ++r) {
+                    int64_t o = A.rp[r * cpad + g];
+                    A.ci[o] = (int)(r * cpad + g);
+                    A.v[o] = 1.0;
+                }
+                continue;
+            }
+            const std::vector<int>& v = nb[g];
+            const int nbk = (int)v.size();
+            blk.assign((size_t)nbk * q * q, 0.0);  // block b: q x q, row-major
+            auto slot = [&](int gp) { return (int)(std::lower_bound(v.begin(), v.end(), gp) - v.begin()); };
+            const int n1 = S.sh.nflux(m.npc(c));
+            const int64_t v0 = S.ofs.v_off[c];
+            const double* Wc = &S.W[S.ofs.w_off[c]];
+            for (int i = 0; i < n1; ++i) {
+                if (S.isb[v0 + i]) continue;
+                const int64_t gd = S.gdof[v0 + i];
+                const double dinv = minv[L.int_of_ref[gd]];
+                // this cell
+                double* bs = &blk[(size_t)slot((int)g) * q * q];
+                for (int r = 0; r < q; ++r)
+                    for (int s = 0; s < q; ++s) bs[r * q + s] += Wc[r * n1 + i] * Wc[s * n1 + i] * dinv;
+                // the other cell of an edge dof
+                if (i >= (S.sh.k + 1) * m.npc(c)) continue;
+                const int ed = m.cell_edges[m.off[c] + i % m.npc(c)];
+                int o = m.edge_cells[2 * ed] == c ? m.edge_cells[2 * ed + 1] : m.edge_cells[2 * ed];
+                if (o < 0) continue;
+                const int n2 = S.sh.nflux(m.npc(o));
+                const int64_t w0 = S.ofs.v_off[o];
+                int i2 = -1;
+                for (int t = 0; t < n2; ++t)
+                    if (S.gdof[w0 + t] == gd) {
+                        i2 = t;
+                        break;
+                    }
+                if (i2 < 0) throw Error(VEMD_ERR_INTERNAL, "edge dof not found in the neighbouring cell");
+                const double* Wo = &S.W[S.ofs.w_off[o]];
+                double* bo = &blk[(size_t)slot(L.gpos_of_cell[o]) * q * q];
+                for (int r = 0; r < q; ++r)
+                    for (int s = 0; s < q; ++s) bo[r * q + s] += Wc[r * n1 + i] * Wo[s * n2 + i2] * dinv;
+            }
+            for (int r = 0; r < q; ++r) {
+                int64_t o = A.rp[r * cpad + g];
+                for (int s = 0; s < q; ++s)
+                    for (int t = 0; t < nbk; ++t) {
+                        A.ci[o] = (int)(s * cpad + v[t]);
+                        A.v[o] = blk[(size_t)t * q * q + r * q + s];
+                        ++o;
+                    }
+            }
+        }
+    });
+    return A;
+}
+
+// Upper estimate of the spectral radius of D^-1 A by power iteration.
+static double spectral_radius(const Csr& A, const std::vector<double>& dinv) {
+    const int64_t n = A.nrows;
+    std::vector<double> x(n), y(n);
+    uint64_t s = 0x9e3779b97f4a7c15ull;
+    for (int64_t i = 0; i < n; ++i) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        x[i] = 0.5 + (double)(s >> 11) * (1.0 / 9007199254740992.0);
+    }
+    double lam = 0.0;
+    for (int it = 0; it < 25; ++it) {
+        csr_spmv(A, x.data(), y.data());
+        double ny = 0.0, nx = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            y[i] *= dinv[i];
+            ny += y[i] * y[i];
+            nx += x[i] * x[i];
+        }
+        if (!(ny > 0.0)) break;
+        lam = std::sqrt(ny / nx);
+        double sc = 1.0 / std::sqrt(ny);
+        for (int64_t i = 0; i < n; ++i) x[i] = y[i] * sc;
+    }
+    return 1.1 * lam;
+}
+
+// Greedy aggregation of the nodes on the strength graph (three passes: whole
+// neighbourhoods, attachment to an adjacent aggregate, leftovers).  Nodes with
+// no strong neighbour stay unaggregated (-1).
+static int aggregate(const std::vector<std::vector<std::pair<int, double>>>& strong, const std::vector<int>& order,
+                     std::vector<int>& agg) {
+    const int nn = (int)strong.size();
+    agg.assign(nn, -1);
+    int na = 0;
+    for (int i : order) {
+        if (agg[i] >= 0 || strong[i].empty()) continue;
+        bool free_nbhd = true;
+        for (auto& p : strong[i])
+            if (agg[p.first] >= 0) {
+                free_nbhd = false;
+                break;
+            }
+        if (!free_nbhd) continue;
+        agg[i] = na;
+        for (auto& p : strong[i]) agg[p.first] = na;
+        ++na;
+    }
+    std::vector<int> first = agg;
+    for (int i : order) {
+        if (agg[i] >= 0 || strong[i].empty()) continue;
+        double best = -1.0;
+        int ba = -1;
+        for (auto& p : strong[i])
+            if (first[p.first] >= 0 && p.second > best) {
+                best = p.second;
+                ba = first[p.first];
+            }
+        if (ba >= 0) agg[i] = ba;
+    }
+    for (int i : order) {
+        if (agg[i] >= 0 || strong[i].empty()) continue;
+        agg[i] = na;
+        for (auto& p : strong[i])
+            if (agg[p.first] < 0) agg[p.first] = na;
+        ++na;
+    }
+    return na;
+}
+
+static void dense_inverse(std::vector<double>& a, int64_t n) {
+    // Gauss-Jordan with partial pivoting (the coarsest operator has a few
+    // hundred rows).
+    std::vector<double> inv(n * n, 0.0);
+    for (int64_t i = 0; i < n; ++i) inv[i * n + i] = 1.0;
+    for (int64_t c = 0; c < n; ++c) {
+        int64_t p = c;
+        for (int64_t r = c + 1; r < n; ++r)
+            if (std::abs(a[r * n + c]) > std::abs(a[p * n + c])) p = r;
+        if (!(std::abs(a[p * n + c]) > 0.0)) throw Error(VEMD_ERR_SINGULAR, "singular coarsest multigrid operator");
+        if (p != c)
+            for (int64_t j = 0; j < n; ++j) {
+                std::swap(a[p * n + j], a[c * n + j]);
+                std::swap(inv[p * n + j], inv[c * n + j]);
+            }
+        double d = 1.0 / a[c * n + c];
+        for (int64_t j = 0; j < n; ++j) {
+            a[c * n + j] *= d;
+            inv[c * n + j] *= d;
+        }
+        for (int64_t r = 0; r < n; ++r) {
+            if (r == c) continue;
+            double f = a[r * n + c];
+            if (f == 0.0) continue;
+            for (int64_t j = 0; j < n; ++j) {
+                a[r * n + j] -= f * a[c * n + j];
+                inv[r * n + j] -= f * inv[c * n + j];
+            }
+        }
+    }
+    a.swap(inv);
+}
+
+Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const std::vector<double>& b00,
+              const std::vector<int>& node_order0, const AmgOptions& o) {
+    Amg H;
+    H.opt = o;
+    std::vector<int> nor = node_of_row0, order = node_order0;
+    std::vector<double> b0 = b00;
+    int nn = nnodes0;
+    double eps = o.strength;
+    for (int lev = 0;; ++lev) {
+        AmgLevel L;
+        const int64_t n = A.nrows;
+        L.dinv.assign(n, 1.0);
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+                if (A.ci[p] == i && A.v[p] != 0.0) L.dinv[i] = 1.0 / A.v[p];
+        L.lmax = spectral_radius(A, L.dinv);
+        bool last = n <= o.coarse_size || lev + 1 >= o.max_levels;
+        int na = 0;
+        std::vector<int> agg;
+        if (!last) {
+            // node-level strength matrix from the near-null vector
+            std::vector<std::vector<int>> rows_of(nn);
+            for (int64_t r = 0; r < n; ++r)
+                if (nor[r] >= 0 && b0[r] != 0.0) rows_of[nor[r]].push_back((int)r);
+            std::vector<std::vector<std::pair<int, double>>> nm(nn), strong(nn);
+            parallel_for(nn, [&](int64_t b, int64_t e) {
+                std::vector<std::pair<int, double>> t;
+                for (int64_t i = b; i < e; ++i) {
+                    t.clear();
+                    for (int r : rows_of[i])
+                        for (int64_t p = A.rp[r]; p < A.rp[r + 1]; ++p) {
+                            int cj = A.ci[p];
+                            if (nor[cj] < 0 || b0[cj] == 0.0) continue;
+                            t.push_back({nor[cj], b0[r] * A.v[p] * b0[cj]});
+                        }
+                    std::sort(t.begin(), t.end(), [](auto& x, auto& y) { return x.first < y.first; });
+                    auto& out = nm[i];
+                    for (auto& pr : t)
+                        if (!out.empty() && out.back().first == pr.first)
+                            out.back().second += pr.second;
+                        else
+                            out.push_back(pr);
+                }
+            });
+            std::vector<double> diag(nn, 0.0);
+            for (int i = 0; i < nn; ++i)
+                for (auto& pr : nm[i])
+                    if (pr.first == i) diag[i] = pr.second;
+            for (int i = 0; i < nn; ++i)
+                for (auto& pr : nm[i]) {
+                    if (pr.first == i) continue;
+                    double s = std::abs(pr.second);
+                    if (s >= eps * std::sqrt(std::abs(diag[i] * diag[pr.first])) && s > 0.0)
+                        strong[i].push_back({pr.first, s / std::sqrt(std::abs(diag[i] * diag[pr.first]))});
+                }
+            na = aggregate(strong, order, agg);
+            if (na == 0 || na >= nn) last = true;
+        }
+        if (last) {
+            L.A = std::move(A);
+            H.lv.push_back(std::move(L));
+            break;
+        }
+        // tentative prolongator: the near-null vector restricted to each
+        // aggregate, normalised; the norms are the coarse near-null vector
+        std::vector<double> bc(na, 0.0);
+        for (int64_t r = 0; r < n; ++r)
+            if (nor[r] >= 0 && agg[nor[r]] >= 0) bc[agg[nor[r]]] += b0[r] * b0[r];
+        for (double& v : bc) v = std::sqrt(v);
+        Csr Pt;
+        Pt.nrows = n;
+        Pt.ncols = na;
+        Pt.rp.assign(n + 1, 0);
+        for (int64_t r = 0; r < n; ++r) {
+            bool on = nor[r] >= 0 && agg[nor[r]] >= 0 && b0[r] != 0.0;
+            Pt.rp[r + 1] = Pt.rp[r] + (on ? 1 : 0);
+            if (on) {
+                int a = agg[nor[r]];
+                Pt.ci.push_back(a);
+                Pt.v.push_back(b0[r] / bc[a]);
+            }
+        }
+        // smoothed prolongator P = (I - omega D^-1 A) Pt
+        const double omega = 4.0 / (3.0 * L.lmax);
+        Csr AP = csr_mult(A, Pt);
+        Csr P;
+        P.nrows = n;
+        P.ncols = na;
+        P.rp.assign(n + 1, 0);
+        for (int64_t r = 0; r < n; ++r) {
+            int64_t p1 = Pt.rp[r], e1 = Pt.rp[r + 1], p2 = AP.rp[r], e2 = AP.rp[r + 1];
+            const double f = omega * L.dinv[r];
+            while (p1 < e1 || p2 < e2) {
+                int c1 = p1 < e1 ? Pt.ci[p1] : INT32_MAX, c2 = p2 < e2 ? AP.ci[p2] : INT32_MAX;
+                double v = 0.0;
+                int c = std::min(c1, c2);
+                if (c1 == c) v += Pt.v[p1++];
+                if (c2 == c) v -= f * AP.v[p2++];
+                P.ci.push_back(c);
+                P.v.push_back(v);
+            }
+            P.rp[r + 1] = (int64_t)P.ci.size();
+        }
+        AP = Csr();
+        Csr R = csr_transpose(P);
+        Csr Ac = csr_mult(R, csr_mult(A, P));
+        L.A = std::move(A);
+        L.P = std::move(P);
+        L.R = std::move(R);
+        H.lv.push_back(std::move(L));
+        A = std::move(Ac);
+        // the next level: one row per aggregate, ordered as created (spatially)
+        nn = na;
+        nor.resize(na);
+        std::iota(nor.begin(), nor.end(), 0);
+        order = nor;
+        b0 = bc;
+        eps *= 0.5;
+    }
+    // dense inverse of the coarsest operator
+    const AmgLevel& C = H.lv.back();
+    H.ncoarse = C.A.nrows;
+    if (H.ncoarse > 8 * (int64_t)std::max(o.coarse_size, 1000))
+        throw Error(VEMD_ERR_INTERNAL, "multigrid coarsening stalled at " + std::to_string(H.ncoarse) + " unknowns");
+    H.coarse_inv.assign(H.ncoarse * H.ncoarse, 0.0);
+    for (int64_t i = 0; i < H.ncoarse; ++i)
+        for (int64_t p = C.A.rp[i]; p < C.A.rp[i + 1]; ++p) H.coarse_inv[i * H.ncoarse + C.A.ci[p]] = C.A.v[p];
+    dense_inverse(H.coarse_inv, H.ncoarse);
+    return H;
+}
+
+}  // namespace vemd

@@ -1,0 +1,63 @@
+// Smoothed-aggregation algebraic multigrid for the pressure block of the
+// block-diagonal MINRES preconditioner: host construction (FP64) of the
+// hierarchy for S = W diag(A)^-1 W^T, the pressure Schur complement with the
+// flux block replaced by its assembled diagonal.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "store.h"
+
+namespace vemd {
+
+// Compressed sparse rows with 32-bit column indices.
+struct Csr {
+    int64_t nrows = 0, ncols = 0;
+    std::vector<int64_t> rp;  // nrows + 1
+    std::vector<int> ci;
+    std::vector<double> v;
+    int64_t nnz() const { return (int64_t)ci.size(); }
+};
+
+struct AmgOptions {
+    int max_levels = 12;
+    int coarse_size = 600;     // the coarsest operator is inverted densely
+    double strength = 0.08;    // strength-of-connection threshold on the finest level (halved per level)
+    int degree0 = 2;           // Chebyshev steps before and after the coarse correction, finest level
+    int degree = 2;            // the same on the coarser levels
+    double cheb_ratio = 20.0;  // smoothing interval [lmax / ratio, lmax] of D^-1 A
+};
+
+struct AmgLevel {
+    Csr A;                     // operator of this level
+    std::vector<double> dinv;  // inverse of its diagonal
+    double lmax = 0.0;         // upper estimate of the spectral radius of D^-1 A
+    Csr P, R;                  // prolongation from the next coarser level and R = P^T (not on the coarsest)
+};
+
+struct Amg {
+    AmgOptions opt;
+    std::vector<AmgLevel> lv;         // lv[0] finest
+    std::vector<double> coarse_inv;   // dense inverse of the coarsest operator, row-major
+    int64_t ncoarse = 0;
+};
+
+// Pressure Schur approximation on the internal pressure block: row r * cpad +
+// gpos is pressure moment r of the cell at store position gpos; padding
+// positions get unit rows.  dflux is the assembled flux diagonal (internal
+// order, exp(-theta) included); essential boundary flux dofs are left out.
+Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::vector<double>& dflux);
+
+// Builds the hierarchy.  Rows are grouped into nodes (node_of_row, -1 = row
+// left to the smoother), b0 is the near-null vector (one value per row) and
+// node_order the order in which aggregates are grown.
+Amg amg_build(Csr A0, const std::vector<int>& node_of_row, int nnodes, const std::vector<double>& b0,
+              const std::vector<int>& node_order, const AmgOptions& o);
+
+// Helpers exposed for tests: y = A x, C = A B, transpose.
+void csr_spmv(const Csr& A, const double* x, double* y);
+Csr csr_mult(const Csr& A, const Csr& B);
+Csr csr_transpose(const Csr& A);
+
+}  // namespace vemd

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -51,6 +52,26 @@ void Solver::refresh_theta_dependent() {
     build_minv(*mesh, *setup, layout, minv);
     build_row_scale(*mesh, *setup, layout, rowscale);
     build_rhs_ref(*mesh, *setup, rhs_ref);
+    build_amg();
+}
+
+void Solver::build_amg() {
+    amg.reset();
+    if (setup->desc.bc != VEMD_BC_DIRICHLET_P) return;
+    auto t0 = std::chrono::steady_clock::now();
+    const Layout& L = layout;
+    Csr S0 = schur_pressure(*mesh, *setup, L, minv);
+    const int64_t n = S0.nrows;
+    std::vector<int> node(n, -1);
+    std::vector<double> b0(n, 0.0);
+    for (int r = 0; r < L.q; ++r)
+        for (int64_t g = 0; g < L.cpad; ++g)
+            if (L.cell_of_gpos[g] >= 0) {
+                node[r * L.cpad + g] = (int)g;
+                b0[r * L.cpad + g] = r == 0 ? 1.0 : 0.0;
+            }
+    amg.reset(new Amg(amg_build(std::move(S0), node, (int)L.cpad, b0, L.spatial_gpos, amg_opt)));
+    amg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // L2 errors of a reference-order solution (computational and physical measure)
@@ -210,7 +231,7 @@ vemd_status vemd_solver_sizes(vemd_solver sh, int64_t* z) {
     z[1] = s.setup->npres_total;
     z[2] = s.setup->ntot;
     z[3] = s.layout.nvec;
-    z[4] = s.layout.ncolours;
+    z[4] = s.layout.ngroups;
     z[5] = s.layout.max_ne;
     API_END
 }
@@ -282,7 +303,15 @@ vemd_status vemd_apply_device(vemd_solver sh, int precision, const void* x, void
     API_BEGIN
     REQUIRE(sh && x && y && x != y, "null or aliased argument");
     REQUIRE(precision == VEMD_FP64 || precision == VEMD_FP32, "precision must be VEMD_FP64 or VEMD_FP32");
-    dev_apply_int(*sh->s, precision, x, y, stream);
+    dev_apply_int(*sh->s, precision, x, y, nullptr, stream);
+    API_END
+}
+
+vemd_status vemd_apply_device_split(vemd_solver sh, int precision, const void* x, void* y, void* y1, void* stream) {
+    API_BEGIN
+    REQUIRE(sh && x && y && y1 && x != y && x != y1 && y != y1, "null or aliased argument");
+    REQUIRE(precision == VEMD_FP64 || precision == VEMD_FP32, "precision must be VEMD_FP64 or VEMD_FP32");
+    dev_apply_int(*sh->s, precision, x, y, y1, stream);
     API_END
 }
 

@@ -1,6 +1,6 @@
-// Device path of libvemd for gfx950: HBM operator store, coloured matrix-free
-// apply, fused preconditioned MINRES with on-device scalar recurrences, and
-// FP64 iterative refinement around an FP32 inner solve.
+// Device path of libvemd for gfx950: HBM operator store, one-launch matrix-free
+// apply with two-sided edge slots, fused preconditioned MINRES with on-device
+// scalar recurrences, and FP64 iterative refinement around an FP32 inner solve.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,14 +13,9 @@
 #include <type_traits>
 #include <utility>
 
+#include "precond.h"
 #include "solver.h"
 
-#define HIPCHK(x)                                                                                   \
-    do {                                                                                            \
-        hipError_t e_ = (x);                                                                        \
-        if (e_ != hipSuccess)                                                                       \
-            throw ::vemd::Error(VEMD_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));  \
-    } while (0)
 
 namespace vemd {
 
@@ -51,16 +46,6 @@ __host__ __device__ constexpr int tri_row(int t) {
     return i;
 }
 
-// Krylov scalar state kept in device memory: the host never reads it inside
-// the iteration loop except for the periodic convergence check.
-struct MState {
-    double beta1, beta, oldb, alfa, dbar, epsln, phibar, cs, sn, rho;
-    double c_az, c_r1, c_r2;
-    double u_v, u_w1, u_w2, u_x;
-    double tol;
-    int itn, maxit, done, pending;
-};
-
 template <typename T>
 struct ApplyArgs {
     const GroupDesc* groups;
@@ -73,8 +58,9 @@ struct ApplyArgs {
     const T* x;          // input when st == nullptr
     const T* xs[2];      // inputs selected by the Krylov iteration parity when st != nullptr
     T* y;
+    T* y1;               // side-1 contributions of edge dofs (cbase values)
     int64_t cbase, cpad, lam_idx, nvec;
-    int tile_base, ntiles, per_xcd;
+    int ntiles, per_xcd;
     double* pdot;        // per-tile partial of x.Kx (nullptr: not needed)
     double* plam;        // per-tile partial of the multiplier row
     const MState* st;
@@ -120,11 +106,11 @@ __device__ __forceinline__ void pin(T (&v)[M]) {
     for (int i = 0; i < M; ++i) asm volatile("" : "+v"(v[i]));
 }
 
-// One cell per lane: gather signed-folded local dofs, y_E = K_E x_E, scatter
-// edge dofs by colour (first writer stores, later colours accumulate).
+// One cell per lane: gather signed-folded local dofs, y_E = K_E x_E, store the
+// edge-dof results to the slot of the cell's side of each edge.
 template <typename T, int K, int NE>
 __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __restrict__ x, const GroupDesc& g, int lane,
-                                          double& dot, double& lamp) {
+                                          int l, double& dot, double& lamp) {
     using V = typename VT<T>::V;
     constexpr int VW = VT<T>::W;
     constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE, N = NED + NI;
@@ -134,10 +120,12 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
     const uint32_t gpos = (uint32_t)(g.gpos0 + lane);
     const Rsrc rx = make_rsrc(x, a.nvec * (int64_t)TS);
     const Rsrc ry = make_rsrc(a.y, a.nvec * (int64_t)TS);
+    const Rsrc ry1 = make_rsrc(a.y1, a.cbase * (int64_t)TS);
     const Rsrc ri = make_rsrc(a.idx + g.i_off, (int64_t)NED * npad * 4);
     const Rsrc rA = make_rsrc(a.A + g.a_off, (int64_t)NAV * npad * 16);
     const Rsrc rW = make_rsrc(a.W + g.w_off, (int64_t)NWV * npad * 16);
     const uint32_t lv = (uint32_t)lane * 16u;  // per-lane offset into the interleaved operator blocks
+    (void)l;
     uint32_t id[NED];
     T u[N], p[Q], acc[N], yp[Q];
 #pragma unroll
@@ -241,8 +229,8 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
 #pragma unroll
     for (int j = 0; j < NED; ++j) {
         const uint32_t off = (id[j] & 0x7fffffffu) * TS;
-        if (id[j] & 0x80000000u)
-            bstore1(bload1(ry, off, 0u, (T*)nullptr) + acc[j], ry, off, 0u);
+        if (id[j] & SIDE1)
+            bstore1(acc[j], ry1, off, 0u);
         else
             bstore1(acc[j], ry, off, 0u);
     }
@@ -258,27 +246,47 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// One launch per (colour, valence) group: the kernel is specialised for the
-// group's polygon size so that its registers fit that size only.
-template <typename T, int K, int NE>
-__global__ __launch_bounds__(64) void k_apply(ApplyArgs<T> a, GroupDesc g) {
+// One launch for all cells: each wave takes one tile of 64 cells of one
+// valence group and runs the body specialised for that polygon size (a
+// wave-uniform branch).  Consecutive tiles go to one XCD so that neighbouring
+// cells share its L2.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_apply(ApplyArgs<T> a) {
     if (a.st && a.st->done) return;
     const int b = blockIdx.x;
-    const int t = (b & 7) * a.per_xcd + (b >> 3);  // consecutive tiles stay on one XCD (one L2)
-    if (t >= a.ntiles) return;
-    const int tile = g.tile0 + t;
-    const int lane = t * 64 + threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile = ((b & 7) * a.per_xcd + (b >> 3)) * 4 + wave;
+    if (tile >= a.ntiles) return;
+    const GroupDesc g = a.groups[__builtin_amdgcn_readfirstlane(a.tile_group[tile])];
+    const int l = threadIdx.x & 63;
+    const int lane = (tile - g.tile0) * 64 + l;
     const T* x = a.st ? a.xs[a.st->itn & 1] : a.x;
     double dot = 0.0, lamp = 0.0;
-    cell_body<T, K, NE>(a, x, g, lane, dot, lamp);
+    switch (g.ne) {
+        case 3: cell_body<T, K, 3>(a, x, g, lane, l, dot, lamp); break;
+        case 4: cell_body<T, K, 4>(a, x, g, lane, l, dot, lamp); break;
+        case 5: cell_body<T, K, 5>(a, x, g, lane, l, dot, lamp); break;
+        case 6: cell_body<T, K, 6>(a, x, g, lane, l, dot, lamp); break;
+        case 7: cell_body<T, K, 7>(a, x, g, lane, l, dot, lamp); break;
+        case 8: cell_body<T, K, 8>(a, x, g, lane, l, dot, lamp); break;
+        case 9: cell_body<T, K, 9>(a, x, g, lane, l, dot, lamp); break;
+        case 10: cell_body<T, K, 10>(a, x, g, lane, l, dot, lamp); break;
+        default: break;
+    }
     if (a.pdot || a.lam_idx >= 0) {
         dot = wave_sum(dot);
         lamp = wave_sum(lamp);
-        if (threadIdx.x == 0) {
+        if (l == 0) {
             if (a.pdot) a.pdot[tile] = dot;
             if (a.plam) a.plam[tile] = lamp;
         }
     }
+}
+
+// y[i] += y1[i] over the edge dofs: the second side of every edge.
+template <typename T>
+__global__ __launch_bounds__(256) void k_combine(T* __restrict__ y, const T* __restrict__ y1, int64_t n) {
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] += y1[i];
 }
 
 // Sums n partials in a fixed order with one block of 256 threads.
@@ -333,11 +341,14 @@ struct UpdArgs {
     T* WB[2];
     T* x;
     const T* Az;
+    const T* Az1;        // side-1 half of the edge dofs of Az
     const T* minv;
-    int64_t n;
+    int64_t n, edof;
     double* p1;
     double* p2;
     const MState* st;
+    int64_t pb, pe;      // pressure block preconditioned by the multigrid cycle (pb == pe: none)
+    T* bp;               // copy of the new Lanczos vector's pressure block (cycle input)
 };
 
 // Lanczos step, preconditioner, and the solution update of the previous
@@ -359,17 +370,23 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
     double d1 = 0.0, d2 = 0.0;
     for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
         T r2i = r2[i];
-        T y = caz * a.Az[i] - cr1 * r1[i] - cr2 * r2i;
-        T zn = a.minv[i] * y;
+        T az = a.Az[i];
+        if (i < a.edof) az += a.Az1[i];
+        T y = caz * az - cr1 * r1[i] - cr2 * r2i;
         if (pend) {
             T w = uv * zo[i] - uw1 * wn[i] - uw2 * wp[i];
             wn[i] = w;
             a.x[i] += ux * w;
         }
         r1[i] = y;
-        zo[i] = zn;
-        d1 += (double)y * (double)zn;
-        d2 += (double)zn * (double)r2i;
+        if (i >= a.pb && i < a.pe) {
+            a.bp[i - a.pb] = y;  // preconditioned by the cycle, dots in k_pdot
+        } else {
+            T zn = a.minv[i] * y;
+            zo[i] = zn;
+            d1 += (double)y * (double)zn;
+            d2 += (double)zn * (double)r2i;
+        }
     }
     s1[threadIdx.x] = d1;
     s2[threadIdx.x] = d2;
@@ -387,11 +404,47 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
     }
 }
 
+// Pressure block of the preconditioned vector after the multigrid cycle:
+// z_p = cycle output, with the partials of y.z and z.r2 over the block (init:
+// z of the right-hand side and b.z).
+template <typename T>
+__global__ __launch_bounds__(256) void k_pdot(UpdArgs<T> a, const T* xp, double* q1, double* q2, int init) {
+    __shared__ double s1[256], s2[256];
+    const MState* st = a.st;
+    if (!init && st->done) return;
+    const int it = init ? 0 : st->itn;
+    T* __restrict__ zo = init ? a.Z[0] : a.Z[(it + 1) & 1];
+    const T* __restrict__ r2 = a.R[it & 1];
+    const int64_t np = a.pe - a.pb;
+    double d1 = 0.0, d2 = 0.0;
+    for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < np; j += (int64_t)gridDim.x * 256) {
+        const T z = xp[j];
+        zo[a.pb + j] = z;
+        d1 += (double)a.bp[j] * (double)z;
+        if (!init) d2 += (double)z * (double)r2[a.pb + j];
+    }
+    s1[threadIdx.x] = d1;
+    s2[threadIdx.x] = d2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        q1[blockIdx.x] = s1[0];
+        q2[blockIdx.x] = s2[0];
+    }
+}
+
 // Givens recurrence of MINRES (Paige and Saunders) and the stopping test.
-__global__ void k_finB(MState* st, const double* p1, const double* p2, int nb) {
+__global__ void k_finB(MState* st, const double* p1, const double* p2, int nb, const double* q1, const double* q2,
+                       int nq) {
     if (st->done) return;
-    double b2 = block_sum_fixed(p1, nb);
-    double rho = block_sum_fixed(p2, nb);
+    double b2 = block_sum_fixed(p1, nb) + block_sum_fixed(q1, nq);
+    double rho = block_sum_fixed(p2, nb) + block_sum_fixed(q2, nq);
     if (threadIdx.x == 0) {
         double beta_new = std::sqrt(b2 > 0.0 ? b2 : 0.0);
         st->rho = rho;
@@ -425,15 +478,19 @@ __global__ __launch_bounds__(256) void k_init(UpdArgs<T> a, const T* b) {
     double d = 0.0;
     for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
         T bi = b[i];
-        T z = a.minv[i] * bi;
         a.R[0][i] = bi;
         a.R[1][i] = T(0);
-        a.Z[0][i] = z;
         a.Z[1][i] = T(0);
         a.WB[0][i] = T(0);
         a.WB[1][i] = T(0);
         a.x[i] = T(0);
-        d += (double)bi * (double)z;
+        if (i >= a.pb && i < a.pe) {
+            a.bp[i - a.pb] = bi;
+        } else {
+            T z = a.minv[i] * bi;
+            a.Z[0][i] = z;
+            d += (double)bi * (double)z;
+        }
     }
     s1[threadIdx.x] = d;
     __syncthreads();
@@ -444,8 +501,8 @@ __global__ __launch_bounds__(256) void k_init(UpdArgs<T> a, const T* b) {
     if (threadIdx.x == 0) a.p1[blockIdx.x] = s1[0];
 }
 
-__global__ void k_finInit(MState* st, const double* p1, int nb, double rtol, int maxit) {
-    double b2 = block_sum_fixed(p1, nb);
+__global__ void k_finInit(MState* st, const double* p1, int nb, const double* q1, int nq, double rtol, int maxit) {
+    double b2 = block_sum_fixed(p1, nb) + block_sum_fixed(q1, nq);
     if (threadIdx.x == 0) {
         double beta1 = std::sqrt(b2 > 0.0 ? b2 : 0.0);
         MState s;
@@ -598,9 +655,11 @@ struct DevKrylov {
     T* WB[2] = {nullptr, nullptr};
     T* x = nullptr;
     T* Az = nullptr;
+    T* Az1 = nullptr;
     T* b = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
+    bool graph_amg = false;
 };
 
 struct DeviceState {
@@ -629,6 +688,13 @@ struct DeviceState {
     double* b64 = nullptr;
     double* bprob = nullptr;  // the problem's own right-hand side, internal order
     float* r32 = nullptr;
+    double* y1_64 = nullptr;  // side-1 scratch of the plain applies
+    float* y1_32 = nullptr;
+    AmgDev<double>* amg64 = nullptr;  // multigrid pressure preconditioner (when built)
+    AmgDev<float>* amg32 = nullptr;
+    double* q1 = nullptr;  // partials of the pressure-block dots
+    double* q2 = nullptr;
+    int nblkp = 0;
     std::vector<void*> allocs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -669,7 +735,7 @@ static void upload_ops(DeviceState& d, const Layout& L, DevOps<T>& o, const std:
 }
 
 template <typename T>
-static void alloc_krylov(DeviceState& d, DevKrylov<T>& k, int64_t n) {
+static void alloc_krylov(DeviceState& d, DevKrylov<T>& k, int64_t n, int64_t edof) {
     for (int i = 0; i < 2; ++i) {
         k.R[i] = (T*)dalloc(d, n * sizeof(T));
         k.Z[i] = (T*)dalloc(d, n * sizeof(T));
@@ -680,10 +746,19 @@ static void alloc_krylov(DeviceState& d, DevKrylov<T>& k, int64_t n) {
     }
     k.x = (T*)dalloc(d, n * sizeof(T));
     k.Az = (T*)dalloc(d, n * sizeof(T));
+    k.Az1 = (T*)dalloc(d, edof * sizeof(T));
     k.b = (T*)dalloc(d, n * sizeof(T));
+    // side-1 slots of boundary edges are never written and stay zero
+    HIPCHK(hipMemset(k.Az1, 0, std::max<int64_t>(edof, 1) * sizeof(T)));
     HIPCHK(hipMemset(k.x, 0, n * sizeof(T)));
     HIPCHK(hipMemset(k.Az, 0, n * sizeof(T)));
     HIPCHK(hipMemset(k.b, 0, n * sizeof(T)));
+}
+
+template <typename T>
+static void destroy_graph(DevKrylov<T>& k) {
+    if (k.graph) (void)hipGraphExecDestroy(k.graph);
+    k.graph = nullptr;
 }
 
 static void upload_theta(Solver& s) {
@@ -697,6 +772,17 @@ static void upload_theta(Solver& s) {
     std::vector<double> bi(s.layout.nvec, 0.0);
     s.to_internal(s.rhs_ref.data(), bi.data(), true);
     HIPCHK(hipMemcpy(d.bprob, bi.data(), bi.size() * sizeof(double), hipMemcpyHostToDevice));
+    // the hierarchy depends on theta through the flux diagonal
+    delete d.amg64;
+    delete d.amg32;
+    d.amg64 = nullptr;
+    d.amg32 = nullptr;
+    destroy_graph(d.k64);
+    destroy_graph(d.k32);
+    if (s.amg) {
+        d.amg64 = amg_upload<double>(*s.amg);
+        d.amg32 = amg_upload<float>(*s.amg);
+    }
 }
 
 void dev_upload(Solver& s, int device) {
@@ -730,13 +816,20 @@ void dev_upload(Solver& s, int device) {
     d.nblk = (int)std::min<int64_t>(2048, (L.nvec + 255) / 256);
     d.p1 = (double*)dalloc(d, d.nblk * sizeof(double));
     d.p2 = (double*)dalloc(d, d.nblk * sizeof(double));
+    d.nblkp = (int)std::min<int64_t>(1024, ((int64_t)L.q * L.cpad + 255) / 256);
+    d.q1 = (double*)dalloc(d, d.nblkp * sizeof(double));
+    d.q2 = (double*)dalloc(d, d.nblkp * sizeof(double));
     d.st = (MState*)dalloc(d, sizeof(MState));
     HIPCHK(hipMemset(d.st, 0, sizeof(MState)));
     HIPCHK(hipHostMalloc((void**)&d.st_host, sizeof(MState), hipHostMallocDefault));
     d.dsum = (double*)dalloc(d, sizeof(double));
     HIPCHK(hipHostMalloc((void**)&d.hsum, sizeof(double), hipHostMallocDefault));
-    alloc_krylov(d, d.k64, L.nvec);
-    alloc_krylov(d, d.k32, L.nvec);
+    alloc_krylov(d, d.k64, L.nvec, L.cbase);
+    alloc_krylov(d, d.k32, L.nvec, L.cbase);
+    d.y1_64 = (double*)dalloc(d, L.cbase * sizeof(double));
+    d.y1_32 = (float*)dalloc(d, L.cbase * sizeof(float));
+    HIPCHK(hipMemset(d.y1_64, 0, std::max<int64_t>(L.cbase, 1) * sizeof(double)));
+    HIPCHK(hipMemset(d.y1_32, 0, std::max<int64_t>(L.cbase, 1) * sizeof(float)));
     d.x64 = (double*)dalloc(d, L.nvec * sizeof(double));
     d.r64 = (double*)dalloc(d, L.nvec * sizeof(double));
     d.y64 = (double*)dalloc(d, L.nvec * sizeof(double));
@@ -756,11 +849,6 @@ void dev_refresh(Solver& s) {
     upload_theta(s);
 }
 
-template <typename T>
-static void destroy_graph(DevKrylov<T>& k) {
-    if (k.graph) (void)hipGraphExecDestroy(k.graph);
-    k.graph = nullptr;
-}
 
 void dev_free(Solver& s) {
     if (!s.dev) return;
@@ -768,6 +856,8 @@ void dev_free(Solver& s) {
     (void)hipSetDevice(d.device);
     destroy_graph(d.k64);
     destroy_graph(d.k32);
+    delete d.amg64;
+    delete d.amg32;
     for (void* p : d.allocs) (void)hipFree(p);
     if (d.st_host) (void)hipHostFree(d.st_host);
     if (d.hsum) (void)hipHostFree(d.hsum);
@@ -790,7 +880,19 @@ const DevOps<float>& ops<float>(DeviceState& d) {
 }
 
 template <typename T>
-static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, const MState* st, const T* x0, const T* x1, bool dot) {
+static T* side1_scratch(DeviceState& d);
+template <>
+double* side1_scratch<double>(DeviceState& d) {
+    return d.y1_64;
+}
+template <>
+float* side1_scratch<float>(DeviceState& d) {
+    return d.y1_32;
+}
+
+template <typename T>
+static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, T* y1, const MState* st, const T* x0, const T* x1,
+                              bool dot) {
     DeviceState& d = *s.dev;
     const DevOps<T>& o = ops<T>(d);
     const Layout& L = s.layout;
@@ -806,6 +908,7 @@ static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, const MState* st, con
     a.xs[0] = x0;
     a.xs[1] = x1;
     a.y = y;
+    a.y1 = y1;
     a.cbase = L.cbase;
     a.cpad = L.cpad;
     a.lam_idx = L.lam ? L.lam_idx : -1;
@@ -813,68 +916,37 @@ static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, const MState* st, con
     a.pdot = dot ? d.pdot : nullptr;
     a.plam = d.plam;
     a.st = st;
-    a.tile_base = 0;
-    a.ntiles = 0;
-    a.per_xcd = 0;
+    a.ntiles = d.ntiles;
+    a.per_xcd = (int)(((d.ntiles + 3) / 4 + 7) / 8);
     return a;
 }
 
+// One apply on the solver stream: a single launch over all tiles.
 template <typename T>
-static const std::vector<GroupDesc>& ops_groups(const Layout& L);
-template <>
-const std::vector<GroupDesc>& ops_groups<double>(const Layout& L) {
-    return L.groups;
-}
-template <>
-const std::vector<GroupDesc>& ops_groups<float>(const Layout& L) {
-    return L.groups32;
+static void launch_apply(Solver& s, const ApplyArgs<T>& a) {
+    dim3 grid(8 * a.per_xcd), block(256);
+    if (s.layout.k == 0)
+        hipLaunchKernelGGL((k_apply<T, 0>), grid, block, 0, s.dev->stream, a);
+    else
+        hipLaunchKernelGGL((k_apply<T, 1>), grid, block, 0, s.dev->stream, a);
 }
 
-template <typename T, int K>
-static void launch_group(ApplyArgs<T> a, const GroupDesc& g, hipStream_t st) {
-    dim3 grid(8 * a.per_xcd), block(64);
-    switch (g.ne) {
-        case 3: hipLaunchKernelGGL((k_apply<T, K, 3>), grid, block, 0, st, a, g); break;
-        case 4: hipLaunchKernelGGL((k_apply<T, K, 4>), grid, block, 0, st, a, g); break;
-        case 5: hipLaunchKernelGGL((k_apply<T, K, 5>), grid, block, 0, st, a, g); break;
-        case 6: hipLaunchKernelGGL((k_apply<T, K, 6>), grid, block, 0, st, a, g); break;
-        case 7: hipLaunchKernelGGL((k_apply<T, K, 7>), grid, block, 0, st, a, g); break;
-        case 8: hipLaunchKernelGGL((k_apply<T, K, 8>), grid, block, 0, st, a, g); break;
-        case 9: hipLaunchKernelGGL((k_apply<T, K, 9>), grid, block, 0, st, a, g); break;
-        case 10: hipLaunchKernelGGL((k_apply<T, K, 10>), grid, block, 0, st, a, g); break;
-        default: throw Error(VEMD_ERR_INTERNAL, "polygon size outside the device kernels");
-    }
-}
-
-// Launches the colour sweeps of one apply on the solver stream: one launch per
-// (colour, valence) group; groups of one colour write disjoint dofs.
+// y = K x in the internal layout: cell launch, second sides of the edges, and
+// the multiplier row.
 template <typename T>
-static void launch_apply(Solver& s, ApplyArgs<T> a) {
-    const Layout& L = s.layout;
-    hipStream_t st = s.dev->stream;
-    const std::vector<GroupDesc>& groups = ops_groups<T>(L);
-    for (const GroupDesc& g : groups) {
-        a.ntiles = g.npad / TILE;
-        a.tile_base = g.tile0;
-        a.per_xcd = (a.ntiles + 7) / 8;
-        if (L.k == 0)
-            launch_group<T, 0>(a, g, st);
-        else
-            launch_group<T, 1>(a, g, st);
-    }
-}
-
-template <typename T>
-static void apply_plain(Solver& s, const T* x, T* y) {
-    ApplyArgs<T> a = make_args<T>(s, x, y, nullptr, nullptr, nullptr, false);
+static void apply_plain(Solver& s, const T* x, T* y, T* y1_out = nullptr) {
+    DeviceState& d = *s.dev;
+    T* y1 = y1_out ? y1_out : side1_scratch<T>(d);
+    ApplyArgs<T> a = make_args<T>(s, x, y, y1, nullptr, nullptr, nullptr, false);
     launch_apply(s, a);
+    if (!y1_out)
+        hipLaunchKernelGGL((k_combine<T>), dim3(d.nblk), dim3(256), 0, d.stream, y, (const T*)y1, s.layout.cbase);
     if (s.layout.lam)
-        hipLaunchKernelGGL((k_lam_row<T>), dim3(1), dim3(256), 0, s.dev->stream, s.dev->plam, s.dev->ntiles, y,
-                           s.layout.lam_idx);
+        hipLaunchKernelGGL((k_lam_row<T>), dim3(1), dim3(256), 0, d.stream, d.plam, d.ntiles, y, s.layout.lam_idx);
     HIPCHK(hipGetLastError());
 }
 
-void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* stream) {
+void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* y1, void* stream) {
     if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
     HIPCHK(hipSetDevice(s.dev->device));
     // the caller's stream; NULL is the null (default) stream, as documented
@@ -882,9 +954,9 @@ void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* strea
     s.dev->stream = (hipStream_t)stream;
     try {
         if (precision == VEMD_FP64)
-            apply_plain<double>(s, (const double*)x, (double*)y);
+            apply_plain<double>(s, (const double*)x, (double*)y, (double*)y1);
         else
-            apply_plain<float>(s, (const float*)x, (float*)y);
+            apply_plain<float>(s, (const float*)x, (float*)y, (float*)y1);
     } catch (...) {
         s.dev->stream = saved;
         throw;
@@ -915,42 +987,82 @@ void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref)
     s.to_reference(yi.data(), y_ref, true);
 }
 
-// One MINRES iteration: colour sweeps, alfa, Lanczos/precondition/update, Givens.
 template <typename T>
-static void enqueue_iteration(Solver& s, DevKrylov<T>& k) {
+static AmgDev<T>* amg_of(DeviceState& d);
+template <>
+AmgDev<double>* amg_of<double>(DeviceState& d) {
+    return d.amg64;
+}
+template <>
+AmgDev<float>* amg_of<float>(DeviceState& d) {
+    return d.amg32;
+}
+
+// Kernel arguments of the Krylov sweeps; with the multigrid cycle the
+// pressure block [pb, pe) of z comes from the cycle instead of the diagonal.
+template <typename T>
+static UpdArgs<T> upd_args(Solver& s, DevKrylov<T>& k, bool amg) {
     DeviceState& d = *s.dev;
     const Layout& L = s.layout;
-    ApplyArgs<T> a = make_args<T>(s, nullptr, k.Az, d.st, k.Z[0], k.Z[1], true);
+    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x, k.Az, k.Az1, ops<T>(d).minv, L.nvec,
+                 L.cbase, d.p1, d.p2, d.st, 0, 0, nullptr};
+    if (amg) {
+        u.pb = L.cbase + (int64_t)L.ni * L.cpad;
+        u.pe = u.pb + (int64_t)L.q * L.cpad;
+        u.bp = amg_of<T>(d)->lv[0].b;
+    }
+    return u;
+}
+
+// One MINRES iteration: apply, alfa, Lanczos/precondition/update, [pressure
+// cycle], Givens.
+template <typename T>
+static void enqueue_iteration(Solver& s, DevKrylov<T>& k, bool amg) {
+    DeviceState& d = *s.dev;
+    const Layout& L = s.layout;
+    ApplyArgs<T> a = make_args<T>(s, nullptr, k.Az, k.Az1, d.st, k.Z[0], k.Z[1], true);
     launch_apply(s, a);
     hipLaunchKernelGGL((k_finA<T>), dim3(1), dim3(256), 0, d.stream, d.st, d.pdot, d.plam, d.ntiles, k.Az, k.Z[0],
                        k.Z[1], L.lam ? L.lam_idx : (int64_t)-1);
-    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x, k.Az, ops<T>(d).minv, L.nvec, d.p1, d.p2, d.st};
+    UpdArgs<T> u = upd_args(s, k, amg);
     hipLaunchKernelGGL((k_update<T>), dim3(d.nblk), dim3(256), 0, d.stream, u);
-    hipLaunchKernelGGL(k_finB, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.p2, d.nblk);
+    if (amg) {
+        const AmgDev<T>& M = *amg_of<T>(d);
+        amg_vcycle(M, d.stream, d.st);
+        hipLaunchKernelGGL((k_pdot<T>), dim3(d.nblkp), dim3(256), 0, d.stream, u, (const T*)M.lv[0].x, d.q1, d.q2, 0);
+    }
+    hipLaunchKernelGGL(k_finB, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.p2, d.nblk, d.q1, d.q2,
+                       amg ? d.nblkp : 0);
 }
 
 // Preconditioned MINRES on K x = b (internal order, precision T): b in k.b,
 // solution in k.x.  Stops when the M^{-1}-norm residual estimate falls to
 // rtol times its initial value.  Returns the iteration count.
 template <typename T>
-static int minres(Solver& s, DevKrylov<T>& k, double rtol, int maxit, int check_every, bool use_graph) {
+static int minres(Solver& s, DevKrylov<T>& k, double rtol, int maxit, int check_every, bool use_graph, bool amg) {
     DeviceState& d = *s.dev;
-    const Layout& L = s.layout;
-    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x, k.Az, ops<T>(d).minv, L.nvec, d.p1, d.p2, d.st};
+    UpdArgs<T> u = upd_args(s, k, amg);
     check_every = std::max(1, check_every);
-    if (use_graph && k.graph_iters != check_every) {
+    if (use_graph && (k.graph_iters != check_every || k.graph_amg != amg || !k.graph)) {
         HIPCHK(hipStreamSynchronize(d.stream));
         destroy_graph(k);
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k);
+        for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k, amg);
         HIPCHK(hipStreamEndCapture(d.stream, &g));
         HIPCHK(hipGraphInstantiate(&k.graph, g, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(g));
         k.graph_iters = check_every;
+        k.graph_amg = amg;
     }
     hipLaunchKernelGGL((k_init<T>), dim3(d.nblk), dim3(256), 0, d.stream, u, (const T*)k.b);
-    hipLaunchKernelGGL(k_finInit, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.nblk, rtol, maxit);
+    if (amg) {
+        const AmgDev<T>& M = *amg_of<T>(d);
+        amg_vcycle(M, d.stream, nullptr);
+        hipLaunchKernelGGL((k_pdot<T>), dim3(d.nblkp), dim3(256), 0, d.stream, u, (const T*)M.lv[0].x, d.q1, d.q2, 1);
+    }
+    hipLaunchKernelGGL(k_finInit, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.nblk, d.q1, amg ? d.nblkp : 0, rtol,
+                       maxit);
     HIPCHK(hipGetLastError());
     static const bool verbose = std::getenv("VEMD_VERBOSE") != nullptr;
     auto last = std::chrono::steady_clock::now();
@@ -958,7 +1070,7 @@ static int minres(Solver& s, DevKrylov<T>& k, double rtol, int maxit, int check_
         if (use_graph)
             HIPCHK(hipGraphLaunch(k.graph, d.stream));
         else
-            for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k);
+            for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k, amg);
         HIPCHK(hipMemcpyAsync(d.st_host, d.st, sizeof(MState), hipMemcpyDeviceToHost, d.stream));
         HIPCHK(hipStreamSynchronize(d.stream));
         if (d.st_host->done) break;
@@ -995,16 +1107,19 @@ static void solve_device(Solver& s, const vemd_solve_opts& o, vemd_solve_info* i
     int maxit = o.max_iters > 0 ? o.max_iters : 1000000;
     bool graph = o.use_graph != 0;
     int chk = o.check_every > 0 ? o.check_every : 32;
+    if (o.precond == VEMD_PC_MULTIGRID && !d.amg64)
+        throw Error(VEMD_ERR_ARG, "no multigrid hierarchy: it is built for Dirichlet pressure problems only");
+    const bool amg = o.precond != VEMD_PC_JACOBI && d.amg64 != nullptr;
     while (rel > o.rtol && iters < maxit && refin < 100) {
         if (o.precision == VEMD_FP64) {
             HIPCHK(hipMemcpyAsync(d.k64.b, d.r64, n * sizeof(double), hipMemcpyDeviceToDevice, d.stream));
             double target = std::min(0.5, 0.5 * o.rtol / rel);
-            iters += minres<double>(s, d.k64, target, maxit - iters, chk, graph);
+            iters += minres<double>(s, d.k64, target, maxit - iters, chk, graph, amg);
             hipLaunchKernelGGL((k_add<double>), dim3(d.nblk), dim3(256), 0, d.stream, d.x64, d.k64.x, n);
         } else {
             hipLaunchKernelGGL((k_cast<double, float>), dim3(d.nblk), dim3(256), 0, d.stream, d.r64, d.k32.b, n);
             double target = std::max(o.inner_rtol > 0 ? o.inner_rtol : 1e-4, 0.5 * o.rtol / rel);
-            iters += minres<float>(s, d.k32, target, maxit - iters, chk, graph);
+            iters += minres<float>(s, d.k32, target, maxit - iters, chk, graph, amg);
             hipLaunchKernelGGL((k_add<float>), dim3(d.nblk), dim3(256), 0, d.stream, d.x64, d.k32.x, n);
         }
         ++refin;
@@ -1042,6 +1157,7 @@ void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, doubl
     }
     auto t1 = std::chrono::steady_clock::now();
     tmp.seconds = std::chrono::duration<double>(t1 - t0).count();
+    tmp.iter_ms = tmp.iterations > 0 ? 1e3 * tmp.seconds / tmp.iterations : 0.0;
     HIPCHK(hipMemcpyAsync(bi.data(), d.x64, L.nvec * sizeof(double), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
     if (sol_ref) s.to_reference(bi.data(), sol_ref, false);
@@ -1070,6 +1186,7 @@ void dev_solve_int(Solver& s, const vemd_solve_opts& o, const void* b_dev, void*
     HIPCHK(hipStreamSynchronize(d.stream));
     auto t1 = std::chrono::steady_clock::now();
     tmp.seconds = std::chrono::duration<double>(t1 - t0).count();
+    tmp.iter_ms = tmp.iterations > 0 ? 1e3 * tmp.seconds / tmp.iterations : 0.0;
     if (info) *info = tmp;
     if (code) throw Error(code, err);
 }
@@ -1093,7 +1210,7 @@ void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, con
     // grad_E = -mu^T (dK_int/dtheta_E) x = mu_E^T (exp(-theta_E) A_E) x_E
     HIPCHK(hipMemcpyAsync(d.y64, xi.data(), L.nvec * sizeof(double), hipMemcpyHostToDevice, d.stream));
     double* g = (double*)dalloc(d, s.mesh->nc * sizeof(double));
-    ApplyArgs<double> a = make_args<double>(s, nullptr, nullptr, nullptr, nullptr, nullptr, false);
+    ApplyArgs<double> a = make_args<double>(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, false);
     if (L.k == 0)
         hipLaunchKernelGGL((k_adjoint<double, 0>), dim3(d.ntiles), dim3(64), 0, d.stream, a, d.x64, d.y64,
                            d.cell_of_gpos, g);

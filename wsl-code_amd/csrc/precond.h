@@ -1,0 +1,75 @@
+// Device side of the multigrid pressure preconditioner: the hierarchy in
+// sliced-ELLPACK form (one wavefront per slice of 64 rows) and the V-cycle
+// (Chebyshev smoothing with the Jacobi diagonal, Galerkin coarse levels, dense
+// coarsest solve).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "amg.h"
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw ::vemd::Error(VEMD_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+namespace vemd {
+
+// Krylov scalar state kept in device memory (MINRES recurrences, stop test);
+// the host reads it only at the periodic convergence check.
+struct MState {
+    double beta1, beta, oldb, alfa, dbar, epsln, phibar, cs, sn, rho;
+    double c_az, c_r1, c_r2;
+    double u_v, u_w1, u_w2, u_x;
+    double tol;
+    int itn, maxit, done, pending;
+};
+
+// Sliced ELLPACK: row s * 64 + l has width[s] entries at off[s] + j * 64 + l;
+// padding entries carry value 0 and column 0.
+template <typename T>
+struct Sell {
+    int64_t nrows = 0;
+    int nslices = 0;
+    const int64_t* off = nullptr;
+    const int* width = nullptr;
+    const int* col = nullptr;
+    const T* val = nullptr;
+};
+
+template <typename T>
+struct AmgLevelDev {
+    int64_t n = 0;
+    Sell<T> A, P, R;          // P: prolongation from the next level, R = P^T
+    T* dinv = nullptr;
+    T *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;
+    double centre = 0.0, half = 0.0;  // Chebyshev interval of D^-1 A
+    int degree = 2;
+};
+
+template <typename T>
+struct AmgDev {
+    std::vector<AmgLevelDev<T>> lv;
+    T* cinv = nullptr;        // dense inverse of the coarsest operator
+    int64_t nc = 0;
+    int64_t bytes = 0;        // device memory held
+    std::vector<void*> allocs;
+    ~AmgDev();
+};
+
+// Uploads the hierarchy in precision T (level buffers included).
+template <typename T>
+AmgDev<T>* amg_upload(const Amg& H);
+
+// Enqueues one V-cycle on the stream: x = B b with b = lv[0].b, x = lv[0].x.
+// With st non-null every kernel returns at once when the Krylov state says
+// done (work skipped inside a captured chunk after convergence).
+template <typename T>
+void amg_vcycle(const AmgDev<T>& D, hipStream_t stream, const MState* st);
+
+}  // namespace vemd

@@ -4,6 +4,7 @@
 #include <memory>
 #include <vector>
 
+#include "amg.h"
 #include "store.h"
 
 namespace vemd {
@@ -18,6 +19,12 @@ struct Solver {
     std::vector<double> minv;      // preconditioner inverse, internal order
     std::vector<double> rowscale;  // internal = rowscale * reference-form rows
     std::vector<double> rhs_ref;   // reference-form right-hand side
+    // multigrid hierarchy of the pressure Schur approximation (Dirichlet
+    // pressure problems; the mean-zero multiplier problem keeps the diagonal
+    // preconditioner because its pressure block is singular)
+    std::unique_ptr<Amg> amg;
+    AmgOptions amg_opt;
+    double amg_setup_s = 0.0;
     DeviceState* dev = nullptr;
     int device = -1;
     double last_apply_ms = 0.0;
@@ -26,6 +33,7 @@ struct Solver {
     void to_internal(const double* ref, double* in, bool scale_rows) const;
     void to_reference(const double* in, double* ref, bool unscale_rows) const;
     void refresh_theta_dependent();
+    void build_amg();
 };
 
 int device_count();
@@ -33,7 +41,8 @@ void dev_upload(Solver& s, int device);
 void dev_refresh(Solver& s);  // cell scales and preconditioner after a theta change
 void dev_free(Solver& s);
 void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref);
-void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* stream);
+// y1 == nullptr: complete apply; otherwise y1 receives the edges' second sides.
+void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* y1, void* stream);
 void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, double* sol_ref, vemd_solve_info* info);
 void dev_solve_int(Solver& s, const vemd_solve_opts& o, const void* b_dev, void* x_dev, vemd_solve_info* info);
 void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, const double* dldx_ref, double* grad,

@@ -1,4 +1,4 @@
-// Cell ordering, colouring and packing of the HBM operator store.
+// Cell ordering, valence grouping, edge sides and packing of the HBM operator store.
 #include "store.h"
 
 #include <algorithm>
@@ -34,79 +34,6 @@ static std::vector<uint64_t> hilbert_keys(const std::vector<double>& px, const s
     return k;
 }
 
-// Colours cells so that no two cells sharing an edge have the same colour:
-// DSATUR (most-constrained cell first, ties to the earlier cell in spatial
-// order), then a balancing pass that moves cells from the largest classes into
-// smaller ones where no neighbour forbids it.  Few, balanced colours mean few,
-// full launches of the apply.
-static std::vector<int> colour_cells(const Mesh& m, const std::vector<int>& order) {
-    const int nc = m.nc;
-    auto neighbours = [&](int c, auto&& f) {
-        for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
-            int e = m.cell_edges[j];
-            for (int t = 0; t < 2; ++t) {
-                int o = m.edge_cells[2 * e + t];
-                if (o >= 0 && o != c) f(o);
-            }
-        }
-    };
-    std::vector<int> rank(nc);
-    for (int i = 0; i < nc; ++i) rank[order[i]] = i;
-    std::vector<int> colour(nc, -1);
-    std::vector<uint32_t> mask(nc, 0);
-    // buckets by saturation; each bucket is a min-heap on spatial rank
-    std::vector<std::vector<std::pair<int, int>>> bucket(33);
-    auto push = [&](int c) {
-        int s = __builtin_popcount(mask[c]);
-        bucket[s].push_back({-rank[c], c});
-        std::push_heap(bucket[s].begin(), bucket[s].end());
-    };
-    for (int c = 0; c < nc; ++c) push(c);
-    int top = 0, done = 0;
-    while (done < nc) {
-        while (top > 0 && bucket[top].empty()) --top;
-        auto& b = bucket[top];
-        if (b.empty()) {
-            ++top;  // cannot happen while cells remain; keeps the loop total
-            if (top > 32) throw Error(VEMD_ERR_INTERNAL, "colouring failed");
-            continue;
-        }
-        std::pop_heap(b.begin(), b.end());
-        int c = b.back().second;
-        b.pop_back();
-        if (colour[c] >= 0 || __builtin_popcount(mask[c]) != top) continue;  // stale entry
-        int col = 0;
-        while (mask[c] & (1u << col)) ++col;
-        if (col >= 32) throw Error(VEMD_ERR_MESH, "cell graph needs more than 32 colours");
-        colour[c] = col;
-        ++done;
-        neighbours(c, [&](int o) {
-            if (colour[o] < 0 && !(mask[o] & (1u << col))) {
-                mask[o] |= 1u << col;
-                push(o);
-                top = std::max(top, __builtin_popcount(mask[o]));
-            }
-        });
-    }
-    int ncol = 1 + *std::max_element(colour.begin(), colour.end());
-    std::vector<int64_t> size(ncol, 0);
-    for (int c = 0; c < nc; ++c) size[colour[c]]++;
-    for (int pass = 0; pass < 4; ++pass)
-        for (int c : order) {
-            uint32_t used = 0;
-            neighbours(c, [&](int o) { used |= 1u << colour[o]; });
-            int best = colour[c];
-            for (int k = 0; k < ncol; ++k)
-                if (!(used & (1u << k)) && size[k] + 1 < size[best]) best = k;
-            if (best != colour[c]) {
-                size[colour[c]]--;
-                size[best]++;
-                colour[c] = best;
-            }
-        }
-    return colour;
-}
-
 Layout layout_build(const Mesh& m, const Setup& S) {
     Layout L;
     const int nc = m.nc;
@@ -127,29 +54,23 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     std::vector<int> order(nc);
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int a, int b) { return hk[a] < hk[b] || (hk[a] == hk[b] && a < b); });
-    std::vector<int> colour = colour_cells(m, order);
-    int ncol = 1 + *std::max_element(colour.begin(), colour.end());
-    L.ncolours = ncol;
-    // store order: colour, valence, spatial
+    // store order: valence (largest first, so the longest tiles start first), then spatial
     std::vector<int> rank(nc);
     for (int i = 0; i < nc; ++i) rank[order[i]] = i;
     std::vector<int> so(nc);
     std::iota(so.begin(), so.end(), 0);
     std::sort(so.begin(), so.end(), [&](int a, int b) {
-        if (colour[a] != colour[b]) return colour[a] < colour[b];
-        if (m.npc(a) != m.npc(b)) return m.npc(a) < m.npc(b);
+        if (m.npc(a) != m.npc(b)) return m.npc(a) > m.npc(b);
         return rank[a] < rank[b];
     });
     // groups with padding to whole tiles
     int gpos = 0, tile = 0;
-    L.colour_tile0.assign(ncol + 1, 0);
     L.cell_of_gpos.clear();
     L.gpos_of_cell.assign(nc, -1);
     for (size_t i = 0; i < so.size();) {
         size_t j = i;
-        while (j < so.size() && colour[so[j]] == colour[so[i]] && m.npc(so[j]) == m.npc(so[i])) ++j;
+        while (j < so.size() && m.npc(so[j]) == m.npc(so[i])) ++j;
         GroupDesc g{};
-        g.colour = colour[so[i]];
         g.ne = m.npc(so[i]);
         g.n = S.sh.nflux(g.ne);
         int cnt = (int)(j - i);
@@ -167,10 +88,11 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         L.groups.push_back(g);
         i = j;
     }
-    for (auto& g : L.groups) L.colour_tile0[g.colour + 1] = g.tile0 + g.npad / TILE;
-    for (int c = 1; c <= ncol; ++c) L.colour_tile0[c] = std::max(L.colour_tile0[c], L.colour_tile0[c - 1]);
+    L.ngroups = (int)L.groups.size();
+    L.spatial_gpos.resize(nc);
+    for (int i = 0; i < nc; ++i) L.spatial_gpos[i] = L.gpos_of_cell[order[i]];
     L.cpad = gpos;
-    // edges numbered by the store position of the first cell that owns them
+    // edges numbered by the spatial rank of the first cell that owns them
     std::vector<int> efirst(m.ne, INT32_MAX);
     for (int c = 0; c < nc; ++c)
         for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
@@ -199,10 +121,12 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         for (int r = 0; r < L.q; ++r) L.int_of_ref[S.nflux_total + (int64_t)c * L.q + r] = L.cbase + (L.ni + r) * L.cpad + gp;
     }
     if (L.lam) L.int_of_ref[S.ntot - 1] = L.lam_idx;
-    // packed edge-dof indices with the accumulate flag for the second writer
-    std::vector<int> first_colour(m.ne, INT32_MAX);
+    // packed edge-dof indices; an interior edge's first side is the adjacent
+    // cell with the lower store position
+    std::vector<int> side0(m.ne, INT32_MAX);
     for (int c = 0; c < nc; ++c)
-        for (int j = m.off[c]; j < m.off[c + 1]; ++j) first_colour[m.cell_edges[j]] = std::min(first_colour[m.cell_edges[j]], colour[c]);
+        for (int j = m.off[c]; j < m.off[c + 1]; ++j)
+            side0[m.cell_edges[j]] = std::min(side0[m.cell_edges[j]], L.gpos_of_cell[c]);
     int64_t ioff = 0, aoff2 = 0, woff2 = 0, aoff4 = 0, woff4 = 0, loff = 0;
     L.a_off4.resize(L.groups.size());
     L.w_off4.resize(L.groups.size());
@@ -233,7 +157,7 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     L.w_len4 = woff4;
     L.l_len = loff;
     L.idx.assign(ioff, (uint32_t)L.trash_idx);
-    if (L.nvec >= (int64_t)ACC) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit dof indices");
+    if (L.nvec >= (int64_t)SIDE1) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit dof indices");
     for (auto& g : L.groups) {
         int ned = kk * g.ne;
         for (int a = 0; a < g.npad; ++a) {
@@ -243,7 +167,7 @@ Layout layout_build(const Mesh& m, const Setup& S) {
                 for (int j = 0; j < g.ne; ++j) {
                     int e = m.cell_edges[m.off[c] + j];
                     uint32_t id = (uint32_t)(kk * L.edge_int[e] + r);
-                    if (colour[c] != first_colour[e]) id |= ACC;
+                    if (L.gpos_of_cell[c] != side0[e]) id |= SIDE1;
                     L.idx[g.i_off + (int64_t)(r * g.ne + j) * g.npad + a] = id;
                 }
             (void)ned;

@@ -1,5 +1,5 @@
-// Host-side description of the HBM operator store: cell order, colouring,
-// valence groups, internal dof numbering and the packed per-precision arrays.
+// Host-side description of the HBM operator store: cell order, valence groups,
+// internal dof numbering, edge sides and the packed per-precision arrays.
 #pragma once
 
 #include <cstdint>
@@ -11,36 +11,39 @@ namespace vemd {
 
 constexpr int TILE = 64;       // cells per tile: one wavefront, one lane per cell
 constexpr int MAX_NE = 10;     // largest polygon the device kernels are built for
-constexpr uint32_t ACC = 0x80000000u;  // index flag: accumulate (a lower colour wrote first)
+// Index flag of an edge dof seen from the second of its two cells: that cell's
+// contribution goes to the side-1 array, the first cell's to the output vector,
+// and the consumer adds the two (one writer per slot, no atomics, fixed order).
+constexpr uint32_t SIDE1 = 0x80000000u;
 
 // POD shared with the device code.
 struct GroupDesc {
-    int colour, ne, n, npad;  // valence, local flux dofs, cells incl. padding
+    int ne, n, npad;          // valence, local flux dofs, cells incl. padding
     int gpos0, tile0;         // first store position, first global tile
     int64_t a_off, w_off, l_off, i_off;  // element offsets of A, W, L and index blocks
 };
 
 struct Layout {
     int k = 1, q = 0, ni = 0;
-    int ncolours = 0, max_ne = 0;
+    int ngroups = 0, max_ne = 0;
     bool lam = false;
     int64_t nedge = 0, ncell = 0, cpad = 0;  // edges, real cells, padded cell slots
-    int64_t cbase = 0;        // first cell-owned slot
+    int64_t cbase = 0;        // first cell-owned slot (= edge dofs = length of the side-1 array)
     int64_t lam_idx = -1, trash_idx = 0, nvec = 0;
     std::vector<GroupDesc> groups;      // offsets of the FP64 store (vector width 2)
     std::vector<GroupDesc> groups32;    // the same groups with FP32 store offsets (width 4)
     std::vector<int> tile_group;        // group of each global tile
-    std::vector<int> colour_tile0;      // first tile of each colour (+ end)
     std::vector<int> cell_of_gpos;      // mesh cell at each store position (-1 padding)
     std::vector<int> gpos_of_cell;
+    std::vector<int> spatial_gpos;      // store positions of the cells in spatial (Hilbert) order
     std::vector<int> edge_int;          // internal number of each mesh edge
     std::vector<int64_t> int_of_ref;    // reference dof -> internal slot
-    std::vector<uint32_t> idx;          // packed edge-dof indices (SoA per group) with ACC flag
+    std::vector<uint32_t> idx;          // packed edge-dof indices (SoA per group) with the SIDE1 flag
     int64_t a_len2 = 0, w_len2 = 0, a_len4 = 0, w_len4 = 0, l_len = 0;  // packed sizes (elements)
     std::vector<int64_t> a_off4, w_off4;  // FP32 offsets per group (vector width 4)
 };
 
-// Builds ordering, colouring and numbering from the mesh and setup.
+// Builds ordering, grouping, numbering and edge sides from the mesh and setup.
 Layout layout_build(const Mesh& m, const Setup& S);
 
 // Packs A (packed symmetric, boundary dofs decoupled), W and L into the

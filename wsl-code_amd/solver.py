@@ -21,10 +21,11 @@ class SolveOptions:
     inner_rtol: float = 1e-4
     check_every: int = 32
     use_graph: int = 1
+    precond: int = 0  # PC_AUTO: multigrid where built (Dirichlet pressure), else Jacobi
 
     def c(self):
         return SolveOptsC(self.precision, self.rtol, self.max_iters, self.inner_rtol, self.check_every,
-                          self.use_graph)
+                          self.use_graph, self.precond)
 
 
 class Solver:
@@ -40,7 +41,7 @@ class Solver:
         check(lib().vemd_solver_create(mesh.h, ctypes.byref(d), ctypes.byref(self.h)))
         s = np.zeros(6, np.int64)
         check(lib().vemd_solver_sizes(self.h, s.ctypes.data_as(c_lp)))
-        self.nflux, self.npres, self.ntot, self.nvec, self.ncolours, self.max_ne = (int(v) for v in s)
+        self.nflux, self.npres, self.ntot, self.nvec, self.ngroups, self.max_ne = (int(v) for v in s)
         self.k, self.bc = k, bc
         self.uploaded = False
         if device is not None:
