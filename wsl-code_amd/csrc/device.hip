@@ -50,8 +50,7 @@ template <typename T>
 struct ApplyArgs {
     const GroupDesc* groups;
     const int* tile_group;
-    const T* A;
-    const T* W;
+    const T* O;          // operator stream (packed A then W per cell, tile-major)
     const T* L;
     const T* scale;
     const uint32_t* idx;
@@ -106,118 +105,150 @@ __device__ __forceinline__ void pin(T (&v)[M]) {
     for (int i = 0; i < M; ++i) asm volatile("" : "+v"(v[i]));
 }
 
-// One cell per lane: gather signed-folded local dofs, y_E = K_E x_E, store the
-// edge-dof results to the slot of the cell's side of each edge.
+// Shape constants of the cell operator of one polygon size.
 template <typename T, int K, int NE>
-__device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __restrict__ x, const GroupDesc& g, int lane,
+struct CellShape {
+    static constexpr int VW = VT<T>::W;
+    static constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE,
+                         N = NED + NI;
+    static constexpr int NA = N * (N + 1) / 2, NAV = (NA + VW - 1) / VW, NW = Q * N, NWV = (NW + VW - 1) / VW;
+    static constexpr int NV = NAV + NWV;  // 16-byte vectors per cell: packed A, then W
+};
+
+#ifndef VEMD_CHUNK
+#define VEMD_CHUNK 8
+#endif
+#ifndef VEMD_PIPE
+#define VEMD_PIPE 1
+#endif
+
+// One cell per lane: gather signed-folded local dofs, y_E = K_E x_E, store the
+// edge-dof results to the slot of the cell's side of each edge.  The cell's
+// operator is one stream of 16-byte vectors (tile-major: vector v of lane l of
+// tile tau at ((tau * NV + v) * 64 + l) * 16 bytes), read in chunks of CH
+// vectors; with PIPE the next chunk is in flight while the current one is
+// consumed.  y_u = s (A u - W^T (p / s)) so that A and W entries can be
+// consumed in stream order.
+template <typename T, int K, int NE>
+__device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __restrict__ x, const GroupDesc& g, int tau,
                                           int l, double& dot, double& lamp) {
+    using SH = CellShape<T, K, NE>;
     using V = typename VT<T>::V;
-    constexpr int VW = VT<T>::W;
-    constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE, N = NED + NI;
-    constexpr int NA = N * (N + 1) / 2, NAV = (NA + VW - 1) / VW, NW = Q * N, NWV = (NW + VW - 1) / VW;
+    constexpr int VW = SH::VW, Q = SH::Q, NI = SH::NI, NED = SH::NED, N = SH::N, NA = SH::NA, NAV = SH::NAV,
+                  NW = SH::NW, NV = SH::NV;
+    constexpr int CH = VEMD_CHUNK, NC = (NV + CH - 1) / CH;
     constexpr uint32_t TS = sizeof(T);
-    const uint32_t npad = (uint32_t)g.npad;
+    const int lane = tau * 64 + l;
     const uint32_t gpos = (uint32_t)(g.gpos0 + lane);
     const Rsrc rx = make_rsrc(x, a.nvec * (int64_t)TS);
     const Rsrc ry = make_rsrc(a.y, a.nvec * (int64_t)TS);
     const Rsrc ry1 = make_rsrc(a.y1, a.cbase * (int64_t)TS);
-    const Rsrc ri = make_rsrc(a.idx + g.i_off, (int64_t)NED * npad * 4);
-    const Rsrc rA = make_rsrc(a.A + g.a_off, (int64_t)NAV * npad * 16);
-    const Rsrc rW = make_rsrc(a.W + g.w_off, (int64_t)NWV * npad * 16);
-    const uint32_t lv = (uint32_t)lane * 16u;  // per-lane offset into the interleaved operator blocks
-    (void)l;
+    const Rsrc ri = make_rsrc(a.idx + g.i_off + (int64_t)tau * NED * 64, (int64_t)NED * 64 * 4);
+    const Rsrc rO = make_rsrc(a.O + g.o_off + (int64_t)tau * NV * 64 * VW, (int64_t)NV * 64 * 16);
+    const uint32_t lv = (uint32_t)l * 16u;
     uint32_t id[NED];
-    T u[N], p[Q], acc[N], yp[Q];
+    T u[N], ps[Q], acc[N], yp[Q];
 #pragma unroll
-    for (int j = 0; j < NED; ++j) id[j] = bload_u32(ri, (uint32_t)lane * 4u, (uint32_t)j * npad * 4u);
+    for (int j = 0; j < NED; ++j) id[j] = bload_u32(ri, (uint32_t)l * 4u, (uint32_t)j * 256u);
 #pragma unroll
     for (int j = 0; j < NED; ++j) u[j] = bload1(rx, (id[j] & 0x7fffffffu) * TS, 0u, (T*)nullptr);
 #pragma unroll
     for (int j = 0; j < NI; ++j) u[NED + j] = bload1(rx, gpos * TS, (uint32_t)((a.cbase + j * a.cpad) * TS), (T*)nullptr);
+    T p[Q];
 #pragma unroll
     for (int r = 0; r < Q; ++r) p[r] = bload1(rx, gpos * TS, (uint32_t)((a.cbase + (NI + r) * a.cpad) * TS), (T*)nullptr);
+    const T s = a.scale[gpos];
+    const T rs = s > T(0) ? T(1) / s : T(0);
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+        ps[r] = p[r] * rs;
+        yp[r] = T(0);
+    }
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] = T(0);
-    // The packed matrix streams through in chunks of CH vector loads; the
-    // scheduling barrier keeps the compiler from hoisting every load of the
-    // cell, which would need more registers than a wave has.
-    constexpr int CH = 8;
-    static_for(
-        [&](auto cc) {
-            constexpr int c0 = decltype(cc)::value * CH;
-            V av[CH];
-            static_for(
-                [&](auto vv) {
-                    constexpr int v = c0 + decltype(vv)::value;
-                    if constexpr (v < NAV) av[decltype(vv)::value] = bload_vec<V>(rA, lv, (uint32_t)v * npad * 16u);
-                },
-                std::make_integer_sequence<int, CH>{});
-            static_for(
-                [&](auto vv) {
-                    constexpr int v = c0 + decltype(vv)::value;
-                    static_for(
-                        [&](auto ll) {
-                            constexpr int l = decltype(ll)::value;
-                            constexpr int t = v * VW + l;
-                            if constexpr (v < NAV && t < NA) {
+    auto load_chunk = [&](auto cc, V(&buf)[CH]) {
+        constexpr int c0 = decltype(cc)::value * CH;
+        static_for(
+            [&](auto vv) {
+                constexpr int v = c0 + decltype(vv)::value;
+                if constexpr (v < NV) buf[decltype(vv)::value] = bload_vec<V>(rO, lv, (uint32_t)v * 1024u);
+            },
+            std::make_integer_sequence<int, CH>{});
+    };
+    auto use_chunk = [&](auto cc, const V(&buf)[CH]) {
+        constexpr int c0 = decltype(cc)::value * CH;
+        static_for(
+            [&](auto vv) {
+                constexpr int v = c0 + decltype(vv)::value;
+                static_for(
+                    [&](auto ll) {
+                        constexpr int li = decltype(ll)::value;
+                        if constexpr (v < NAV) {
+                            constexpr int t = v * VW + li;
+                            if constexpr (t < NA) {
                                 constexpr int i = tri_row(t);
                                 constexpr int j = t - i * (i + 1) / 2;
-                                const T aij = av[decltype(vv)::value][l];
+                                const T aij = buf[decltype(vv)::value][li];
                                 acc[i] += aij * u[j];
                                 if constexpr (i != j) acc[j] += aij * u[i];
                             }
-                        },
-                        std::make_integer_sequence<int, VW>{});
-                },
-                std::make_integer_sequence<int, CH>{});
-            pin(acc);
-            __builtin_amdgcn_sched_barrier(0);
-        },
-        std::make_integer_sequence<int, (NAV + CH - 1) / CH>{});
-    const T s = a.scale[gpos];
-#pragma unroll
-    for (int i = 0; i < N; ++i) acc[i] *= s;
-#pragma unroll
-    for (int r = 0; r < Q; ++r) yp[r] = T(0);
-    static_for(
-        [&](auto cc) {
-            constexpr int c0 = decltype(cc)::value * CH;
-            V wv[CH];
-            static_for(
-                [&](auto vv) {
-                    constexpr int v = c0 + decltype(vv)::value;
-                    if constexpr (v < NWV) wv[decltype(vv)::value] = bload_vec<V>(rW, lv, (uint32_t)v * npad * 16u);
-                },
-                std::make_integer_sequence<int, CH>{});
-            static_for(
-                [&](auto vv) {
-                    constexpr int v = c0 + decltype(vv)::value;
-                    static_for(
-                        [&](auto ll) {
-                            constexpr int l = decltype(ll)::value;
-                            constexpr int t = v * VW + l;
-                            if constexpr (v < NWV && t < NW) {
+                        } else if constexpr (v < NV) {
+                            constexpr int t = (v - NAV) * VW + li;
+                            if constexpr (t < NW) {
                                 constexpr int r = t / N, i = t % N;
-                                const T w = wv[decltype(vv)::value][l];
-                                acc[i] -= w * p[r];
+                                const T w = buf[decltype(vv)::value][li];
+                                acc[i] -= w * ps[r];
                                 yp[r] -= w * u[i];
                             }
-                        },
-                        std::make_integer_sequence<int, VW>{});
-                },
-                std::make_integer_sequence<int, CH>{});
+                        }
+                    },
+                    std::make_integer_sequence<int, VW>{});
+            },
+            std::make_integer_sequence<int, CH>{});
+    };
+    V b0[CH], b1[CH];
+#if VEMD_PIPE
+    load_chunk(std::integral_constant<int, 0>{}, b0);
+    static_for(
+        [&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            if constexpr (c + 1 < NC) {
+                if constexpr ((c & 1) == 0)
+                    load_chunk(std::integral_constant<int, c + 1>{}, b1);
+                else
+                    load_chunk(std::integral_constant<int, c + 1>{}, b0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr ((c & 1) == 0)
+                use_chunk(cc, b0);
+            else
+                use_chunk(cc, b1);
             pin(acc);
             pin(yp);
             __builtin_amdgcn_sched_barrier(0);
         },
-        std::make_integer_sequence<int, (NWV + CH - 1) / CH>{});
+        std::make_integer_sequence<int, NC>{});
+#else
+    (void)b1;
+    static_for(
+        [&](auto cc) {
+            load_chunk(cc, b0);
+            use_chunk(cc, b0);
+            pin(acc);
+            pin(yp);
+            __builtin_amdgcn_sched_barrier(0);
+        },
+        std::make_integer_sequence<int, NC>{});
+#endif
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] *= s;
     if (a.lam_idx >= 0) {
         const T xl = x[a.lam_idx];
 #pragma unroll
         for (int r = 0; r < Q; ++r) {
-            T l = a.L[g.l_off + r * (int64_t)npad + lane];
-            yp[r] -= l * xl;
-            lamp -= (double)l * (double)p[r];
+            T lr = a.L[g.l_off + r * (int64_t)g.npad + lane];
+            yp[r] -= lr * xl;
+            lamp -= (double)lr * (double)p[r];
         }
     }
     if (a.pdot) {
@@ -259,18 +290,18 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs<T> a) {
     if (tile >= a.ntiles) return;
     const GroupDesc g = a.groups[__builtin_amdgcn_readfirstlane(a.tile_group[tile])];
     const int l = threadIdx.x & 63;
-    const int lane = (tile - g.tile0) * 64 + l;
+    const int tau = tile - g.tile0;
     const T* x = a.st ? a.xs[a.st->itn & 1] : a.x;
     double dot = 0.0, lamp = 0.0;
     switch (g.ne) {
-        case 3: cell_body<T, K, 3>(a, x, g, lane, l, dot, lamp); break;
-        case 4: cell_body<T, K, 4>(a, x, g, lane, l, dot, lamp); break;
-        case 5: cell_body<T, K, 5>(a, x, g, lane, l, dot, lamp); break;
-        case 6: cell_body<T, K, 6>(a, x, g, lane, l, dot, lamp); break;
-        case 7: cell_body<T, K, 7>(a, x, g, lane, l, dot, lamp); break;
-        case 8: cell_body<T, K, 8>(a, x, g, lane, l, dot, lamp); break;
-        case 9: cell_body<T, K, 9>(a, x, g, lane, l, dot, lamp); break;
-        case 10: cell_body<T, K, 10>(a, x, g, lane, l, dot, lamp); break;
+        case 3: cell_body<T, K, 3>(a, x, g, tau, l, dot, lamp); break;
+        case 4: cell_body<T, K, 4>(a, x, g, tau, l, dot, lamp); break;
+        case 5: cell_body<T, K, 5>(a, x, g, tau, l, dot, lamp); break;
+        case 6: cell_body<T, K, 6>(a, x, g, tau, l, dot, lamp); break;
+        case 7: cell_body<T, K, 7>(a, x, g, tau, l, dot, lamp); break;
+        case 8: cell_body<T, K, 8>(a, x, g, tau, l, dot, lamp); break;
+        case 9: cell_body<T, K, 9>(a, x, g, tau, l, dot, lamp); break;
+        case 10: cell_body<T, K, 10>(a, x, g, tau, l, dot, lamp); break;
         default: break;
     }
     if (a.pdot || a.lam_idx >= 0) {
@@ -570,30 +601,29 @@ __global__ __launch_bounds__(256) void k_add(double* x, const S* d, int64_t n) {
 // Adjoint contraction per cell: g_E = lambda_E^T (exp(-theta_E) A_E) x_E.
 template <typename T, int K, int NE>
 __device__ __forceinline__ double cell_quad(const ApplyArgs<T>& a, const T* lam, const T* x, const GroupDesc& g,
-                                            int lane) {
+                                            int tau, int l) {
+    using SH = CellShape<T, K, NE>;
     using V = typename VT<T>::V;
-    constexpr int VW = VT<T>::W;
-    constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE, N = NED + NI;
-    constexpr int NA = N * (N + 1) / 2, NAV = (NA + VW - 1) / VW;
-    const int64_t npad = g.npad, gpos = g.gpos0 + lane;
-    T u[N], l[N];
+    constexpr int VW = SH::VW, NI = SH::NI, NED = SH::NED, N = SH::N, NA = SH::NA, NAV = SH::NAV, NV = SH::NV;
+    const int64_t gpos = g.gpos0 + tau * 64 + l;
+    T u[N], m[N];
 #pragma unroll
     for (int j = 0; j < NED; ++j) {
-        uint32_t k = a.idx[g.i_off + j * npad + lane] & 0x7fffffffu;
+        uint32_t k = a.idx[g.i_off + ((int64_t)tau * NED + j) * 64 + l] & 0x7fffffffu;
         u[j] = x[k];
-        l[j] = lam[k];
+        m[j] = lam[k];
     }
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
         u[NED + j] = x[a.cbase + j * a.cpad + gpos];
-        l[NED + j] = lam[a.cbase + j * a.cpad + gpos];
+        m[NED + j] = lam[a.cbase + j * a.cpad + gpos];
     }
     double s = 0.0;
-    const V* Av = reinterpret_cast<const V*>(a.A + g.a_off) + lane;
+    const V* Av = reinterpret_cast<const V*>(a.O + g.o_off) + (int64_t)tau * NV * 64 + l;
     static_for(
         [&](auto vv) {
             constexpr int v = decltype(vv)::value;
-            V av = Av[v * npad];
+            V av = Av[v * 64];
             static_for(
                 [&](auto ll) {
                     constexpr int li = decltype(ll)::value;
@@ -602,8 +632,8 @@ __device__ __forceinline__ double cell_quad(const ApplyArgs<T>& a, const T* lam,
                         constexpr int i = tri_row(t);
                         constexpr int j = t - i * (i + 1) / 2;
                         double av_ = (double)av[li];
-                        s += av_ * (double)l[i] * (double)u[j];
-                        if constexpr (i != j) s += av_ * (double)l[j] * (double)u[i];
+                        s += av_ * (double)m[i] * (double)u[j];
+                        if constexpr (i != j) s += av_ * (double)m[j] * (double)u[i];
                     }
                 },
                 std::make_integer_sequence<int, VW>{});
@@ -617,20 +647,20 @@ __global__ __launch_bounds__(64) void k_adjoint(ApplyArgs<T> a, const T* lam, co
                                               double* grad) {
     const int tile = blockIdx.x;
     const GroupDesc g = a.groups[a.tile_group[tile]];
-    const int lane = (tile - g.tile0) * 64 + threadIdx.x;
+    const int tau = tile - g.tile0, l = threadIdx.x;
     double v = 0.0;
     switch (g.ne) {
-        case 3: v = cell_quad<T, K, 3>(a, lam, x, g, lane); break;
-        case 4: v = cell_quad<T, K, 4>(a, lam, x, g, lane); break;
-        case 5: v = cell_quad<T, K, 5>(a, lam, x, g, lane); break;
-        case 6: v = cell_quad<T, K, 6>(a, lam, x, g, lane); break;
-        case 7: v = cell_quad<T, K, 7>(a, lam, x, g, lane); break;
-        case 8: v = cell_quad<T, K, 8>(a, lam, x, g, lane); break;
-        case 9: v = cell_quad<T, K, 9>(a, lam, x, g, lane); break;
-        case 10: v = cell_quad<T, K, 10>(a, lam, x, g, lane); break;
+        case 3: v = cell_quad<T, K, 3>(a, lam, x, g, tau, l); break;
+        case 4: v = cell_quad<T, K, 4>(a, lam, x, g, tau, l); break;
+        case 5: v = cell_quad<T, K, 5>(a, lam, x, g, tau, l); break;
+        case 6: v = cell_quad<T, K, 6>(a, lam, x, g, tau, l); break;
+        case 7: v = cell_quad<T, K, 7>(a, lam, x, g, tau, l); break;
+        case 8: v = cell_quad<T, K, 8>(a, lam, x, g, tau, l); break;
+        case 9: v = cell_quad<T, K, 9>(a, lam, x, g, tau, l); break;
+        case 10: v = cell_quad<T, K, 10>(a, lam, x, g, tau, l); break;
         default: break;
     }
-    int c = cell_of_gpos[g.gpos0 + lane];
+    int c = cell_of_gpos[g.gpos0 + tau * 64 + l];
     if (c >= 0) grad[c] = v;
 }
 
@@ -640,8 +670,7 @@ __global__ __launch_bounds__(64) void k_adjoint(ApplyArgs<T> a, const T* lam, co
 
 template <typename T>
 struct DevOps {
-    T* A = nullptr;
-    T* W = nullptr;
+    T* O = nullptr;
     T* L = nullptr;
     T* scale = nullptr;
     T* minv = nullptr;
@@ -720,18 +749,11 @@ int device_count() {
 }
 
 template <typename T>
-static void upload_ops(DeviceState& d, const Layout& L, DevOps<T>& o, const std::vector<T>& A, const std::vector<T>& W,
-                       const std::vector<T>& Lv, bool wide) {
-    o.A = dupload(d, A);
-    o.W = dupload(d, W);
+static void upload_ops(DeviceState& d, const Layout& L, DevOps<T>& o, const std::vector<T>& O, const std::vector<T>& Lv,
+                       bool wide) {
+    o.O = dupload(d, O);
     o.L = dupload(d, Lv);
-    std::vector<GroupDesc> g = L.groups;
-    if (wide)
-        for (size_t i = 0; i < g.size(); ++i) {
-            g[i].a_off = L.a_off4[i];
-            g[i].w_off = L.w_off4[i];
-        }
-    o.groups = dupload(d, g);
+    o.groups = dupload(d, wide ? L.groups32 : L.groups);
 }
 
 template <typename T>
@@ -797,11 +819,11 @@ void dev_upload(Solver& s, int device) {
     const Layout& L = s.layout;
     HIPCHK(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
     {
-        std::vector<double> A64, W64, L64;
-        std::vector<float> A32, W32, L32;
-        pack_store(*s.mesh, *s.setup, L, A64, W64, L64, A32, W32, L32);
-        upload_ops(d, L, d.o64, A64, W64, L64, false);
-        upload_ops(d, L, d.o32, A32, W32, L32, true);
+        std::vector<double> O64, L64;
+        std::vector<float> O32, L32;
+        pack_store(*s.mesh, *s.setup, L, O64, L64, O32, L32);
+        upload_ops(d, L, d.o64, O64, L64, false);
+        upload_ops(d, L, d.o32, O32, L32, true);
     }
     d.idx = dupload(d, L.idx);
     d.tile_group = dupload(d, L.tile_group);
@@ -899,8 +921,7 @@ static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, T* y1, const MState* 
     ApplyArgs<T> a;
     a.groups = o.groups;
     a.tile_group = d.tile_group;
-    a.A = o.A;
-    a.W = o.W;
+    a.O = o.O;
     a.L = o.L;
     a.scale = o.scale;
     a.idx = d.idx;

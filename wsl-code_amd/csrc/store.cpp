@@ -127,34 +127,28 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     for (int c = 0; c < nc; ++c)
         for (int j = m.off[c]; j < m.off[c + 1]; ++j)
             side0[m.cell_edges[j]] = std::min(side0[m.cell_edges[j]], L.gpos_of_cell[c]);
-    int64_t ioff = 0, aoff2 = 0, woff2 = 0, aoff4 = 0, woff4 = 0, loff = 0;
-    L.a_off4.resize(L.groups.size());
-    L.w_off4.resize(L.groups.size());
-    for (size_t gi = 0; gi < L.groups.size(); ++gi) {
-        GroupDesc& g = L.groups[gi];
-        int ned = kk * g.ne, na = g.n * (g.n + 1) / 2, nw = L.q * g.n;
-        g.i_off = ioff;
-        g.a_off = aoff2;
-        g.w_off = woff2;
-        g.l_off = loff;
-        L.a_off4[gi] = aoff4;
-        L.w_off4[gi] = woff4;
-        ioff += (int64_t)ned * g.npad;
-        aoff2 += (int64_t)((na + 1) / 2 * 2) * g.npad;
-        woff2 += (int64_t)((nw + 1) / 2 * 2) * g.npad;
-        aoff4 += (int64_t)((na + 3) / 4 * 4) * g.npad;
-        woff4 += (int64_t)((nw + 3) / 4 * 4) * g.npad;
-        loff += (int64_t)L.q * g.npad;
-    }
+    // per-cell stream lengths in 16-byte vectors: packed A then W, each padded
+    auto nvec_of = [&](int n, int vw) {
+        int na = n * (n + 1) / 2, nw = L.q * n;
+        return (na + vw - 1) / vw + (nw + vw - 1) / vw;
+    };
+    int64_t ioff = 0, ooff2 = 0, ooff4 = 0, loff = 0;
     L.groups32 = L.groups;
     for (size_t gi = 0; gi < L.groups.size(); ++gi) {
-        L.groups32[gi].a_off = L.a_off4[gi];
-        L.groups32[gi].w_off = L.w_off4[gi];
+        GroupDesc& g = L.groups[gi];
+        int ned = kk * g.ne;
+        g.i_off = ioff;
+        g.o_off = ooff2;
+        g.l_off = loff;
+        L.groups32[gi] = g;
+        L.groups32[gi].o_off = ooff4;
+        ioff += (int64_t)ned * g.npad;
+        ooff2 += (int64_t)nvec_of(g.n, 2) * 2 * g.npad;
+        ooff4 += (int64_t)nvec_of(g.n, 4) * 4 * g.npad;
+        loff += (int64_t)L.q * g.npad;
     }
-    L.a_len2 = aoff2;
-    L.w_len2 = woff2;
-    L.a_len4 = aoff4;
-    L.w_len4 = woff4;
+    L.o_len2 = ooff2;
+    L.o_len4 = ooff4;
     L.l_len = loff;
     L.idx.assign(ioff, (uint32_t)L.trash_idx);
     if (L.nvec >= (int64_t)SIDE1) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit dof indices");
@@ -168,7 +162,7 @@ Layout layout_build(const Mesh& m, const Setup& S) {
                     int e = m.cell_edges[m.off[c] + j];
                     uint32_t id = (uint32_t)(kk * L.edge_int[e] + r);
                     if (L.gpos_of_cell[c] != side0[e]) id |= SIDE1;
-                    L.idx[g.i_off + (int64_t)(r * g.ne + j) * g.npad + a] = id;
+                    L.idx[g.i_off + ((int64_t)(a / TILE) * ned + r * g.ne + j) * TILE + a % TILE] = id;
                 }
             (void)ned;
         }
@@ -176,12 +170,10 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     return L;
 }
 
-void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& A64, std::vector<double>& W64,
-                std::vector<double>& L64, std::vector<float>& A32, std::vector<float>& W32, std::vector<float>& L32) {
-    A64.assign(L.a_len2, 0.0);
-    W64.assign(L.w_len2, 0.0);
-    A32.assign(L.a_len4, 0.0f);
-    W32.assign(L.w_len4, 0.0f);
+void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& O64, std::vector<double>& L64,
+                std::vector<float>& O32, std::vector<float>& L32) {
+    O64.assign(L.o_len2, 0.0);
+    O32.assign(L.o_len4, 0.0f);
     L64.assign(L.l_len, 0.0);
     L32.assign(L.l_len, 0.0f);
     const int q = L.q;
@@ -189,10 +181,21 @@ void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<doub
         std::vector<double> a;
         for (int64_t gi = gb; gi < ge; ++gi) {
             const GroupDesc& g = L.groups[gi];
-            int n = g.n, na = n * (n + 1) / 2;
+            const int64_t off2 = g.o_off, off4 = L.groups32[gi].o_off;
+            const int n = g.n, na = n * (n + 1) / 2, nw = q * n;
+            const int nav2 = (na + 1) / 2, nv2 = nav2 + (nw + 1) / 2;
+            const int nav4 = (na + 3) / 4, nv4 = nav4 + (nw + 3) / 4;
             for (int lane = 0; lane < g.npad; ++lane) {
                 int c = L.cell_of_gpos[g.gpos0 + lane];
                 if (c < 0) continue;
+                const int64_t tau = lane / TILE, l = lane % TILE;
+                // element t of the stream (A entries first, W from the next whole vector)
+                auto put = [&](int t_a, int t_w, double v) {
+                    int t2 = t_a >= 0 ? t_a : nav2 * 2 + t_w;
+                    int t4 = t_a >= 0 ? t_a : nav4 * 4 + t_w;
+                    O64[off2 + ((tau * nv2 + t2 / 2) * TILE + l) * 2 + t2 % 2] = v;
+                    O32[off4 + ((tau * nv4 + t4 / 4) * TILE + l) * 4 + t4 % 4] = (float)v;
+                };
                 const double* Ac = &S.A[S.ofs.a_off[c]];
                 const double* Wc = &S.W[S.ofs.w_off[c]];
                 const char* isb = &S.isb[S.ofs.v_off[c]];
@@ -201,15 +204,8 @@ void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<doub
                 for (int i = 0; i < n; ++i)
                     for (int j = 0; j <= i; ++j)
                         if (isb[i] || isb[j]) a[i * (i + 1) / 2 + j] = (i == j) ? 1.0 : 0.0;
-                for (int t = 0; t < na; ++t) {
-                    A64[g.a_off + ((int64_t)(t / 2) * g.npad + lane) * 2 + t % 2] = a[t];
-                    A32[L.a_off4[gi] + ((int64_t)(t / 4) * g.npad + lane) * 4 + t % 4] = (float)a[t];
-                }
-                for (int t = 0; t < q * n; ++t) {
-                    double w = isb[t % n] ? 0.0 : Wc[t];
-                    W64[g.w_off + ((int64_t)(t / 2) * g.npad + lane) * 2 + t % 2] = w;
-                    W32[L.w_off4[gi] + ((int64_t)(t / 4) * g.npad + lane) * 4 + t % 4] = (float)w;
-                }
+                for (int t = 0; t < na; ++t) put(t, -1, a[t]);
+                for (int t = 0; t < nw; ++t) put(-1, t, isb[t % n] ? 0.0 : Wc[t]);
                 for (int r = 0; r < q; ++r) {
                     L64[g.l_off + (int64_t)r * g.npad + lane] = S.L[(size_t)c * q + r];
                     L32[g.l_off + (int64_t)r * g.npad + lane] = (float)S.L[(size_t)c * q + r];

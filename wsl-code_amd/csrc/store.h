@@ -20,7 +20,7 @@ constexpr uint32_t SIDE1 = 0x80000000u;
 struct GroupDesc {
     int ne, n, npad;          // valence, local flux dofs, cells incl. padding
     int gpos0, tile0;         // first store position, first global tile
-    int64_t a_off, w_off, l_off, i_off;  // element offsets of A, W, L and index blocks
+    int64_t o_off, l_off, i_off;  // element offsets of the operator stream, L and index blocks
 };
 
 struct Layout {
@@ -39,17 +39,18 @@ struct Layout {
     std::vector<int> edge_int;          // internal number of each mesh edge
     std::vector<int64_t> int_of_ref;    // reference dof -> internal slot
     std::vector<uint32_t> idx;          // packed edge-dof indices (SoA per group) with the SIDE1 flag
-    int64_t a_len2 = 0, w_len2 = 0, a_len4 = 0, w_len4 = 0, l_len = 0;  // packed sizes (elements)
-    std::vector<int64_t> a_off4, w_off4;  // FP32 offsets per group (vector width 4)
+    int64_t o_len2 = 0, o_len4 = 0, l_len = 0;  // packed sizes (elements) of the FP64 / FP32 streams
 };
 
 // Builds ordering, grouping, numbering and edge sides from the mesh and setup.
 Layout layout_build(const Mesh& m, const Setup& S);
 
-// Packs A (packed symmetric, boundary dofs decoupled), W and L into the
-// interleaved layouts: vector width 2 for FP64, 4 for FP32.
-void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& A64, std::vector<double>& W64,
-                std::vector<double>& L64, std::vector<float>& A32, std::vector<float>& W32, std::vector<float>& L32);
+// Packs each cell's operator stream -- A (packed symmetric lower triangle,
+// essential boundary dofs decoupled) then W (q x n), each padded to whole
+// 16-byte vectors (2 FP64 / 4 FP32 values) -- tile-major: vector v of lane l
+// of tile tau of a group at ((tau * nv + v) * 64 + l) * width; and L.
+void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& O64, std::vector<double>& L64,
+                std::vector<float>& O32, std::vector<float>& L32);
 
 // Per-store-position cell scale exp(-theta) (padding 0).
 void pack_scale(const Setup& S, const Layout& L, std::vector<double>& s);
