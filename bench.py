@@ -122,6 +122,62 @@ def cpu_baseline(pkg, cells, seed):
     return t1 - t0, its, rel
 
 
+def adjoint_bench(pkg, torch, cells, seed, ndir):
+    """Adjoint permeability gradient on a curved disk: theta ~ N(0, 1) per
+    cell, loss 0.5 ||C p_h - d||^2 with C the pressure at 400 seeded interior
+    points and d the observations of a hidden theta* plus 1% noise.  Times the
+    forward solve and the backward pass (adjoint solve + per-cell contraction)
+    through the torch.autograd.Function, and checks the gradient along seeded
+    unit directions against central differences (h = 1e-6)."""
+    from wsl_code_amd.autograd import PressureSampler, interior_points, make_function
+    t0 = time.time()
+    m = pkg.generate_mesh(cells, domain=pkg.DOMAIN_DISK, lloyd_iters=20, seed=seed, min_edge_frac=0.05)
+    rng = np.random.default_rng(seed + 7)
+    theta = rng.standard_normal(m.num_cells)
+    theta_star = rng.standard_normal(m.num_cells)
+    s = pkg.Solver(m, k=1, map_kind=pkg.MAP_ARCS, solution=pkg.SOL_SINSIN, bc=pkg.BC_DIRICHLET_P, theta=theta_star,
+                   device=0)
+    opts = pkg.SolveOptions(precision=pkg.FP64, rtol=1e-12, check_every=16)
+    C = PressureSampler(s, interior_points(s, 400, seed + 8, r_max=0.9))
+    xs, _ = s.solve(opts)
+    d = C(xs)
+    d = d + 0.01 * np.abs(d).mean() * rng.standard_normal(len(d))
+    log("adjoint setup %.1fs (%d cells, %d unknowns)" % (time.time() - t0, m.num_cells, s.ntot))
+    f = make_function(s, C, opts)
+    dt = torch.from_numpy(d)
+
+    def loss_of(th_np):
+        th = torch.tensor(th_np, dtype=torch.float64, requires_grad=True)
+        y = f(th)
+        return 0.5 * ((y - dt) ** 2).sum(), th
+
+    t1 = time.time()
+    L0, th = loss_of(theta)
+    t2 = time.time()
+    L0.backward()
+    t3 = time.time()
+    g = th.grad.numpy()
+    h = 1e-6
+    errs = []
+    for k in range(ndir):
+        v = rng.standard_normal(len(g))
+        v /= np.linalg.norm(v)
+        lp = loss_of(theta + h * v)[0].item()
+        lm = loss_of(theta - h * v)[0].item()
+        fd = (lp - lm) / (2 * h)
+        errs.append(abs(fd - float(g @ v)) / max(abs(fd), 1e-30))
+        log("direction %d: central difference %.10e, adjoint %.10e" % (k, fd, float(g @ v)))
+    return {"metric": "adjoint gradient dL/dtheta: forward + backward seconds", "value": round(t3 - t1, 4),
+            "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp64",
+            "data": "synthetic: seeded Voronoi disk, theta ~ N(0,1) per cell, 400 point observations of a hidden "
+                    "field plus 1% noise",
+            "config": {"workload": "disk, %d-cell Lloyd Voronoi, k=1, point-pressure least squares" % m.num_cells,
+                       "cells": m.num_cells, "unknowns": s.ntot},
+            "forward_s": round(t2 - t1, 4), "backward_s": round(t3 - t2, 4), "loss": L0.item(),
+            "fd_directions": ndir, "fd_h": h, "fd_rel_err_max": max(errs) if errs else None,
+            "fd_rel_err_median": float(np.median(errs)) if errs else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,13 +186,14 @@ def main():
     ap.add_argument("--cells", type=int, default=1000000)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--apply-reps", type=int, default=50)
-    ap.add_argument("--mode", choices=["solve", "apply"], default="solve",
+    ap.add_argument("--mode", choices=["solve", "apply", "adjoint"], default="solve",
                     help="apply: only the operator-apply timing (for profiling runs)")
     ap.add_argument("--cpu-cells", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precond", choices=["auto", "jacobi", "multigrid"], default="auto",
                     help="MINRES preconditioner: diagonal (the reference's) or the multigrid pressure cycle")
     ap.add_argument("--check-every", type=int, default=16)
+    ap.add_argument("--fd-directions", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; a figure at %d GPUs is not produced (one process per GPU "
@@ -145,6 +202,10 @@ def main():
     import torch
     pkg = load_pkg()
     assert torch.cuda.is_available() and pkg.device_count() > 0, "no HIP device"
+    if args.mode == "adjoint":
+        print(json.dumps(adjoint_bench(pkg, torch, args.cells if args.cells != 1000000 else 250000, args.seed,
+                                       args.fd_directions)))
+        return
     mesh, arr, theta, s = build(pkg, args.cells, args.seed)
     t0 = time.time()
     s.upload(0)

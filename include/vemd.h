@@ -84,6 +84,14 @@ typedef struct {
  * generator is a Python script on the unit square, vem/gmesh_py/get_gmesh.py:15-76). */
 vemd_status vemd_mesh_generate(const vemd_mesh_opts* opts, vemd_mesh* out);
 
+/* Batch of independent problems: the disjoint union of n meshes (node and
+ * cell numbering concatenated in order).  A solver created on it solves every
+ * instance at once -- one operator store, one captured Krylov loop -- with
+ * separate Krylov recurrences, stopping tests and refinement per instance
+ * (Dirichlet-pressure condition).  desc->theta then covers all cells of the
+ * union.  No reference counterpart. */
+vemd_status vemd_mesh_concat(int n, const vemd_mesh* meshes, vemd_mesh* out);
+
 /* counts: [num_nodes, num_cells, num_edges, total cell-node entries, num_arcs] */
 vemd_status vemd_mesh_counts(vemd_mesh m, int64_t* counts);
 
@@ -239,6 +247,11 @@ vemd_status vemd_adjoint_theta(vemd_solver s, const vemd_solve_opts* opts, const
  * CPU/solver/DarcySolver.h:54-73). */
 vemd_status vemd_element_matrices(vemd_solver s, int cell, int* n, double* A, double* W, double* L, double* f,
                                   double* sign, int64_t* gdof);
+
+/* Per-instance results of the last solve on a batch (n = number of
+ * instances, 1 for a plain mesh): iterations and true relative residual;
+ * the other fields are those of the whole solve. */
+vemd_status vemd_batch_info(vemd_solver s, int n, vemd_solve_info* per_instance);
 
 void vemd_solver_destroy(vemd_solver s);
 

@@ -1,0 +1,58 @@
+"""Batched solves: several independent problems on meshes of uneven size in
+one operator store and one captured Krylov loop, each with its own
+recurrences, stopping test and refinement.  Every instance must agree with a
+solve of that instance alone."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES, SEEDS = [400, 1500, 3000, 6000], [31, 32, 33, 34]
+
+
+@pytest.fixture(scope="module")
+def batch(vemd):
+    from wsl_code_amd.fields import grf_theta
+    assert vemd.device_count() > 0, "no HIP device"
+    print("batch of", SIZES, flush=True)
+    meshes, thetas = [], []
+    for n, sd in zip(SIZES, SEEDS):
+        m = vemd.generate_mesh(n, domain=vemd.DOMAIN_DISK, lloyd_iters=20, seed=sd, min_edge_frac=0.05)
+        a = m.arrays()
+        sigma, ell = 0.5 + 1.5 * (sd % 4) / 3.0, 0.02 + 0.08 * (sd % 3) / 2.0
+        meshes.append(m)
+        thetas.append(grf_theta(a["cx"], a["cy"], sigma=sigma, ell=ell, seed=sd + 100))
+    u = vemd.Mesh.concat(meshes)
+    kw = dict(k=1, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P)
+    sb = vemd.Solver(u, theta=np.concatenate(thetas), device=0, **kw)
+    singles = [vemd.Solver(m, theta=t, device=0, **kw) for m, t in zip(meshes, thetas)]
+    return sb, singles
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_matches_single_solves(vemd, batch, prec):
+    sb, singles = batch
+    p = vemd.FP64 if prec == "fp64" else vemd.MIXED
+    xb, info = sb.solve(vemd.SolveOptions(precision=p, rtol=1e-10))
+    per = sb.batch_info()
+    print("batch", info, per, flush=True)
+    assert len(per) == len(singles)
+    assert all(r["rel_residual"] <= 1e-10 for r in per)
+    for xi, s, r in zip(sb.instance_vectors(xb), singles, per):
+        xs, i1 = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12))
+        rel = np.linalg.norm(xi - xs) / np.linalg.norm(xs)
+        print("instance", s.mesh.num_cells, "cells: its", r["iterations"], "alone", i1["iterations"], "diff", rel,
+              flush=True)
+        assert rel <= 1e-7
+        # the error functionals of each instance agree with its own solve
+        eb, es = s.errors(xi), s.errors(xs)
+        assert abs(eb["u_phys"] - es["u_phys"]) <= 1e-9 * max(1.0, es["u_phys"])
+
+
+def test_batch_of_one_equals_plain_solve(vemd, batch):
+    _, singles = batch
+    s = singles[1]
+    x1, i1 = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-10))
+    x2, i2 = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-10))
+    assert i1["iterations"] == i2["iterations"] and np.array_equal(x1, x2)  # deterministic
+    assert s.batch_info()[0]["iterations"] == i2["iterations"]

@@ -77,6 +77,8 @@ SIGNATURES = {
     "vemd_errors": [c_vp, c_dp, ctypes.c_int, c_dp],
     "vemd_adjoint_theta": [c_vp, ctypes.POINTER(SolveOptsC), c_dp, c_dp, c_dp, ctypes.POINTER(SolveInfo)],
     "vemd_element_matrices": [c_vp, ctypes.c_int, c_ip, c_dp, c_dp, c_dp, c_dp, c_dp, c_lp],
+    "vemd_mesh_concat": [ctypes.c_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)],
+    "vemd_batch_info": [c_vp, ctypes.c_int, ctypes.POINTER(SolveInfo)],
     "vemd_solver_destroy": [c_vp],
 }
 _RESTYPE = {"vemd_last_error": ctypes.c_char_p, "vemd_mesh_destroy": None, "vemd_solver_destroy": None}

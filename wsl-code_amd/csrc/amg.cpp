@@ -120,9 +120,8 @@ Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::ve
     Csr A;
     A.nrows = A.ncols = n;
     A.rp.assign(n + 1, 0);
-    for (int r = 0; r < q; ++r)
-        for (int64_t g = 0; g < cpad; ++g)
-            A.rp[r * cpad + g + 1] = L.cell_of_gpos[g] < 0 ? 1 : (int64_t)q * nb[g].size();
+    for (int64_t g = 0; g < cpad; ++g)
+        for (int r = 0; r < q; ++r) A.rp[g * q + r + 1] = L.cell_of_gpos[g] < 0 ? 1 : (int64_t)q * nb[g].size();
     for (int64_t i = 0; i < n; ++i) A.rp[i + 1] += A.rp[i];
     A.ci.resize(A.rp[n]);
     A.v.assign(A.rp[n], 0.0);
@@ -132,8 +131,8 @@ Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::ve
             int c = L.cell_of_gpos[g];
             if (c < 0) {
                 for (int r = 0; r < q; ++r) {
-                    int64_t o = A.rp[r * cpad + g];
-                    A.ci[o] = (int)(r * cpad + g);
+                    int64_t o = A.rp[g * q + r];
+                    A.ci[o] = (int)(g * q + r);
                     A.v[o] = 1.0;
                 }
                 continue;
@@ -173,10 +172,10 @@ Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::ve
                     for (int s = 0; s < q; ++s) bo[r * q + s] += Wc[r * n1 + i] * Wo[s * n2 + i2] * dinv;
             }
             for (int r = 0; r < q; ++r) {
-                int64_t o = A.rp[r * cpad + g];
-                for (int s = 0; s < q; ++s)
-                    for (int t = 0; t < nbk; ++t) {
-                        A.ci[o] = (int)(s * cpad + v[t]);
+                int64_t o = A.rp[g * q + r];
+                for (int t = 0; t < nbk; ++t)
+                    for (int s = 0; s < q; ++s) {
+                        A.ci[o] = (int)((int64_t)v[t] * q + s);
                         A.v[o] = blk[(size_t)t * q * q + r * q + s];
                         ++o;
                     }

@@ -38,14 +38,15 @@ struct AmgLevel {
 
 struct Amg {
     AmgOptions opt;
+    int bs0 = 1;                      // block size of the finest level (pressure moments per cell)
     std::vector<AmgLevel> lv;         // lv[0] finest
     std::vector<double> coarse_inv;   // dense inverse of the coarsest operator, row-major
     int64_t ncoarse = 0;
 };
 
-// Pressure Schur approximation on the internal pressure block: row r * cpad +
-// gpos is pressure moment r of the cell at store position gpos; padding
-// positions get unit rows.  dflux is the assembled flux diagonal (internal
+// Pressure Schur approximation on the cells' pressure moments, cell-major:
+// row gpos * q + r is moment r of the cell at store position gpos (q x q
+// blocks per pair of cells sharing an edge); padding positions get unit rows.  dflux is the assembled flux diagonal (internal
 // order, exp(-theta) included); essential boundary flux dofs are left out.
 Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::vector<double>& dflux);
 

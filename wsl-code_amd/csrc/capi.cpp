@@ -64,13 +64,14 @@ void Solver::build_amg() {
     const int64_t n = S0.nrows;
     std::vector<int> node(n, -1);
     std::vector<double> b0(n, 0.0);
-    for (int r = 0; r < L.q; ++r)
-        for (int64_t g = 0; g < L.cpad; ++g)
-            if (L.cell_of_gpos[g] >= 0) {
-                node[r * L.cpad + g] = (int)g;
-                b0[r * L.cpad + g] = r == 0 ? 1.0 : 0.0;
+    for (int64_t g = 0; g < L.cpad; ++g)
+        if (L.cell_of_gpos[g] >= 0)
+            for (int r = 0; r < L.q; ++r) {
+                node[g * L.q + r] = (int)g;
+                b0[g * L.q + r] = r == 0 ? 1.0 : 0.0;
             }
     amg.reset(new Amg(amg_build(std::move(S0), node, (int)L.cpad, b0, L.spatial_gpos, amg_opt)));
+    amg->bs0 = L.q;
     amg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -139,6 +140,19 @@ vemd_status vemd_mesh_generate(const vemd_mesh_opts* opts, vemd_mesh* out) {
     API_BEGIN
     REQUIRE(opts && out, "null argument");
     Mesh* m = mesh_generate(*opts);
+    *out = new vemd_mesh_s{std::shared_ptr<Mesh>(m)};
+    API_END
+}
+
+vemd_status vemd_mesh_concat(int n, const vemd_mesh* meshes, vemd_mesh* out) {
+    API_BEGIN
+    REQUIRE(n > 0 && meshes && out, "null argument or empty batch");
+    std::vector<const Mesh*> parts;
+    for (int i = 0; i < n; ++i) {
+        REQUIRE(meshes[i], "null mesh in the batch");
+        parts.push_back(meshes[i]->m.get());
+    }
+    Mesh* m = mesh_concat(parts);
     *out = new vemd_mesh_s{std::shared_ptr<Mesh>(m)};
     API_END
 }
@@ -357,6 +371,20 @@ vemd_status vemd_element_matrices(vemd_solver sh, int cell, int* n_out, double* 
     if (f) std::copy(&S.f[(size_t)cell * q], &S.f[(size_t)cell * q] + q, f);
     if (sign) std::copy(&S.sgn[S.ofs.v_off[cell]], &S.sgn[S.ofs.v_off[cell]] + n, sign);
     if (gdof) std::copy(&S.gdof[S.ofs.v_off[cell]], &S.gdof[S.ofs.v_off[cell]] + n, gdof);
+    API_END
+}
+
+vemd_status vemd_batch_info(vemd_solver sh, int n, vemd_solve_info* out) {
+    API_BEGIN
+    REQUIRE(sh && out, "null argument");
+    const Solver& s = *sh->s;
+    REQUIRE(n == s.mesh->ninst(), "n must be the number of instances");
+    REQUIRE((int)s.inst_iterations.size() == n, "no solve has run");
+    for (int i = 0; i < n; ++i) {
+        out[i] = s.last_info;
+        out[i].iterations = s.inst_iterations[i];
+        out[i].rel_residual = s.inst_rel[i];
+    }
     API_END
 }
 

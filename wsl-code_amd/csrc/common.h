@@ -42,6 +42,10 @@ struct Mesh {
     std::vector<int> edge_cells;        // two cells per edge, second = -1 on the boundary
     // Circular boundary arcs centred at the origin: key (min << 32 | max) -> radius.
     std::unordered_map<uint64_t, double> arcs;
+    // Disjoint instances of a batch: cells [inst_cell0[i], inst_cell0[i + 1])
+    // (empty: one instance).
+    std::vector<int> inst_cell0;
+    int ninst() const { return inst_cell0.empty() ? 1 : (int)inst_cell0.size() - 1; }
 
     int npc(int c) const { return off[c + 1] - off[c]; }
     static uint64_t key(int a, int b) {
@@ -57,6 +61,8 @@ Mesh* mesh_read_vtk(const char* path);
 Mesh* mesh_from_arrays(int nn, const double* xy, int nc, const int* off, const int* nodes, int narc,
                        const int* arc_nodes, const double* arc_r);
 Mesh* mesh_generate(const vemd_mesh_opts& o);
+// Disjoint union of meshes (node and cell numbering concatenated in order).
+Mesh* mesh_concat(const std::vector<const Mesh*>& parts);
 
 // ---------------------------------------------------------------------------
 // Element setup output (FP64, reference DOF conventions, direction signs

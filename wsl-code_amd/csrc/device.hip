@@ -283,15 +283,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 // cells share its L2.
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs<T> a) {
-    if (a.st && a.st->done) return;
     const int b = blockIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int tile = ((b & 7) * a.per_xcd + (b >> 3)) * 4 + wave;
     if (tile >= a.ntiles) return;
     const GroupDesc g = a.groups[__builtin_amdgcn_readfirstlane(a.tile_group[tile])];
+    const MState* st = a.st ? a.st + g.inst : nullptr;  // the Krylov state of the tile's instance
+    if (st && st->done) return;
     const int l = threadIdx.x & 63;
     const int tau = tile - g.tile0;
-    const T* x = a.st ? a.xs[a.st->itn & 1] : a.x;
+    const T* x = st ? a.xs[st->itn & 1] : a.x;
     double dot = 0.0, lamp = 0.0;
     switch (g.ne) {
         case 3: cell_body<T, K, 3>(a, x, g, tau, l, dot, lamp); break;
@@ -343,13 +344,35 @@ __global__ void k_lam_row(const double* plam, int ntiles, T* y, int64_t lam_idx)
     if (threadIdx.x == 0) y[lam_idx] = (T)s;
 }
 
+// Sums the partials p[b..e) in a fixed order with one block of 256 threads.
+__device__ double block_sum_range(const double* p, int b, int e) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = b + (int)threadIdx.x; i < e; i += 256) s += p[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// The Krylov recurrences run per batch instance: every instance has its own
+// MState; per-tile partials of the apply and per-chunk partials of the vector
+// sweeps are summed over the instance's contiguous tile and chunk ranges by
+// one block per instance.
+
 // After the apply of iteration itn: alfa and the Lanczos coefficients.
 template <typename T>
-__global__ void k_finA(MState* st, const double* pdot, const double* plam, int ntiles, T* Az, const T* Z0,
-                       const T* Z1, int64_t lam_idx) {
+__global__ void k_finA(MState* sts, const double* pdot, const double* plam, const int* inst_tile0, T* Az,
+                       const T* Z0, const T* Z1, int64_t lam_idx) {
+    MState* st = sts + blockIdx.x;
     if (st->done) return;
-    double zaz = block_sum_fixed(pdot, ntiles);
-    double yl = lam_idx >= 0 ? block_sum_fixed(plam, ntiles) : 0.0;
+    double zaz = block_sum_range(pdot, inst_tile0[blockIdx.x], inst_tile0[blockIdx.x + 1]);
+    double yl = lam_idx >= 0 ? block_sum_range(plam, inst_tile0[blockIdx.x], inst_tile0[blockIdx.x + 1]) : 0.0;
     if (threadIdx.x == 0) {
         if (lam_idx >= 0) {
             const T* z = (st->itn & 1) ? Z1 : Z0;
@@ -374,20 +397,50 @@ struct UpdArgs {
     const T* Az;
     const T* Az1;        // side-1 half of the edge dofs of Az
     const T* minv;
-    int64_t n, edof;
-    double* p1;
+    int64_t edof;
+    const VecChunk* chunks;
+    double* p1;          // per-chunk partials
     double* p2;
-    const MState* st;
-    int64_t pb, pe;      // pressure block preconditioned by the multigrid cycle (pb == pe: none)
-    T* bp;               // copy of the new Lanczos vector's pressure block (cycle input)
+    MState* st;          // per instance
+    int amg;             // pressure chunks are preconditioned by the multigrid cycle
+    int64_t pb;          // first entry of the pressure block
+    T* bp;               // cycle input, cell-major: bp[gpos * q + r]
+    int64_t cpad;
+    int q;
 };
 
+// Cycle index of entry i of the pressure block (slot r = moment r).
+template <typename T>
+__device__ __forceinline__ int64_t cyc_index(const UpdArgs<T>& a, int64_t i, int r) {
+    return (i - a.pb - (int64_t)r * a.cpad) * a.q + r;
+}
+
+// Block partials of two sums (256 threads), written by thread 0.
+__device__ __forceinline__ void block_pair(double d1, double d2, double* o1, double* o2) {
+    __shared__ double s1[256], s2[256];
+    s1[threadIdx.x] = d1;
+    s2[threadIdx.x] = d2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *o1 = s1[0];
+        if (o2) *o2 = s2[0];
+    }
+}
+
 // Lanczos step, preconditioner, and the solution update of the previous
-// iteration (delayed by one so that it needs no extra pass), in one sweep.
+// iteration (delayed by one so that it needs no extra pass), in one sweep of
+// one chunk per block.
 template <typename T>
 __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
-    __shared__ double s1[256], s2[256];
-    const MState* st = a.st;
+    const VecChunk ch = a.chunks[blockIdx.x];
+    const MState* st = a.st + ch.inst;
     if (st->done) return;
     const int it = st->itn;
     T* __restrict__ r2 = a.R[it & 1];
@@ -398,8 +451,10 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
     const T caz = (T)st->c_az, cr1 = (T)st->c_r1, cr2 = (T)st->c_r2;
     const bool pend = st->pending;
     const T uv = (T)st->u_v, uw1 = (T)st->u_w1, uw2 = (T)st->u_w2, ux = (T)st->u_x;
+    const bool cyc = a.amg && ch.pslot;
+    const int pr = ch.pslot - 1;
     double d1 = 0.0, d2 = 0.0;
-    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
+    for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
         T r2i = r2[i];
         T az = a.Az[i];
         if (i < a.edof) az += a.Az1[i];
@@ -410,8 +465,8 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
             a.x[i] += ux * w;
         }
         r1[i] = y;
-        if (i >= a.pb && i < a.pe) {
-            a.bp[i - a.pb] = y;  // preconditioned by the cycle, dots in k_pdot
+        if (cyc) {
+            a.bp[cyc_index(a, i, pr)] = y;  // preconditioned by the cycle, dots in k_pdot
         } else {
             T zn = a.minv[i] * y;
             zo[i] = zn;
@@ -419,63 +474,42 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
             d2 += (double)zn * (double)r2i;
         }
     }
-    s1[threadIdx.x] = d1;
-    s2[threadIdx.x] = d2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            s1[threadIdx.x] += s1[threadIdx.x + o];
-            s2[threadIdx.x] += s2[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        a.p1[blockIdx.x] = s1[0];
-        a.p2[blockIdx.x] = s2[0];
-    }
+    block_pair(d1, d2, a.p1 + blockIdx.x, a.p2 + blockIdx.x);
 }
 
-// Pressure block of the preconditioned vector after the multigrid cycle:
-// z_p = cycle output, with the partials of y.z and z.r2 over the block (init:
-// z of the right-hand side and b.z).
+// Pressure chunks after the multigrid cycle: z_p = cycle output, with the
+// partials of y.z and z.r2 (init: z of the right-hand side and b.z).
 template <typename T>
-__global__ __launch_bounds__(256) void k_pdot(UpdArgs<T> a, const T* xp, double* q1, double* q2, int init) {
-    __shared__ double s1[256], s2[256];
-    const MState* st = a.st;
+__global__ __launch_bounds__(256) void k_pdot(UpdArgs<T> a, const int* pchunks, const T* xp, double* q1, double* q2,
+                                            int init) {
+    const int c = pchunks[blockIdx.x];
+    const VecChunk ch = a.chunks[c];
+    const MState* st = a.st + ch.inst;
     if (!init && st->done) return;
     const int it = init ? 0 : st->itn;
     T* __restrict__ zo = init ? a.Z[0] : a.Z[(it + 1) & 1];
     const T* __restrict__ r2 = a.R[it & 1];
-    const int64_t np = a.pe - a.pb;
+    const int pr = ch.pslot - 1;
     double d1 = 0.0, d2 = 0.0;
-    for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < np; j += (int64_t)gridDim.x * 256) {
+    for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
+        const int64_t j = cyc_index(a, i, pr);
         const T z = xp[j];
-        zo[a.pb + j] = z;
+        zo[i] = z;
         d1 += (double)a.bp[j] * (double)z;
-        if (!init) d2 += (double)z * (double)r2[a.pb + j];
+        if (!init) d2 += (double)z * (double)r2[i];
     }
-    s1[threadIdx.x] = d1;
-    s2[threadIdx.x] = d2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            s1[threadIdx.x] += s1[threadIdx.x + o];
-            s2[threadIdx.x] += s2[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        q1[blockIdx.x] = s1[0];
-        q2[blockIdx.x] = s2[0];
-    }
+    block_pair(d1, d2, q1 + c, q2 + c);
 }
 
-// Givens recurrence of MINRES (Paige and Saunders) and the stopping test.
-__global__ void k_finB(MState* st, const double* p1, const double* p2, int nb, const double* q1, const double* q2,
-                       int nq) {
+// Givens recurrence of MINRES (Paige and Saunders) and the stopping test, one
+// block per instance.
+__global__ void k_finB(MState* sts, const double* p1, const double* p2, const double* q1, const double* q2,
+                       const int* inst_chunk0, int useq) {
+    MState* st = sts + blockIdx.x;
     if (st->done) return;
-    double b2 = block_sum_fixed(p1, nb) + block_sum_fixed(q1, nq);
-    double rho = block_sum_fixed(p2, nb) + block_sum_fixed(q2, nq);
+    const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
+    double b2 = block_sum_range(p1, c0, c1) + (useq ? block_sum_range(q1, c0, c1) : 0.0);
+    double rho = block_sum_range(p2, c0, c1) + (useq ? block_sum_range(q2, c0, c1) : 0.0);
     if (threadIdx.x == 0) {
         double beta_new = std::sqrt(b2 > 0.0 ? b2 : 0.0);
         st->rho = rho;
@@ -502,12 +536,25 @@ __global__ void k_finB(MState* st, const double* p1, const double* p2, int nb, c
     }
 }
 
-// r2 = b, r1 = 0, z = M^{-1} b, w = 0, x = 0 and partials of b.z.
+// all_done = every instance done (lets the multigrid cycle skip its work).
+__global__ void k_alldone(const MState* sts, int ninst, int* all_done) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < ninst; i += blockDim.x)
+        if (!sts[i].done) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) *all_done = any ? 0 : 1;
+}
+
+// r2 = b, r1 = 0, z = M^{-1} b, w = 0, x = 0 and partials of b.z, per chunk.
 template <typename T>
 __global__ __launch_bounds__(256) void k_init(UpdArgs<T> a, const T* b) {
-    __shared__ double s1[256];
+    const VecChunk ch = a.chunks[blockIdx.x];
+    const bool cyc = a.amg && ch.pslot;
+    const int pr = ch.pslot - 1;
     double d = 0.0;
-    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
+    for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
         T bi = b[i];
         a.R[0][i] = bi;
         a.R[1][i] = T(0);
@@ -515,25 +562,23 @@ __global__ __launch_bounds__(256) void k_init(UpdArgs<T> a, const T* b) {
         a.WB[0][i] = T(0);
         a.WB[1][i] = T(0);
         a.x[i] = T(0);
-        if (i >= a.pb && i < a.pe) {
-            a.bp[i - a.pb] = bi;
+        if (cyc) {
+            a.bp[cyc_index(a, i, pr)] = bi;
         } else {
             T z = a.minv[i] * bi;
             a.Z[0][i] = z;
             d += (double)bi * (double)z;
         }
     }
-    s1[threadIdx.x] = d;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.p1[blockIdx.x] = s1[0];
+    block_pair(d, 0.0, a.p1 + blockIdx.x, a.p2 + blockIdx.x);
 }
 
-__global__ void k_finInit(MState* st, const double* p1, int nb, const double* q1, int nq, double rtol, int maxit) {
-    double b2 = block_sum_fixed(p1, nb) + block_sum_fixed(q1, nq);
+// Initial state per instance: beta1 = sqrt(b.z), tolerance rtol_i * beta1;
+// an instance with rtol_i >= 1 or a zero right-hand side starts done.
+__global__ void k_finInit(MState* sts, const double* p1, const double* q1, const int* inst_chunk0, int useq,
+                          const double* rtol, int maxit) {
+    const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
+    double b2 = block_sum_range(p1, c0, c1) + (useq ? block_sum_range(q1, c0, c1) : 0.0);
     if (threadIdx.x == 0) {
         double beta1 = std::sqrt(b2 > 0.0 ? b2 : 0.0);
         MState s;
@@ -542,50 +587,50 @@ __global__ void k_finInit(MState* st, const double* p1, int nb, const double* q1
         s.beta = beta1;
         s.phibar = beta1;
         s.cs = -1.0;
-        s.tol = rtol * beta1;
+        s.tol = rtol[blockIdx.x] * beta1;
         s.maxit = maxit;
-        s.done = (beta1 > 0.0 && maxit > 0) ? 0 : 1;
-        *st = s;
+        s.done = (beta1 > 0.0 && maxit > 0 && rtol[blockIdx.x] < 1.0) ? 0 : 1;
+        sts[blockIdx.x] = s;
     }
 }
 
-// Pending solution update of the last iteration.
+// Pending solution update of the last iteration, per chunk.
 template <typename T>
 __global__ __launch_bounds__(256) void k_flush(UpdArgs<T> a) {
-    const MState* st = a.st;
+    const VecChunk ch = a.chunks[blockIdx.x];
+    const MState* st = a.st + ch.inst;
     if (!st->pending) return;
     const int m = st->itn - 1;
     const T* z = a.Z[m & 1];
     const T* wpp = a.WB[m & 1];
     const T* wp = a.WB[(m + 1) & 1];
     const T uv = (T)st->u_v, uw1 = (T)st->u_w1, uw2 = (T)st->u_w2, ux = (T)st->u_x;
-    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) {
+    for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
         T w = uv * z[i] - uw1 * wpp[i] - uw2 * wp[i];
         a.x[i] += ux * w;
     }
 }
 
-// r = b - y with partial sums of |r|^2 (FP64).
-__global__ __launch_bounds__(256) void k_residual(const double* b, const double* y, double* r, int64_t n, double* p) {
-    __shared__ double s1[256];
+// r = b - y over one chunk with the partial of |r|^2 (FP64); y's edge dofs
+// are the sum of both sides.
+__global__ __launch_bounds__(256) void k_residual(const VecChunk* chunks, const double* b, const double* y,
+                                                  const double* y1, int64_t edof, double* r, double* p) {
+    const VecChunk ch = chunks[blockIdx.x];
     double d = 0.0;
-    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        double v = b[i] - y[i];
+    for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
+        double yi = y[i];
+        if (i < edof) yi += y1[i];
+        double v = b[i] - yi;
         r[i] = v;
         d += v * v;
     }
-    s1[threadIdx.x] = d;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) p[blockIdx.x] = s1[0];
+    block_pair(d, 0.0, p + blockIdx.x, nullptr);
 }
 
-__global__ void k_sum(const double* p, int n, double* out) {
-    double s = block_sum_fixed(p, n);
-    if (threadIdx.x == 0) *out = s;
+// Per-instance sums of chunk partials.
+__global__ void k_inst_sum(const double* p, const int* inst_chunk0, double* out) {
+    double s = block_sum_range(p, inst_chunk0[blockIdx.x], inst_chunk0[blockIdx.x + 1]);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 template <typename S, typename D>
@@ -702,12 +747,22 @@ struct DeviceState {
     int ntiles = 0;
     double* pdot = nullptr;
     double* plam = nullptr;
-    double* p1 = nullptr;
+    double* p1 = nullptr;       // per-chunk partials of the Krylov sweeps
     double* p2 = nullptr;
-    int nblk = 0;
-    MState* st = nullptr;
-    MState* st_host = nullptr;  // pinned
-    double* dsum = nullptr;
+    int nblk = 0;               // grid of the plain elementwise kernels
+    int ninst = 1;
+    VecChunk* chunks = nullptr;
+    int nchunks = 0;
+    int* pchunks = nullptr;
+    int npchunks = 0;
+    int* inst_tile0 = nullptr;
+    int* inst_chunk0 = nullptr;
+    MState* st = nullptr;       // per instance
+    MState* st_host = nullptr;  // pinned copy
+    int* all_done = nullptr;
+    double* rtol = nullptr;     // per-instance inner tolerances
+    double* rtol_host = nullptr;  // pinned
+    double* dsum = nullptr;     // per-instance sums
     double* hsum = nullptr;     // pinned
     DevKrylov<double> k64;
     DevKrylov<float> k32;
@@ -721,9 +776,8 @@ struct DeviceState {
     float* y1_32 = nullptr;
     AmgDev<double>* amg64 = nullptr;  // multigrid pressure preconditioner (when built)
     AmgDev<float>* amg32 = nullptr;
-    double* q1 = nullptr;  // partials of the pressure-block dots
+    double* q1 = nullptr;  // per-chunk partials of the pressure-block dots (zero outside it)
     double* q2 = nullptr;
-    int nblkp = 0;
     std::vector<void*> allocs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -836,16 +890,27 @@ void dev_upload(Solver& s, int device) {
     d.pdot = (double*)dalloc(d, d.ntiles * sizeof(double));
     d.plam = (double*)dalloc(d, d.ntiles * sizeof(double));
     d.nblk = (int)std::min<int64_t>(2048, (L.nvec + 255) / 256);
-    d.p1 = (double*)dalloc(d, d.nblk * sizeof(double));
-    d.p2 = (double*)dalloc(d, d.nblk * sizeof(double));
-    d.nblkp = (int)std::min<int64_t>(1024, ((int64_t)L.q * L.cpad + 255) / 256);
-    d.q1 = (double*)dalloc(d, d.nblkp * sizeof(double));
-    d.q2 = (double*)dalloc(d, d.nblkp * sizeof(double));
-    d.st = (MState*)dalloc(d, sizeof(MState));
-    HIPCHK(hipMemset(d.st, 0, sizeof(MState)));
-    HIPCHK(hipHostMalloc((void**)&d.st_host, sizeof(MState), hipHostMallocDefault));
-    d.dsum = (double*)dalloc(d, sizeof(double));
-    HIPCHK(hipHostMalloc((void**)&d.hsum, sizeof(double), hipHostMallocDefault));
+    d.ninst = L.ninst;
+    d.nchunks = (int)L.chunks.size();
+    d.npchunks = (int)L.pchunks.size();
+    d.chunks = dupload(d, L.chunks);
+    d.pchunks = dupload(d, L.pchunks);
+    d.inst_tile0 = dupload(d, L.inst_tile0);
+    d.inst_chunk0 = dupload(d, L.inst_chunk0);
+    d.p1 = (double*)dalloc(d, d.nchunks * sizeof(double));
+    d.p2 = (double*)dalloc(d, d.nchunks * sizeof(double));
+    d.q1 = (double*)dalloc(d, d.nchunks * sizeof(double));
+    d.q2 = (double*)dalloc(d, d.nchunks * sizeof(double));
+    for (double* p : {d.p1, d.p2, d.q1, d.q2}) HIPCHK(hipMemset(p, 0, d.nchunks * sizeof(double)));
+    d.st = (MState*)dalloc(d, d.ninst * sizeof(MState));
+    HIPCHK(hipMemset(d.st, 0, d.ninst * sizeof(MState)));
+    HIPCHK(hipHostMalloc((void**)&d.st_host, d.ninst * sizeof(MState), hipHostMallocDefault));
+    d.all_done = (int*)dalloc(d, sizeof(int));
+    HIPCHK(hipMemset(d.all_done, 0, sizeof(int)));
+    d.rtol = (double*)dalloc(d, d.ninst * sizeof(double));
+    HIPCHK(hipHostMalloc((void**)&d.rtol_host, d.ninst * sizeof(double), hipHostMallocDefault));
+    d.dsum = (double*)dalloc(d, d.ninst * sizeof(double));
+    HIPCHK(hipHostMalloc((void**)&d.hsum, d.ninst * sizeof(double), hipHostMallocDefault));
     alloc_krylov(d, d.k64, L.nvec, L.cbase);
     alloc_krylov(d, d.k32, L.nvec, L.cbase);
     d.y1_64 = (double*)dalloc(d, L.cbase * sizeof(double));
@@ -883,6 +948,7 @@ void dev_free(Solver& s) {
     for (void* p : d.allocs) (void)hipFree(p);
     if (d.st_host) (void)hipHostFree(d.st_host);
     if (d.hsum) (void)hipHostFree(d.hsum);
+    if (d.rtol_host) (void)hipHostFree(d.rtol_host);
     if (d.e0) (void)hipEventDestroy(d.e0);
     if (d.e1) (void)hipEventDestroy(d.e1);
     if (d.stream) (void)hipStreamDestroy(d.stream);
@@ -1020,47 +1086,51 @@ AmgDev<float>* amg_of<float>(DeviceState& d) {
 }
 
 // Kernel arguments of the Krylov sweeps; with the multigrid cycle the
-// pressure block [pb, pe) of z comes from the cycle instead of the diagonal.
+// pressure chunks of z come from the cycle instead of the diagonal.
 template <typename T>
 static UpdArgs<T> upd_args(Solver& s, DevKrylov<T>& k, bool amg) {
     DeviceState& d = *s.dev;
     const Layout& L = s.layout;
-    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x, k.Az, k.Az1, ops<T>(d).minv, L.nvec,
-                 L.cbase, d.p1, d.p2, d.st, 0, 0, nullptr};
+    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x,  k.Az,      k.Az1, ops<T>(d).minv,
+                 L.cbase,          d.chunks,          d.p1,              d.p2, d.st, amg ? 1 : 0, 0, nullptr,
+                 L.cpad,           L.q};
     if (amg) {
         u.pb = L.cbase + (int64_t)L.ni * L.cpad;
-        u.pe = u.pb + (int64_t)L.q * L.cpad;
         u.bp = amg_of<T>(d)->lv[0].b;
     }
     return u;
 }
 
-// One MINRES iteration: apply, alfa, Lanczos/precondition/update, [pressure
-// cycle], Givens.
+// One MINRES iteration of every unfinished instance: apply, alfa,
+// Lanczos/precondition/update, [pressure cycle], Givens.
 template <typename T>
 static void enqueue_iteration(Solver& s, DevKrylov<T>& k, bool amg) {
     DeviceState& d = *s.dev;
     const Layout& L = s.layout;
     ApplyArgs<T> a = make_args<T>(s, nullptr, k.Az, k.Az1, d.st, k.Z[0], k.Z[1], true);
     launch_apply(s, a);
-    hipLaunchKernelGGL((k_finA<T>), dim3(1), dim3(256), 0, d.stream, d.st, d.pdot, d.plam, d.ntiles, k.Az, k.Z[0],
-                       k.Z[1], L.lam ? L.lam_idx : (int64_t)-1);
+    hipLaunchKernelGGL((k_finA<T>), dim3(d.ninst), dim3(256), 0, d.stream, d.st, d.pdot, d.plam, d.inst_tile0, k.Az,
+                       k.Z[0], k.Z[1], L.lam ? L.lam_idx : (int64_t)-1);
     UpdArgs<T> u = upd_args(s, k, amg);
-    hipLaunchKernelGGL((k_update<T>), dim3(d.nblk), dim3(256), 0, d.stream, u);
+    hipLaunchKernelGGL((k_update<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
     if (amg) {
         const AmgDev<T>& M = *amg_of<T>(d);
-        amg_vcycle(M, d.stream, d.st);
-        hipLaunchKernelGGL((k_pdot<T>), dim3(d.nblkp), dim3(256), 0, d.stream, u, (const T*)M.lv[0].x, d.q1, d.q2, 0);
+        amg_vcycle(M, d.stream, d.all_done);
+        hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
+                           (const T*)M.lv[0].x, d.q1, d.q2, 0);
     }
-    hipLaunchKernelGGL(k_finB, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.p2, d.nblk, d.q1, d.q2,
-                       amg ? d.nblkp : 0);
+    hipLaunchKernelGGL(k_finB, dim3(d.ninst), dim3(256), 0, d.stream, d.st, d.p1, d.p2, d.q1, d.q2, d.inst_chunk0,
+                       amg ? 1 : 0);
+    hipLaunchKernelGGL(k_alldone, dim3(1), dim3(256), 0, d.stream, d.st, d.ninst, d.all_done);
 }
 
-// Preconditioned MINRES on K x = b (internal order, precision T): b in k.b,
-// solution in k.x.  Stops when the M^{-1}-norm residual estimate falls to
-// rtol times its initial value.  Returns the iteration count.
+// Preconditioned MINRES on K x = b for every instance (internal order,
+// precision T): b in k.b, solution in k.x.  Instance i stops when its
+// M^{-1}-norm residual estimate falls to rtol[i] times its initial value
+// (rtol[i] >= 1: left untouched, x_i = 0).  Fills its[i]; returns the largest.
 template <typename T>
-static int minres(Solver& s, DevKrylov<T>& k, double rtol, int maxit, int check_every, bool use_graph, bool amg) {
+static int minres(Solver& s, DevKrylov<T>& k, const std::vector<double>& rtol, int maxit, int check_every,
+                  bool use_graph, bool amg, std::vector<int>& its) {
     DeviceState& d = *s.dev;
     UpdArgs<T> u = upd_args(s, k, amg);
     check_every = std::max(1, check_every);
@@ -1076,54 +1146,78 @@ static int minres(Solver& s, DevKrylov<T>& k, double rtol, int maxit, int check_
         k.graph_iters = check_every;
         k.graph_amg = amg;
     }
-    hipLaunchKernelGGL((k_init<T>), dim3(d.nblk), dim3(256), 0, d.stream, u, (const T*)k.b);
+    HIPCHK(hipStreamSynchronize(d.stream));  // the pinned tolerance buffer is reused
+    for (int i = 0; i < d.ninst; ++i) d.rtol_host[i] = rtol[i];
+    HIPCHK(hipMemcpyAsync(d.rtol, d.rtol_host, d.ninst * sizeof(double), hipMemcpyHostToDevice, d.stream));
+    hipLaunchKernelGGL((k_init<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u, (const T*)k.b);
     if (amg) {
         const AmgDev<T>& M = *amg_of<T>(d);
         amg_vcycle(M, d.stream, nullptr);
-        hipLaunchKernelGGL((k_pdot<T>), dim3(d.nblkp), dim3(256), 0, d.stream, u, (const T*)M.lv[0].x, d.q1, d.q2, 1);
+        hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
+                           (const T*)M.lv[0].x, d.q1, d.q2, 1);
     }
-    hipLaunchKernelGGL(k_finInit, dim3(1), dim3(256), 0, d.stream, d.st, d.p1, d.nblk, d.q1, amg ? d.nblkp : 0, rtol,
-                       maxit);
+    hipLaunchKernelGGL(k_finInit, dim3(d.ninst), dim3(256), 0, d.stream, d.st, d.p1, d.q1, d.inst_chunk0, amg ? 1 : 0,
+                       (const double*)d.rtol, maxit);
+    hipLaunchKernelGGL(k_alldone, dim3(1), dim3(256), 0, d.stream, d.st, d.ninst, d.all_done);
     HIPCHK(hipGetLastError());
     static const bool verbose = std::getenv("VEMD_VERBOSE") != nullptr;
     auto last = std::chrono::steady_clock::now();
     while (true) {
+        HIPCHK(hipMemcpyAsync(d.st_host, d.st, d.ninst * sizeof(MState), hipMemcpyDeviceToHost, d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));
+        bool all = true;
+        for (int i = 0; i < d.ninst && all; ++i) all = d.st_host[i].done != 0;
+        if (all) break;
+        if (verbose && std::chrono::steady_clock::now() - last > std::chrono::seconds(10)) {
+            last = std::chrono::steady_clock::now();
+            const MState& m = d.st_host[0];
+            std::fprintf(stderr, "[vemd] minres %s: %d iterations, residual estimate %.3e of %.3e (instance 0 of %d)\n",
+                         sizeof(T) == 8 ? "fp64" : "fp32", m.itn, m.phibar / m.beta1, rtol[0], d.ninst);
+        }
         if (use_graph)
             HIPCHK(hipGraphLaunch(k.graph, d.stream));
         else
             for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k, amg);
-        HIPCHK(hipMemcpyAsync(d.st_host, d.st, sizeof(MState), hipMemcpyDeviceToHost, d.stream));
-        HIPCHK(hipStreamSynchronize(d.stream));
-        if (d.st_host->done) break;
-        if (verbose && std::chrono::steady_clock::now() - last > std::chrono::seconds(10)) {
-            last = std::chrono::steady_clock::now();
-            std::fprintf(stderr, "[vemd] minres %s: %d iterations, residual estimate %.3e of %.3e\n",
-                         sizeof(T) == 8 ? "fp64" : "fp32", d.st_host->itn, d.st_host->phibar / d.st_host->beta1, rtol);
-        }
     }
-    hipLaunchKernelGGL((k_flush<T>), dim3(d.nblk), dim3(256), 0, d.stream, u);
+    hipLaunchKernelGGL((k_flush<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
     HIPCHK(hipGetLastError());
-    return d.st_host->itn;
+    its.assign(d.ninst, 0);
+    int mx = 0;
+    for (int i = 0; i < d.ninst; ++i) {
+        its[i] = d.st_host[i].itn;
+        mx = std::max(mx, its[i]);
+    }
+    return mx;
 }
 
-// r64 = b64 - K x64 and returns ||r64||_2.
-static double residual64(Solver& s) {
+// r64 = b64 - K x64; returns the 2-norm of every instance's block of r64.
+static std::vector<double> residual64(Solver& s) {
     DeviceState& d = *s.dev;
-    apply_plain<double>(s, d.x64, d.y64);
-    hipLaunchKernelGGL(k_residual, dim3(d.nblk), dim3(256), 0, d.stream, d.b64, d.y64, d.r64, s.layout.nvec, d.p1);
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, d.stream, d.p1, d.nblk, d.dsum);
-    HIPCHK(hipMemcpyAsync(d.hsum, d.dsum, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    apply_plain<double>(s, d.x64, d.y64, d.y1_64);
+    hipLaunchKernelGGL(k_residual, dim3(d.nchunks), dim3(256), 0, d.stream, (const VecChunk*)d.chunks,
+                       (const double*)d.b64, (const double*)d.y64, (const double*)d.y1_64, s.layout.cbase, d.r64,
+                       d.p1);
+    hipLaunchKernelGGL(k_inst_sum, dim3(d.ninst), dim3(256), 0, d.stream, (const double*)d.p1,
+                       (const int*)d.inst_chunk0, d.dsum);
+    HIPCHK(hipMemcpyAsync(d.hsum, d.dsum, d.ninst * sizeof(double), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
-    return std::sqrt(*d.hsum);
+    std::vector<double> r(d.ninst);
+    for (int i = 0; i < d.ninst; ++i) r[i] = std::sqrt(d.hsum[i]);
+    return r;
 }
 
-// Solve with b64 already on the device; solution in x64.
+// Solve with b64 already on the device; solution in x64.  Iterative
+// refinement per instance: FP64 residuals, inner MINRES (FP64, or FP32 for
+// VEMD_MIXED) with per-instance tolerances, until every instance's true
+// relative residual is below rtol.
 static void solve_device(Solver& s, const vemd_solve_opts& o, vemd_solve_info* info) {
     DeviceState& d = *s.dev;
     const int64_t n = s.layout.nvec;
+    const int ni = d.ninst;
     HIPCHK(hipMemsetAsync(d.x64, 0, n * sizeof(double), d.stream));
-    double bnorm = residual64(s);  // r = b
-    double rel = bnorm > 0 ? 1.0 : 0.0;
+    std::vector<double> bn = residual64(s), rel(ni);  // r = b
+    for (int i = 0; i < ni; ++i) rel[i] = bn[i] > 0 ? 1.0 : 0.0;
+    std::vector<int> it_tot(ni, 0), its;
     int iters = 0, refin = 0;
     int maxit = o.max_iters > 0 ? o.max_iters : 1000000;
     bool graph = o.use_graph != 0;
@@ -1131,28 +1225,40 @@ static void solve_device(Solver& s, const vemd_solve_opts& o, vemd_solve_info* i
     if (o.precond == VEMD_PC_MULTIGRID && !d.amg64)
         throw Error(VEMD_ERR_ARG, "no multigrid hierarchy: it is built for Dirichlet pressure problems only");
     const bool amg = o.precond != VEMD_PC_JACOBI && d.amg64 != nullptr;
-    while (rel > o.rtol && iters < maxit && refin < 100) {
+    auto worst = [&]() { return *std::max_element(rel.begin(), rel.end()); };
+    std::vector<double> target(ni);
+    while (worst() > o.rtol && iters < maxit && refin < 100) {
+        for (int i = 0; i < ni; ++i) {
+            if (!(rel[i] > o.rtol))
+                target[i] = 1.0;  // converged: no inner iterations
+            else if (o.precision == VEMD_FP64)
+                target[i] = std::min(0.5, 0.5 * o.rtol / rel[i]);
+            else
+                target[i] = std::max(o.inner_rtol > 0 ? o.inner_rtol : 1e-4, 0.5 * o.rtol / rel[i]);
+        }
         if (o.precision == VEMD_FP64) {
             HIPCHK(hipMemcpyAsync(d.k64.b, d.r64, n * sizeof(double), hipMemcpyDeviceToDevice, d.stream));
-            double target = std::min(0.5, 0.5 * o.rtol / rel);
-            iters += minres<double>(s, d.k64, target, maxit - iters, chk, graph, amg);
+            iters += minres<double>(s, d.k64, target, maxit - iters, chk, graph, amg, its);
             hipLaunchKernelGGL((k_add<double>), dim3(d.nblk), dim3(256), 0, d.stream, d.x64, d.k64.x, n);
         } else {
             hipLaunchKernelGGL((k_cast<double, float>), dim3(d.nblk), dim3(256), 0, d.stream, d.r64, d.k32.b, n);
-            double target = std::max(o.inner_rtol > 0 ? o.inner_rtol : 1e-4, 0.5 * o.rtol / rel);
-            iters += minres<float>(s, d.k32, target, maxit - iters, chk, graph, amg);
+            iters += minres<float>(s, d.k32, target, maxit - iters, chk, graph, amg, its);
             hipLaunchKernelGGL((k_add<float>), dim3(d.nblk), dim3(256), 0, d.stream, d.x64, d.k32.x, n);
         }
+        for (int i = 0; i < ni; ++i) it_tot[i] += its[i];
         ++refin;
-        rel = residual64(s) / bnorm;
+        std::vector<double> rn = residual64(s);
+        for (int i = 0; i < ni; ++i) rel[i] = bn[i] > 0 ? rn[i] / bn[i] : 0.0;
     }
+    s.inst_iterations = it_tot;
+    s.inst_rel = rel;
     if (info) {
         info->iterations = iters;
         info->refinements = refin;
-        info->rel_residual = rel;
+        info->rel_residual = worst();
     }
-    if (rel > o.rtol)
-        throw Error(VEMD_ERR_NOT_CONVERGED, "not converged: relative residual " + std::to_string(rel) + " after " +
+    if (worst() > o.rtol)
+        throw Error(VEMD_ERR_NOT_CONVERGED, "not converged: relative residual " + std::to_string(worst()) + " after " +
                                                 std::to_string(iters) + " iterations");
 }
 
@@ -1179,6 +1285,7 @@ void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, doubl
     auto t1 = std::chrono::steady_clock::now();
     tmp.seconds = std::chrono::duration<double>(t1 - t0).count();
     tmp.iter_ms = tmp.iterations > 0 ? 1e3 * tmp.seconds / tmp.iterations : 0.0;
+    s.last_info = tmp;
     HIPCHK(hipMemcpyAsync(bi.data(), d.x64, L.nvec * sizeof(double), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
     if (sol_ref) s.to_reference(bi.data(), sol_ref, false);
@@ -1208,6 +1315,7 @@ void dev_solve_int(Solver& s, const vemd_solve_opts& o, const void* b_dev, void*
     auto t1 = std::chrono::steady_clock::now();
     tmp.seconds = std::chrono::duration<double>(t1 - t0).count();
     tmp.iter_ms = tmp.iterations > 0 ? 1e3 * tmp.seconds / tmp.iterations : 0.0;
+    s.last_info = tmp;
     if (info) *info = tmp;
     if (code) throw Error(code, err);
 }

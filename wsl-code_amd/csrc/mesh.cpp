@@ -258,6 +258,36 @@ Mesh* mesh_read_vtk(const char* path) {
     return m;
 }
 
+Mesh* mesh_concat(const std::vector<const Mesh*>& parts) {
+    if (parts.empty()) throw Error(VEMD_ERR_ARG, "no meshes to concatenate");
+    std::vector<double> xy;
+    std::vector<int> off{0}, nodes, arc_nodes, cell0{0};
+    std::vector<double> arc_r;
+    int nn = 0;
+    for (const Mesh* p : parts) {
+        if (!p->inst_cell0.empty()) throw Error(VEMD_ERR_ARG, "a part is already a batch");
+        xy.insert(xy.end(), p->xy.begin(), p->xy.end());
+        for (int c = 0; c < p->nc; ++c) {
+            for (int j = p->off[c]; j < p->off[c + 1]; ++j) nodes.push_back(p->nodes[j] + nn);
+            off.push_back((int)nodes.size());
+        }
+        // arcs in a fixed order (sorted keys) so that the union is reproducible
+        std::vector<std::pair<uint64_t, double>> arcs(p->arcs.begin(), p->arcs.end());
+        std::sort(arcs.begin(), arcs.end());
+        for (auto& kv : arcs) {
+            arc_nodes.push_back((int)(kv.first >> 32) + nn);
+            arc_nodes.push_back((int)(kv.first & 0xffffffffu) + nn);
+            arc_r.push_back(kv.second);
+        }
+        nn += p->nn;
+        cell0.push_back(cell0.back() + p->nc);
+    }
+    Mesh* m = mesh_from_arrays(nn, xy.data(), (int)off.size() - 1, off.data(), nodes.data(), (int)arc_r.size(),
+                               arc_nodes.data(), arc_r.data());
+    if (parts.size() > 1) m->inst_cell0 = cell0;
+    return m;
+}
+
 Mesh* mesh_from_arrays(int nn, const double* xy, int nc, const int* off, const int* nodes, int narc,
                        const int* arc_nodes, const double* arc_r) {
     if (nn <= 0 || nc <= 0 || !xy || !off || !nodes) throw Error(VEMD_ERR_ARG, "empty mesh arrays");

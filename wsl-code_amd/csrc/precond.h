@@ -30,11 +30,14 @@ struct MState {
     int itn, maxit, done, pending;
 };
 
-// Sliced ELLPACK: row s * 64 + l has width[s] entries at off[s] + j * 64 + l;
-// padding entries carry value 0 and column 0.
+// Block sliced ELLPACK with square blocks of bs rows: block row s * 64 + l
+// has width[s] block entries; entry j sits at e = off[s] + j * 64 + l with
+// block column col[e] and values val[e * bs * bs + r * bs + c]; padding
+// entries carry zero blocks and column 0.  bs = 1 is sliced ELLPACK.
 template <typename T>
 struct Sell {
-    int64_t nrows = 0;
+    int64_t nrows = 0;        // block rows
+    int bs = 1;
     int nslices = 0;
     const int64_t* off = nullptr;
     const int* width = nullptr;
@@ -67,9 +70,9 @@ template <typename T>
 AmgDev<T>* amg_upload(const Amg& H);
 
 // Enqueues one V-cycle on the stream: x = B b with b = lv[0].b, x = lv[0].x.
-// With st non-null every kernel returns at once when the Krylov state says
-// done (work skipped inside a captured chunk after convergence).
+// With done non-null every kernel returns at once while *done is set (work
+// skipped inside a captured chunk of iterations after convergence).
 template <typename T>
-void amg_vcycle(const AmgDev<T>& D, hipStream_t stream, const MState* st);
+void amg_vcycle(const AmgDev<T>& D, hipStream_t stream, const int* done);
 
 }  // namespace vemd

@@ -27,7 +27,10 @@ struct Solver {
     double amg_setup_s = 0.0;
     DeviceState* dev = nullptr;
     int device = -1;
-    double last_apply_ms = 0.0;
+    // per-instance results of the last solve (iterations, true relative residual)
+    std::vector<int> inst_iterations;
+    std::vector<double> inst_rel;
+    vemd_solve_info last_info{};
     ~Solver();
     // host-side conversions between reference order and internal order
     void to_internal(const double* ref, double* in, bool scale_rows) const;

@@ -49,17 +49,27 @@ Layout layout_build(const Mesh& m, const Setup& S) {
             throw Error(VEMD_ERR_MESH, "cell " + std::to_string(c) + " has more than 10 edges (device limit)");
     }
     if (L.k > 1) throw Error(VEMD_ERR_ARG, "the device path supports k = 0 and k = 1");
-    // spatial order of cells
+    // batch instance of every cell
+    L.ninst = m.ninst();
+    std::vector<int> inst(nc, 0);
+    for (int i = 0; i < L.ninst && !m.inst_cell0.empty(); ++i)
+        for (int c = m.inst_cell0[i]; c < m.inst_cell0[i + 1]; ++c) inst[c] = i;
+    // spatial order of cells (instance-major)
     std::vector<uint64_t> hk = hilbert_keys(m.cx, m.cy);
     std::vector<int> order(nc);
     std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return hk[a] < hk[b] || (hk[a] == hk[b] && a < b); });
-    // store order: valence (largest first, so the longest tiles start first), then spatial
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        if (inst[a] != inst[b]) return inst[a] < inst[b];
+        return hk[a] < hk[b] || (hk[a] == hk[b] && a < b);
+    });
+    // store order: instance, valence (largest first, so the longest tiles
+    // start first), then spatial
     std::vector<int> rank(nc);
     for (int i = 0; i < nc; ++i) rank[order[i]] = i;
     std::vector<int> so(nc);
     std::iota(so.begin(), so.end(), 0);
     std::sort(so.begin(), so.end(), [&](int a, int b) {
+        if (inst[a] != inst[b]) return inst[a] < inst[b];
         if (m.npc(a) != m.npc(b)) return m.npc(a) > m.npc(b);
         return rank[a] < rank[b];
     });
@@ -67,11 +77,14 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     int gpos = 0, tile = 0;
     L.cell_of_gpos.clear();
     L.gpos_of_cell.assign(nc, -1);
+    L.inst_tile0.assign(L.ninst + 1, 0);
+    L.inst_gpos0.assign(L.ninst + 1, 0);
     for (size_t i = 0; i < so.size();) {
         size_t j = i;
-        while (j < so.size() && m.npc(so[j]) == m.npc(so[i])) ++j;
+        while (j < so.size() && m.npc(so[j]) == m.npc(so[i]) && inst[so[j]] == inst[so[i]]) ++j;
         GroupDesc g{};
         g.ne = m.npc(so[i]);
+        g.inst = inst[so[i]];
         g.n = S.sh.nflux(g.ne);
         int cnt = (int)(j - i);
         g.npad = (cnt + TILE - 1) / TILE * TILE;
@@ -89,6 +102,14 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         i = j;
     }
     L.ngroups = (int)L.groups.size();
+    for (const GroupDesc& g : L.groups) {
+        L.inst_tile0[g.inst + 1] = g.tile0 + g.npad / TILE;
+        L.inst_gpos0[g.inst + 1] = g.gpos0 + g.npad;
+    }
+    for (int i = 0; i < L.ninst; ++i) {
+        L.inst_tile0[i + 1] = std::max(L.inst_tile0[i + 1], L.inst_tile0[i]);
+        L.inst_gpos0[i + 1] = std::max(L.inst_gpos0[i + 1], L.inst_gpos0[i]);
+    }
     L.spatial_gpos.resize(nc);
     for (int i = 0; i < nc; ++i) L.spatial_gpos[i] = L.gpos_of_cell[order[i]];
     L.cpad = gpos;
@@ -104,6 +125,10 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     std::sort(eo.begin(), eo.end(), [&](int a, int b) { return efirst[a] < efirst[b] || (efirst[a] == efirst[b] && a < b); });
     L.edge_int.assign(m.ne, 0);
     for (int i = 0; i < m.ne; ++i) L.edge_int[eo[i]] = i;
+    // edges of an instance are contiguous (owners' ranks are instance-major)
+    L.inst_edge0.assign(L.ninst + 1, 0);
+    for (int e = 0; e < m.ne; ++e) L.inst_edge0[inst[m.edge_cells[2 * e]] + 1]++;
+    for (int i = 0; i < L.ninst; ++i) L.inst_edge0[i + 1] += L.inst_edge0[i];
     // internal vector layout
     int kk = L.k + 1;
     L.cbase = (int64_t)kk * m.ne;
@@ -111,6 +136,25 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     L.lam_idx = L.lam ? L.cbase + (int64_t)nslot * L.cpad : -1;
     L.trash_idx = L.cbase + (int64_t)nslot * L.cpad + (L.lam ? 1 : 0);
     L.nvec = L.trash_idx + 1;
+    // Krylov sweep chunks, instance-major
+    if (L.ninst > 1 && L.lam) throw Error(VEMD_ERR_ARG, "a batch needs the Dirichlet-pressure condition");
+    L.chunks.clear();
+    L.pchunks.clear();
+    L.inst_chunk0.assign(L.ninst + 1, 0);
+    auto add = [&](int64_t b, int64_t e, int i, int pslot) {
+        for (int64_t s = b; s < e; s += CHUNK) {
+            if (pslot) L.pchunks.push_back((int)L.chunks.size());
+            L.chunks.push_back(VecChunk{s, (int)std::min<int64_t>(CHUNK, e - s), i, pslot});
+        }
+    };
+    for (int i = 0; i < L.ninst; ++i) {
+        add((int64_t)kk * L.inst_edge0[i], (int64_t)kk * L.inst_edge0[i + 1], i, 0);
+        for (int sl = 0; sl < nslot; ++sl)
+            add(L.cbase + (int64_t)sl * L.cpad + L.inst_gpos0[i], L.cbase + (int64_t)sl * L.cpad + L.inst_gpos0[i + 1], i,
+                sl >= L.ni ? 1 + sl - L.ni : 0);
+        if (i + 1 == L.ninst) add(L.cbase + (int64_t)nslot * L.cpad, L.nvec, i, 0);  // multiplier and trash slot
+        L.inst_chunk0[i + 1] = (int)L.chunks.size();
+    }
     // reference -> internal map
     L.int_of_ref.assign(S.ntot, -1);
     for (int e = 0; e < m.ne; ++e)

@@ -19,8 +19,17 @@ constexpr uint32_t SIDE1 = 0x80000000u;
 // POD shared with the device code.
 struct GroupDesc {
     int ne, n, npad;          // valence, local flux dofs, cells incl. padding
+    int inst;                 // batch instance of the group's cells
     int gpos0, tile0;         // first store position, first global tile
     int64_t o_off, l_off, i_off;  // element offsets of the operator stream, L and index blocks
+};
+
+constexpr int CHUNK = 2048;    // entries per Krylov sweep chunk (256 threads x 8)
+
+struct VecChunk {
+    int64_t start;
+    int len, inst;
+    int pslot;  // 1 + pressure moment of the chunk's slot; 0 outside the pressure block
 };
 
 struct Layout {
@@ -40,6 +49,16 @@ struct Layout {
     std::vector<int64_t> int_of_ref;    // reference dof -> internal slot
     std::vector<uint32_t> idx;          // packed edge-dof indices (SoA per group) with the SIDE1 flag
     int64_t o_len2 = 0, o_len4 = 0, l_len = 0;  // packed sizes (elements) of the FP64 / FP32 streams
+    // Batch instances (one for a plain mesh): each owns a contiguous range of
+    // tiles, store positions and internal edges.
+    int ninst = 1;
+    std::vector<int> inst_tile0, inst_gpos0, inst_edge0;  // ninst + 1 each
+    // Krylov vector sweeps: fixed chunks of at most CHUNK entries, each inside
+    // one instance and one segment; an instance's chunks are contiguous
+    // (inst_chunk0), so per-instance reductions sum a fixed range of partials.
+    std::vector<VecChunk> chunks;
+    std::vector<int> inst_chunk0;
+    std::vector<int> pchunks;  // the chunks of the pressure block
 };
 
 // Builds ordering, grouping, numbering and edge sides from the mesh and setup.

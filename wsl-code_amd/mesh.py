@@ -26,6 +26,17 @@ class Mesh:
         self.num_nodes, self.num_cells, self.num_edges, self._ntot, self.num_arcs = (int(v) for v in c)
 
     @classmethod
+    def concat(cls, meshes):
+        """Disjoint union of meshes: a batch of independent instances."""
+        arr = (c_vp * len(meshes))(*[m.h.value for m in meshes])
+        h = c_vp()
+        check(lib().vemd_mesh_concat(len(meshes), arr, ctypes.byref(h)))
+        out = cls(h)
+        out.instance_cells = np.cumsum([0] + [m.num_cells for m in meshes])
+        out.parts = list(meshes)
+        return out
+
+    @classmethod
     def read(cls, path, vtk=False):
         h = c_vp()
         check(lib().vemd_mesh_read(str(path).encode(), 1 if vtk else 0, ctypes.byref(h)))

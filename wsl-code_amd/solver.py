@@ -79,6 +79,36 @@ class Solver:
         return x, dict(iterations=info.iterations, refinements=info.refinements, rel_residual=info.rel_residual,
                        seconds=info.seconds)
 
+    def instance_vectors(self, x):
+        """Splits a reference-order vector of a batch into the instances'
+        own reference-order vectors (edge moments r * E + e, internal flux
+        dofs, pressure moments; edges and cells of instance i follow those
+        of instances < i)."""
+        parts = self.mesh.parts
+        kk = self.k + 1
+        q = (self.k + 1) * (self.k + 2) // 2
+        ni = (q - 1) + (q - kk)
+        E = self.mesh.num_edges
+        C = self.mesh.num_cells
+        nflux = kk * E + ni * C
+        out, e0, c0 = [], 0, 0
+        for m in parts:
+            e, c = m.num_edges, m.num_cells
+            v = [x[r * E + e0:r * E + e0 + e] for r in range(kk)]
+            v.append(x[kk * E + ni * c0:kk * E + ni * (c0 + c)])
+            v.append(x[nflux + q * c0:nflux + q * (c0 + c)])
+            out.append(np.concatenate(v))
+            e0 += e
+            c0 += c
+        return out
+
+    def batch_info(self):
+        """Per-instance iterations and true relative residuals of the last solve."""
+        n = len(self.mesh.instance_cells) - 1 if hasattr(self.mesh, "instance_cells") else 1
+        arr = (SolveInfo * n)()
+        check(lib().vemd_batch_info(self.h, n, arr))
+        return [dict(iterations=a.iterations, rel_residual=a.rel_residual) for a in arr]
+
     def adjoint_theta(self, x, dldx, opts=None):
         opts = opts or SolveOptions()
         x = np.ascontiguousarray(x, np.float64)
