@@ -178,6 +178,171 @@ def adjoint_bench(pkg, torch, cells, seed, ndir):
             "fd_rel_err_median": float(np.median(errs)) if errs else None}
 
 
+def batch_bench(pkg, torch, ninst, cmin, cmax, seed):
+    """Batched solves: ninst independent instances on seeded Lloyd Voronoi
+    disks with sizes log-uniform in [cmin, cmax] cells, each with its own
+    log-normal permeability (sigma in [0.5, 2], correlation length in
+    [0.02, 0.1]) and manufactured right-hand side, solved together (one
+    operator store, one captured Krylov loop, per-instance recurrences) in
+    mixed precision to 1e-8; compared with the FP64 batched solve."""
+    from wsl_code_amd.fields import grf_theta
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    meshes, thetas = [], []
+    for i in range(ninst):
+        c = int(np.exp(rng.uniform(np.log(cmin), np.log(cmax))))
+        m = pkg.generate_mesh(c, domain=pkg.DOMAIN_DISK, lloyd_iters=20, seed=seed * 1000 + i, min_edge_frac=0.05)
+        a = m.arrays()
+        thetas.append(grf_theta(a["cx"], a["cy"], sigma=rng.uniform(0.5, 2.0), ell=rng.uniform(0.02, 0.1),
+                                seed=seed * 1000 + i + 500000, features=256))
+        meshes.append(m)
+    u = pkg.Mesh.concat(meshes)
+    t1 = time.time()
+    s = pkg.Solver(u, k=1, map_kind=pkg.MAP_ARCS, solution=pkg.SOL_SINSIN, bc=pkg.BC_DIRICHLET_P,
+                   theta=np.concatenate(thetas))
+    t2 = time.time()
+    s.upload(0)
+    log("batch: %d instances, %d cells, %d unknowns; meshes+theta %.1fs, setup %.1fs, upload %.1fs"
+        % (ninst, u.num_cells, s.ntot, t1 - t0, t2 - t1, time.time() - t2))
+    L = pkg.lib()
+    x = torch.empty(s.nvec, dtype=torch.float64, device="cuda")
+
+    def solve(prec):
+        o = pkg.SolveOptions(precision=prec, rtol=1e-8, inner_rtol=1e-4, check_every=16).c()
+        info = pkg._lib.SolveInfo()
+        rc = L.vemd_solve_device(s.h, ctypes.byref(o), None, ctypes.c_void_p(x.data_ptr()), ctypes.byref(info))
+        if rc:
+            raise RuntimeError(L.vemd_last_error().decode())
+        return info, s.batch_info()
+
+    solve(pkg.MIXED)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    im, pm = solve(pkg.MIXED)
+    tm = time.time() - t0
+    xm = x.clone()
+    t0 = time.time()
+    i6, p6 = solve(pkg.FP64)
+    t6 = time.time() - t0
+    its = [p["iterations"] for p in pm]
+    log("batch mixed %.3fs (max %d its, median %d), fp64 %.3fs" % (tm, max(its), int(np.median(its)), t6))
+    # per-instance flux-error difference mixed vs FP64 (physical measure)
+    hm, h64 = xm.cpu().numpy(), x.cpu().numpy()
+    rm, r64 = np.zeros(s.ntot), np.zeros(s.ntot)
+    L.vemd_to_reference(s.h, hm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        rm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0)
+    L.vemd_to_reference(s.h, h64.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        r64.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0)
+    diffs = []
+    for k, (a, b) in enumerate(zip(s.instance_vectors(rm), s.instance_vectors(r64))):
+        if k % max(1, ninst // 16) == 0:  # a sample of the instances (host error integration)
+            sk = pkg.Solver(meshes[k], k=1, map_kind=pkg.MAP_ARCS, solution=pkg.SOL_SINSIN,
+                            bc=pkg.BC_DIRICHLET_P, theta=thetas[k])
+            diffs.append(abs(sk.errors(a)["u_phys"] - sk.errors(b)["u_phys"]))
+    return {"metric": "batched time-to-solve at 1e-8 rel. residual (all instances), mixed vs FP64", "value": round(tm, 4),
+            "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp32 iterations, fp64 refinement",
+            "data": "synthetic: seeded Voronoi disks, per-instance GRF log-permeability, manufactured sin*sin data",
+            "config": {"workload": "%d instances, cells log-uniform in [%d, %d], k=1, Dirichlet p" % (ninst, cmin, cmax),
+                       "cells": u.num_cells, "unknowns": s.ntot},
+            "fp64_seconds": round(t6, 4), "speedup_mixed_vs_fp64": round(t6 / tm, 3),
+            "iterations": {"max": max(its), "median": int(np.median(its)), "min": min(its)},
+            "max_rel_residual": max(p["rel_residual"] for p in pm),
+            "flux_err_diff_max_sampled": max(diffs), "instances_sampled_for_errors": len(diffs)}
+
+
+def multi_bench(pkg, torch, cells, seed, nrhs):
+    """32 right-hand sides on one annulus mesh: seeded random pressure sources
+    and scaled manufactured data, solved simultaneously (FP32 MFMA element
+    apply and MINRES inside FP64 refinement to 1e-10); every column compared
+    with an FP64 single solve; the MFMA apply timed alone."""
+    mesh, arr, theta, s = build(pkg, cells, seed)
+    s.upload(0)
+    L = pkg.lib()
+    rng = np.random.default_rng(seed + 11)
+    base = s.rhs()
+    Bi = np.zeros((nrhs, s.nvec))
+    for j in range(nrhs):
+        if j % 2 == 0:
+            b = base * (1.0 + 0.05 * j)
+        else:
+            b = np.zeros(s.ntot)
+            b[s.nflux:] = rng.standard_normal(s.ntot - s.nflux)
+        L.vemd_to_internal(s.h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           Bi[j].ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1)
+    Bd = torch.from_numpy(Bi).cuda()
+    Xd = torch.zeros_like(Bd)
+    info = (pkg._lib.SolveInfo * nrhs)()
+
+    def msolve():
+        o = pkg.SolveOptions(precision=pkg.MIXED, rtol=1e-10, inner_rtol=1e-4, check_every=16).c()
+        rc = L.vemd_solve_multi_device(s.h, ctypes.byref(o), nrhs, ctypes.c_void_p(Bd.data_ptr()),
+                                       ctypes.c_void_p(Xd.data_ptr()), info)
+        if rc:
+            raise RuntimeError(L.vemd_last_error().decode())
+
+    msolve()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    msolve()
+    tm = time.time() - t0
+    its = [info[j].iterations for j in range(nrhs)]
+    log("multi-rhs: %d columns %.3fs (max %d its), max rel residual %.2e"
+        % (nrhs, tm, max(its), max(info[j].rel_residual for j in range(nrhs))))
+    # FP64 single solves of every column
+    x1 = torch.empty(s.nvec, dtype=torch.float64, device="cuda")
+    t0 = time.time()
+    errs = []
+    for j in range(nrhs):
+        o = pkg.SolveOptions(precision=pkg.FP64, rtol=1e-10, check_every=16).c()
+        inf1 = pkg._lib.SolveInfo()
+        rc = L.vemd_solve_device(s.h, ctypes.byref(o), ctypes.c_void_p(Bd[j].data_ptr()), ctypes.c_void_p(x1.data_ptr()),
+                                 ctypes.byref(inf1))
+        if rc:
+            raise RuntimeError(L.vemd_last_error().decode())
+        errs.append(float(torch.linalg.norm(Xd[j] - x1) / torch.linalg.norm(x1)))
+    t64 = time.time() - t0
+    # MFMA apply alone (the kernel of the block apply)
+    X = torch.randn(s.nvec, 32, dtype=torch.float32, device="cuda")
+    Y, Y1 = torch.empty_like(X), torch.zeros_like(X)
+    st = torch.cuda.current_stream()
+    args = [ctypes.c_void_p(t) for t in (X.data_ptr(), Y.data_ptr(), Y1.data_ptr(), st.cuda_stream)]
+    for _ in range(3):
+        L.vemd_apply_multi_device(s.h, *args)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(10):
+        L.vemd_apply_multi_device(s.h, *args)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    valences = np.bincount(np.diff(arr["cell_off"]))
+    q = 3
+    byt, fl_alg, fl_mfma = 0, 0, 0
+    for ne, cnt in enumerate(valences):
+        if cnt == 0:
+            continue
+        n = 2 * ne + 3
+        byt += cnt * (4 * (n * (n + 1) // 2 + q * n + 1) + 4 * n)
+        fl_alg += cnt * 32 * (2 * n * n + 4 * q * n)
+        fl_mfma += cnt * ((n + q + 1) // 2) * 2 * 32 * 32 * 2
+    byt += 4 * 2 * s.ntot * 32
+    log("MFMA apply %.3f ms: %.0f GB/s algorithmic, %.1f TFLOP/s algorithmic, %.1f TFLOP/s issued on MFMA"
+        % (ms, byt / ms / 1e6, fl_alg / ms / 1e9, fl_mfma / ms / 1e9))
+    return {"metric": "multi-RHS time-to-solve at 1e-10 rel. residual (32 RHS), FP32 MFMA block apply",
+            "value": round(tm, 4), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "fp32 MFMA iterations, fp64 refinement", "data": "synthetic annulus, GRF log-permeability",
+            "config": {"workload": "annulus, %d-cell Lloyd Voronoi, k=1, %d right-hand sides" % (mesh.num_cells, nrhs),
+                       "cells": mesh.num_cells, "unknowns_per_rhs": s.ntot},
+            "iterations_max": max(its), "fp64_single_solves_seconds": round(t64, 3),
+            "rel_diff_vs_fp64_max": max(errs),
+            "mfma_apply": {"ms": round(ms, 4), "bytes": int(byt), "achieved_GBps": round(byt / ms / 1e6, 1),
+                           "hbm_frac": round(byt / ms / 1e6 / HBM_PEAK_GBS, 4),
+                           "flops_algorithmic": int(fl_alg), "flops_mfma_issued": int(fl_mfma),
+                           "mfma_TFLOPs_issued": round(fl_mfma / ms / 1e9, 2),
+                           "mfma_frac_of_fp32_peak": round(fl_mfma / ms / 1e9 / 157.3, 4)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,7 +351,7 @@ def main():
     ap.add_argument("--cells", type=int, default=1000000)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--apply-reps", type=int, default=50)
-    ap.add_argument("--mode", choices=["solve", "apply", "adjoint"], default="solve",
+    ap.add_argument("--mode", choices=["solve", "apply", "adjoint", "batch", "multi"], default="solve",
                     help="apply: only the operator-apply timing (for profiling runs)")
     ap.add_argument("--cpu-cells", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -194,6 +359,9 @@ def main():
                     help="MINRES preconditioner: diagonal (the reference's) or the multigrid pressure cycle")
     ap.add_argument("--check-every", type=int, default=16)
     ap.add_argument("--fd-directions", type=int, default=16)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--inst-cells", type=int, nargs=2, default=[10000, 200000])
+    ap.add_argument("--nrhs", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; a figure at %d GPUs is not produced (one process per GPU "
@@ -205,6 +373,13 @@ def main():
     if args.mode == "adjoint":
         print(json.dumps(adjoint_bench(pkg, torch, args.cells if args.cells != 1000000 else 250000, args.seed,
                                        args.fd_directions)))
+        return
+    if args.mode == "batch":
+        print(json.dumps(batch_bench(pkg, torch, args.instances, args.inst_cells[0], args.inst_cells[1], args.seed)))
+        return
+    if args.mode == "multi":
+        print(json.dumps(multi_bench(pkg, torch, args.cells if args.cells != 1000000 else 2000000, args.seed,
+                                     args.nrhs)))
         return
     mesh, arr, theta, s = build(pkg, args.cells, args.seed)
     t0 = time.time()

@@ -222,6 +222,23 @@ vemd_status vemd_apply_device(vemd_solver s, int precision, const void* x_dev, v
 vemd_status vemd_apply_device_split(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* y1_dev,
                                     void* stream);
 
+/* Simultaneous solves with 1..32 right-hand sides: b_dev and x_dev hold nrhs
+ * columns of sizes[3] doubles each (column-major, internal layout, as
+ * vemd_solve_device).  The Krylov iterations run in FP32 on 32 interleaved
+ * columns with an MFMA element apply (precision must be VEMD_MIXED), inside
+ * FP64 iterative refinement per column to opts->rtol; info receives nrhs
+ * entries.  Dirichlet-pressure problems on one mesh.  No reference
+ * counterpart. */
+vemd_status vemd_solve_multi_device(vemd_solver s, const vemd_solve_opts* opts, int nrhs, const void* b_dev,
+                                    void* x_dev, vemd_solve_info* info);
+
+/* The MFMA element apply of the multi-right-hand-side solves on 32
+ * interleaved FP32 columns (entry i of column j at i * 32 + j, sizes[3] * 32
+ * floats): y_dev receives the first sides of the edge dofs and all other
+ * rows, y1_dev (also sizes[3] * 32 floats, leading edge-dof block used) the
+ * second sides; zero y1_dev once.  On the given stream. */
+vemd_status vemd_apply_multi_device(vemd_solver s, const void* x_dev, void* y_dev, void* y1_dev, void* stream);
+
 /* The problem's right-hand side in reference order (ntot values), after the
  * essential-boundary lift. */
 vemd_status vemd_rhs(vemd_solver s, double* rhs);

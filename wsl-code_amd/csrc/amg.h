@@ -26,7 +26,7 @@ struct AmgOptions {
     double strength = 0.08;    // strength-of-connection threshold on the finest level (halved per level)
     int degree0 = 2;           // Chebyshev steps before and after the coarse correction, finest level
     int degree = 2;            // the same on the coarser levels
-    double cheb_ratio = 20.0;  // smoothing interval [lmax / ratio, lmax] of D^-1 A
+    double cheb_ratio = 7.0;   // smoothing interval [lmax / ratio, lmax] of D^-1 A
 };
 
 struct AmgLevel {

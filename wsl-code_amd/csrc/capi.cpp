@@ -59,6 +59,11 @@ void Solver::build_amg() {
     amg.reset();
     if (setup->desc.bc != VEMD_BC_DIRICHLET_P) return;
     auto t0 = std::chrono::steady_clock::now();
+    // tuning overrides: smoothing degrees, Chebyshev interval ratio, strength threshold
+    if (const char* e = std::getenv("VEMD_MG_DEGREE0")) amg_opt.degree0 = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VEMD_MG_DEGREE")) amg_opt.degree = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VEMD_MG_RATIO")) amg_opt.cheb_ratio = std::max(2.0, std::atof(e));
+    if (const char* e = std::getenv("VEMD_MG_STRENGTH")) amg_opt.strength = std::max(0.0, std::atof(e));
     const Layout& L = layout;
     Csr S0 = schur_pressure(*mesh, *setup, L, minv);
     const int64_t n = S0.nrows;
@@ -326,6 +331,22 @@ vemd_status vemd_apply_device_split(vemd_solver sh, int precision, const void* x
     REQUIRE(sh && x && y && y1 && x != y && x != y1 && y != y1, "null or aliased argument");
     REQUIRE(precision == VEMD_FP64 || precision == VEMD_FP32, "precision must be VEMD_FP64 or VEMD_FP32");
     dev_apply_int(*sh->s, precision, x, y, y1, stream);
+    API_END
+}
+
+vemd_status vemd_solve_multi_device(vemd_solver sh, const vemd_solve_opts* o, int nrhs, const void* b, void* x,
+                                    vemd_solve_info* info) {
+    API_BEGIN
+    REQUIRE(sh && o && b, "null argument");
+    REQUIRE(o->rtol > 0 && o->rtol < 1, "rtol must be in (0, 1)");
+    dev_solve_multi(*sh->s, *o, nrhs, (const double*)b, (double*)x, info);
+    API_END
+}
+
+vemd_status vemd_apply_multi_device(vemd_solver sh, const void* x, void* y, void* y1, void* stream) {
+    API_BEGIN
+    REQUIRE(sh && x && y && y1 && x != y && x != y1 && y != y1, "null or aliased argument");
+    dev_apply_multi(*sh->s, x, y, y1, stream);
     API_END
 }
 

@@ -13,8 +13,7 @@
 #include <type_traits>
 #include <utility>
 
-#include "precond.h"
-#include "solver.h"
+#include "devstate.h"
 
 
 namespace vemd {
@@ -344,20 +343,29 @@ __global__ void k_lam_row(const double* plam, int ntiles, T* y, int64_t lam_idx)
     if (threadIdx.x == 0) y[lam_idx] = (T)s;
 }
 
-// Sums the partials p[b..e) in a fixed order with one block of 256 threads.
-__device__ double block_sum_range(const double* p, int b, int e) {
-    __shared__ double sh[256];
-    double s = 0.0;
-    for (int i = b + (int)threadIdx.x; i < e; i += 256) s += p[i];
-    sh[threadIdx.x] = s;
+// Sums of the partials a[b..e) and c[b..e) (c may be null) in a fixed order
+// by one block of FIN threads; every thread gets the results.
+constexpr int FIN = 1024;
+__device__ void block_sums(const double* a, const double* c, int b, int e, double& sa, double& sc) {
+    __shared__ double sh[2][FIN];
+    double x = 0.0, y = 0.0;
+    for (int i = b + (int)threadIdx.x; i < e; i += FIN) {
+        x += a[i];
+        if (c) y += c[i];
+    }
+    sh[0][threadIdx.x] = x;
+    sh[1][threadIdx.x] = y;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    for (int o = FIN / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+        }
         __syncthreads();
     }
-    double r = sh[0];
+    sa = sh[0][0];
+    sc = sh[1][0];
     __syncthreads();
-    return r;
 }
 
 // The Krylov recurrences run per batch instance: every instance has its own
@@ -371,8 +379,8 @@ __global__ void k_finA(MState* sts, const double* pdot, const double* plam, cons
                        const T* Z0, const T* Z1, int64_t lam_idx) {
     MState* st = sts + blockIdx.x;
     if (st->done) return;
-    double zaz = block_sum_range(pdot, inst_tile0[blockIdx.x], inst_tile0[blockIdx.x + 1]);
-    double yl = lam_idx >= 0 ? block_sum_range(plam, inst_tile0[blockIdx.x], inst_tile0[blockIdx.x + 1]) : 0.0;
+    double zaz, yl;
+    block_sums(pdot, lam_idx >= 0 ? plam : nullptr, inst_tile0[blockIdx.x], inst_tile0[blockIdx.x + 1], zaz, yl);
     if (threadIdx.x == 0) {
         if (lam_idx >= 0) {
             const T* z = (st->itn & 1) ? Z1 : Z0;
@@ -508,8 +516,11 @@ __global__ void k_finB(MState* sts, const double* p1, const double* p2, const do
     MState* st = sts + blockIdx.x;
     if (st->done) return;
     const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
-    double b2 = block_sum_range(p1, c0, c1) + (useq ? block_sum_range(q1, c0, c1) : 0.0);
-    double rho = block_sum_range(p2, c0, c1) + (useq ? block_sum_range(q2, c0, c1) : 0.0);
+    double b2, rho, qb2 = 0.0, qrho = 0.0;
+    block_sums(p1, p2, c0, c1, b2, rho);
+    if (useq) block_sums(q1, q2, c0, c1, qb2, qrho);
+    b2 += qb2;
+    rho += qrho;
     if (threadIdx.x == 0) {
         double beta_new = std::sqrt(b2 > 0.0 ? b2 : 0.0);
         st->rho = rho;
@@ -578,7 +589,10 @@ __global__ __launch_bounds__(256) void k_init(UpdArgs<T> a, const T* b) {
 __global__ void k_finInit(MState* sts, const double* p1, const double* q1, const int* inst_chunk0, int useq,
                           const double* rtol, int maxit) {
     const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
-    double b2 = block_sum_range(p1, c0, c1) + (useq ? block_sum_range(q1, c0, c1) : 0.0);
+    double b2, qb2 = 0.0, unused;
+    block_sums(p1, nullptr, c0, c1, b2, unused);
+    if (useq) block_sums(q1, nullptr, c0, c1, qb2, unused);
+    b2 += qb2;
     if (threadIdx.x == 0) {
         double beta1 = std::sqrt(b2 > 0.0 ? b2 : 0.0);
         MState s;
@@ -629,7 +643,8 @@ __global__ __launch_bounds__(256) void k_residual(const VecChunk* chunks, const 
 
 // Per-instance sums of chunk partials.
 __global__ void k_inst_sum(const double* p, const int* inst_chunk0, double* out) {
-    double s = block_sum_range(p, inst_chunk0[blockIdx.x], inst_chunk0[blockIdx.x + 1]);
+    double s, unused;
+    block_sums(p, nullptr, inst_chunk0[blockIdx.x], inst_chunk0[blockIdx.x + 1], s, unused);
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
@@ -713,89 +728,6 @@ __global__ __launch_bounds__(64) void k_adjoint(ApplyArgs<T> a, const T* lam, co
 // Host side of the device path.
 // ---------------------------------------------------------------------------
 
-template <typename T>
-struct DevOps {
-    T* O = nullptr;
-    T* L = nullptr;
-    T* scale = nullptr;
-    T* minv = nullptr;
-    GroupDesc* groups = nullptr;
-};
-
-template <typename T>
-struct DevKrylov {
-    T* R[2] = {nullptr, nullptr};
-    T* Z[2] = {nullptr, nullptr};
-    T* WB[2] = {nullptr, nullptr};
-    T* x = nullptr;
-    T* Az = nullptr;
-    T* Az1 = nullptr;
-    T* b = nullptr;
-    hipGraphExec_t graph = nullptr;
-    int graph_iters = 0;
-    bool graph_amg = false;
-};
-
-struct DeviceState {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    DevOps<double> o64;
-    DevOps<float> o32;
-    uint32_t* idx = nullptr;
-    int* tile_group = nullptr;
-    int* cell_of_gpos = nullptr;
-    int ntiles = 0;
-    double* pdot = nullptr;
-    double* plam = nullptr;
-    double* p1 = nullptr;       // per-chunk partials of the Krylov sweeps
-    double* p2 = nullptr;
-    int nblk = 0;               // grid of the plain elementwise kernels
-    int ninst = 1;
-    VecChunk* chunks = nullptr;
-    int nchunks = 0;
-    int* pchunks = nullptr;
-    int npchunks = 0;
-    int* inst_tile0 = nullptr;
-    int* inst_chunk0 = nullptr;
-    MState* st = nullptr;       // per instance
-    MState* st_host = nullptr;  // pinned copy
-    int* all_done = nullptr;
-    double* rtol = nullptr;     // per-instance inner tolerances
-    double* rtol_host = nullptr;  // pinned
-    double* dsum = nullptr;     // per-instance sums
-    double* hsum = nullptr;     // pinned
-    DevKrylov<double> k64;
-    DevKrylov<float> k32;
-    double* x64 = nullptr;  // outer solution
-    double* r64 = nullptr;  // outer residual
-    double* y64 = nullptr;
-    double* b64 = nullptr;
-    double* bprob = nullptr;  // the problem's own right-hand side, internal order
-    float* r32 = nullptr;
-    double* y1_64 = nullptr;  // side-1 scratch of the plain applies
-    float* y1_32 = nullptr;
-    AmgDev<double>* amg64 = nullptr;  // multigrid pressure preconditioner (when built)
-    AmgDev<float>* amg32 = nullptr;
-    double* q1 = nullptr;  // per-chunk partials of the pressure-block dots (zero outside it)
-    double* q2 = nullptr;
-    std::vector<void*> allocs;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-};
-
-static void* dalloc(DeviceState& d, size_t bytes) {
-    void* p = nullptr;
-    HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
-    d.allocs.push_back(p);
-    return p;
-}
-
-template <typename T>
-static T* dupload(DeviceState& d, const std::vector<T>& v) {
-    T* p = (T*)dalloc(d, v.size() * sizeof(T));
-    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return p;
-}
-
 int device_count() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -855,6 +787,7 @@ static void upload_theta(Solver& s) {
     d.amg32 = nullptr;
     destroy_graph(d.k64);
     destroy_graph(d.k32);
+    multi_invalidate(d);
     if (s.amg) {
         d.amg64 = amg_upload<double>(*s.amg);
         d.amg32 = amg_upload<float>(*s.amg);
@@ -943,6 +876,7 @@ void dev_free(Solver& s) {
     (void)hipSetDevice(d.device);
     destroy_graph(d.k64);
     destroy_graph(d.k32);
+    multi_free(d);
     delete d.amg64;
     delete d.amg32;
     for (void* p : d.allocs) (void)hipFree(p);
@@ -1109,7 +1043,7 @@ static void enqueue_iteration(Solver& s, DevKrylov<T>& k, bool amg) {
     const Layout& L = s.layout;
     ApplyArgs<T> a = make_args<T>(s, nullptr, k.Az, k.Az1, d.st, k.Z[0], k.Z[1], true);
     launch_apply(s, a);
-    hipLaunchKernelGGL((k_finA<T>), dim3(d.ninst), dim3(256), 0, d.stream, d.st, d.pdot, d.plam, d.inst_tile0, k.Az,
+    hipLaunchKernelGGL((k_finA<T>), dim3(d.ninst), dim3(FIN), 0, d.stream, d.st, d.pdot, d.plam, d.inst_tile0, k.Az,
                        k.Z[0], k.Z[1], L.lam ? L.lam_idx : (int64_t)-1);
     UpdArgs<T> u = upd_args(s, k, amg);
     hipLaunchKernelGGL((k_update<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
@@ -1119,7 +1053,7 @@ static void enqueue_iteration(Solver& s, DevKrylov<T>& k, bool amg) {
         hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
                            (const T*)M.lv[0].x, d.q1, d.q2, 0);
     }
-    hipLaunchKernelGGL(k_finB, dim3(d.ninst), dim3(256), 0, d.stream, d.st, d.p1, d.p2, d.q1, d.q2, d.inst_chunk0,
+    hipLaunchKernelGGL(k_finB, dim3(d.ninst), dim3(FIN), 0, d.stream, d.st, d.p1, d.p2, d.q1, d.q2, d.inst_chunk0,
                        amg ? 1 : 0);
     hipLaunchKernelGGL(k_alldone, dim3(1), dim3(256), 0, d.stream, d.st, d.ninst, d.all_done);
 }
@@ -1156,7 +1090,7 @@ static int minres(Solver& s, DevKrylov<T>& k, const std::vector<double>& rtol, i
         hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
                            (const T*)M.lv[0].x, d.q1, d.q2, 1);
     }
-    hipLaunchKernelGGL(k_finInit, dim3(d.ninst), dim3(256), 0, d.stream, d.st, d.p1, d.q1, d.inst_chunk0, amg ? 1 : 0,
+    hipLaunchKernelGGL(k_finInit, dim3(d.ninst), dim3(FIN), 0, d.stream, d.st, d.p1, d.q1, d.inst_chunk0, amg ? 1 : 0,
                        (const double*)d.rtol, maxit);
     hipLaunchKernelGGL(k_alldone, dim3(1), dim3(256), 0, d.stream, d.st, d.ninst, d.all_done);
     HIPCHK(hipGetLastError());
@@ -1197,7 +1131,7 @@ static std::vector<double> residual64(Solver& s) {
     hipLaunchKernelGGL(k_residual, dim3(d.nchunks), dim3(256), 0, d.stream, (const VecChunk*)d.chunks,
                        (const double*)d.b64, (const double*)d.y64, (const double*)d.y1_64, s.layout.cbase, d.r64,
                        d.p1);
-    hipLaunchKernelGGL(k_inst_sum, dim3(d.ninst), dim3(256), 0, d.stream, (const double*)d.p1,
+    hipLaunchKernelGGL(k_inst_sum, dim3(d.ninst), dim3(FIN), 0, d.stream, (const double*)d.p1,
                        (const int*)d.inst_chunk0, d.dsum);
     HIPCHK(hipMemcpyAsync(d.hsum, d.dsum, d.ninst * sizeof(double), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));

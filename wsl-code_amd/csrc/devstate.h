@@ -1,0 +1,107 @@
+// Device-side state of a solver (shared by the HIP translation units):
+// operator store, Krylov vectors, per-instance recurrences and the
+// multigrid hierarchy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "precond.h"
+#include "solver.h"
+
+namespace vemd {
+
+struct MultiState;
+
+template <typename T>
+struct DevOps {
+    T* O = nullptr;
+    T* L = nullptr;
+    T* scale = nullptr;
+    T* minv = nullptr;
+    GroupDesc* groups = nullptr;
+};
+
+template <typename T>
+struct DevKrylov {
+    T* R[2] = {nullptr, nullptr};
+    T* Z[2] = {nullptr, nullptr};
+    T* WB[2] = {nullptr, nullptr};
+    T* x = nullptr;
+    T* Az = nullptr;
+    T* Az1 = nullptr;
+    T* b = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+    bool graph_amg = false;
+};
+
+struct DeviceState {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevOps<double> o64;
+    DevOps<float> o32;
+    uint32_t* idx = nullptr;
+    int* tile_group = nullptr;
+    int* cell_of_gpos = nullptr;
+    int ntiles = 0;
+    double* pdot = nullptr;
+    double* plam = nullptr;
+    double* p1 = nullptr;       // per-chunk partials of the Krylov sweeps
+    double* p2 = nullptr;
+    int nblk = 0;               // grid of the plain elementwise kernels
+    int ninst = 1;
+    VecChunk* chunks = nullptr;
+    int nchunks = 0;
+    int* pchunks = nullptr;
+    int npchunks = 0;
+    int* inst_tile0 = nullptr;
+    int* inst_chunk0 = nullptr;
+    MState* st = nullptr;       // per instance
+    MState* st_host = nullptr;  // pinned copy
+    int* all_done = nullptr;
+    double* rtol = nullptr;     // per-instance inner tolerances
+    double* rtol_host = nullptr;  // pinned
+    double* dsum = nullptr;     // per-instance sums
+    double* hsum = nullptr;     // pinned
+    DevKrylov<double> k64;
+    DevKrylov<float> k32;
+    double* x64 = nullptr;  // outer solution
+    double* r64 = nullptr;  // outer residual
+    double* y64 = nullptr;
+    double* b64 = nullptr;
+    double* bprob = nullptr;  // the problem's own right-hand side, internal order
+    float* r32 = nullptr;
+    double* y1_64 = nullptr;  // side-1 scratch of the plain applies
+    float* y1_32 = nullptr;
+    AmgDev<double>* amg64 = nullptr;  // multigrid pressure preconditioner (when built)
+    AmgDev<float>* amg32 = nullptr;
+    double* q1 = nullptr;  // per-chunk partials of the pressure-block dots (zero outside it)
+    double* q2 = nullptr;
+    std::vector<void*> allocs;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MultiState* multi = nullptr;  // multi-right-hand-side solver state (allocated on first use)
+};
+
+inline void* dalloc(DeviceState& d, size_t bytes) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+    d.allocs.push_back(p);
+    return p;
+}
+
+template <typename T>
+inline T* dupload(DeviceState& d, const std::vector<T>& v) {
+    T* p = (T*)dalloc(d, v.size() * sizeof(T));
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+
+// Multi-right-hand-side path (MFMA element apply, simultaneous MINRES).
+void multi_free(DeviceState& d);
+void multi_invalidate(DeviceState& d);  // drops the captured iteration graph
+
+}  // namespace vemd

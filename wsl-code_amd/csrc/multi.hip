@@ -1,0 +1,761 @@
+// Multi-right-hand-side path on gfx950: up to 32 right-hand sides solved at
+// once.  The element apply multiplies each cell's saddle-point block
+// [[s A, -W^T], [-W, 0]] (m = n + q rows, m <= 26) by the cell's 32-column
+// slice of the Krylov vectors with v_mfma_f32_32x32x2f32 (one wavefront per
+// cell, FP32 products and accumulation); the Krylov vectors are interleaved
+// ([entry][32] floats, 128 bytes per entry) so that every gathered row is one
+// cache line.  The 32 MINRES recurrences run side by side with their own
+// scalars and stopping tests, preconditioned by the same block-diagonal
+// preconditioner (flux diagonal and a 32-column multigrid cycle), inside
+// FP64 iterative refinement per column.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "devstate.h"
+
+namespace vemd {
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+constexpr int NR = MCOLS;     // right-hand-side columns
+constexpr int OPMAX = 352;    // floats of the largest cell block (n = 23: A 276 + W 69, padded to 4)
+
+struct MultiState {
+    float* op = nullptr;            // per cell: packed A (lower triangle) then W (q x n), padded to 4 floats
+    int64_t* gop = nullptr;         // first float of every group's block of cells
+    int nwaves = 0;
+    int64_t cells_per_wave = 0;
+    float *R[2] = {nullptr, nullptr}, *Z[2] = {nullptr, nullptr}, *WB[2] = {nullptr, nullptr};
+    float *x = nullptr, *Az = nullptr, *Az1 = nullptr, *b = nullptr;  // [entry][NR]
+    double *x64 = nullptr, *r64 = nullptr, *y64 = nullptr, *y1_64 = nullptr;  // [NR][nvec]
+    MState* st = nullptr;
+    MState* st_host = nullptr;
+    double* rtol = nullptr;
+    double* rtol_host = nullptr;
+    double *pw = nullptr, *p1 = nullptr, *p2 = nullptr, *q1 = nullptr, *q2 = nullptr;
+    double *part = nullptr, *dsum = nullptr, *hsum = nullptr;
+    int* all_done = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+    bool graph_amg = false;
+    std::vector<void*> allocs;
+};
+
+static void* malloc_m(MultiState& m, size_t bytes) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+    m.allocs.push_back(p);
+    return p;
+}
+
+void multi_invalidate(DeviceState& d) {
+    if (!d.multi) return;
+    if (d.multi->graph) (void)hipGraphExecDestroy(d.multi->graph);
+    d.multi->graph = nullptr;
+}
+
+void multi_free(DeviceState& d) {
+    if (!d.multi) return;
+    MultiState& m = *d.multi;
+    if (m.graph) (void)hipGraphExecDestroy(m.graph);
+    for (void* p : m.allocs) (void)hipFree(p);
+    if (m.st_host) (void)hipHostFree(m.st_host);
+    if (m.rtol_host) (void)hipHostFree(m.rtol_host);
+    if (m.hsum) (void)hipHostFree(m.hsum);
+    delete d.multi;
+    d.multi = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA element apply
+// ---------------------------------------------------------------------------
+
+struct MrArgs {
+    const GroupDesc* groups;
+    const int* tile_group;
+    const int* cell_of_gpos;
+    const float* op;
+    const int64_t* gop;
+    const uint32_t* idx;
+    const float* scale;
+    const float* X[2];        // input; with st, the buffer of each column's iteration parity
+    float* Y;
+    float* Y1;                // second sides of the edge dofs
+    int64_t cbase, cpad;
+    int64_t cells_per_wave;
+    double* pw;               // per-wave, per-column partials of x.Kx (nullptr: not needed)
+    const MState* st;         // per column
+    const int* all_done;
+};
+
+// Entry (i, c) of the cell block [[s A, -W^T], [-W, 0]] from the packed
+// operator in LDS.
+template <int N, int NA>
+__device__ __forceinline__ float kval(const float* op, int i, int c, float s) {
+    if (i < N && c < N) {
+        const int a = i > c ? i : c, b = i > c ? c : i;
+        return s * op[a * (a + 1) / 2 + b];
+    }
+    if (i < N) return -op[NA + (c - N) * N + i];
+    if (c < N) return -op[NA + (i - N) * N + c];
+    return 0.0f;
+}
+
+template <int K, int NE>
+__device__ __forceinline__ void mr_cell(const MrArgs& a, const GroupDesc& g, int gi, int64_t gpos, int l,
+                                        float* lop, uint32_t* lslot, const float* Xj, double& dot) {
+    constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE, N = NED + NI;
+    constexpr int M = N + Q, NA = N * (N + 1) / 2, SZ = NA + Q * N, SZ4 = (SZ + 3) / 4 * 4;
+    static_assert(SZ4 <= OPMAX && M <= 32, "cell block larger than the MFMA tile");
+    const int64_t cg = gpos - g.gpos0;
+    const float4* src = reinterpret_cast<const float4*>(a.op + a.gop[gi] + cg * SZ4);
+    for (int t = l; t < SZ4 / 4; t += 64) reinterpret_cast<float4*>(lop)[t] = src[t];
+    if (l < M) {
+        uint32_t sl;
+        if (l < NED)
+            sl = a.idx[g.i_off + ((cg >> 6) * NED + l) * 64 + (cg & 63)];
+        else if (l < N)
+            sl = (uint32_t)(a.cbase + (l - NED) * a.cpad + gpos);
+        else
+            sl = (uint32_t)(a.cbase + (NI + l - N) * a.cpad + gpos);
+        lslot[l] = sl;
+    }
+    // the wave's LDS writes are complete before any lane reads them
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    const float s = a.scale[gpos];
+    const int i = l & 31, hi = l >> 5;
+    f16v acc = {};
+#pragma unroll
+    for (int kk = 0; kk < (M + 1) / 2; ++kk) {
+        const int c = 2 * kk + hi;
+        float av = 0.0f, bv = 0.0f;
+        if (c < M) {
+            if (i < M) av = kval<N, NA>(lop, i, c, s);
+            bv = Xj[(int64_t)(lslot[c] & 0x7fffffffu) * NR + i];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+    }
+    // D[row][col]: col = lane % 32, row = 8 (r / 4) + 4 (lane / 32) + r % 4
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * hi + (r & 3);
+        if (row < M) {
+            const uint32_t sl = lslot[row];
+            const int64_t e = (int64_t)(sl & 0x7fffffffu) * NR + i;
+            if (sl & SIDE1)
+                a.Y1[e] = acc[r];
+            else
+                a.Y[e] = acc[r];
+            if (a.pw) dot += (double)Xj[e] * (double)acc[r];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();  // LDS reuse by the next cell
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_apply_mr(MrArgs a) {
+    if (a.all_done && *a.all_done) return;
+    __shared__ __attribute__((aligned(16))) float lop[4][OPMAX];
+    __shared__ uint32_t lslot[4][32];
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int wave = blockIdx.x * 4 + w;
+    const int l = threadIdx.x & 63;
+    const int col = l & 31;
+    const float* Xj = a.st ? a.X[a.st[col].itn & 1] : a.X[0];
+    double dot = 0.0;
+    const int64_t g0 = (int64_t)wave * a.cells_per_wave;
+    const int64_t g1 = g0 + a.cells_per_wave < a.cpad ? g0 + a.cells_per_wave : a.cpad;
+    for (int64_t gpos = g0; gpos < g1; ++gpos) {
+        if (a.cell_of_gpos[gpos] < 0) continue;
+        const int gi = __builtin_amdgcn_readfirstlane(a.tile_group[gpos >> 6]);
+        const GroupDesc g = a.groups[gi];
+        switch (g.ne) {
+            case 3: mr_cell<K, 3>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
+            case 4: mr_cell<K, 4>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
+            case 5: mr_cell<K, 5>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
+            case 6: mr_cell<K, 6>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
+            case 7: mr_cell<K, 7>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
+            case 8: mr_cell<K, 8>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
+            case 9: mr_cell<K, 9>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
+            case 10: mr_cell<K, 10>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
+            default: break;
+        }
+    }
+    if (a.pw) {
+        dot += __shfl_xor(dot, 32, 64);
+        if (l < 32) a.pw[(int64_t)wave * NR + col] = dot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Simultaneous MINRES over NR interleaved columns (one MState per column)
+// ---------------------------------------------------------------------------
+
+struct MrUpd {
+    float* R[2];
+    float* Z[2];
+    float* WB[2];
+    float* x;
+    const float* Az;
+    const float* Az1;
+    const float* minv;
+    int64_t edof;
+    const VecChunk* chunks;
+    double* p1;  // [chunk][NR]
+    double* p2;
+    MState* st;
+    int amg;
+    int64_t pb, cpad;
+    int q;
+    float* bp;   // multigrid input, [cell-major pressure row][NR]
+};
+
+// Column partials of a block of 8 row groups x NR columns.
+__device__ __forceinline__ void col_pair(double d1, double d2, double* o1, double* o2) {
+    __shared__ double s1[8][NR], s2[8][NR];
+    const int j = threadIdx.x & (NR - 1), r0 = threadIdx.x >> 5;
+    s1[r0][j] = d1;
+    s2[r0][j] = d2;
+    __syncthreads();
+    if (r0 == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < 8; ++k) {
+            a += s1[k][j];
+            b += s2[k][j];
+        }
+        o1[j] = a;
+        if (o2) o2[j] = b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_update_mr(MrUpd a) {
+    const VecChunk ch = a.chunks[blockIdx.x];
+    const int j = threadIdx.x & (NR - 1), r0 = threadIdx.x >> 5;
+    const MState* st = a.st + j;
+    double d1 = 0.0, d2 = 0.0;
+    if (!st->done) {
+        const int it = st->itn;
+        float* __restrict__ r2 = a.R[it & 1];
+        float* __restrict__ r1 = a.R[(it + 1) & 1];
+        float* __restrict__ zo = a.Z[(it + 1) & 1];
+        float* __restrict__ wn = a.WB[(it + 1) & 1];
+        const float* __restrict__ wp = a.WB[it & 1];
+        const float caz = (float)st->c_az, cr1 = (float)st->c_r1, cr2 = (float)st->c_r2;
+        const bool pend = st->pending;
+        const float uv = (float)st->u_v, uw1 = (float)st->u_w1, uw2 = (float)st->u_w2, ux = (float)st->u_x;
+        const bool cyc = a.amg && ch.pslot;
+        const int pr = ch.pslot - 1;
+        for (int64_t row = ch.start + r0; row < ch.start + ch.len; row += 8) {
+            const int64_t e = row * NR + j;
+            float r2i = r2[e];
+            float az = a.Az[e];
+            if (row < a.edof) az += a.Az1[e];
+            float y = caz * az - cr1 * r1[e] - cr2 * r2i;
+            if (pend) {
+                float w = uv * zo[e] - uw1 * wn[e] - uw2 * wp[e];
+                wn[e] = w;
+                a.x[e] += ux * w;
+            }
+            r1[e] = y;
+            if (cyc) {
+                a.bp[((row - a.pb - (int64_t)pr * a.cpad) * a.q + pr) * NR + j] = y;
+            } else {
+                float zn = a.minv[row] * y;
+                zo[e] = zn;
+                d1 += (double)y * (double)zn;
+                d2 += (double)zn * (double)r2i;
+            }
+        }
+    }
+    col_pair(d1, d2, a.p1 + (int64_t)blockIdx.x * NR, a.p2 + (int64_t)blockIdx.x * NR);
+}
+
+__global__ __launch_bounds__(256) void k_pdot_mr(MrUpd a, const int* pchunks, const float* xp, double* q1,
+                                               double* q2, int init) {
+    const int c = pchunks[blockIdx.x];
+    const VecChunk ch = a.chunks[c];
+    const int j = threadIdx.x & (NR - 1), r0 = threadIdx.x >> 5;
+    const MState* st = a.st + j;
+    double d1 = 0.0, d2 = 0.0;
+    if (init || !st->done) {
+        const int it = init ? 0 : st->itn;
+        float* __restrict__ zo = init ? a.Z[0] : a.Z[(it + 1) & 1];
+        const float* __restrict__ r2 = a.R[it & 1];
+        const int pr = ch.pslot - 1;
+        for (int64_t row = ch.start + r0; row < ch.start + ch.len; row += 8) {
+            const int64_t e = row * NR + j;
+            const int64_t k = ((row - a.pb - (int64_t)pr * a.cpad) * a.q + pr) * NR + j;
+            const float z = xp[k];
+            zo[e] = z;
+            d1 += (double)a.bp[k] * (double)z;
+            if (!init) d2 += (double)z * (double)r2[e];
+        }
+    }
+    col_pair(d1, d2, q1 + (int64_t)c * NR, q2 + (int64_t)c * NR);
+}
+
+// Column sums of partials p[c][j] over c in [0, n) for the block's column j.
+__device__ double col_sum(const double* p, int n, int j) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int c = threadIdx.x; c < n; c += 256) s += p[(int64_t)c * NR + j];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void k_finA_mr(MState* sts, const double* pw, int nwaves) {
+    MState* st = sts + blockIdx.x;
+    if (st->done) return;
+    double zaz = col_sum(pw, nwaves, blockIdx.x);
+    if (threadIdx.x == 0) {
+        double beta = st->beta;
+        double alfa = zaz / (beta * beta) - (st->itn > 0 ? st->rho / st->oldb : 0.0);
+        st->alfa = alfa;
+        st->c_az = 1.0 / beta;
+        st->c_r1 = st->itn > 0 ? beta / st->oldb : 0.0;
+        st->c_r2 = alfa / beta;
+    }
+}
+
+__global__ void k_finB_mr(MState* sts, const double* p1, const double* p2, const double* q1, const double* q2,
+                          int nchunks, int useq) {
+    MState* st = sts + blockIdx.x;
+    if (st->done) return;
+    const int j = blockIdx.x;
+    double b2 = col_sum(p1, nchunks, j) + (useq ? col_sum(q1, nchunks, j) : 0.0);
+    double rho = col_sum(p2, nchunks, j) + (useq ? col_sum(q2, nchunks, j) : 0.0);
+    if (threadIdx.x == 0) {
+        double beta_new = std::sqrt(b2 > 0.0 ? b2 : 0.0);
+        st->rho = rho;
+        st->oldb = st->beta;
+        st->beta = beta_new;
+        double oldeps = st->epsln;
+        double delta = st->cs * st->dbar + st->sn * st->alfa;
+        double gbar = st->sn * st->dbar - st->cs * st->alfa;
+        st->epsln = st->sn * beta_new;
+        st->dbar = -st->cs * beta_new;
+        double gamma = std::sqrt(gbar * gbar + beta_new * beta_new);
+        if (gamma < 1e-300) gamma = 1e-300;
+        st->cs = gbar / gamma;
+        st->sn = beta_new / gamma;
+        double phi = st->cs * st->phibar;
+        st->phibar = st->sn * st->phibar;
+        st->u_v = 1.0 / (st->oldb * gamma);
+        st->u_w1 = oldeps / gamma;
+        st->u_w2 = delta / gamma;
+        st->u_x = phi;
+        st->pending = 1;
+        st->itn += 1;
+        if (st->phibar <= st->tol || st->itn >= st->maxit || !(beta_new > 0.0)) st->done = 1;
+    }
+}
+
+__global__ void k_alldone_mr(const MState* sts, int* all_done) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (threadIdx.x < NR && !sts[threadIdx.x].done) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) *all_done = any ? 0 : 1;
+}
+
+__global__ __launch_bounds__(256) void k_init_mr(MrUpd a, const float* b) {
+    const VecChunk ch = a.chunks[blockIdx.x];
+    const int j = threadIdx.x & (NR - 1), r0 = threadIdx.x >> 5;
+    const bool cyc = a.amg && ch.pslot;
+    const int pr = ch.pslot - 1;
+    double d = 0.0;
+    for (int64_t row = ch.start + r0; row < ch.start + ch.len; row += 8) {
+        const int64_t e = row * NR + j;
+        float bi = b[e];
+        a.R[0][e] = bi;
+        a.R[1][e] = 0.0f;
+        a.Z[1][e] = 0.0f;
+        a.WB[0][e] = 0.0f;
+        a.WB[1][e] = 0.0f;
+        a.x[e] = 0.0f;
+        if (cyc) {
+            a.bp[((row - a.pb - (int64_t)pr * a.cpad) * a.q + pr) * NR + j] = bi;
+        } else {
+            float z = a.minv[row] * bi;
+            a.Z[0][e] = z;
+            d += (double)bi * (double)z;
+        }
+    }
+    col_pair(d, 0.0, a.p1 + (int64_t)blockIdx.x * NR, nullptr);
+}
+
+__global__ void k_finInit_mr(MState* sts, const double* p1, const double* q1, int nchunks, int useq,
+                             const double* rtol, int maxit) {
+    const int j = blockIdx.x;
+    double b2 = col_sum(p1, nchunks, j) + (useq ? col_sum(q1, nchunks, j) : 0.0);
+    if (threadIdx.x == 0) {
+        double beta1 = std::sqrt(b2 > 0.0 ? b2 : 0.0);
+        MState s = {};
+        s.beta1 = beta1;
+        s.beta = beta1;
+        s.phibar = beta1;
+        s.cs = -1.0;
+        s.tol = rtol[j] * beta1;
+        s.maxit = maxit;
+        s.done = (beta1 > 0.0 && maxit > 0 && rtol[j] < 1.0) ? 0 : 1;
+        sts[j] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_flush_mr(MrUpd a) {
+    const VecChunk ch = a.chunks[blockIdx.x];
+    const int j = threadIdx.x & (NR - 1), r0 = threadIdx.x >> 5;
+    const MState* st = a.st + j;
+    if (!st->pending) return;
+    const int m = st->itn - 1;
+    const float* z = a.Z[m & 1];
+    const float* wpp = a.WB[m & 1];
+    const float* wp = a.WB[(m + 1) & 1];
+    const float uv = (float)st->u_v, uw1 = (float)st->u_w1, uw2 = (float)st->u_w2, ux = (float)st->u_x;
+    for (int64_t row = ch.start + r0; row < ch.start + ch.len; row += 8) {
+        const int64_t e = row * NR + j;
+        float w = uv * z[e] - uw1 * wpp[e] - uw2 * wp[e];
+        a.x[e] += ux * w;
+    }
+}
+
+// FP64 outer loop: column-major [NR][n] <-> interleaved [n][NR].
+__global__ __launch_bounds__(256) void k_cm_to_il(const double* r, float* b, int64_t n, int ncol) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n * NR) return;
+    const int64_t row = t / NR;
+    const int j = (int)(t % NR);
+    b[t] = j < ncol ? (float)r[(int64_t)j * n + row] : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_il_add_cm(double* x, const float* d, int64_t n, int ncol) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n * NR) return;
+    const int64_t row = t / NR;
+    const int j = (int)(t % NR);
+    if (j < ncol) x[(int64_t)j * n + row] += (double)d[t];
+}
+
+// r = b - y (y's edge dofs: both sides) and block partials of |r|^2.
+__global__ __launch_bounds__(256) void k_resid_col(const double* b, const double* y, const double* y1, int64_t edof,
+                                                   int64_t n, double* r, double* part) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        double yi = y[i];
+        if (i < edof) yi += y1[i];
+        double v = b[i] - yi;
+        r[i] = v;
+        s += v * v;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_to(const double* part, int n, double* out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static MultiState& multi_setup(Solver& s) {
+    DeviceState& d = *s.dev;
+    if (d.multi) return *d.multi;
+    const Layout& L = s.layout;
+    const Setup& S = *s.setup;
+    if (L.lam || L.ninst != 1) throw Error(VEMD_ERR_ARG, "multi-right-hand-side solves need one Dirichlet-pressure mesh");
+    d.multi = new MultiState();
+    MultiState& m = *d.multi;
+    // FP32 cell blocks: packed A then W, per group in store order
+    std::vector<int64_t> gop(L.groups.size());
+    int64_t tot = 0;
+    for (size_t gi = 0; gi < L.groups.size(); ++gi) {
+        const GroupDesc& g = L.groups[gi];
+        const int n = g.n, sz = n * (n + 1) / 2 + L.q * n, sz4 = (sz + 3) / 4 * 4;
+        gop[gi] = tot;
+        tot += (int64_t)sz4 * g.npad;
+    }
+    std::vector<float> op(tot, 0.0f);
+    parallel_for((int64_t)L.groups.size(), [&](int64_t b, int64_t e) {
+        for (int64_t gi = b; gi < e; ++gi) {
+            const GroupDesc& g = L.groups[gi];
+            const int n = g.n, na = n * (n + 1) / 2, sz = na + L.q * n, sz4 = (sz + 3) / 4 * 4;
+            for (int cg = 0; cg < g.npad; ++cg) {
+                const int c = L.cell_of_gpos[g.gpos0 + cg];
+                if (c < 0) continue;
+                float* o = &op[gop[gi] + (int64_t)cg * sz4];
+                const double* A = &S.A[S.ofs.a_off[c]];
+                const double* W = &S.W[S.ofs.w_off[c]];
+                for (int t = 0; t < na; ++t) o[t] = (float)A[t];
+                for (int t = 0; t < L.q * n; ++t) o[na + t] = (float)W[t];
+            }
+        }
+    });
+    m.op = (float*)malloc_m(m, op.size() * sizeof(float));
+    HIPCHK(hipMemcpy(m.op, op.data(), op.size() * sizeof(float), hipMemcpyHostToDevice));
+    m.gop = (int64_t*)malloc_m(m, gop.size() * sizeof(int64_t));
+    HIPCHK(hipMemcpy(m.gop, gop.data(), gop.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    // persistent waves: about 8 per CU of the chip
+    int dev = 0, ncu = 256;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    m.nwaves = std::max(4, ncu * 8) / 4 * 4;
+    m.cells_per_wave = (L.cpad + m.nwaves - 1) / m.nwaves;
+    const size_t fv = (size_t)L.nvec * NR * sizeof(float);
+    for (int i = 0; i < 2; ++i) {
+        m.R[i] = (float*)malloc_m(m, fv);
+        m.Z[i] = (float*)malloc_m(m, fv);
+        m.WB[i] = (float*)malloc_m(m, fv);
+    }
+    m.x = (float*)malloc_m(m, fv);
+    m.Az = (float*)malloc_m(m, fv);
+    m.b = (float*)malloc_m(m, fv);
+    m.Az1 = (float*)malloc_m(m, (size_t)std::max<int64_t>(L.cbase, 1) * NR * sizeof(float));
+    HIPCHK(hipMemset(m.Az1, 0, (size_t)std::max<int64_t>(L.cbase, 1) * NR * sizeof(float)));
+    for (float* p : {m.R[0], m.R[1], m.Z[0], m.Z[1], m.WB[0], m.WB[1], m.x, m.Az, m.b}) HIPCHK(hipMemset(p, 0, fv));
+    const size_t dv = (size_t)L.nvec * NR * sizeof(double);
+    m.x64 = (double*)malloc_m(m, dv);
+    m.r64 = (double*)malloc_m(m, dv);
+    m.y64 = (double*)malloc_m(m, (size_t)L.nvec * sizeof(double));
+    m.y1_64 = (double*)malloc_m(m, (size_t)std::max<int64_t>(L.cbase, 1) * sizeof(double));
+    HIPCHK(hipMemset(m.y1_64, 0, (size_t)std::max<int64_t>(L.cbase, 1) * sizeof(double)));
+    m.st = (MState*)malloc_m(m, NR * sizeof(MState));
+    HIPCHK(hipMemset(m.st, 0, NR * sizeof(MState)));
+    HIPCHK(hipHostMalloc((void**)&m.st_host, NR * sizeof(MState), hipHostMallocDefault));
+    m.rtol = (double*)malloc_m(m, NR * sizeof(double));
+    HIPCHK(hipHostMalloc((void**)&m.rtol_host, NR * sizeof(double), hipHostMallocDefault));
+    const int nch = (int)L.chunks.size();
+    m.pw = (double*)malloc_m(m, (size_t)m.nwaves * NR * sizeof(double));
+    for (double** p : {&m.p1, &m.p2, &m.q1, &m.q2}) {
+        *p = (double*)malloc_m(m, (size_t)nch * NR * sizeof(double));
+        HIPCHK(hipMemset(*p, 0, (size_t)nch * NR * sizeof(double)));
+    }
+    m.part = (double*)malloc_m(m, (size_t)d.nblk * sizeof(double));
+    m.dsum = (double*)malloc_m(m, NR * sizeof(double));
+    HIPCHK(hipHostMalloc((void**)&m.hsum, NR * sizeof(double), hipHostMallocDefault));
+    m.all_done = (int*)malloc_m(m, sizeof(int));
+    HIPCHK(hipMemset(m.all_done, 0, sizeof(int)));
+    if (d.amg32) amg_multi_alloc(*d.amg32);
+    return m;
+}
+
+static MrArgs mr_args(Solver& s, MultiState& m, const float* x0, const float* x1, float* y, float* y1, bool solve) {
+    DeviceState& d = *s.dev;
+    MrArgs a;
+    a.groups = d.o32.groups;
+    a.tile_group = d.tile_group;
+    a.cell_of_gpos = d.cell_of_gpos;
+    a.op = m.op;
+    a.gop = m.gop;
+    a.idx = d.idx;
+    a.scale = d.o32.scale;
+    a.X[0] = x0;
+    a.X[1] = x1;
+    a.Y = y;
+    a.Y1 = y1;
+    a.cbase = s.layout.cbase;
+    a.cpad = s.layout.cpad;
+    a.cells_per_wave = m.cells_per_wave;
+    a.pw = solve ? m.pw : nullptr;
+    a.st = solve ? m.st : nullptr;
+    a.all_done = solve ? m.all_done : nullptr;
+    return a;
+}
+
+static void launch_apply_mr(Solver& s, MultiState& m, const MrArgs& a, hipStream_t st) {
+    const dim3 grid(m.nwaves / 4), block(256);
+    if (s.layout.k == 0)
+        hipLaunchKernelGGL((k_apply_mr<0>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((k_apply_mr<1>), grid, block, 0, st, a);
+}
+
+void dev_apply_multi(Solver& s, const void* x, void* y, void* y1, void* stream) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    HIPCHK(hipSetDevice(s.dev->device));
+    MultiState& m = multi_setup(s);
+    MrArgs a = mr_args(s, m, (const float*)x, (const float*)x, (float*)y, (float*)y1, false);
+    launch_apply_mr(s, m, a, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+}
+
+static MrUpd mr_upd(Solver& s, MultiState& m, bool amg) {
+    DeviceState& d = *s.dev;
+    const Layout& L = s.layout;
+    MrUpd u{{m.R[0], m.R[1]}, {m.Z[0], m.Z[1]}, {m.WB[0], m.WB[1]}, m.x, m.Az, m.Az1, d.o32.minv, L.cbase,
+            d.chunks,         m.p1,           m.p2,           m.st, amg ? 1 : 0, 0,      L.cpad,        L.q,
+            nullptr};
+    if (amg) {
+        u.pb = L.cbase + (int64_t)L.ni * L.cpad;
+        u.bp = d.amg32->lv[0].mb;
+    }
+    return u;
+}
+
+static void enqueue_iteration_mr(Solver& s, MultiState& m, bool amg) {
+    DeviceState& d = *s.dev;
+    MrArgs a = mr_args(s, m, m.Z[0], m.Z[1], m.Az, m.Az1, true);
+    launch_apply_mr(s, m, a, d.stream);
+    hipLaunchKernelGGL(k_finA_mr, dim3(NR), dim3(256), 0, d.stream, m.st, (const double*)m.pw, m.nwaves);
+    MrUpd u = mr_upd(s, m, amg);
+    hipLaunchKernelGGL(k_update_mr, dim3(d.nchunks), dim3(256), 0, d.stream, u);
+    if (amg) {
+        amg_vcycle_multi(*d.amg32, d.stream, m.all_done);
+        hipLaunchKernelGGL(k_pdot_mr, dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
+                           (const float*)d.amg32->lv[0].mx, m.q1, m.q2, 0);
+    }
+    hipLaunchKernelGGL(k_finB_mr, dim3(NR), dim3(256), 0, d.stream, m.st, (const double*)m.p1, (const double*)m.p2,
+                       (const double*)m.q1, (const double*)m.q2, d.nchunks, amg ? 1 : 0);
+    hipLaunchKernelGGL(k_alldone_mr, dim3(1), dim3(64), 0, d.stream, (const MState*)m.st, m.all_done);
+}
+
+// Simultaneous preconditioned MINRES (FP32) on the NR columns of m.b; column
+// j stops at rtol[j] (>= 1: untouched).  Returns the largest iteration count.
+static int minres_mr(Solver& s, MultiState& m, const std::vector<double>& rtol, int maxit, int chk, bool graph,
+                     bool amg, std::vector<int>& its) {
+    DeviceState& d = *s.dev;
+    MrUpd u = mr_upd(s, m, amg);
+    if (graph && (m.graph_iters != chk || m.graph_amg != amg || !m.graph)) {
+        HIPCHK(hipStreamSynchronize(d.stream));
+        if (m.graph) (void)hipGraphExecDestroy(m.graph);
+        m.graph = nullptr;
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < chk; ++i) enqueue_iteration_mr(s, m, amg);
+        HIPCHK(hipStreamEndCapture(d.stream, &g));
+        HIPCHK(hipGraphInstantiate(&m.graph, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        m.graph_iters = chk;
+        m.graph_amg = amg;
+    }
+    HIPCHK(hipStreamSynchronize(d.stream));
+    for (int j = 0; j < NR; ++j) m.rtol_host[j] = rtol[j];
+    HIPCHK(hipMemcpyAsync(m.rtol, m.rtol_host, NR * sizeof(double), hipMemcpyHostToDevice, d.stream));
+    hipLaunchKernelGGL(k_init_mr, dim3(d.nchunks), dim3(256), 0, d.stream, u, (const float*)m.b);
+    if (amg) {
+        amg_vcycle_multi(*d.amg32, d.stream, nullptr);
+        hipLaunchKernelGGL(k_pdot_mr, dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
+                           (const float*)d.amg32->lv[0].mx, m.q1, m.q2, 1);
+    }
+    hipLaunchKernelGGL(k_finInit_mr, dim3(NR), dim3(256), 0, d.stream, m.st, (const double*)m.p1, (const double*)m.q1,
+                       d.nchunks, amg ? 1 : 0, (const double*)m.rtol, maxit);
+    hipLaunchKernelGGL(k_alldone_mr, dim3(1), dim3(64), 0, d.stream, (const MState*)m.st, m.all_done);
+    HIPCHK(hipGetLastError());
+    while (true) {
+        HIPCHK(hipMemcpyAsync(m.st_host, m.st, NR * sizeof(MState), hipMemcpyDeviceToHost, d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));
+        bool all = true;
+        for (int j = 0; j < NR && all; ++j) all = m.st_host[j].done != 0;
+        if (all) break;
+        if (graph)
+            HIPCHK(hipGraphLaunch(m.graph, d.stream));
+        else
+            for (int i = 0; i < chk; ++i) enqueue_iteration_mr(s, m, amg);
+    }
+    hipLaunchKernelGGL(k_flush_mr, dim3(d.nchunks), dim3(256), 0, d.stream, u);
+    HIPCHK(hipGetLastError());
+    its.assign(NR, 0);
+    int mx = 0;
+    for (int j = 0; j < NR; ++j) {
+        its[j] = m.st_host[j].itn;
+        mx = std::max(mx, its[j]);
+    }
+    return mx;
+}
+
+// r_j = b_j - K x_j (FP64, column-major) for j < ncol; returns |r_j|.
+static std::vector<double> residual_mr(Solver& s, MultiState& m, const double* b, int ncol) {
+    DeviceState& d = *s.dev;
+    const int64_t n = s.layout.nvec;
+    for (int j = 0; j < ncol; ++j) {
+        dev_apply_int(s, VEMD_FP64, m.x64 + (int64_t)j * n, m.y64, m.y1_64, d.stream);
+        hipLaunchKernelGGL(k_resid_col, dim3(d.nblk), dim3(256), 0, d.stream, b + (int64_t)j * n, (const double*)m.y64,
+                           (const double*)m.y1_64, s.layout.cbase, n, m.r64 + (int64_t)j * n, m.part);
+        hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, d.stream, (const double*)m.part, d.nblk, m.dsum + j);
+    }
+    HIPCHK(hipMemcpyAsync(m.hsum, m.dsum, ncol * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    std::vector<double> r(ncol);
+    for (int j = 0; j < ncol; ++j) r[j] = std::sqrt(m.hsum[j]);
+    return r;
+}
+
+void dev_solve_multi(Solver& s, const vemd_solve_opts& o, int ncol, const double* b, double* x,
+                     vemd_solve_info* info) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    DeviceState& d = *s.dev;
+    HIPCHK(hipSetDevice(d.device));
+    if (ncol < 1 || ncol > NR) throw Error(VEMD_ERR_ARG, "between 1 and 32 right-hand sides");
+    if (o.precision != VEMD_MIXED)
+        throw Error(VEMD_ERR_ARG, "multi-right-hand-side solves run FP32 MFMA iterations inside FP64 refinement "
+                                  "(VEMD_MIXED)");
+    if (o.precond == VEMD_PC_MULTIGRID && !d.amg32)
+        throw Error(VEMD_ERR_ARG, "no multigrid hierarchy: it is built for Dirichlet pressure problems only");
+    MultiState& m = multi_setup(s);
+    if (d.amg32) amg_multi_alloc(*d.amg32);  // the hierarchy is rebuilt when theta changes
+    const bool amg = o.precond != VEMD_PC_JACOBI && d.amg32 != nullptr;
+    const int64_t n = s.layout.nvec;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipMemsetAsync(m.x64, 0, (size_t)ncol * n * sizeof(double), d.stream));
+    std::vector<double> bn = residual_mr(s, m, b, ncol), rel(ncol);
+    for (int j = 0; j < ncol; ++j) rel[j] = bn[j] > 0 ? 1.0 : 0.0;
+    std::vector<int> tot(ncol, 0), its;
+    int iters = 0, refin = 0;
+    const int maxit = o.max_iters > 0 ? o.max_iters : 1000000;
+    const int chk = o.check_every > 0 ? o.check_every : 32;
+    auto worst = [&]() { return *std::max_element(rel.begin(), rel.end()); };
+    std::vector<double> target(NR, 1.0);
+    while (worst() > o.rtol && iters < maxit && refin < 100) {
+        for (int j = 0; j < ncol; ++j)
+            target[j] = rel[j] > o.rtol ? std::max(o.inner_rtol > 0 ? o.inner_rtol : 1e-4, 0.5 * o.rtol / rel[j]) : 1.0;
+        hipLaunchKernelGGL(k_cm_to_il, dim3((unsigned)((n * NR + 255) / 256)), dim3(256), 0, d.stream,
+                           (const double*)m.r64, m.b, n, ncol);
+        iters += minres_mr(s, m, target, maxit - iters, chk, o.use_graph != 0, amg, its);
+        hipLaunchKernelGGL(k_il_add_cm, dim3((unsigned)((n * NR + 255) / 256)), dim3(256), 0, d.stream, m.x64,
+                           (const float*)m.x, n, ncol);
+        for (int j = 0; j < ncol; ++j) tot[j] += its[j];
+        ++refin;
+        std::vector<double> rn = residual_mr(s, m, b, ncol);
+        for (int j = 0; j < ncol; ++j) rel[j] = bn[j] > 0 ? rn[j] / bn[j] : 0.0;
+    }
+    if (x) HIPCHK(hipMemcpyAsync(x, m.x64, (size_t)ncol * n * sizeof(double), hipMemcpyDeviceToDevice, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (info)
+        for (int j = 0; j < ncol; ++j) {
+            info[j].iterations = tot[j];
+            info[j].refinements = refin;
+            info[j].rel_residual = rel[j];
+            info[j].seconds = secs;
+            info[j].iter_ms = iters > 0 ? 1e3 * secs / iters : 0.0;
+        }
+    if (worst() > o.rtol)
+        throw Error(VEMD_ERR_NOT_CONVERGED, "not converged: largest relative residual " + std::to_string(worst()));
+}
+
+}  // namespace vemd

@@ -45,19 +45,33 @@ struct Sell {
     const T* val = nullptr;
 };
 
+// Compressed sparse rows (restriction: few long rows).
+template <typename T>
+struct CsrDev {
+    int64_t nrows = 0;
+    const int64_t* rp = nullptr;
+    const int* ci = nullptr;
+    const T* v = nullptr;
+};
+
 template <typename T>
 struct AmgLevelDev {
     int64_t n = 0;
-    Sell<T> A, P, R;          // P: prolongation from the next level, R = P^T
+    Sell<T> A, P;             // P: prolongation from the next level
+    CsrDev<T> R;              // R = P^T
     T* dinv = nullptr;
-    T *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;
+    T *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;       // one column
+    T *mb = nullptr, *mx = nullptr, *mr = nullptr, *md0 = nullptr, *md1 = nullptr;  // MCOLS interleaved columns
     double centre = 0.0, half = 0.0;  // Chebyshev interval of D^-1 A
     int degree = 2;
 };
 
+constexpr int MCOLS = 32;  // columns of the multi-vector cycle (right-hand sides)
+
 template <typename T>
 struct AmgDev {
     std::vector<AmgLevelDev<T>> lv;
+    bool multi = false;       // level buffers for MCOLS columns allocated
     T* cinv = nullptr;        // dense inverse of the coarsest operator
     int64_t nc = 0;
     int64_t bytes = 0;        // device memory held
@@ -74,5 +88,12 @@ AmgDev<T>* amg_upload(const Amg& H);
 // skipped inside a captured chunk of iterations after convergence).
 template <typename T>
 void amg_vcycle(const AmgDev<T>& D, hipStream_t stream, const int* done);
+
+// The same cycle applied to MCOLS interleaved columns (entry (row, j) at
+// row * MCOLS + j): b = lv[0].mb, x = lv[0].mx.  amg_multi_alloc first.
+template <typename T>
+void amg_multi_alloc(AmgDev<T>& D);
+template <typename T>
+void amg_vcycle_multi(const AmgDev<T>& D, hipStream_t stream, const int* done);
 
 }  // namespace vemd

@@ -13,22 +13,26 @@
 
 namespace vemd {
 
-// out = (A (x + y))_brow for one block row of bs = B rows (y may be null).
-template <typename T, int B>
-__device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, const T* __restrict__ x, T (&out)[B],
+// out = (A (x + y))_brow, column j of NC interleaved columns (vector entry
+// (row, j) at row * NC + j), for one block row of bs = B rows (y may be null).
+template <typename T, int B, int NC>
+__device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x, T (&out)[B],
                                          const T* __restrict__ y = nullptr) {
     const int64_t s = brow >> 6;
     const int w = A.width[s];
     const int64_t o = A.off[s] + (brow & 63);
 #pragma unroll
     for (int r = 0; r < B; ++r) out[r] = T(0);
-    for (int j = 0; j < w; ++j) {
-        const int64_t e = o + (int64_t)j * 64;
+    for (int k = 0; k < w; ++k) {
+        const int64_t e = o + (int64_t)k * 64;
         const int64_t c = (int64_t)A.col[e] * B;
         const T* v = A.val + e * (B * B);
         T xc[B];
 #pragma unroll
-        for (int cc = 0; cc < B; ++cc) xc[cc] = y ? x[c + cc] + y[c + cc] : x[c + cc];
+        for (int cc = 0; cc < B; ++cc) {
+            const int64_t xi = (c + cc) * NC + j;
+            xc[cc] = y ? x[xi] + y[xi] : x[xi];
+        }
 #pragma unroll
         for (int r = 0; r < B; ++r)
 #pragma unroll
@@ -36,118 +40,138 @@ __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, const T
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T sell_dot(const Sell<T>& A, int64_t row, const T* __restrict__ x) {
-    T o[1];
-    sell_row<T, 1>(A, row, x, o);
-    return o[0];
-}
-
+// Thread t of an elementwise sweep over n rows x NC columns: entry t, row t / NC.
 #define AMG_ROW                                                  \
     if (done && *done) return;                                   \
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;   \
-    if (i >= n) return;
+    if (i >= n * NC) return;                                     \
+    const int64_t row = i / NC;
 
+// Thread t of a block-row sweep: block row t / NC, column t % NC.
 #define AMG_BROW                                                 \
     if (done && *done) return;                                   \
-    const int64_t bi = (int64_t)blockIdx.x * 256 + threadIdx.x;  \
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;   \
+    const int64_t bi = t / NC;                                   \
+    const int j = (int)(t % NC);                                 \
     if (bi >= A.nrows) return;
 
 // First pre-smoothing step from x = 0: r = b, d = D^-1 b / centre, x = 0.
-template <typename T>
+template <typename T, int NC>
 __global__ __launch_bounds__(256) void k_pre0(int64_t n, const T* b, const T* dinv, T* r, T* d, T* x, T c,
                                             const int* done) {
     AMG_ROW
     const T bi = b[i];
     r[i] = bi;
-    d[i] = dinv[i] * bi * c;
+    d[i] = dinv[row] * bi * c;
     x[i] = T(0);
 }
 
 // Chebyshev step per block row: r -= A d, d' = c1 d + c2 D^-1 r, x += d.
-template <typename T, int B>
+template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, const T* din, T* dout, T* x, T c1, T c2,
                                             const int* done) {
     AMG_BROW
     T ad[B];
-    sell_row<T, B>(A, bi, din, ad);
+    sell_row<T, B, NC>(A, bi, j, din, ad);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-        const int64_t i = bi * B + k;
+        const int64_t i = (bi * B + k) * NC + j;
         const T ri = r[i] - ad[k];
         r[i] = ri;
         const T di = din[i];
-        dout[i] = c1 * di + c2 * dinv[i] * ri;
+        dout[i] = c1 * di + c2 * dinv[bi * B + k] * ri;
         x[i] += di;
     }
 }
 
-// Residual of the lagged iterate: r = b - A x - A d.
-template <typename T, int B>
+// Residual of the lagged iterate: r = b - A (x + d).
+template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_resid(Sell<T> A, const T* b, const T* x, const T* d, T* r, const int* done) {
     AMG_BROW
     T ax[B];
-    sell_row<T, B>(A, bi, x, ax, d);
+    sell_row<T, B, NC>(A, bi, j, x, ax, d);
 #pragma unroll
-    for (int k = 0; k < B; ++k) r[bi * B + k] = b[bi * B + k] - ax[k];
+    for (int k = 0; k < B; ++k) {
+        const int64_t i = (bi * B + k) * NC + j;
+        r[i] = b[i] - ax[k];
+    }
 }
 
 // First post-smoothing step: r = b - A x, d = D^-1 r / centre (x unchanged).
-template <typename T, int B>
+template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_post0(Sell<T> A, const T* b, const T* x, const T* dinv, T* r, T* d, T c,
                                              const int* done) {
     AMG_BROW
     T ax[B];
-    sell_row<T, B>(A, bi, x, ax);
+    sell_row<T, B, NC>(A, bi, j, x, ax);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-        const int64_t i = bi * B + k;
+        const int64_t i = (bi * B + k) * NC + j;
         const T ri = b[i] - ax[k];
         r[i] = ri;
-        d[i] = dinv[i] * ri * c;
+        d[i] = dinv[bi * B + k] * ri * c;
     }
 }
 
-// y = M v (restriction).
-template <typename T>
-__global__ __launch_bounds__(256) void k_spmv(int64_t n, Sell<T> M, const T* v, T* y, const int* done) {
-    AMG_ROW
-    y[i] = sell_dot(M, i, v);
+// y = R v (restriction).  One column: 32 lanes per row, partial sums
+// combined in a fixed butterfly order; NC columns: one thread per (row, column).
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void k_restrict(CsrDev<T> R, const T* v, T* y, const int* done) {
+    if (done && *done) return;
+    if constexpr (NC == 1) {
+        const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 5;
+        const int lane = threadIdx.x & 31;
+        T acc = T(0);
+        if (row < R.nrows)
+            for (int64_t p = R.rp[row] + lane; p < R.rp[row + 1]; p += 32) acc += R.v[p] * v[R.ci[p]];
+        for (int o = 16; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 32);
+        if (row < R.nrows && lane == 0) y[row] = acc;
+    } else {
+        const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        const int64_t row = t / NC;
+        const int j = (int)(t % NC);
+        if (row >= R.nrows) return;
+        T acc = T(0);
+        for (int64_t p = R.rp[row]; p < R.rp[row + 1]; ++p) acc += R.v[p] * v[(int64_t)R.ci[p] * NC + j];
+        y[t] = acc;
+    }
 }
 
 // Coarse correction of the lagged iterate: x += d + P xc.
-template <typename T>
-__global__ __launch_bounds__(256) void k_prol(int64_t n, Sell<T> P, const T* xc, T* x, const T* d,
-                                            const int* done) {
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void k_prol(Sell<T> P, const T* xc, T* x, const T* d, const int* done) {
+    const int64_t n = P.nrows;
     AMG_ROW
-    x[i] += d[i] + sell_dot(P, i, xc);
+    T px[1];
+    sell_row<T, 1, NC>(P, row, (int)(i % NC), xc, px);
+    x[i] += d[i] + px[0];
 }
 
-// First post-smoothing step: r = b - A x, d = D^-1 r / centre (x unchanged).
-template <typename T>
-__global__ __launch_bounds__(256) void k_post0(int64_t n, Sell<T> A, const T* b, const T* x, const T* dinv, T* r,
-                                             T* d, T c, const int* done) {
-    AMG_ROW
-    const T ri = b[i] - sell_dot(A, i, x);
-    r[i] = ri;
-    d[i] = dinv[i] * ri * c;
-}
-
-template <typename T>
+template <typename T, int NC>
 __global__ __launch_bounds__(256) void k_addto(int64_t n, T* x, const T* d, const int* done) {
     AMG_ROW
+    (void)row;
     x[i] += d[i];
 }
 
-// Coarsest level: x = C^-1 b with the dense inverse, one wavefront per row.
-template <typename T>
+// Coarsest level: x = C^-1 b with the dense inverse; one block per row, one
+// wavefront reduction (NC = 1) or one thread per column.
+template <typename T, int NC>
 __global__ __launch_bounds__(64) void k_dense(int64_t n, const T* cinv, const T* b, T* x, const int* done) {
     if (done && *done) return;
     const int64_t i = blockIdx.x;
-    double acc = 0.0;
-    for (int64_t j = threadIdx.x; j < n; j += 64) acc += (double)cinv[i * n + j] * (double)b[j];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    if (threadIdx.x == 0) x[i] = (T)acc;
+    if constexpr (NC == 1) {
+        double acc = 0.0;
+        for (int64_t j = threadIdx.x; j < n; j += 64) acc += (double)cinv[i * n + j] * (double)b[j];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if (threadIdx.x == 0) x[i] = (T)acc;
+    } else {
+        for (int j = threadIdx.x; j < NC; j += 64) {
+            double acc = 0.0;
+            for (int64_t k = 0; k < n; ++k) acc += (double)cinv[i * n + k] * (double)b[k * NC + j];
+            x[i * NC + j] = (T)acc;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -216,6 +240,17 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs) {
 }
 
 template <typename T>
+static CsrDev<T> to_csr(AmgDev<T>& D, const Csr& A) {
+    CsrDev<T> c;
+    c.nrows = A.nrows;
+    c.rp = amg_copy(D, A.rp);
+    c.ci = amg_copy(D, A.ci);
+    std::vector<T> v(A.v.begin(), A.v.end());
+    c.v = amg_copy(D, v);
+    return c;
+}
+
+template <typename T>
 AmgDev<T>* amg_upload(const Amg& H) {
     AmgDev<T>* D = new AmgDev<T>();
     try {
@@ -229,7 +264,7 @@ AmgDev<T>* amg_upload(const Amg& H) {
             d.A = to_sell(*D, h.A, bs);
             if (l + 1 < H.lv.size()) {
                 d.P = to_sell(*D, h.P, 1);
-                d.R = to_sell(*D, h.R, 1);
+                d.R = to_csr(*D, h.R);
             }
             std::vector<T> di(h.dinv.begin(), h.dinv.end());
             d.dinv = amg_copy(*D, di);
@@ -255,65 +290,103 @@ AmgDev<T>* amg_upload(const Amg& H) {
     return D;
 }
 
-template <typename T, int B>
-static void smooth_steps(const AmgLevelDev<T>& L, hipStream_t st, const int* ms, T*& din) {
+// Vectors of one level for NC columns.
+template <typename T>
+struct LevelVecs {
+    T *b, *x, *r, *d0, *d1;
+};
+template <typename T, int NC>
+static LevelVecs<T> vecs(const AmgLevelDev<T>& L) {
+    if constexpr (NC == 1)
+        return {L.b, L.x, L.r, L.d0, L.d1};
+    else
+        return {L.mb, L.mx, L.mr, L.md0, L.md1};
+}
+
+static dim3 grid_of(int64_t threads) { return dim3((unsigned)((threads + 255) / 256)); }
+
+template <typename T, int B, int NC>
+static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStream_t st, const int* ms, T*& din) {
     const double sigma = L.centre / L.half;
     double rho = 1.0 / sigma;
-    T* dout = din == L.d0 ? L.d1 : L.d0;
-    const dim3 g((unsigned)((L.A.nrows + 255) / 256)), b(256);
+    T* dout = din == v.d0 ? v.d1 : v.d0;
     for (int k = 1; k < L.degree; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
-        hipLaunchKernelGGL((k_step<T, B>), g, b, 0, st, L.A, L.r, (const T*)L.dinv, (const T*)din, dout, L.x,
-                           (T)(rn * rho), (T)(2.0 * rn / L.half), ms);
+        hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC), dim3(256), 0, st, L.A, v.r, (const T*)L.dinv,
+                           (const T*)din, dout, v.x, (T)(rn * rho), (T)(2.0 * rn / L.half), ms);
         std::swap(din, dout);
         rho = rn;
     }
 }
 
-template <typename T>
+template <typename T, int NC>
 static void vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms);
 
 // Pre-smoothing, residual, restriction, coarse cycle, prolongation and
 // post-smoothing of one level whose operator has blocks of B rows.
-template <typename T, int B>
+template <typename T, int B, int NC>
 static void level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms) {
     const AmgLevelDev<T>& L = D.lv[l];
     const AmgLevelDev<T>& C = D.lv[l + 1];
-    const dim3 g((unsigned)((L.n + 255) / 256)), gb((unsigned)((L.A.nrows + 255) / 256)), b(256),
-        gc((unsigned)((C.n + 255) / 256));
+    const LevelVecs<T> v = vecs<T, NC>(L), vc = vecs<T, NC>(C);
+    const dim3 g = grid_of(L.n * NC), gb = grid_of(L.A.nrows * NC), b(256);
+    const dim3 gr = NC == 1 ? grid_of(C.n * 32) : grid_of(C.n * NC);
     const T c0 = (T)(1.0 / L.centre);
-    T* din = L.d0;
-    hipLaunchKernelGGL((k_pre0<T>), g, b, 0, st, L.n, (const T*)L.b, (const T*)L.dinv, L.r, L.d0, L.x, c0, ms);
-    smooth_steps<T, B>(L, st, ms, din);
-    hipLaunchKernelGGL((k_resid<T, B>), gb, b, 0, st, L.A, (const T*)L.b, (const T*)L.x, (const T*)din, L.r, ms);
-    hipLaunchKernelGGL((k_spmv<T>), gc, b, 0, st, C.n, L.R, (const T*)L.r, C.b, ms);
-    vcycle_level(D, l + 1, st, ms);
-    hipLaunchKernelGGL((k_prol<T>), g, b, 0, st, L.n, L.P, (const T*)C.x, L.x, (const T*)din, ms);
-    din = L.d0;
-    hipLaunchKernelGGL((k_post0<T, B>), gb, b, 0, st, L.A, (const T*)L.b, (const T*)L.x, (const T*)L.dinv, L.r, L.d0,
-                       c0, ms);
-    smooth_steps<T, B>(L, st, ms, din);
-    hipLaunchKernelGGL((k_addto<T>), g, b, 0, st, L.n, L.x, (const T*)din, ms);
+    T* din = v.d0;
+    hipLaunchKernelGGL((k_pre0<T, NC>), g, b, 0, st, L.n, (const T*)v.b, (const T*)L.dinv, v.r, v.d0, v.x, c0, ms);
+    smooth_steps<T, B, NC>(L, v, st, ms, din);
+    hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, (const T*)v.x, (const T*)din, v.r, ms);
+    hipLaunchKernelGGL((k_restrict<T, NC>), gr, b, 0, st, L.R, (const T*)v.r, vc.b, ms);
+    vcycle_level<T, NC>(D, l + 1, st, ms);
+    hipLaunchKernelGGL((k_prol<T, NC>), g, b, 0, st, L.P, (const T*)vc.x, v.x, (const T*)din, ms);
+    din = v.d0;
+    hipLaunchKernelGGL((k_post0<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, (const T*)v.x, (const T*)L.dinv, v.r,
+                       v.d0, c0, ms);
+    smooth_steps<T, B, NC>(L, v, st, ms, din);
+    hipLaunchKernelGGL((k_addto<T, NC>), g, b, 0, st, L.n, v.x, (const T*)din, ms);
 }
 
-template <typename T>
+template <typename T, int NC>
 static void vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms) {
     const AmgLevelDev<T>& L = D.lv[l];
     if (l + 1 == D.lv.size()) {
-        hipLaunchKernelGGL((k_dense<T>), dim3((unsigned)D.nc), dim3(64), 0, st, D.nc, (const T*)D.cinv,
-                           (const T*)L.b, L.x, ms);
+        const LevelVecs<T> v = vecs<T, NC>(L);
+        hipLaunchKernelGGL((k_dense<T, NC>), dim3((unsigned)D.nc), dim3(64), 0, st, D.nc, (const T*)D.cinv,
+                           (const T*)v.b, v.x, ms);
         return;
     }
     switch (L.A.bs) {
-        case 1: level_cycle<T, 1>(D, l, st, ms); break;
-        case 3: level_cycle<T, 3>(D, l, st, ms); break;
+        case 1: level_cycle<T, 1, NC>(D, l, st, ms); break;
+        case 3: level_cycle<T, 3, NC>(D, l, st, ms); break;
         default: throw Error(VEMD_ERR_INTERNAL, "unsupported multigrid block size");
     }
 }
 
 template <typename T>
 void amg_vcycle(const AmgDev<T>& D, hipStream_t st, const int* ms) {
-    vcycle_level(D, 0, st, ms);
+    vcycle_level<T, 1>(D, 0, st, ms);
+    HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+void amg_multi_alloc(AmgDev<T>& D) {
+    if (D.multi) return;
+    for (AmgLevelDev<T>& L : D.lv) {
+        const size_t vb = (size_t)L.n * MCOLS * sizeof(T);
+        L.mb = (T*)amg_alloc(D, vb);
+        L.mx = (T*)amg_alloc(D, vb);
+        L.mr = (T*)amg_alloc(D, vb);
+        L.md0 = (T*)amg_alloc(D, vb);
+        L.md1 = (T*)amg_alloc(D, vb);
+        for (T* p : {L.mb, L.mx, L.mr, L.md0, L.md1}) HIPCHK(hipMemset(p, 0, vb));
+    }
+    D.multi = true;
+}
+
+template <typename T>
+void amg_vcycle_multi(const AmgDev<T>& D, hipStream_t st, const int* ms) {
+    if (!D.multi) throw Error(VEMD_ERR_STATE, "multi-column cycle buffers not allocated");
+    vcycle_level<T, MCOLS>(D, 0, st, ms);
     HIPCHK(hipGetLastError());
 }
 
@@ -323,5 +396,9 @@ template AmgDev<double>* amg_upload<double>(const Amg&);
 template AmgDev<float>* amg_upload<float>(const Amg&);
 template void amg_vcycle<double>(const AmgDev<double>&, hipStream_t, const int*);
 template void amg_vcycle<float>(const AmgDev<float>&, hipStream_t, const int*);
+template void amg_multi_alloc<double>(AmgDev<double>&);
+template void amg_multi_alloc<float>(AmgDev<float>&);
+template void amg_vcycle_multi<double>(const AmgDev<double>&, hipStream_t, const int*);
+template void amg_vcycle_multi<float>(const AmgDev<float>&, hipStream_t, const int*);
 
 }  // namespace vemd

@@ -48,6 +48,11 @@ void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref)
 void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* y1, void* stream);
 void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, double* sol_ref, vemd_solve_info* info);
 void dev_solve_int(Solver& s, const vemd_solve_opts& o, const void* b_dev, void* x_dev, vemd_solve_info* info);
+// Up to 32 right-hand sides at once (FP32 MFMA iterations, FP64 refinement):
+// b and x are ncol column-major device vectors of the internal layout.
+void dev_solve_multi(Solver& s, const vemd_solve_opts& o, int ncol, const double* b, double* x, vemd_solve_info* info);
+// MFMA apply on 32 interleaved FP32 columns (second sides of edge dofs in y1).
+void dev_apply_multi(Solver& s, const void* x, void* y, void* y1, void* stream);
 void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, const double* dldx_ref, double* grad,
                  vemd_solve_info* info);
 
