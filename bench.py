@@ -406,6 +406,14 @@ def main():
                          "frac": round(gbs32 / HBM_PEAK_GBS, 4), "bytes_per_apply": int(b32),
                          "ms_per_apply": round(ms32, 4)},
                 "complete_apply_ms": {"fp64": round(ms64f, 4), "fp32": round(ms32f, 4)}}
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes of
+    # this workload (counters cannot be read from inside the timed run)
+    tpath = os.path.join(ROOT, "profiles", "k_apply_1m_annulus_traffic.json")
+    if args.cells == 1000000 and args.seed == 2 and os.path.exists(tpath):
+        t = json.load(open(tpath))
+        roofline["traffic"] = t["traffic_bytes"]["fp64"]
+        roofline["fp32"]["traffic"] = t["traffic_bytes"]["fp32"]
+        roofline["traffic_source"] = "profiles/k_apply_1m_annulus_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
     if args.mode == "apply":
         # calibration for hardware byte counters: a plain copy of known size
         src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")

@@ -96,6 +96,38 @@ __device__ __forceinline__ void bstore1(float v, Rsrc r, uint32_t voff, uint32_t
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
 }
 
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+// The k + 1 adjacent moments of one edge: one 4-, 8- or 16-byte access.
+template <typename T, int KK>
+__device__ __forceinline__ void load_edge(Rsrc r, uint32_t off, T (&v)[KK]) {
+    if constexpr (sizeof(T) * KK == 16) {
+        d2v w = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0u, 0));
+        v[0] = w[0];
+        v[1] = w[1];
+    } else if constexpr (sizeof(T) * KK == 8 && KK == 2) {
+        f2v w = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
+        v[0] = w[0];
+        v[1] = w[1];
+    } else {
+#pragma unroll
+        for (int i = 0; i < KK; ++i) v[i] = bload1(r, off + i * (uint32_t)sizeof(T), 0u, (T*)nullptr);
+    }
+}
+template <typename T, int KK>
+__device__ __forceinline__ void store_edge(Rsrc r, uint32_t off, const T (&v)[KK]) {
+    if constexpr (sizeof(T) * KK == 16) {
+        d2v w = {v[0], v[1]};
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4v, w), r, off, 0u, 0);
+    } else if constexpr (sizeof(T) * KK == 8 && KK == 2) {
+        f2v w = {v[0], v[1]};
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, w), r, off, 0u, 0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < KK; ++i) bstore1(v[i], r, off + i * (uint32_t)sizeof(T), 0u);
+    }
+}
+
 // Marks register values as used and redefined here (an empty asm statement),
 // so the arithmetic that produces them cannot sink below this point.
 template <typename T, int M>
@@ -142,15 +174,21 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
     const Rsrc rx = make_rsrc(x, a.nvec * (int64_t)TS);
     const Rsrc ry = make_rsrc(a.y, a.nvec * (int64_t)TS);
     const Rsrc ry1 = make_rsrc(a.y1, a.cbase * (int64_t)TS);
-    const Rsrc ri = make_rsrc(a.idx + g.i_off + (int64_t)tau * NED * 64, (int64_t)NED * 64 * 4);
+    const Rsrc ri = make_rsrc(a.idx + g.i_off + (int64_t)tau * NE * 64, (int64_t)NE * 64 * 4);
     const Rsrc rO = make_rsrc(a.O + g.o_off + (int64_t)tau * NV * 64 * VW, (int64_t)NV * 64 * 16);
     const uint32_t lv = (uint32_t)l * 16u;
-    uint32_t id[NED];
+    constexpr int KK = SH::KK;
+    uint32_t id[NE];
     T u[N], ps[Q], acc[N], yp[Q];
 #pragma unroll
-    for (int j = 0; j < NED; ++j) id[j] = bload_u32(ri, (uint32_t)l * 4u, (uint32_t)j * 256u);
+    for (int j = 0; j < NE; ++j) id[j] = bload_u32(ri, (uint32_t)l * 4u, (uint32_t)j * 256u);
 #pragma unroll
-    for (int j = 0; j < NED; ++j) u[j] = bload1(rx, (id[j] & 0x7fffffffu) * TS, 0u, (T*)nullptr);
+    for (int j = 0; j < NE; ++j) {
+        T v[KK];
+        load_edge<T, KK>(rx, (id[j] & 0x7fffffffu) * TS, v);
+#pragma unroll
+        for (int r = 0; r < KK; ++r) u[r * NE + j] = v[r];
+    }
 #pragma unroll
     for (int j = 0; j < NI; ++j) u[NED + j] = bload1(rx, gpos * TS, (uint32_t)((a.cbase + j * a.cpad) * TS), (T*)nullptr);
     T p[Q];
@@ -257,12 +295,15 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
         for (int r = 0; r < Q; ++r) dot += (double)p[r] * (double)yp[r];
     }
 #pragma unroll
-    for (int j = 0; j < NED; ++j) {
+    for (int j = 0; j < NE; ++j) {
         const uint32_t off = (id[j] & 0x7fffffffu) * TS;
+        T v[KK];
+#pragma unroll
+        for (int r = 0; r < KK; ++r) v[r] = acc[r * NE + j];
         if (id[j] & SIDE1)
-            bstore1(acc[j], ry1, off, 0u);
+            store_edge<T, KK>(ry1, off, v);
         else
-            bstore1(acc[j], ry, off, 0u);
+            store_edge<T, KK>(ry, off, v);
     }
 #pragma unroll
     for (int j = 0; j < NI; ++j) bstore1(acc[NED + j], ry, gpos * TS, (uint32_t)((a.cbase + j * a.cpad) * TS));
@@ -669,7 +710,8 @@ __device__ __forceinline__ double cell_quad(const ApplyArgs<T>& a, const T* lam,
     T u[N], m[N];
 #pragma unroll
     for (int j = 0; j < NED; ++j) {
-        uint32_t k = a.idx[g.i_off + ((int64_t)tau * NED + j) * 64 + l] & 0x7fffffffu;
+        const int e = j % NE, r = j / NE;
+        uint32_t k = (a.idx[g.i_off + ((int64_t)tau * NE + e) * 64 + l] & 0x7fffffffu) + r;
         u[j] = x[k];
         m[j] = lam[k];
     }

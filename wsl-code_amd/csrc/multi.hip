@@ -112,8 +112,10 @@ __device__ __forceinline__ void mr_cell(const MrArgs& a, const GroupDesc& g, int
     for (int t = l; t < SZ4 / 4; t += 64) reinterpret_cast<float4*>(lop)[t] = src[t];
     if (l < M) {
         uint32_t sl;
-        if (l < NED)
-            sl = a.idx[g.i_off + ((cg >> 6) * NED + l) * 64 + (cg & 63)];
+        if (l < NED) {
+            const uint32_t id = a.idx[g.i_off + ((cg >> 6) * NE + l % NE) * 64 + (cg & 63)];
+            sl = (id & SIDE1) | ((id & 0x7fffffffu) + (uint32_t)(l / NE));
+        }
         else if (l < N)
             sl = (uint32_t)(a.cbase + (l - NED) * a.cpad + gpos);
         else

@@ -133,9 +133,11 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     int kk = L.k + 1;
     L.cbase = (int64_t)kk * m.ne;
     int nslot = L.ni + L.q;
-    L.lam_idx = L.lam ? L.cbase + (int64_t)nslot * L.cpad : -1;
-    L.trash_idx = L.cbase + (int64_t)nslot * L.cpad + (L.lam ? 1 : 0);
-    L.nvec = L.trash_idx + 1;
+    // trash slots of padding lanes (k + 1 entries, aligned like an edge's
+    // moments since cbase and cpad are even), then the multiplier
+    L.trash_idx = L.cbase + (int64_t)nslot * L.cpad;
+    L.lam_idx = L.lam ? L.trash_idx + kk : -1;
+    L.nvec = L.trash_idx + kk + (L.lam ? 1 : 0);
     // Krylov sweep chunks, instance-major
     if (L.ninst > 1 && L.lam) throw Error(VEMD_ERR_ARG, "a batch needs the Dirichlet-pressure condition");
     L.chunks.clear();
@@ -180,13 +182,12 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     L.groups32 = L.groups;
     for (size_t gi = 0; gi < L.groups.size(); ++gi) {
         GroupDesc& g = L.groups[gi];
-        int ned = kk * g.ne;
         g.i_off = ioff;
         g.o_off = ooff2;
         g.l_off = loff;
         L.groups32[gi] = g;
         L.groups32[gi].o_off = ooff4;
-        ioff += (int64_t)ned * g.npad;
+        ioff += (int64_t)g.ne * g.npad;
         ooff2 += (int64_t)nvec_of(g.n, 2) * 2 * g.npad;
         ooff4 += (int64_t)nvec_of(g.n, 4) * 4 * g.npad;
         loff += (int64_t)L.q * g.npad;
@@ -196,19 +197,18 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     L.l_len = loff;
     L.idx.assign(ioff, (uint32_t)L.trash_idx);
     if (L.nvec >= (int64_t)SIDE1) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit dof indices");
+    // one index per local edge: the edge's first moment (its k + 1 moments are
+    // adjacent), flagged when the cell is the edge's second side
     for (auto& g : L.groups) {
-        int ned = kk * g.ne;
         for (int a = 0; a < g.npad; ++a) {
             int c = L.cell_of_gpos[g.gpos0 + a];
             if (c < 0) continue;
-            for (int r = 0; r < kk; ++r)
-                for (int j = 0; j < g.ne; ++j) {
-                    int e = m.cell_edges[m.off[c] + j];
-                    uint32_t id = (uint32_t)(kk * L.edge_int[e] + r);
-                    if (L.gpos_of_cell[c] != side0[e]) id |= SIDE1;
-                    L.idx[g.i_off + ((int64_t)(a / TILE) * ned + r * g.ne + j) * TILE + a % TILE] = id;
-                }
-            (void)ned;
+            for (int j = 0; j < g.ne; ++j) {
+                int e = m.cell_edges[m.off[c] + j];
+                uint32_t id = (uint32_t)(kk * L.edge_int[e]);
+                if (L.gpos_of_cell[c] != side0[e]) id |= SIDE1;
+                L.idx[g.i_off + ((int64_t)(a / TILE) * g.ne + j) * TILE + a % TILE] = id;
+            }
         }
     }
     return L;

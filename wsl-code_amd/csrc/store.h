@@ -47,7 +47,7 @@ struct Layout {
     std::vector<int> spatial_gpos;      // store positions of the cells in spatial (Hilbert) order
     std::vector<int> edge_int;          // internal number of each mesh edge
     std::vector<int64_t> int_of_ref;    // reference dof -> internal slot
-    std::vector<uint32_t> idx;          // packed edge-dof indices (SoA per group) with the SIDE1 flag
+    std::vector<uint32_t> idx;          // first moment of every local edge (tile-major per group), SIDE1 flag
     int64_t o_len2 = 0, o_len4 = 0, l_len = 0;  // packed sizes (elements) of the FP64 / FP32 streams
     // Batch instances (one for a plain mesh): each owns a contiguous range of
     // tiles, store positions and internal edges.
