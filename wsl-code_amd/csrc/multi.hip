@@ -88,101 +88,145 @@ struct MrArgs {
     const int* all_done;
 };
 
-// Entry (i, c) of the cell block [[s A, -W^T], [-W, 0]] from the packed
-// operator in LDS.
-template <int N, int NA>
-__device__ __forceinline__ float kval(const float* op, int i, int c, float s) {
-    if (i < N && c < N) {
+// Entry (i, c) of the cell block [[s A, -W^T], [-W, 0]] (n flux rows, m rows
+// in all) from the packed operator in LDS.
+__device__ __forceinline__ float kval(const float* op, int n, int na, int m, int i, int c, float s) {
+    if (i >= m || c >= m) return 0.0f;
+    if (i < n && c < n) {
         const int a = i > c ? i : c, b = i > c ? c : i;
         return s * op[a * (a + 1) / 2 + b];
     }
-    if (i < N) return -op[NA + (c - N) * N + i];
-    if (c < N) return -op[NA + (i - N) * N + c];
+    if (i < n) return -op[na + (c - n) * n + i];
+    if (c < n) return -op[na + (i - n) * n + c];
     return 0.0f;
 }
 
-template <int K, int NE>
-__device__ __forceinline__ void mr_cell(const MrArgs& a, const GroupDesc& g, int gi, int64_t gpos, int l,
-                                        float* lop, uint32_t* lslot, const float* Xj, double& dot) {
-    constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE, N = NED + NI;
-    constexpr int M = N + Q, NA = N * (N + 1) / 2, SZ = NA + Q * N, SZ4 = (SZ + 3) / 4 * 4;
-    static_assert(SZ4 <= OPMAX && M <= 32, "cell block larger than the MFMA tile");
+constexpr int KSTEPS = 13;  // ceil(m / 2) <= 13 for polygons of up to 10 edges (k = 1)
+
+// What a wave needs of one cell: the shape (wave-uniform), two float4 of the
+// packed operator per lane and the lane's row slot (rows < m).
+struct CellMeta {
+    int64_t gpos;    // -1: none
+    int n, m, na, nv4;
+    float s;
+    float4 o0, o1;
+    uint32_t slot;
+};
+
+template <int K>
+__device__ __forceinline__ void cell_meta(const MrArgs& a, int64_t gpos, int l, CellMeta& c) {
+    constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1));
+    c.gpos = gpos;
+    if (gpos < 0) return;
+    const int gi = __builtin_amdgcn_readfirstlane(a.tile_group[gpos >> 6]);
+    const GroupDesc g = a.groups[gi];
+    const int ne = g.ne, n = g.n, ned = KK * ne;
+    c.n = n;
+    c.m = n + Q;
+    c.na = n * (n + 1) / 2;
+    c.nv4 = (c.na + Q * n + 3) / 4;
     const int64_t cg = gpos - g.gpos0;
-    const float4* src = reinterpret_cast<const float4*>(a.op + a.gop[gi] + cg * SZ4);
-    for (int t = l; t < SZ4 / 4; t += 64) reinterpret_cast<float4*>(lop)[t] = src[t];
-    if (l < M) {
-        uint32_t sl;
-        if (l < NED) {
-            const uint32_t id = a.idx[g.i_off + ((cg >> 6) * NE + l % NE) * 64 + (cg & 63)];
-            sl = (id & SIDE1) | ((id & 0x7fffffffu) + (uint32_t)(l / NE));
-        }
-        else if (l < N)
-            sl = (uint32_t)(a.cbase + (l - NED) * a.cpad + gpos);
-        else
-            sl = (uint32_t)(a.cbase + (NI + l - N) * a.cpad + gpos);
-        lslot[l] = sl;
+    const float4* src = reinterpret_cast<const float4*>(a.op + a.gop[gi]) + cg * c.nv4;
+    const float4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+    c.o0 = l < c.nv4 ? src[l] : z;
+    c.o1 = l + 64 < c.nv4 ? src[l + 64] : z;
+    c.s = a.scale[gpos];
+    uint32_t sl = 0;
+    if (l < ned) {
+        const uint32_t id = a.idx[g.i_off + ((cg >> 6) * ne + l % ne) * 64 + (cg & 63)];
+        sl = (id & SIDE1) | ((id & 0x7fffffffu) + (uint32_t)(l / ne));
+    } else if (l < n) {
+        sl = (uint32_t)(a.cbase + (l - ned) * a.cpad + gpos);
+    } else if (l < c.m) {
+        sl = (uint32_t)(a.cbase + (NI + l - n) * a.cpad + gpos);
     }
-    // the wave's LDS writes are complete before any lane reads them
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __builtin_amdgcn_wave_barrier();
-    const float s = a.scale[gpos];
-    const int i = l & 31, hi = l >> 5;
-    f16v acc = {};
-#pragma unroll
-    for (int kk = 0; kk < (M + 1) / 2; ++kk) {
-        const int c = 2 * kk + hi;
-        float av = 0.0f, bv = 0.0f;
-        if (c < M) {
-            if (i < M) av = kval<N, NA>(lop, i, c, s);
-            bv = Xj[(int64_t)(lslot[c] & 0x7fffffffu) * NR + i];
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
-    }
-    // D[row][col]: col = lane % 32, row = 8 (r / 4) + 4 (lane / 32) + r % 4
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = 8 * (r >> 2) + 4 * hi + (r & 3);
-        if (row < M) {
-            const uint32_t sl = lslot[row];
-            const int64_t e = (int64_t)(sl & 0x7fffffffu) * NR + i;
-            if (sl & SIDE1)
-                a.Y1[e] = acc[r];
-            else
-                a.Y[e] = acc[r];
-            if (a.pw) dot += (double)Xj[e] * (double)acc[r];
-        }
-    }
-    __builtin_amdgcn_wave_barrier();  // LDS reuse by the next cell
+    c.slot = sl;
 }
 
+// Next store position at or after p with a cell, or -1.
+__device__ __forceinline__ int64_t next_cell(const MrArgs& a, int64_t p, int64_t end) {
+    while (p < end && a.cell_of_gpos[p] < 0) ++p;
+    return p < end ? p : -1;
+}
+
+// Persistent waves, each over a contiguous range of store positions.  The cell
+// loop is software-pipelined: while cell c is multiplied, the rows of cell
+// c + 1 are being gathered and the operator of cell c + 2 is being loaded.
 template <int K>
 __global__ __launch_bounds__(256) void k_apply_mr(MrArgs a) {
     if (a.all_done && *a.all_done) return;
-    __shared__ __attribute__((aligned(16))) float lop[4][OPMAX];
-    __shared__ uint32_t lslot[4][32];
+    __shared__ __attribute__((aligned(16))) float lop[4][2][OPMAX];
+    __shared__ uint32_t lslot[4][2][32];
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int wave = blockIdx.x * 4 + w;
     const int l = threadIdx.x & 63;
-    const int col = l & 31;
+    const int col = l & 31, hi = l >> 5;
     const float* Xj = a.st ? a.X[a.st[col].itn & 1] : a.X[0];
     double dot = 0.0;
     const int64_t g0 = (int64_t)wave * a.cells_per_wave;
     const int64_t g1 = g0 + a.cells_per_wave < a.cpad ? g0 + a.cells_per_wave : a.cpad;
-    for (int64_t gpos = g0; gpos < g1; ++gpos) {
-        if (a.cell_of_gpos[gpos] < 0) continue;
-        const int gi = __builtin_amdgcn_readfirstlane(a.tile_group[gpos >> 6]);
-        const GroupDesc g = a.groups[gi];
-        switch (g.ne) {
-            case 3: mr_cell<K, 3>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
-            case 4: mr_cell<K, 4>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
-            case 5: mr_cell<K, 5>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
-            case 6: mr_cell<K, 6>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
-            case 7: mr_cell<K, 7>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
-            case 8: mr_cell<K, 8>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
-            case 9: mr_cell<K, 9>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
-            case 10: mr_cell<K, 10>(a, g, gi, gpos, l, lop[w], lslot[w], Xj, dot); break;
-            default: break;
+    // stage a cell's operator and slots into LDS buffer b and issue its row gathers
+    auto stage = [&](const CellMeta& c, int b, float (&B)[KSTEPS]) {
+        float4* d = reinterpret_cast<float4*>(lop[w][b]);
+        if (l < c.nv4) d[l] = c.o0;
+        if (l + 64 < c.nv4) d[l + 64] = c.o1;
+        if (l < c.m) lslot[w][b][l] = c.slot;
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+        const int ks = (c.m + 1) / 2;
+#pragma unroll
+        for (int kk = 0; kk < KSTEPS; ++kk) {
+            B[kk] = 0.0f;
+            const int cc = 2 * kk + hi;
+            if (kk < ks && cc < c.m) B[kk] = Xj[(int64_t)(lslot[w][b][cc] & 0x7fffffffu) * NR + col];
         }
+    };
+    CellMeta cur, nxt, nn;
+    float Bc[KSTEPS], Bn[KSTEPS];
+    cell_meta<K>(a, next_cell(a, g0, g1), l, cur);
+    int b = 0;
+    if (cur.gpos >= 0) {
+        stage(cur, b, Bc);
+        cell_meta<K>(a, next_cell(a, cur.gpos + 1, g1), l, nxt);
+    }
+    while (cur.gpos >= 0) {
+        if (nxt.gpos >= 0) {
+            stage(nxt, b ^ 1, Bn);
+            cell_meta<K>(a, next_cell(a, nxt.gpos + 1, g1), l, nn);
+        } else {
+            nn.gpos = -1;
+        }
+        // multiply cell cur
+        const float* op = lop[w][b];
+        const int ks = (cur.m + 1) / 2;
+        f16v acc = {};
+#pragma unroll
+        for (int kk = 0; kk < KSTEPS; ++kk) {
+            if (kk < ks) {
+                const float av = kval(op, cur.n, cur.na, cur.m, col, 2 * kk + hi, cur.s);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Bc[kk], acc, 0, 0, 0);
+            }
+        }
+        // D[row][col]: col = lane % 32, row = 8 (r / 4) + 4 (lane / 32) + r % 4
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = 8 * (r >> 2) + 4 * hi + (r & 3);
+            if (row < cur.m) {
+                const uint32_t sl = lslot[w][b][row];
+                const int64_t e = (int64_t)(sl & 0x7fffffffu) * NR + col;
+                if (sl & SIDE1)
+                    a.Y1[e] = acc[r];
+                else
+                    a.Y[e] = acc[r];
+                if (a.pw) dot += (double)Xj[e] * (double)acc[r];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();  // buffer b is free for the cell after next
+        cur = nxt;
+        nxt = nn;
+#pragma unroll
+        for (int kk = 0; kk < KSTEPS; ++kk) Bc[kk] = Bn[kk];
+        b ^= 1;
     }
     if (a.pw) {
         dot += __shfl_xor(dot, 32, 64);
@@ -526,7 +570,7 @@ static MultiState& multi_setup(Solver& s) {
     int dev = 0, ncu = 256;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    m.nwaves = std::max(4, ncu * 8) / 4 * 4;
+    m.nwaves = std::max(4, ncu * 16) / 4 * 4;
     m.cells_per_wave = (L.cpad + m.nwaves - 1) / m.nwaves;
     const size_t fv = (size_t)L.nvec * NR * sizeof(float);
     for (int i = 0; i < 2; ++i) {
