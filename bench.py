@@ -51,6 +51,9 @@ def algorithmic_bytes(valences, k, ntot, w):
     return total + w * 2 * ntot
 
 
+SETUP_DEVICE = None  # GPU index for the element setup (None: host cores)
+
+
 def build(pkg, cells, seed):
     from wsl_code_amd.fields import grf_theta
     t0 = time.time()
@@ -61,10 +64,11 @@ def build(pkg, cells, seed):
     theta = grf_theta(a["cx"], a["cy"], sigma=1.0, ell=0.05, seed=seed + 1)
     t2 = time.time()
     s = pkg.Solver(mesh, k=1, nq2d=9, nq1d=9, map_kind=pkg.MAP_ARCS, solution=pkg.SOL_SINSIN, bc=pkg.BC_DIRICHLET_P,
-                   theta=theta)
+                   theta=theta, setup_device=SETUP_DEVICE)
     t3 = time.time()
-    log("mesh %d cells %.1fs, theta %.1fs, FP64 element setup %.1fs; unknowns %d, valence groups %d"
-        % (mesh.num_cells, t1 - t0, t2 - t1, t3 - t2, s.ntot, s.ngroups))
+    log("mesh %d cells %.1fs, theta %.1fs, FP64 element setup (%s) and store layout %.1fs; unknowns %d, "
+        "valence groups %d" % (mesh.num_cells, t1 - t0, t2 - t1, "host" if SETUP_DEVICE is None else "GPU",
+                               t3 - t2, s.ntot, s.ngroups))
     return mesh, a, theta, s
 
 
@@ -196,11 +200,14 @@ def batch_bench(pkg, torch, ninst, cmin, cmax, seed):
         thetas.append(grf_theta(a["cx"], a["cy"], sigma=rng.uniform(0.5, 2.0), ell=rng.uniform(0.02, 0.1),
                                 seed=seed * 1000 + i + 500000, features=256))
         meshes.append(m)
+        if (i + 1) % 32 == 0:
+            log("batch: %d of %d instances generated (%.0fs)" % (i + 1, ninst, time.time() - t0))
     u = pkg.Mesh.concat(meshes)
     t1 = time.time()
     s = pkg.Solver(u, k=1, map_kind=pkg.MAP_ARCS, solution=pkg.SOL_SINSIN, bc=pkg.BC_DIRICHLET_P,
-                   theta=np.concatenate(thetas))
+                   theta=np.concatenate(thetas), setup_device=SETUP_DEVICE)
     t2 = time.time()
+    log("batch: element setup of %d cells %.1fs" % (u.num_cells, t2 - t1))
     s.upload(0)
     log("batch: %d instances, %d cells, %d unknowns; meshes+theta %.1fs, setup %.1fs, upload %.1fs"
         % (ninst, u.num_cells, s.ntot, t1 - t0, t2 - t1, time.time() - t2))
@@ -362,6 +369,8 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--inst-cells", type=int, nargs=2, default=[10000, 200000])
     ap.add_argument("--nrhs", type=int, default=32)
+    ap.add_argument("--setup", choices=["host", "device"], default="host",
+                    help="where the FP64 element operators are built (not part of the timed solve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; a figure at %d GPUs is not produced (one process per GPU "
@@ -370,6 +379,8 @@ def main():
     import torch
     pkg = load_pkg()
     assert torch.cuda.is_available() and pkg.device_count() > 0, "no HIP device"
+    global SETUP_DEVICE
+    SETUP_DEVICE = 0 if args.setup == "device" else None
     if args.mode == "adjoint":
         print(json.dumps(adjoint_bench(pkg, torch, args.cells if args.cells != 1000000 else 250000, args.seed,
                                        args.fd_directions)))

@@ -135,6 +135,14 @@ typedef struct {
  * (CPU/solver/DarcySolver.h:21-25, CPU/solver/DarcySolver.cpp:402-500). */
 vemd_status vemd_solver_create(vemd_mesh m, const vemd_problem_desc* desc, vemd_solver* out);
 
+/* The same solver with the element operators built on a GPU (one thread per
+ * cell, FP64; k = 0 or 1, polygons of up to 10 edges) and copied back into
+ * the host arrays; agrees with vemd_solver_create to rounding.  Replaces the
+ * same constructor and the reference GPU port's per-element integration from
+ * host-packed quadrature samples (GPU/src/hdiv_gpu.cu:45-135,
+ * GPU/src/mesh_pack.cpp:31-142).  Does not upload the solver. */
+vemd_status vemd_solver_create_device(vemd_mesh m, const vemd_problem_desc* desc, int device, vemd_solver* out);
+
 /* sizes: [total flux dofs, total pressure dofs, total unknowns (incl. multiplier),
  *         internal device vector length, number of valence groups, max valence].
  * Replaces HdivMatrix::getTotalDofFlux / getTotalDofConc (CPU/solver/HdivMatrix.h:69-71). */

@@ -257,8 +257,8 @@ static int aggregate(const std::vector<std::vector<std::pair<int, double>>>& str
 }
 
 static void dense_inverse(std::vector<double>& a, int64_t n) {
-    // Gauss-Jordan with partial pivoting (the coarsest operator has a few
-    // hundred rows).
+    // Gauss-Jordan with partial pivoting (the coarsest operator has at most a
+    // few thousand rows); the row eliminations of each pivot run in parallel.
     std::vector<double> inv(n * n, 0.0);
     for (int64_t i = 0; i < n; ++i) inv[i * n + i] = 1.0;
     for (int64_t c = 0; c < n; ++c) {
@@ -276,15 +276,21 @@ static void dense_inverse(std::vector<double>& a, int64_t n) {
             a[c * n + j] *= d;
             inv[c * n + j] *= d;
         }
-        for (int64_t r = 0; r < n; ++r) {
-            if (r == c) continue;
-            double f = a[r * n + c];
-            if (f == 0.0) continue;
-            for (int64_t j = 0; j < n; ++j) {
-                a[r * n + j] -= f * a[c * n + j];
-                inv[r * n + j] -= f * inv[c * n + j];
+        auto eliminate = [&](int64_t rb, int64_t re) {
+            for (int64_t r = rb; r < re; ++r) {
+                if (r == c) continue;
+                double f = a[r * n + c];
+                if (f == 0.0) continue;
+                for (int64_t j = 0; j < n; ++j) {
+                    a[r * n + j] -= f * a[c * n + j];
+                    inv[r * n + j] -= f * inv[c * n + j];
+                }
             }
-        }
+        };
+        if (n >= 512)
+            parallel_for(n, eliminate);
+        else
+            eliminate(0, n);
     }
     a.swap(inv);
 }

@@ -22,7 +22,7 @@ struct Csr {
 
 struct AmgOptions {
     int max_levels = 12;
-    int coarse_size = 600;     // the coarsest operator is inverted densely
+    int coarse_size = 1500;    // the coarsest operator is inverted densely
     double strength = 0.08;    // strength-of-connection threshold on the finest level (halved per level)
     int degree0 = 2;           // Chebyshev steps before and after the coarse correction, finest level
     int degree = 2;            // the same on the coarser levels

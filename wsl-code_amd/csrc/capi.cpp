@@ -78,6 +78,12 @@ void Solver::build_amg() {
     amg.reset(new Amg(amg_build(std::move(S0), node, (int)L.cpad, b0, L.spatial_gpos, amg_opt)));
     amg->bs0 = L.q;
     amg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (std::getenv("VEMD_VERBOSE")) {
+        std::fprintf(stderr, "[vemd] multigrid setup %.2fs:", amg_setup_s);
+        for (const AmgLevel& l : amg->lv)
+            std::fprintf(stderr, " %lld rows/%lld nnz", (long long)l.A.nrows, (long long)l.A.nnz());
+        std::fprintf(stderr, "\n");
+    }
 }
 
 // L2 errors of a reference-order solution (computational and physical measure)
@@ -238,6 +244,25 @@ vemd_status vemd_solver_create(vemd_mesh mh, const vemd_problem_desc* d, vemd_so
     } else {
         build_rhs_ref(*s->mesh, *s->setup, s->rhs_ref);
     }
+    *out = new vemd_solver_s{std::move(s)};
+    API_END
+}
+
+vemd_status vemd_solver_create_device(vemd_mesh mh, const vemd_problem_desc* d, int device, vemd_solver* out) {
+    API_BEGIN
+    REQUIRE(mh && d && out, "null argument");
+    std::unique_ptr<Solver> s(new Solver());
+    s->mesh = mh->m;
+    auto t0 = std::chrono::steady_clock::now();
+    s->setup.reset(setup_build_device(*s->mesh, *d, device));
+    auto t1 = std::chrono::steady_clock::now();
+    s->layout = layout_build(*s->mesh, *s->setup);
+    auto t2 = std::chrono::steady_clock::now();
+    s->refresh_theta_dependent();
+    if (std::getenv("VEMD_VERBOSE"))
+        std::fprintf(stderr, "[vemd] device element setup %.2fs, store layout %.2fs, theta-dependent data %.2fs\n",
+                     std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t2 - t1).count(),
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t2).count());
     *out = new vemd_solver_s{std::move(s)};
     API_END
 }

@@ -100,7 +100,20 @@ struct Setup {
     std::vector<int64_t> p_off;
 };
 
+// Host (FP64, all cores) element setup.
 Setup* setup_build(const Mesh& m, const vemd_problem_desc& d);
+// The same element data computed on a GPU (k = 0 and k = 1, polygons of up to
+// 10 edges) and copied back into the host arrays.
+Setup* setup_build_device(const Mesh& m, const vemd_problem_desc& d, int device);
+// Pieces shared by both: validation, offsets and sized arrays; the global dof
+// map and signs; the quadrature rules.
+Setup* setup_prepare(const Mesh& m, const vemd_problem_desc& d);
+void setup_finish(const Mesh& m, Setup* S);
+struct QuadRules {
+    std::vector<double> ta, tb, tw;  // triangle rule on (0,0), (1,0), (0,1)
+    std::vector<double> glx, glw;    // Gauss-Legendre on [-1, 1]
+};
+QuadRules quad_rules(int nq2d, int nq1d);
 
 }  // namespace vemd
 

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 
 #include "common.h"
 
@@ -687,19 +688,29 @@ static void build_cell(const Mesh& m, const vemd_problem_desc& d, const Shape& S
     std::memcpy(o.P, w.P.data(), sizeof(double) * p * n);
 }
 
-Setup* setup_build(const Mesh& m, const vemd_problem_desc& d) {
+QuadRules quad_rules(int nq2d, int nq1d) {
+    QuadRules r;
+    Rule t = tri_rule(nq2d);
+    r.ta = t.a;
+    r.tb = t.b;
+    r.tw = t.w;
+    gauss_legendre(nq1d, r.glx, r.glw);
+    return r;
+}
+
+Setup* setup_prepare(const Mesh& m, const vemd_problem_desc& d) {
     if (d.k < 0 || d.k > 3) throw Error(VEMD_ERR_ARG, "k must be between 0 and 3");
     if (d.k == 3 && d.nq2d != 12) throw Error(VEMD_ERR_ARG, "k=3 requires 12 triangle points");
     if (d.map_kind != VEMD_MAP_SIN && d.map_kind != VEMD_MAP_ARCS) throw Error(VEMD_ERR_ARG, "map_kind");
     if (d.bc != VEMD_BC_FLUX_MEANZERO && d.bc != VEMD_BC_DIRICHLET_P) throw Error(VEMD_ERR_ARG, "bc");
+    (void)tri_rule(d.nq2d);  // validates nq2d
+    std::vector<double> glx, glw;
+    gauss_legendre(d.nq1d, glx, glw);
     Setup* S = new Setup();
     S->sh = Shape(d.k);
     S->desc = d;
     S->desc.theta = nullptr;
     const Shape& sh = S->sh;
-    Rule rule = tri_rule(d.nq2d);
-    std::vector<double> glx, glw;
-    gauss_legendre(d.nq1d, glx, glw);
     int nc = m.nc;
     auto& O = S->ofs;
     O.a_off.resize(nc + 1);
@@ -709,7 +720,10 @@ Setup* setup_build(const Mesh& m, const vemd_problem_desc& d) {
     O.a_off[0] = O.w_off[0] = O.v_off[0] = S->p_off[0] = 0;
     for (int c = 0; c < nc; ++c) {
         int n = sh.nflux(m.npc(c));
-        if (n > 63) throw Error(VEMD_ERR_MESH, "cell " + std::to_string(c) + " has too many edges");
+        if (n > 63) {
+            delete S;
+            throw Error(VEMD_ERR_MESH, "cell " + std::to_string(c) + " has too many edges");
+        }
         O.a_off[c + 1] = O.a_off[c] + n * (n + 1) / 2;
         O.w_off[c + 1] = O.w_off[c] + (int64_t)sh.q * n;
         O.v_off[c + 1] = O.v_off[c] + n;
@@ -726,15 +740,14 @@ Setup* setup_build(const Mesh& m, const vemd_problem_desc& d) {
     S->alpha.resize(nc);
     S->theta.assign(nc, 0.0);
     if (d.theta) S->theta.assign(d.theta, d.theta + nc);
-    parallel_for(nc, [&](int64_t b, int64_t e) {
-        Work w;
-        for (int64_t c = b; c < e; ++c) {
-            CellOut o{&S->A[O.a_off[c]], &S->W[O.w_off[c]], &S->L[c * sh.q], &S->f[c * sh.q],
-                      &S->ru[O.v_off[c]], &S->ub[O.v_off[c]], &S->isb[O.v_off[c]], &S->P[S->p_off[c]],
-                      &S->alpha[c]};
-            build_cell(m, S->desc, sh, rule, glx, glw, (int)c, w, o);
-        }
-    });
+    return S;
+}
+
+void setup_finish(const Mesh& m, Setup* S) {
+    const vemd_problem_desc& d = S->desc;
+    const Shape& sh = S->sh;
+    const int nc = m.nc;
+    const auto& O = S->ofs;
     // reference global dof of each local flux dof, and its sign
     S->gdof.resize(O.v_off[nc]);
     S->sgn.resize(O.v_off[nc]);
@@ -756,7 +769,27 @@ Setup* setup_build(const Mesh& m, const vemd_problem_desc& d) {
     S->nflux_total = (int64_t)m.ne * (d.k + 1) + (int64_t)nc * sh.ni;
     S->npres_total = (int64_t)nc * sh.q;
     S->ntot = S->nflux_total + S->npres_total + (d.bc == VEMD_BC_FLUX_MEANZERO ? 1 : 0);
-    return S;
+}
+
+Setup* setup_build(const Mesh& m, const vemd_problem_desc& d) {
+    std::unique_ptr<Setup> S(setup_prepare(m, d));
+    const Shape& sh = S->sh;
+    const auto& O = S->ofs;
+    Rule rule = tri_rule(d.nq2d);
+    std::vector<double> glx, glw;
+    gauss_legendre(d.nq1d, glx, glw);
+    Setup* Sp = S.get();
+    parallel_for(m.nc, [&](int64_t b, int64_t e) {
+        Work w;
+        for (int64_t c = b; c < e; ++c) {
+            CellOut o{&Sp->A[O.a_off[c]], &Sp->W[O.w_off[c]], &Sp->L[c * sh.q], &Sp->f[c * sh.q],
+                      &Sp->ru[O.v_off[c]], &Sp->ub[O.v_off[c]], &Sp->isb[O.v_off[c]], &Sp->P[Sp->p_off[c]],
+                      &Sp->alpha[c]};
+            build_cell(m, Sp->desc, sh, rule, glx, glw, (int)c, w, o);
+        }
+    });
+    setup_finish(m, Sp);
+    return S.release();
 }
 
 // L2 errors against the manufactured solution (CPU/solver/DarcySolver.cpp:942-1131):

@@ -20,9 +20,19 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 constexpr int NR = MCOLS;     // right-hand-side columns
 constexpr int OPMAX = 352;    // floats of the largest cell block (n = 23: A 276 + W 69, padded to 4)
 
+// A valence group as the multi-RHS apply sees it: its real cells occupy the
+// store positions [gpos0, gpos0 + count) (tile padding follows them).
+struct MrGroup {
+    int gpos0, count, ne, n;
+    int64_t gop;    // first float of the group's cell blocks
+    int64_t i_off;  // first entry of the group's edge-index block
+};
+constexpr int MR_MAXG = 32;
+
 struct MultiState {
     float* op = nullptr;            // per cell: packed A (lower triangle) then W (q x n), padded to 4 floats
-    int64_t* gop = nullptr;         // first float of every group's block of cells
+    MrGroup* mg = nullptr;          // group table of the apply
+    int ngroups = 0;
     int nwaves = 0;
     int64_t cells_per_wave = 0;
     float *R[2] = {nullptr, nullptr}, *Z[2] = {nullptr, nullptr}, *WB[2] = {nullptr, nullptr};
@@ -71,11 +81,9 @@ void multi_free(DeviceState& d) {
 // ---------------------------------------------------------------------------
 
 struct MrArgs {
-    const GroupDesc* groups;
-    const int* tile_group;
-    const int* cell_of_gpos;
+    const MrGroup* mg;
+    int ngroups;
     const float* op;
-    const int64_t* gop;
     const uint32_t* idx;
     const float* scale;
     const float* X[2];        // input; with st, the buffer of each column's iteration parity
@@ -113,20 +121,20 @@ struct CellMeta {
     uint32_t slot;
 };
 
+// Issues the loads of cell gpos of group g (all independent of each other:
+// the group table sits in LDS).
 template <int K>
-__device__ __forceinline__ void cell_meta(const MrArgs& a, int64_t gpos, int l, CellMeta& c) {
+__device__ __forceinline__ void cell_meta(const MrArgs& a, const MrGroup& g, int64_t gpos, int l, CellMeta& c) {
     constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1));
     c.gpos = gpos;
     if (gpos < 0) return;
-    const int gi = __builtin_amdgcn_readfirstlane(a.tile_group[gpos >> 6]);
-    const GroupDesc g = a.groups[gi];
     const int ne = g.ne, n = g.n, ned = KK * ne;
     c.n = n;
     c.m = n + Q;
     c.na = n * (n + 1) / 2;
     c.nv4 = (c.na + Q * n + 3) / 4;
     const int64_t cg = gpos - g.gpos0;
-    const float4* src = reinterpret_cast<const float4*>(a.op + a.gop[gi]) + cg * c.nv4;
+    const float4* src = reinterpret_cast<const float4*>(a.op + g.gop) + cg * c.nv4;
     const float4 z = {0.0f, 0.0f, 0.0f, 0.0f};
     c.o0 = l < c.nv4 ? src[l] : z;
     c.o1 = l + 64 < c.nv4 ? src[l + 64] : z;
@@ -143,10 +151,14 @@ __device__ __forceinline__ void cell_meta(const MrArgs& a, int64_t gpos, int l, 
     c.slot = sl;
 }
 
-// Next store position at or after p with a cell, or -1.
-__device__ __forceinline__ int64_t next_cell(const MrArgs& a, int64_t p, int64_t end) {
-    while (p < end && a.cell_of_gpos[p] < 0) ++p;
-    return p < end ? p : -1;
+// First real cell at or after store position p and before end, moving the
+// group cursor gi along (wave-uniform); -1 when there is none.
+__device__ __forceinline__ int64_t next_cell(const MrGroup* lg, int ng, int64_t p, int64_t end, int& gi) {
+    while (gi < ng && p >= (int64_t)lg[gi].gpos0 + lg[gi].count) {
+        ++gi;
+        if (gi < ng && p < lg[gi].gpos0) p = lg[gi].gpos0;
+    }
+    return gi < ng && p < end ? p : -1;
 }
 
 // Persistent waves, each over a contiguous range of store positions.  The cell
@@ -157,6 +169,10 @@ __global__ __launch_bounds__(256) void k_apply_mr(MrArgs a) {
     if (a.all_done && *a.all_done) return;
     __shared__ __attribute__((aligned(16))) float lop[4][2][OPMAX];
     __shared__ uint32_t lslot[4][2][32];
+    __shared__ MrGroup lg[MR_MAXG];
+    if ((int)threadIdx.x < a.ngroups) lg[threadIdx.x] = a.mg[threadIdx.x];
+    __syncthreads();
+    const int ng = a.ngroups;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int wave = blockIdx.x * 4 + w;
     const int l = threadIdx.x & 63;
@@ -183,16 +199,22 @@ __global__ __launch_bounds__(256) void k_apply_mr(MrArgs a) {
     };
     CellMeta cur, nxt, nn;
     float Bc[KSTEPS], Bn[KSTEPS];
-    cell_meta<K>(a, next_cell(a, g0, g1), l, cur);
+    int gi = 0;
+    int64_t p = next_cell(lg, ng, g0, g1, gi);
+    cell_meta<K>(a, lg[gi < ng ? gi : 0], p, l, cur);
     int b = 0;
     if (cur.gpos >= 0) {
         stage(cur, b, Bc);
-        cell_meta<K>(a, next_cell(a, cur.gpos + 1, g1), l, nxt);
+        p = next_cell(lg, ng, cur.gpos + 1, g1, gi);
+        cell_meta<K>(a, lg[gi < ng ? gi : 0], p, l, nxt);
+    } else {
+        nxt.gpos = -1;
     }
     while (cur.gpos >= 0) {
         if (nxt.gpos >= 0) {
             stage(nxt, b ^ 1, Bn);
-            cell_meta<K>(a, next_cell(a, nxt.gpos + 1, g1), l, nn);
+            p = next_cell(lg, ng, nxt.gpos + 1, g1, gi);
+            cell_meta<K>(a, lg[gi < ng ? gi : 0], p, l, nn);
         } else {
             nn.gpos = -1;
         }
@@ -564,8 +586,20 @@ static MultiState& multi_setup(Solver& s) {
     });
     m.op = (float*)malloc_m(m, op.size() * sizeof(float));
     HIPCHK(hipMemcpy(m.op, op.data(), op.size() * sizeof(float), hipMemcpyHostToDevice));
-    m.gop = (int64_t*)malloc_m(m, gop.size() * sizeof(int64_t));
-    HIPCHK(hipMemcpy(m.gop, gop.data(), gop.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    if ((int)L.groups.size() > MR_MAXG) throw Error(VEMD_ERR_ARG, "too many valence groups for the multi-RHS apply");
+    std::vector<MrGroup> mg(L.groups.size());
+    for (size_t gi = 0; gi < L.groups.size(); ++gi) {
+        const GroupDesc& g = L.groups[gi];
+        int count = g.npad;
+        while (count > 0 && L.cell_of_gpos[g.gpos0 + count - 1] < 0) --count;
+        for (int cg = 0; cg < count; ++cg)
+            if (L.cell_of_gpos[g.gpos0 + cg] < 0)
+                throw Error(VEMD_ERR_INTERNAL, "padding inside a valence group");
+        mg[gi] = MrGroup{g.gpos0, count, g.ne, g.n, gop[gi], g.i_off};
+    }
+    m.ngroups = (int)mg.size();
+    m.mg = (MrGroup*)malloc_m(m, mg.size() * sizeof(MrGroup));
+    HIPCHK(hipMemcpy(m.mg, mg.data(), mg.size() * sizeof(MrGroup), hipMemcpyHostToDevice));
     // persistent waves: about 8 per CU of the chip
     int dev = 0, ncu = 256;
     HIPCHK(hipGetDevice(&dev));
@@ -613,11 +647,9 @@ static MultiState& multi_setup(Solver& s) {
 static MrArgs mr_args(Solver& s, MultiState& m, const float* x0, const float* x1, float* y, float* y1, bool solve) {
     DeviceState& d = *s.dev;
     MrArgs a;
-    a.groups = d.o32.groups;
-    a.tile_group = d.tile_group;
-    a.cell_of_gpos = d.cell_of_gpos;
+    a.mg = m.mg;
+    a.ngroups = m.ngroups;
     a.op = m.op;
-    a.gop = m.gop;
     a.idx = d.idx;
     a.scale = d.o32.scale;
     a.X[0] = x0;

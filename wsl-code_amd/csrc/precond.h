@@ -32,8 +32,14 @@ struct MState {
 
 // Block sliced ELLPACK with square blocks of bs rows: block row s * 64 + l
 // has width[s] block entries; entry j sits at e = off[s] + j * 64 + l with
-// block column col[e] and values val[e * bs * bs + r * bs + c]; padding
-// entries carry zero blocks and column 0.  bs = 1 is sliced ELLPACK.
+// block column col[e]; value (r, c) of its block is
+// val[(off[s] + j * 64) * bs * bs + (r * bs + c) * 64 + l], so that the 64
+// rows of a slice read every block component as 64 consecutive values.
+// Padding entries carry zero blocks and column 0.  bs = 1 is sliced ELLPACK.
+// Half-precision form (hval non-null, val null): the stored values are those
+// of the symmetrically scaled matrix D^-1/2 A D^-1/2 (D = diag A, entries in
+// [-1, 1]), and A = D^1/2 (stored) D^1/2 is applied with rs = D^1/2 and
+// ris = D^-1/2 per row.
 template <typename T>
 struct Sell {
     int64_t nrows = 0;        // block rows
@@ -43,6 +49,9 @@ struct Sell {
     const int* width = nullptr;
     const int* col = nullptr;
     const T* val = nullptr;
+    const _Float16* hval = nullptr;
+    const T* rs = nullptr;
+    const T* ris = nullptr;
 };
 
 // Compressed sparse rows (restriction: few long rows).

@@ -8,35 +8,63 @@
 // vector that other rows of the same kernel read.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "precond.h"
 
 namespace vemd {
 
 // out = (A (x + y))_brow, column j of NC interleaved columns (vector entry
-// (row, j) at row * NC + j), for one block row of bs = B rows (y may be null).
+// (row, j) at row * NC + j), for one block row of bs = B rows (y may be null);
+// with ds non-null the gathered entries are scaled by ds[row] first, where ds
+// is the level's D^-1 (the only scaling the cycle needs).
 template <typename T, int B, int NC>
 __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x, T (&out)[B],
-                                         const T* __restrict__ y = nullptr) {
+                                         const T* __restrict__ y = nullptr, const T* __restrict__ ds = nullptr) {
     const int64_t s = brow >> 6;
     const int w = A.width[s];
     const int64_t o = A.off[s] + (brow & 63);
 #pragma unroll
     for (int r = 0; r < B; ++r) out[r] = T(0);
+    if (A.hval) {
+        // D^1/2 (scaled values) D^1/2; D^1/2 D^-1 = D^-1/2 for the ds gathers
+        const _Float16* hs = A.hval + A.off[s] * (B * B) + (brow & 63);
+        const T* __restrict__ cs = ds ? A.ris : A.rs;
+        for (int k = 0; k < w; ++k) {
+            const int64_t e = o + (int64_t)k * 64;
+            const int64_t c = (int64_t)A.col[e] * B;
+            const _Float16* v = hs + (int64_t)k * 64 * (B * B);
+            T xc[B];
+#pragma unroll
+            for (int cc = 0; cc < B; ++cc) {
+                const int64_t xi = (c + cc) * NC + j;
+                xc[cc] = (y ? x[xi] + y[xi] : x[xi]) * cs[c + cc];
+            }
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int cc = 0; cc < B; ++cc) out[r] += (T)v[(r * B + cc) * 64] * xc[cc];
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r) out[r] *= A.rs[brow * B + r];
+        return;
+    }
+    const T* vs = A.val + A.off[s] * (B * B) + (brow & 63);
     for (int k = 0; k < w; ++k) {
         const int64_t e = o + (int64_t)k * 64;
         const int64_t c = (int64_t)A.col[e] * B;
-        const T* v = A.val + e * (B * B);
+        const T* v = vs + (int64_t)k * 64 * (B * B);
         T xc[B];
 #pragma unroll
         for (int cc = 0; cc < B; ++cc) {
             const int64_t xi = (c + cc) * NC + j;
             xc[cc] = y ? x[xi] + y[xi] : x[xi];
+            if (ds) xc[cc] *= ds[c + cc];
         }
 #pragma unroll
         for (int r = 0; r < B; ++r)
 #pragma unroll
-            for (int cc = 0; cc < B; ++cc) out[r] += v[r * B + cc] * xc[cc];
+            for (int cc = 0; cc < B; ++cc) out[r] += v[(r * B + cc) * 64] * xc[cc];
     }
 }
 
@@ -67,9 +95,11 @@ __global__ __launch_bounds__(256) void k_pre0(int64_t n, const T* b, const T* di
 }
 
 // Chebyshev step per block row: r -= A d, d' = c1 d + c2 D^-1 r, x += d.
+// The last step of a sweep (last != 0) completes the iterate instead,
+// x += d + d', and leaves r and d' unwritten (nothing reads them).
 template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, const T* din, T* dout, T* x, T c1, T c2,
-                                            const int* done) {
+                                            int last, const int* done) {
     AMG_BROW
     T ad[B];
     sell_row<T, B, NC>(A, bi, j, din, ad);
@@ -77,14 +107,44 @@ __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, co
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
         const T ri = r[i] - ad[k];
-        r[i] = ri;
         const T di = din[i];
-        dout[i] = c1 * di + c2 * dinv[bi * B + k] * ri;
-        x[i] += di;
+        const T dn = c1 * di + c2 * dinv[bi * B + k] * ri;
+        if (last) {
+            x[i] += di + dn;
+        } else {
+            r[i] = ri;
+            dout[i] = dn;
+            x[i] += di;
+        }
     }
 }
 
-// Residual of the lagged iterate: r = b - A (x + d).
+// First Chebyshev step from x = 0, with d0 = c0 D^-1 b formed on the fly:
+// r = b - A d0, d' = c1 d0 + c2 D^-1 r, x = d0 (last: x = d0 + d' only).
+template <typename T, int B, int NC>
+__global__ __launch_bounds__(256) void k_step0(Sell<T> A, const T* b, const T* dinv, T* r, T* dout, T* x, T c0, T c1,
+                                             T c2, int last, const int* done) {
+    AMG_BROW
+    T ad[B];
+    sell_row<T, B, NC>(A, bi, j, b, ad, nullptr, dinv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+        const int64_t i = (bi * B + k) * NC + j;
+        const T bv = b[i], dv = dinv[bi * B + k];
+        const T ri = bv - c0 * ad[k];
+        const T di = c0 * dv * bv;
+        const T dn = c1 * di + c2 * dv * ri;
+        if (last) {
+            x[i] = di + dn;
+        } else {
+            r[i] = ri;
+            dout[i] = dn;
+            x[i] = di;
+        }
+    }
+}
+
+// Residual of the lagged iterate: r = b - A (x + d) (d null: x is complete).
 template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_resid(Sell<T> A, const T* b, const T* x, const T* d, T* r, const int* done) {
     AMG_BROW
@@ -137,14 +197,16 @@ __global__ __launch_bounds__(256) void k_restrict(CsrDev<T> R, const T* v, T* y,
     }
 }
 
-// Coarse correction of the lagged iterate: x += d + P xc.
+// Coarse correction of the lagged iterate: x += d + P (xc + dc), with d this
+// level's and dc the coarse level's pending Chebyshev update (null where the
+// iterate is complete).
 template <typename T, int NC>
-__global__ __launch_bounds__(256) void k_prol(Sell<T> P, const T* xc, T* x, const T* d, const int* done) {
+__global__ __launch_bounds__(256) void k_prol(Sell<T> P, const T* xc, const T* dc, T* x, const T* d, const int* done) {
     const int64_t n = P.nrows;
     AMG_ROW
     T px[1];
-    sell_row<T, 1, NC>(P, row, (int)(i % NC), xc, px);
-    x[i] += d[i] + px[0];
+    sell_row<T, 1, NC>(P, row, (int)(i % NC), xc, px, dc);
+    x[i] += d ? d[i] + px[0] : px[0];
 }
 
 template <typename T, int NC>
@@ -197,8 +259,9 @@ static S* amg_copy(AmgDev<T>& D, const std::vector<S>& v) {
     return p;
 }
 
+// dsq non-null: half-precision values of D^-1/2 A D^-1/2 with dsq = D^1/2.
 template <typename T>
-static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs) {
+static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<double>* dsq = nullptr) {
     Sell<T> s;
     s.bs = bs;
     s.nrows = A.nrows / bs;
@@ -221,7 +284,8 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs) {
         off[sl + 1] = off[sl] + (int64_t)w * 64;
     }
     std::vector<int> col(off[s.nslices], 0);
-    std::vector<T> val(off[s.nslices] * bs * bs, T(0));
+    std::vector<T> val(dsq ? 0 : off[s.nslices] * bs * bs, T(0));
+    std::vector<_Float16> hval(dsq ? off[s.nslices] * bs * bs : 0, (_Float16)0.0f);
     for (int64_t br = 0; br < s.nrows; ++br) {
         const int64_t sl = br >> 6, l = br & 63;
         const std::vector<int>& v = bc[br];
@@ -229,13 +293,28 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs) {
         for (int k = 0; k < bs; ++k)
             for (int64_t p = A.rp[br * bs + k]; p < A.rp[br * bs + k + 1]; ++p) {
                 const int64_t j = std::lower_bound(v.begin(), v.end(), A.ci[p] / bs) - v.begin();
-                val[(off[sl] + j * 64 + l) * bs * bs + k * bs + A.ci[p] % bs] = (T)A.v[p];
+                const int64_t at = (off[sl] + j * 64) * bs * bs + (k * bs + A.ci[p] % bs) * 64 + l;
+                if (dsq)
+                    hval[at] = (_Float16)(A.v[p] / ((*dsq)[br * bs + k] * (*dsq)[A.ci[p]]));
+                else
+                    val[at] = (T)A.v[p];
             }
     }
     s.off = amg_copy(D, off);
     s.width = amg_copy(D, width);
     s.col = amg_copy(D, col);
-    s.val = amg_copy(D, val);
+    if (dsq) {
+        s.hval = amg_copy(D, hval);
+        std::vector<T> rs(A.nrows), ris(A.nrows);
+        for (int64_t i = 0; i < A.nrows; ++i) {
+            rs[i] = (T)(*dsq)[i];
+            ris[i] = (T)(1.0 / (*dsq)[i]);
+        }
+        s.rs = amg_copy(D, rs);
+        s.ris = amg_copy(D, ris);
+    } else {
+        s.val = amg_copy(D, val);
+    }
     return s;
 }
 
@@ -261,7 +340,20 @@ AmgDev<T>* amg_upload(const Amg& H) {
             d.n = h.A.nrows;
             const int bs = l == 0 ? H.bs0 : 1;
             if (h.A.nrows % bs) throw Error(VEMD_ERR_INTERNAL, "finest multigrid level is not blocked");
-            d.A = to_sell(*D, h.A, bs);
+            // FP32 cycles keep the finest operator in half precision (scaled to
+            // a unit diagonal); VEMD_MG_HALF=0 keeps it in FP32
+            const char* he = std::getenv("VEMD_MG_HALF");
+            bool half = l == 0 && sizeof(T) == 4 && !(he && he[0] == '0');
+            std::vector<double> dsq(half ? h.dinv.size() : 0);
+            for (size_t i = 0; i < dsq.size(); ++i) {
+                dsq[i] = 1.0 / std::sqrt(h.dinv[i]);
+                if (!(h.dinv[i] > 0.0) || !std::isfinite(dsq[i])) half = false;
+            }
+            if (half) {
+                d.A = to_sell(*D, h.A, bs, &dsq);
+            } else {
+                d.A = to_sell(*D, h.A, bs);
+            }
             if (l + 1 < H.lv.size()) {
                 d.P = to_sell(*D, h.P, 1);
                 d.R = to_csr(*D, h.R);
@@ -305,27 +397,45 @@ static LevelVecs<T> vecs(const AmgLevelDev<T>& L) {
 
 static dim3 grid_of(int64_t threads) { return dim3((unsigned)((threads + 255) / 256)); }
 
+// The Chebyshev steps of one sweep after its first update: steps 2..degree
+// (fused = false: the first update d = din is already formed) or 1..degree
+// with the first update formed on the fly from x = 0 (fused = true).  The
+// last step completes the iterate, so on return din is null; with a single
+// step and no fused start the update stays pending in din.
 template <typename T, int B, int NC>
-static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStream_t st, const int* ms, T*& din) {
+static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStream_t st, const int* ms, T*& din,
+                         bool fused) {
     const double sigma = L.centre / L.half;
     double rho = 1.0 / sigma;
     T* dout = din == v.d0 ? v.d1 : v.d0;
     for (int k = 1; k < L.degree; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
-        hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC), dim3(256), 0, st, L.A, v.r, (const T*)L.dinv,
-                           (const T*)din, dout, v.x, (T)(rn * rho), (T)(2.0 * rn / L.half), ms);
-        std::swap(din, dout);
+        const int last = k + 1 == L.degree;
+        if (fused && k == 1) {
+            hipLaunchKernelGGL((k_step0<T, B, NC>), grid_of(L.A.nrows * NC), dim3(256), 0, st, L.A, (const T*)v.b,
+                               (const T*)L.dinv, v.r, din, v.x, (T)(1.0 / L.centre), (T)(rn * rho),
+                               (T)(2.0 * rn / L.half), last, ms);
+        } else {
+            hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC), dim3(256), 0, st, L.A, v.r,
+                               (const T*)L.dinv, (const T*)din, dout, v.x, (T)(rn * rho), (T)(2.0 * rn / L.half),
+                               last, ms);
+            std::swap(din, dout);
+        }
+        if (last) din = nullptr;
         rho = rn;
     }
 }
 
 template <typename T, int NC>
-static void vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms);
+static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms);
 
 // Pre-smoothing, residual, restriction, coarse cycle, prolongation and
-// post-smoothing of one level whose operator has blocks of B rows.
+// post-smoothing of one level whose operator has blocks of B rows.  The
+// result is x + d with d the returned pending update: null unless the level
+// smooths with a single step (then coarser levels leave the sum to the
+// prolongation that reads them and the finest level adds it).
 template <typename T, int B, int NC>
-static void level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms) {
+static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms) {
     const AmgLevelDev<T>& L = D.lv[l];
     const AmgLevelDev<T>& C = D.lv[l + 1];
     const LevelVecs<T> v = vecs<T, NC>(L), vc = vecs<T, NC>(C);
@@ -333,31 +443,37 @@ static void level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const int*
     const dim3 gr = NC == 1 ? grid_of(C.n * 32) : grid_of(C.n * NC);
     const T c0 = (T)(1.0 / L.centre);
     T* din = v.d0;
-    hipLaunchKernelGGL((k_pre0<T, NC>), g, b, 0, st, L.n, (const T*)v.b, (const T*)L.dinv, v.r, v.d0, v.x, c0, ms);
-    smooth_steps<T, B, NC>(L, v, st, ms, din);
+    // scalar levels form the first update inside the first step; block levels
+    // (gathers of B consecutive entries) are cheaper with it materialised
+    const bool fused = B == 1 && L.degree >= 2;
+    if (!fused)
+        hipLaunchKernelGGL((k_pre0<T, NC>), g, b, 0, st, L.n, (const T*)v.b, (const T*)L.dinv, v.r, v.d0, v.x, c0, ms);
+    smooth_steps<T, B, NC>(L, v, st, ms, din, fused);
     hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, (const T*)v.x, (const T*)din, v.r, ms);
     hipLaunchKernelGGL((k_restrict<T, NC>), gr, b, 0, st, L.R, (const T*)v.r, vc.b, ms);
-    vcycle_level<T, NC>(D, l + 1, st, ms);
-    hipLaunchKernelGGL((k_prol<T, NC>), g, b, 0, st, L.P, (const T*)vc.x, v.x, (const T*)din, ms);
+    const T* dc = vcycle_level<T, NC>(D, l + 1, st, ms);
+    hipLaunchKernelGGL((k_prol<T, NC>), g, b, 0, st, L.P, (const T*)vc.x, dc, v.x, (const T*)din, ms);
     din = v.d0;
     hipLaunchKernelGGL((k_post0<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, (const T*)v.x, (const T*)L.dinv, v.r,
                        v.d0, c0, ms);
-    smooth_steps<T, B, NC>(L, v, st, ms, din);
+    smooth_steps<T, B, NC>(L, v, st, ms, din, false);
+    if (l > 0 || !din) return din;
     hipLaunchKernelGGL((k_addto<T, NC>), g, b, 0, st, L.n, v.x, (const T*)din, ms);
+    return nullptr;
 }
 
 template <typename T, int NC>
-static void vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms) {
+static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms) {
     const AmgLevelDev<T>& L = D.lv[l];
     if (l + 1 == D.lv.size()) {
         const LevelVecs<T> v = vecs<T, NC>(L);
         hipLaunchKernelGGL((k_dense<T, NC>), dim3((unsigned)D.nc), dim3(64), 0, st, D.nc, (const T*)D.cinv,
                            (const T*)v.b, v.x, ms);
-        return;
+        return nullptr;
     }
     switch (L.A.bs) {
-        case 1: level_cycle<T, 1, NC>(D, l, st, ms); break;
-        case 3: level_cycle<T, 3, NC>(D, l, st, ms); break;
+        case 1: return level_cycle<T, 1, NC>(D, l, st, ms);
+        case 3: return level_cycle<T, 3, NC>(D, l, st, ms);
         default: throw Error(VEMD_ERR_INTERNAL, "unsupported multigrid block size");
     }
 }

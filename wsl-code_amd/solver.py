@@ -32,13 +32,18 @@ class Solver:
     """Element setup (host FP64) and device solve for one mesh and problem."""
 
     def __init__(self, mesh, k=1, nq2d=9, nq1d=9, map_kind=MAP_SIN, eps=0.05, solution=SOL_SINCOS,
-                 bc=BC_FLUX_MEANZERO, theta=None, device=None):
+                 bc=BC_FLUX_MEANZERO, theta=None, device=None, setup_device=None):
+        """setup_device: build the element operators on that GPU instead of
+        the host cores (k <= 1, polygons of up to 10 edges)."""
         self.mesh = mesh
         self._theta = None if theta is None else np.ascontiguousarray(theta, np.float64)
         d = ProblemDesc(k, nq2d, nq1d, map_kind, eps, solution, bc,
                         None if self._theta is None else _d(self._theta))
         self.h = c_vp()
-        check(lib().vemd_solver_create(mesh.h, ctypes.byref(d), ctypes.byref(self.h)))
+        if setup_device is None:
+            check(lib().vemd_solver_create(mesh.h, ctypes.byref(d), ctypes.byref(self.h)))
+        else:
+            check(lib().vemd_solver_create_device(mesh.h, ctypes.byref(d), int(setup_device), ctypes.byref(self.h)))
         s = np.zeros(6, np.int64)
         check(lib().vemd_solver_sizes(self.h, s.ctypes.data_as(c_lp)))
         self.nflux, self.npres, self.ntot, self.nvec, self.ngroups, self.max_ne = (int(v) for v in s)
