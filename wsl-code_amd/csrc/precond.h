@@ -36,10 +36,7 @@ struct MState {
 // val[(off[s] + j * 64) * bs * bs + (r * bs + c) * 64 + l], so that the 64
 // rows of a slice read every block component as 64 consecutive values.
 // Padding entries carry zero blocks and column 0.  bs = 1 is sliced ELLPACK.
-// Half-precision form (hval non-null, val null): the stored values are those
-// of the symmetrically scaled matrix D^-1/2 A D^-1/2 (D = diag A, entries in
-// [-1, 1]), and A = D^1/2 (stored) D^1/2 is applied with rs = D^1/2 and
-// ris = D^-1/2 per row.
+// Values are in val, or in half precision in hval (val null).
 template <typename T>
 struct Sell {
     int64_t nrows = 0;        // block rows
@@ -50,8 +47,6 @@ struct Sell {
     const int* col = nullptr;
     const T* val = nullptr;
     const _Float16* hval = nullptr;
-    const T* rs = nullptr;
-    const T* ris = nullptr;
 };
 
 // Compressed sparse rows (restriction: few long rows).
@@ -68,7 +63,12 @@ struct AmgLevelDev {
     int64_t n = 0;
     Sell<T> A, P;             // P: prolongation from the next level
     CsrDev<T> R;              // R = P^T
-    T* dinv = nullptr;
+    T* dinv = nullptr;        // null: unit diagonal
+    // Scaled level (sc non-null): the operator is D^-1/2 A D^-1/2 (unit
+    // diagonal, entries in [-1, 1], stored in half precision) acting on
+    // D^1/2 x; sc = D^-1/2 scales the level's right-hand side on entry and its
+    // solution on exit, and P, R of the level carry D^1/2.
+    T* sc = nullptr;
     T *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;       // one column
     T *mb = nullptr, *mx = nullptr, *mr = nullptr, *md0 = nullptr, *md1 = nullptr;  // MCOLS interleaved columns
     double centre = 0.0, half = 0.0;  // Chebyshev interval of D^-1 A

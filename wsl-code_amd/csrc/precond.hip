@@ -14,58 +14,73 @@
 
 namespace vemd {
 
+// Accumulates entries [0, w) of a slice row into out: U entries at a time,
+// with their column indices, then their gathers, issued together (the
+// gathers depend on the indices; batching keeps several in flight per lane).
+// Gathered entries are x (+ y) times cs[column] when cs is non-null; the sum
+// runs in entry order.
+template <typename T, typename V, int B, int NC, int U>
+__device__ __forceinline__ void sell_acc(const int* __restrict__ col, const V* __restrict__ vs, int64_t o, int w, int j,
+                                         const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ cs,
+                                         T (&out)[B]) {
+    int k = 0;
+    for (; k + U <= w; k += U) {
+        int64_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = (int64_t)col[o + (int64_t)(k + u) * 64] * B;
+        T xc[U][B];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int cc = 0; cc < B; ++cc) {
+                const int64_t xi = (c[u] + cc) * NC + j;
+                T xv = y ? x[xi] + y[xi] : x[xi];
+                if (cs) xv *= cs[c[u] + cc];
+                xc[u][cc] = xv;
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const V* v = vs + (int64_t)(k + u) * 64 * (B * B);
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int cc = 0; cc < B; ++cc) out[r] += (T)v[(r * B + cc) * 64] * xc[u][cc];
+        }
+    }
+    for (; k < w; ++k) {
+        const int64_t c = (int64_t)col[o + (int64_t)k * 64] * B;
+        const V* v = vs + (int64_t)k * 64 * (B * B);
+        T xc[B];
+#pragma unroll
+        for (int cc = 0; cc < B; ++cc) {
+            const int64_t xi = (c + cc) * NC + j;
+            T xv = y ? x[xi] + y[xi] : x[xi];
+            if (cs) xv *= cs[c + cc];
+            xc[cc] = xv;
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int cc = 0; cc < B; ++cc) out[r] += (T)v[(r * B + cc) * 64] * xc[cc];
+    }
+}
+
 // out = (A (x + y))_brow, column j of NC interleaved columns (vector entry
 // (row, j) at row * NC + j), for one block row of bs = B rows (y may be null);
-// with ds non-null the gathered entries are scaled by ds[row] first, where ds
-// is the level's D^-1 (the only scaling the cycle needs).
+// with ds non-null the gathered entries are scaled by ds[row] first.
 template <typename T, int B, int NC>
 __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x, T (&out)[B],
                                          const T* __restrict__ y = nullptr, const T* __restrict__ ds = nullptr) {
+    constexpr int U = B == 1 ? 4 : 2;
     const int64_t s = brow >> 6;
     const int w = A.width[s];
     const int64_t o = A.off[s] + (brow & 63);
 #pragma unroll
     for (int r = 0; r < B; ++r) out[r] = T(0);
-    if (A.hval) {
-        // D^1/2 (scaled values) D^1/2; D^1/2 D^-1 = D^-1/2 for the ds gathers
-        const _Float16* hs = A.hval + A.off[s] * (B * B) + (brow & 63);
-        const T* __restrict__ cs = ds ? A.ris : A.rs;
-        for (int k = 0; k < w; ++k) {
-            const int64_t e = o + (int64_t)k * 64;
-            const int64_t c = (int64_t)A.col[e] * B;
-            const _Float16* v = hs + (int64_t)k * 64 * (B * B);
-            T xc[B];
-#pragma unroll
-            for (int cc = 0; cc < B; ++cc) {
-                const int64_t xi = (c + cc) * NC + j;
-                xc[cc] = (y ? x[xi] + y[xi] : x[xi]) * cs[c + cc];
-            }
-#pragma unroll
-            for (int r = 0; r < B; ++r)
-#pragma unroll
-                for (int cc = 0; cc < B; ++cc) out[r] += (T)v[(r * B + cc) * 64] * xc[cc];
-        }
-#pragma unroll
-        for (int r = 0; r < B; ++r) out[r] *= A.rs[brow * B + r];
-        return;
-    }
-    const T* vs = A.val + A.off[s] * (B * B) + (brow & 63);
-    for (int k = 0; k < w; ++k) {
-        const int64_t e = o + (int64_t)k * 64;
-        const int64_t c = (int64_t)A.col[e] * B;
-        const T* v = vs + (int64_t)k * 64 * (B * B);
-        T xc[B];
-#pragma unroll
-        for (int cc = 0; cc < B; ++cc) {
-            const int64_t xi = (c + cc) * NC + j;
-            xc[cc] = y ? x[xi] + y[xi] : x[xi];
-            if (ds) xc[cc] *= ds[c + cc];
-        }
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int cc = 0; cc < B; ++cc) out[r] += v[(r * B + cc) * 64] * xc[cc];
-    }
+    if (A.hval)
+        sell_acc<T, _Float16, B, NC, U>(A.col, A.hval + A.off[s] * (B * B) + (brow & 63), o, w, j, x, y, ds, out);
+    else
+        sell_acc<T, T, B, NC, U>(A.col, A.val + A.off[s] * (B * B) + (brow & 63), o, w, j, x, y, ds, out);
 }
 
 // Thread t of an elementwise sweep over n rows x NC columns: entry t, row t / NC.
@@ -83,23 +98,35 @@ __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, 
     const int j = (int)(t % NC);                                 \
     if (bi >= A.nrows) return;
 
+// Diagonal entry of D^-1 (null: unit diagonal) and the right-hand side of a
+// scaled level (es non-null: es * b).
+template <typename T>
+__device__ __forceinline__ T dval(const T* dinv, int64_t row) {
+    return dinv ? dinv[row] : T(1);
+}
+template <typename T>
+__device__ __forceinline__ T bval(const T* b, const T* es, int64_t i, int64_t row) {
+    return es ? b[i] * es[row] : b[i];
+}
+
 // First pre-smoothing step from x = 0: r = b, d = D^-1 b / centre, x = 0.
 template <typename T, int NC>
-__global__ __launch_bounds__(256) void k_pre0(int64_t n, const T* b, const T* dinv, T* r, T* d, T* x, T c,
-                                            const int* done) {
+__global__ __launch_bounds__(256) void k_pre0(int64_t n, const T* b, const T* es, const T* dinv, T* r, T* d, T* x,
+                                            T c, const int* done) {
     AMG_ROW
-    const T bi = b[i];
+    const T bi = bval(b, es, i, row);
     r[i] = bi;
-    d[i] = dinv[row] * bi * c;
+    d[i] = dval(dinv, row) * bi * c;
     x[i] = T(0);
 }
 
 // Chebyshev step per block row: r -= A d, d' = c1 d + c2 D^-1 r, x += d.
 // The last step of a sweep (last != 0) completes the iterate instead,
 // x += d + d', and leaves r and d' unwritten (nothing reads them).
+// With xs non-null the completed iterate leaves the scaled space: x = xs (x + d + d').
 template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, const T* din, T* dout, T* x, T c1, T c2,
-                                            int last, const int* done) {
+                                            int last, const T* xs, const int* done) {
     AMG_BROW
     T ad[B];
     sell_row<T, B, NC>(A, bi, j, din, ad);
@@ -108,9 +135,10 @@ __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, co
         const int64_t i = (bi * B + k) * NC + j;
         const T ri = r[i] - ad[k];
         const T di = din[i];
-        const T dn = c1 * di + c2 * dinv[bi * B + k] * ri;
+        const T dn = c1 * di + c2 * dval(dinv, bi * B + k) * ri;
         if (last) {
-            x[i] += di + dn;
+            const T xn = x[i] + di + dn;
+            x[i] = xs ? xn * xs[bi * B + k] : xn;
         } else {
             r[i] = ri;
             dout[i] = dn;
@@ -130,7 +158,7 @@ __global__ __launch_bounds__(256) void k_step0(Sell<T> A, const T* b, const T* d
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
-        const T bv = b[i], dv = dinv[bi * B + k];
+        const T bv = b[i], dv = dval(dinv, bi * B + k);
         const T ri = bv - c0 * ad[k];
         const T di = c0 * dv * bv;
         const T dn = c1 * di + c2 * dv * ri;
@@ -146,30 +174,31 @@ __global__ __launch_bounds__(256) void k_step0(Sell<T> A, const T* b, const T* d
 
 // Residual of the lagged iterate: r = b - A (x + d) (d null: x is complete).
 template <typename T, int B, int NC>
-__global__ __launch_bounds__(256) void k_resid(Sell<T> A, const T* b, const T* x, const T* d, T* r, const int* done) {
+__global__ __launch_bounds__(256) void k_resid(Sell<T> A, const T* b, const T* es, const T* x, const T* d, T* r,
+                                             const int* done) {
     AMG_BROW
     T ax[B];
     sell_row<T, B, NC>(A, bi, j, x, ax, d);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
-        r[i] = b[i] - ax[k];
+        r[i] = bval(b, es, i, bi * B + k) - ax[k];
     }
 }
 
 // First post-smoothing step: r = b - A x, d = D^-1 r / centre (x unchanged).
 template <typename T, int B, int NC>
-__global__ __launch_bounds__(256) void k_post0(Sell<T> A, const T* b, const T* x, const T* dinv, T* r, T* d, T c,
-                                             const int* done) {
+__global__ __launch_bounds__(256) void k_post0(Sell<T> A, const T* b, const T* es, const T* x, const T* dinv, T* r,
+                                             T* d, T c, const int* done) {
     AMG_BROW
     T ax[B];
     sell_row<T, B, NC>(A, bi, j, x, ax);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
-        const T ri = b[i] - ax[k];
+        const T ri = bval(b, es, i, bi * B + k) - ax[k];
         r[i] = ri;
-        d[i] = dinv[bi * B + k] * ri * c;
+        d[i] = dval(dinv, bi * B + k) * ri * c;
     }
 }
 
@@ -210,10 +239,10 @@ __global__ __launch_bounds__(256) void k_prol(Sell<T> P, const T* xc, const T* d
 }
 
 template <typename T, int NC>
-__global__ __launch_bounds__(256) void k_addto(int64_t n, T* x, const T* d, const int* done) {
+__global__ __launch_bounds__(256) void k_addto(int64_t n, T* x, const T* d, const T* xs, const int* done) {
     AMG_ROW
-    (void)row;
-    x[i] += d[i];
+    const T xn = x[i] + d[i];
+    x[i] = xs ? xn * xs[row] : xn;
 }
 
 // Coarsest level: x = C^-1 b with the dense inverse; one block per row, one
@@ -303,18 +332,10 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<dou
     s.off = amg_copy(D, off);
     s.width = amg_copy(D, width);
     s.col = amg_copy(D, col);
-    if (dsq) {
+    if (dsq)
         s.hval = amg_copy(D, hval);
-        std::vector<T> rs(A.nrows), ris(A.nrows);
-        for (int64_t i = 0; i < A.nrows; ++i) {
-            rs[i] = (T)(*dsq)[i];
-            ris[i] = (T)(1.0 / (*dsq)[i]);
-        }
-        s.rs = amg_copy(D, rs);
-        s.ris = amg_copy(D, ris);
-    } else {
+    else
         s.val = amg_copy(D, val);
-    }
     return s;
 }
 
@@ -340,10 +361,10 @@ AmgDev<T>* amg_upload(const Amg& H) {
             d.n = h.A.nrows;
             const int bs = l == 0 ? H.bs0 : 1;
             if (h.A.nrows % bs) throw Error(VEMD_ERR_INTERNAL, "finest multigrid level is not blocked");
-            // FP32 cycles keep the finest operator in half precision (scaled to
-            // a unit diagonal); VEMD_MG_HALF=0 keeps it in FP32
+            // FP32 cycles run the finest level scaled to a unit diagonal with
+            // its operator in half precision; VEMD_MG_HALF=0 keeps it in FP32
             const char* he = std::getenv("VEMD_MG_HALF");
-            bool half = l == 0 && sizeof(T) == 4 && !(he && he[0] == '0');
+            bool half = l == 0 && sizeof(T) == 4 && l + 1 < H.lv.size() && !(he && he[0] == '0');
             std::vector<double> dsq(half ? h.dinv.size() : 0);
             for (size_t i = 0; i < dsq.size(); ++i) {
                 dsq[i] = 1.0 / std::sqrt(h.dinv[i]);
@@ -351,15 +372,25 @@ AmgDev<T>* amg_upload(const Amg& H) {
             }
             if (half) {
                 d.A = to_sell(*D, h.A, bs, &dsq);
+                Csr P = h.P, R = h.R;  // P <- D^1/2 P, R <- R D^1/2
+                for (int64_t i = 0; i < P.nrows; ++i)
+                    for (int64_t p = P.rp[i]; p < P.rp[i + 1]; ++p) P.v[p] *= dsq[i];
+                for (int64_t p = 0; p < R.nnz(); ++p) R.v[p] *= dsq[R.ci[p]];
+                d.P = to_sell(*D, P, 1);
+                d.R = to_csr(*D, R);
+                std::vector<T> sc(dsq.size());
+                for (size_t i = 0; i < dsq.size(); ++i) sc[i] = (T)(1.0 / dsq[i]);
+                d.sc = amg_copy(*D, sc);
+                d.dinv = nullptr;
             } else {
                 d.A = to_sell(*D, h.A, bs);
+                if (l + 1 < H.lv.size()) {
+                    d.P = to_sell(*D, h.P, 1);
+                    d.R = to_csr(*D, h.R);
+                }
+                std::vector<T> di(h.dinv.begin(), h.dinv.end());
+                d.dinv = amg_copy(*D, di);
             }
-            if (l + 1 < H.lv.size()) {
-                d.P = to_sell(*D, h.P, 1);
-                d.R = to_csr(*D, h.R);
-            }
-            std::vector<T> di(h.dinv.begin(), h.dinv.end());
-            d.dinv = amg_copy(*D, di);
             const size_t vb = (size_t)d.n * sizeof(T);
             d.b = (T*)amg_alloc(*D, vb);
             d.x = (T*)amg_alloc(*D, vb);
@@ -404,7 +435,7 @@ static dim3 grid_of(int64_t threads) { return dim3((unsigned)((threads + 255) / 
 // step and no fused start the update stays pending in din.
 template <typename T, int B, int NC>
 static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStream_t st, const int* ms, T*& din,
-                         bool fused) {
+                         bool fused, const T* xs) {
     const double sigma = L.centre / L.half;
     double rho = 1.0 / sigma;
     T* dout = din == v.d0 ? v.d1 : v.d0;
@@ -418,7 +449,7 @@ static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStre
         } else {
             hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC), dim3(256), 0, st, L.A, v.r,
                                (const T*)L.dinv, (const T*)din, dout, v.x, (T)(rn * rho), (T)(2.0 * rn / L.half),
-                               last, ms);
+                               last, last ? xs : nullptr, ms);
             std::swap(din, dout);
         }
         if (last) din = nullptr;
@@ -445,20 +476,23 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
     T* din = v.d0;
     // scalar levels form the first update inside the first step; block levels
     // (gathers of B consecutive entries) are cheaper with it materialised
-    const bool fused = B == 1 && L.degree >= 2;
+    const bool fused = B == 1 && L.degree >= 2 && !L.sc;
+    const T* sc = L.sc;
     if (!fused)
-        hipLaunchKernelGGL((k_pre0<T, NC>), g, b, 0, st, L.n, (const T*)v.b, (const T*)L.dinv, v.r, v.d0, v.x, c0, ms);
-    smooth_steps<T, B, NC>(L, v, st, ms, din, fused);
-    hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, (const T*)v.x, (const T*)din, v.r, ms);
+        hipLaunchKernelGGL((k_pre0<T, NC>), g, b, 0, st, L.n, (const T*)v.b, sc, (const T*)L.dinv, v.r, v.d0, v.x, c0,
+                           ms);
+    smooth_steps<T, B, NC>(L, v, st, ms, din, fused, (const T*)nullptr);
+    hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)din, v.r,
+                       ms);
     hipLaunchKernelGGL((k_restrict<T, NC>), gr, b, 0, st, L.R, (const T*)v.r, vc.b, ms);
     const T* dc = vcycle_level<T, NC>(D, l + 1, st, ms);
     hipLaunchKernelGGL((k_prol<T, NC>), g, b, 0, st, L.P, (const T*)vc.x, dc, v.x, (const T*)din, ms);
     din = v.d0;
-    hipLaunchKernelGGL((k_post0<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, (const T*)v.x, (const T*)L.dinv, v.r,
-                       v.d0, c0, ms);
-    smooth_steps<T, B, NC>(L, v, st, ms, din, false);
+    hipLaunchKernelGGL((k_post0<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)L.dinv,
+                       v.r, v.d0, c0, ms);
+    smooth_steps<T, B, NC>(L, v, st, ms, din, false, sc);
     if (l > 0 || !din) return din;
-    hipLaunchKernelGGL((k_addto<T, NC>), g, b, 0, st, L.n, v.x, (const T*)din, ms);
+    hipLaunchKernelGGL((k_addto<T, NC>), g, b, 0, st, L.n, v.x, (const T*)din, sc, ms);
     return nullptr;
 }
 
