@@ -41,14 +41,29 @@ def test_mfma_apply_matches_single_column_apply(vemd, setup):
     ne = (s.k + 1) * s.mesh.num_edges
     Ym = Y.clone()
     Ym[:ne] += Y1[:ne]
+    # slots of real dofs (the MFMA apply leaves padding positions and the trash
+    # slot untouched)
+    ones = np.ones(s.ntot)
+    live = np.zeros(s.nvec)
+    L.vemd_to_internal(s.h, ones.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                       live.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0)
+    live = torch.from_numpy(live != 0).cuda()
     for j in range(NR):
+        # both FP32 products against the FP64 apply of the same column: the
+        # MFMA path is as accurate as the single-column FP32 path
         x = X[:, j].contiguous()
         y = torch.empty_like(x)
         assert L.vemd_apply_device(s.h, vemd.FP32, _ptr(x), _ptr(y), None) == 0
+        x64 = x.double()
+        y64 = torch.empty_like(x64)
+        assert L.vemd_apply_device(s.h, vemd.FP64, _ptr(x64), _ptr(y64), None) == 0
         torch.cuda.synchronize()
-        scale = y.abs().max().item()
-        err = (Ym[:, j] - y).abs().max().item()
-        assert err <= 2e-5 * scale, (j, err, scale)
+        y64, y = y64[live], y[live]
+        scale = y64.abs().max().item()
+        err_single = (y.double() - y64).abs().max().item()
+        err_mfma = (Ym[:, j][live].double() - y64).abs().max().item()
+        assert err_single <= 1e-4 * scale, (j, err_single, scale)
+        assert err_mfma <= 2.0 * err_single + 1e-6 * scale, (j, err_mfma, err_single, scale)
 
 
 def test_multi_rhs_solve_matches_single_solves(vemd, setup):

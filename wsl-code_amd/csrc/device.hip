@@ -53,6 +53,7 @@ struct ApplyArgs {
     const T* L;
     const T* scale;
     const uint32_t* idx;
+    const uint32_t* wmask;  // row 0 of W per store position (bit j: -1, bit 16 + j: 0)
     const T* x;          // input when st == nullptr
     const T* xs[2];      // inputs selected by the Krylov iteration parity when st != nullptr
     T* y;
@@ -142,7 +143,9 @@ struct CellShape {
     static constexpr int VW = VT<T>::W;
     static constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1)), NED = KK * NE,
                          N = NED + NI;
-    static constexpr int NA = N * (N + 1) / 2, NAV = (NA + VW - 1) / VW, NW = Q * N, NWV = (NW + VW - 1) / VW;
+    // streamed part of W: rows 1..Q-1 at the NED edge columns
+    static constexpr int NA = N * (N + 1) / 2, NAV = (NA + VW - 1) / VW, NW = (Q - 1) * NED,
+                         NWV = (NW + VW - 1) / VW;
     static constexpr int NV = NAV + NWV;  // 16-byte vectors per cell: packed A, then W
 };
 
@@ -159,7 +162,8 @@ struct CellShape {
 // tile tau at ((tau * NV + v) * 64 + l) * 16 bytes), read in chunks of CH
 // vectors; with PIPE the next chunk is in flight while the current one is
 // consumed.  y_u = s (A u - W^T (p / s)) so that A and W entries can be
-// consumed in stream order.
+// consumed in stream order; the exact part of W (row 0 from the cell's mask,
+// the -1 entries of the internal columns) is applied after the stream.
 template <typename T, int K, int NE>
 __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __restrict__ x, const GroupDesc& g, int tau,
                                           int l, double& dot, double& lamp) {
@@ -195,6 +199,7 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
 #pragma unroll
     for (int r = 0; r < Q; ++r) p[r] = bload1(rx, gpos * TS, (uint32_t)((a.cbase + (NI + r) * a.cpad) * TS), (T*)nullptr);
     const T s = a.scale[gpos];
+    const uint32_t wm = a.wmask[gpos];
     const T rs = s > T(0) ? T(1) / s : T(0);
 #pragma unroll
     for (int r = 0; r < Q; ++r) {
@@ -232,7 +237,7 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
                         } else if constexpr (v < NV) {
                             constexpr int t = (v - NAV) * VW + li;
                             if constexpr (t < NW) {
-                                constexpr int r = t / N, i = t % N;
+                                constexpr int r = 1 + t / NED, i = t % NED;
                                 const T w = buf[decltype(vv)::value][li];
                                 acc[i] -= w * ps[r];
                                 yp[r] -= w * u[i];
@@ -277,6 +282,19 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
         },
         std::make_integer_sequence<int, NC>{});
 #endif
+    // exact part of W: row 0 (+-1 or 0 at the first moment of each edge) and
+    // W(r + 1, NED + r) = -1
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+        const T w0 = (wm >> (16 + j) & 1u) ? T(0) : ((wm >> j & 1u) ? T(-1) : T(1));
+        acc[j] -= w0 * ps[0];
+        yp[0] -= w0 * u[j];
+    }
+#pragma unroll
+    for (int r = 0; r + 1 < Q; ++r) {
+        acc[NED + r] += ps[r + 1];
+        yp[r + 1] += u[NED + r];
+    }
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] *= s;
     if (a.lam_idx >= 0) {
@@ -850,9 +868,11 @@ void dev_upload(Solver& s, int device) {
     {
         std::vector<double> O64, L64;
         std::vector<float> O32, L32;
-        pack_store(*s.mesh, *s.setup, L, O64, L64, O32, L32);
+        std::vector<uint32_t> wmask;
+        pack_store(*s.mesh, *s.setup, L, O64, L64, O32, L32, wmask);
         upload_ops(d, L, d.o64, O64, L64, false);
         upload_ops(d, L, d.o32, O32, L32, true);
+        d.wmask = dupload(d, wmask);
     }
     d.idx = dupload(d, L.idx);
     d.tile_group = dupload(d, L.tile_group);
@@ -967,6 +987,7 @@ static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, T* y1, const MState* 
     a.L = o.L;
     a.scale = o.scale;
     a.idx = d.idx;
+    a.wmask = d.wmask;
     a.x = x;
     a.xs[0] = x0;
     a.xs[1] = x1;

@@ -44,6 +44,7 @@ struct DeviceState {
     DevOps<double> o64;
     DevOps<float> o32;
     uint32_t* idx = nullptr;
+    uint32_t* wmask = nullptr;
     int* tile_group = nullptr;
     int* cell_of_gpos = nullptr;
     int ntiles = 0;

@@ -173,9 +173,10 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     for (int c = 0; c < nc; ++c)
         for (int j = m.off[c]; j < m.off[c + 1]; ++j)
             side0[m.cell_edges[j]] = std::min(side0[m.cell_edges[j]], L.gpos_of_cell[c]);
-    // per-cell stream lengths in 16-byte vectors: packed A then W, each padded
-    auto nvec_of = [&](int n, int vw) {
-        int na = n * (n + 1) / 2, nw = L.q * n;
+    // per-cell stream lengths in 16-byte vectors: packed A, then the rows
+    // 1..q-1 of W at the edge columns, each padded
+    auto nvec_of = [&](const GroupDesc& g, int vw) {
+        int na = g.n * (g.n + 1) / 2, nw = (L.q - 1) * (L.k + 1) * g.ne;
         return (na + vw - 1) / vw + (nw + vw - 1) / vw;
     };
     int64_t ioff = 0, ooff2 = 0, ooff4 = 0, loff = 0;
@@ -188,8 +189,8 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         L.groups32[gi] = g;
         L.groups32[gi].o_off = ooff4;
         ioff += (int64_t)g.ne * g.npad;
-        ooff2 += (int64_t)nvec_of(g.n, 2) * 2 * g.npad;
-        ooff4 += (int64_t)nvec_of(g.n, 4) * 4 * g.npad;
+        ooff2 += (int64_t)nvec_of(g, 2) * 2 * g.npad;
+        ooff4 += (int64_t)nvec_of(g, 4) * 4 * g.npad;
         loff += (int64_t)L.q * g.npad;
     }
     L.o_len2 = ooff2;
@@ -215,18 +216,19 @@ Layout layout_build(const Mesh& m, const Setup& S) {
 }
 
 void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& O64, std::vector<double>& L64,
-                std::vector<float>& O32, std::vector<float>& L32) {
+                std::vector<float>& O32, std::vector<float>& L32, std::vector<uint32_t>& wmask) {
     O64.assign(L.o_len2, 0.0);
     O32.assign(L.o_len4, 0.0f);
     L64.assign(L.l_len, 0.0);
     L32.assign(L.l_len, 0.0f);
+    wmask.assign(L.cpad, 0u);
     const int q = L.q;
     parallel_for((int64_t)L.groups.size(), [&](int64_t gb, int64_t ge) {
         std::vector<double> a;
         for (int64_t gi = gb; gi < ge; ++gi) {
             const GroupDesc& g = L.groups[gi];
             const int64_t off2 = g.o_off, off4 = L.groups32[gi].o_off;
-            const int n = g.n, na = n * (n + 1) / 2, nw = q * n;
+            const int n = g.n, ned = (L.k + 1) * g.ne, na = n * (n + 1) / 2, nw = (q - 1) * ned;
             const int nav2 = (na + 1) / 2, nv2 = nav2 + (nw + 1) / 2;
             const int nav4 = (na + 3) / 4, nv4 = nav4 + (nw + 3) / 4;
             for (int lane = 0; lane < g.npad; ++lane) {
@@ -249,7 +251,31 @@ void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<doub
                     for (int j = 0; j <= i; ++j)
                         if (isb[i] || isb[j]) a[i * (i + 1) / 2 + j] = (i == j) ? 1.0 : 0.0;
                 for (int t = 0; t < na; ++t) put(t, -1, a[t]);
-                for (int t = 0; t < nw; ++t) put(-1, t, isb[t % n] ? 0.0 : Wc[t]);
+                for (int t = 0; t < nw; ++t) {
+                    const int r = 1 + t / ned, i = t % ned;
+                    put(-1, t, isb[i] ? 0.0 : Wc[r * n + i]);
+                }
+                // the exact part of W: row 0 and the internal columns
+                uint32_t mk = 0;
+                for (int i = 0; i < n; ++i)
+                    for (int r = 0; r < q; ++r) {
+                        double want;
+                        if (i >= ned)
+                            want = (i - ned < q - 1 && r == i - ned + 1) ? -1.0 : 0.0;
+                        else if (r > 0)
+                            continue;
+                        else if (i >= g.ne)
+                            want = 0.0;
+                        else
+                            want = std::fabs(Wc[i]) == 1.0 ? Wc[i] : 2.0;
+                        if (Wc[r * n + i] != want)
+                            throw Error(VEMD_ERR_INTERNAL, "unexpected divergence entry in cell " + std::to_string(c));
+                    }
+                for (int j = 0; j < g.ne; ++j) {
+                    if (isb[j]) mk |= 1u << (16 + j);
+                    else if (Wc[j] < 0.0) mk |= 1u << j;
+                }
+                wmask[g.gpos0 + lane] = mk;
                 for (int r = 0; r < q; ++r) {
                     L64[g.l_off + (int64_t)r * g.npad + lane] = S.L[(size_t)c * q + r];
                     L32[g.l_off + (int64_t)r * g.npad + lane] = (float)S.L[(size_t)c * q + r];

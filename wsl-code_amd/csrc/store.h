@@ -65,11 +65,17 @@ struct Layout {
 Layout layout_build(const Mesh& m, const Setup& S);
 
 // Packs each cell's operator stream -- A (packed symmetric lower triangle,
-// essential boundary dofs decoupled) then W (q x n), each padded to whole
-// 16-byte vectors (2 FP64 / 4 FP32 values) -- tile-major: vector v of lane l
-// of tile tau of a group at ((tau * nv + v) * 64 + l) * width; and L.
+// essential boundary dofs decoupled) then the geometric part of W, rows
+// 1..q-1 at the (k+1) ne edge columns, each padded to whole 16-byte vectors
+// (2 FP64 / 4 FP32 values) -- tile-major: vector v of lane l of tile tau of a
+// group at ((tau * nv + v) * 64 + l) * width; and L.  The rest of W is exact
+// and not streamed: row 0 is +1 / -1 (direction sign) / 0 (decoupled boundary
+// dof) at the first moment of each edge and 0 elsewhere -- recorded per store
+// position in wmask (bit j: -1, bit 16 + j: 0, for local edge j) -- and the
+// internal columns carry W(r + 1, edof + r) = -1 for r < q - 1 and zeros
+// (the packing checks both).
 void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& O64, std::vector<double>& L64,
-                std::vector<float>& O32, std::vector<float>& L32);
+                std::vector<float>& O32, std::vector<float>& L32, std::vector<uint32_t>& wmask);
 
 // Per-store-position cell scale exp(-theta) (padding 0).
 void pack_scale(const Setup& S, const Layout& L, std::vector<double>& s);
