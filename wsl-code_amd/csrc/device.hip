@@ -474,6 +474,9 @@ struct UpdArgs {
     T* bp;               // cycle input, cell-major: bp[gpos * q + r]
     int64_t cpad;
     int q;
+    T* bd0;              // non-null: also the cycle's first update bd0 = c0 * bsc * bp
+    const T* bsc;
+    T c0;
 };
 
 // Cycle index of entry i of the pressure block (slot r = moment r).
@@ -533,7 +536,9 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
         }
         r1[i] = y;
         if (cyc) {
-            a.bp[cyc_index(a, i, pr)] = y;  // preconditioned by the cycle, dots in k_pdot
+            const int64_t j = cyc_index(a, i, pr);
+            a.bp[j] = y;  // preconditioned by the cycle, dots in k_pdot
+            if (a.bd0) a.bd0[j] = a.c0 * a.bsc[j] * y;
         } else {
             T zn = a.minv[i] * y;
             zo[i] = zn;
@@ -570,8 +575,10 @@ __global__ __launch_bounds__(256) void k_pdot(UpdArgs<T> a, const int* pchunks, 
 
 // Givens recurrence of MINRES (Paige and Saunders) and the stopping test, one
 // block per instance.
+// With all_done non-null (a single instance) the block also publishes the
+// instance's stop flag as the all-done flag.
 __global__ void k_finB(MState* sts, const double* p1, const double* p2, const double* q1, const double* q2,
-                       const int* inst_chunk0, int useq) {
+                       const int* inst_chunk0, int useq, int* all_done) {
     MState* st = sts + blockIdx.x;
     if (st->done) return;
     const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
@@ -603,6 +610,7 @@ __global__ void k_finB(MState* sts, const double* p1, const double* p2, const do
         st->pending = 1;
         st->itn += 1;
         if (st->phibar <= st->tol || st->itn >= st->maxit || !(beta_new > 0.0)) st->done = 1;
+        if (all_done) *all_done = st->done;
     }
 }
 
@@ -1090,10 +1098,16 @@ static UpdArgs<T> upd_args(Solver& s, DevKrylov<T>& k, bool amg) {
     const Layout& L = s.layout;
     UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x,  k.Az,      k.Az1, ops<T>(d).minv,
                  L.cbase,          d.chunks,          d.p1,              d.p2, d.st, amg ? 1 : 0, 0, nullptr,
-                 L.cpad,           L.q};
+                 L.cpad,           L.q,               nullptr,           nullptr, T(0)};
     if (amg) {
+        const AmgDev<T>& M = *amg_of<T>(d);
         u.pb = L.cbase + (int64_t)L.ni * L.cpad;
-        u.bp = amg_of<T>(d)->lv[0].b;
+        u.bp = M.lv[0].b;
+        if (amg_pre_fusable(M)) {  // the cycle's first update formed here
+            u.bd0 = M.lv[0].d0;
+            u.bsc = M.lv[0].sc ? M.lv[0].sc : M.lv[0].dinv;
+            u.c0 = (T)(1.0 / M.lv[0].centre);
+        }
     }
     return u;
 }
@@ -1112,13 +1126,13 @@ static void enqueue_iteration(Solver& s, DevKrylov<T>& k, bool amg) {
     hipLaunchKernelGGL((k_update<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
     if (amg) {
         const AmgDev<T>& M = *amg_of<T>(d);
-        amg_vcycle(M, d.stream, d.all_done);
+        amg_vcycle(M, d.stream, d.all_done, u.bd0 != nullptr);
         hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
                            (const T*)M.lv[0].x, d.q1, d.q2, 0);
     }
     hipLaunchKernelGGL(k_finB, dim3(d.ninst), dim3(FIN), 0, d.stream, d.st, d.p1, d.p2, d.q1, d.q2, d.inst_chunk0,
-                       amg ? 1 : 0);
-    hipLaunchKernelGGL(k_alldone, dim3(1), dim3(256), 0, d.stream, d.st, d.ninst, d.all_done);
+                       amg ? 1 : 0, d.ninst == 1 ? d.all_done : nullptr);
+    if (d.ninst > 1) hipLaunchKernelGGL(k_alldone, dim3(1), dim3(256), 0, d.stream, d.st, d.ninst, d.all_done);
 }
 
 // Preconditioned MINRES on K x = b for every instance (internal order,
@@ -1246,6 +1260,11 @@ static void solve_device(Solver& s, const vemd_solve_opts& o, vemd_solve_info* i
         ++refin;
         std::vector<double> rn = residual64(s);
         for (int i = 0; i < ni; ++i) rel[i] = bn[i] > 0 ? rn[i] / bn[i] : 0.0;
+        static const bool verbose = std::getenv("VEMD_VERBOSE") != nullptr;
+        if (verbose)
+            std::fprintf(stderr, "[vemd] refinement %d: inner target %.2e (instance 0), %d iterations, true relative "
+                                 "residual %.3e (worst %.3e)\n",
+                         refin, target[0], its[0], rel[0], worst());
     }
     s.inst_iterations = it_tot;
     s.inst_rel = rel;

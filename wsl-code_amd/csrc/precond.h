@@ -94,9 +94,14 @@ AmgDev<T>* amg_upload(const Amg& H);
 
 // Enqueues one V-cycle on the stream: x = B b with b = lv[0].b, x = lv[0].x.
 // With done non-null every kernel returns at once while *done is set (work
-// skipped inside a captured chunk of iterations after convergence).
+// skipped inside a captured chunk of iterations after convergence).  With
+// pre_done the producer of b has also written lv[0].d0 = b / centre times
+// lv[0].sc (scaled level) or lv[0].dinv, the cycle's first update, which
+// saves the cycle's first sweep over the level (only if amg_pre_fusable).
 template <typename T>
-void amg_vcycle(const AmgDev<T>& D, hipStream_t stream, const int* done);
+void amg_vcycle(const AmgDev<T>& D, hipStream_t stream, const int* done, bool pre_done = false);
+template <typename T>
+bool amg_pre_fusable(const AmgDev<T>& D);
 
 // The same cycle applied to MCOLS interleaved columns (entry (row, j) at
 // row * MCOLS + j): b = lv[0].mb, x = lv[0].mx.  amg_multi_alloc first.

@@ -124,25 +124,28 @@ __global__ __launch_bounds__(256) void k_pre0(int64_t n, const T* b, const T* es
 // The last step of a sweep (last != 0) completes the iterate instead,
 // x += d + d', and leaves r and d' unwritten (nothing reads them).
 // With xs non-null the completed iterate leaves the scaled space: x = xs (x + d + d').
+// With b0 non-null this is the first step of the sweep from x = 0 with r = b0
+// (times es on a scaled level) and d = din formed by the caller.
 template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, const T* din, T* dout, T* x, T c1, T c2,
-                                            int last, const T* xs, const int* done) {
+                                            int last, const T* xs, const T* b0, const T* es, const int* done) {
     AMG_BROW
     T ad[B];
     sell_row<T, B, NC>(A, bi, j, din, ad);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
-        const T ri = r[i] - ad[k];
+        const T ri = (b0 ? bval(b0, es, i, bi * B + k) : r[i]) - ad[k];
         const T di = din[i];
         const T dn = c1 * di + c2 * dval(dinv, bi * B + k) * ri;
+        const T xo = b0 ? T(0) : x[i];
         if (last) {
-            const T xn = x[i] + di + dn;
+            const T xn = xo + di + dn;
             x[i] = xs ? xn * xs[bi * B + k] : xn;
         } else {
             r[i] = ri;
             dout[i] = dn;
-            x[i] += di;
+            x[i] = xo + di;
         }
     }
 }
@@ -435,7 +438,7 @@ static dim3 grid_of(int64_t threads) { return dim3((unsigned)((threads + 255) / 
 // step and no fused start the update stays pending in din.
 template <typename T, int B, int NC>
 static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStream_t st, const int* ms, T*& din,
-                         bool fused, const T* xs) {
+                         bool fused, const T* xs, bool from_b = false) {
     const double sigma = L.centre / L.half;
     double rho = 1.0 / sigma;
     T* dout = din == v.d0 ? v.d1 : v.d0;
@@ -447,9 +450,11 @@ static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStre
                                (const T*)L.dinv, v.r, din, v.x, (T)(1.0 / L.centre), (T)(rn * rho),
                                (T)(2.0 * rn / L.half), last, ms);
         } else {
+            const bool b0 = from_b && k == 1;
             hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC), dim3(256), 0, st, L.A, v.r,
                                (const T*)L.dinv, (const T*)din, dout, v.x, (T)(rn * rho), (T)(2.0 * rn / L.half),
-                               last, last ? xs : nullptr, ms);
+                               last, last ? xs : nullptr, b0 ? (const T*)v.b : nullptr, b0 ? (const T*)L.sc : nullptr,
+                               ms);
             std::swap(din, dout);
         }
         if (last) din = nullptr;
@@ -458,7 +463,7 @@ static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStre
 }
 
 template <typename T, int NC>
-static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms);
+static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done = false);
 
 // Pre-smoothing, residual, restriction, coarse cycle, prolongation and
 // post-smoothing of one level whose operator has blocks of B rows.  The
@@ -466,7 +471,7 @@ static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const
 // smooths with a single step (then coarser levels leave the sum to the
 // prolongation that reads them and the finest level adds it).
 template <typename T, int B, int NC>
-static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms) {
+static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done) {
     const AmgLevelDev<T>& L = D.lv[l];
     const AmgLevelDev<T>& C = D.lv[l + 1];
     const LevelVecs<T> v = vecs<T, NC>(L), vc = vecs<T, NC>(C);
@@ -478,10 +483,11 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
     // (gathers of B consecutive entries) are cheaper with it materialised
     const bool fused = B == 1 && L.degree >= 2 && !L.sc;
     const T* sc = L.sc;
-    if (!fused)
+    // pre_done: the caller has written d0 = b / centre (D^-1 or sc applied)
+    if (!fused && !pre_done)
         hipLaunchKernelGGL((k_pre0<T, NC>), g, b, 0, st, L.n, (const T*)v.b, sc, (const T*)L.dinv, v.r, v.d0, v.x, c0,
                            ms);
-    smooth_steps<T, B, NC>(L, v, st, ms, din, fused, (const T*)nullptr);
+    smooth_steps<T, B, NC>(L, v, st, ms, din, fused, (const T*)nullptr, pre_done);
     hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)din, v.r,
                        ms);
     hipLaunchKernelGGL((k_restrict<T, NC>), gr, b, 0, st, L.R, (const T*)v.r, vc.b, ms);
@@ -497,7 +503,7 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
 }
 
 template <typename T, int NC>
-static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms) {
+static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done) {
     const AmgLevelDev<T>& L = D.lv[l];
     if (l + 1 == D.lv.size()) {
         const LevelVecs<T> v = vecs<T, NC>(L);
@@ -506,15 +512,21 @@ static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const
         return nullptr;
     }
     switch (L.A.bs) {
-        case 1: return level_cycle<T, 1, NC>(D, l, st, ms);
-        case 3: return level_cycle<T, 3, NC>(D, l, st, ms);
+        case 1: return level_cycle<T, 1, NC>(D, l, st, ms, pre_done);
+        case 3: return level_cycle<T, 3, NC>(D, l, st, ms, pre_done);
         default: throw Error(VEMD_ERR_INTERNAL, "unsupported multigrid block size");
     }
 }
 
 template <typename T>
-void amg_vcycle(const AmgDev<T>& D, hipStream_t st, const int* ms) {
-    vcycle_level<T, 1>(D, 0, st, ms);
+bool amg_pre_fusable(const AmgDev<T>& D) {
+    return D.lv.size() > 1 && D.lv[0].degree >= 2 && !(D.lv[0].A.bs == 1 && !D.lv[0].sc);
+}
+
+template <typename T>
+void amg_vcycle(const AmgDev<T>& D, hipStream_t st, const int* ms, bool pre_done) {
+    if (pre_done && !amg_pre_fusable(D)) throw Error(VEMD_ERR_INTERNAL, "multigrid cycle cannot take a formed start");
+    vcycle_level<T, 1>(D, 0, st, ms, pre_done);
     HIPCHK(hipGetLastError());
 }
 
@@ -544,8 +556,10 @@ template struct AmgDev<double>;
 template struct AmgDev<float>;
 template AmgDev<double>* amg_upload<double>(const Amg&);
 template AmgDev<float>* amg_upload<float>(const Amg&);
-template void amg_vcycle<double>(const AmgDev<double>&, hipStream_t, const int*);
-template void amg_vcycle<float>(const AmgDev<float>&, hipStream_t, const int*);
+template void amg_vcycle<double>(const AmgDev<double>&, hipStream_t, const int*, bool);
+template void amg_vcycle<float>(const AmgDev<float>&, hipStream_t, const int*, bool);
+template bool amg_pre_fusable<double>(const AmgDev<double>&);
+template bool amg_pre_fusable<float>(const AmgDev<float>&);
 template void amg_multi_alloc<double>(AmgDev<double>&);
 template void amg_multi_alloc<float>(AmgDev<float>&);
 template void amg_vcycle_multi<double>(const AmgDev<double>&, hipStream_t, const int*);
