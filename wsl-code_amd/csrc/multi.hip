@@ -10,6 +10,7 @@
 // FP64 iterative refinement per column.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "devstate.h"
@@ -604,7 +605,15 @@ static MultiState& multi_setup(Solver& s) {
     int dev = 0, ncu = 256;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    m.nwaves = std::max(4, ncu * 16) / 4 * 4;
+    // exactly the waves that fit on the chip at once (a second round of
+    // persistent waves would run after the first and double the time)
+    int bpc = 0;
+    if (s.layout.k == 0)
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_apply_mr<0>, 256, 0));
+    else
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_apply_mr<1>, 256, 0));
+    if (const char* e = std::getenv("VEMD_MR_BLOCKS_PER_CU")) bpc = std::max(1, std::atoi(e));
+    m.nwaves = std::max(1, ncu * std::max(bpc, 1)) * 4;
     m.cells_per_wave = (L.cpad + m.nwaves - 1) / m.nwaves;
     const size_t fv = (size_t)L.nvec * NR * sizeof(float);
     for (int i = 0; i < 2; ++i) {
