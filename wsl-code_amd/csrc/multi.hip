@@ -241,7 +241,25 @@ __global__ __launch_bounds__(256) void k_apply_mr(MrArgs a) {
                     a.Y1[e] = acc[r];
                 else
                     a.Y[e] = acc[r];
-                if (a.pw) dot += (double)Xj[e] * (double)acc[r];
+            }
+        }
+        if (a.pw) {
+            // x.Kx of the cell from registers: row 2 kk + h of column col is
+            // Bc[kk] of lane (col, h); the other half-wave's rows come across
+            float P[KSTEPS];
+#pragma unroll
+            for (int kk = 0; kk < KSTEPS; ++kk) P[kk] = __shfl_xor(Bc[kk], 32, 64);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 8 * (r >> 2) + 4 * hi + (r & 3);
+                const int i0 = 4 * (r >> 2) + ((r & 3) >> 1);  // row / 2 on the lower half-wave
+                float xv = 0.0f;
+                if (hi) {
+                    if (i0 + 2 < KSTEPS) xv = (r & 1) ? Bc[i0 + 2 < KSTEPS ? i0 + 2 : 0] : P[i0 + 2 < KSTEPS ? i0 + 2 : 0];
+                } else {
+                    if (i0 < KSTEPS) xv = (r & 1) ? P[i0 < KSTEPS ? i0 : 0] : Bc[i0 < KSTEPS ? i0 : 0];
+                }
+                if (row < cur.m) dot += (double)xv * (double)acc[r];
             }
         }
         __builtin_amdgcn_wave_barrier();  // buffer b is free for the cell after next
