@@ -105,3 +105,23 @@ def test_solve_without_device_fails_loudly(vemd):
     with pytest.raises(vemd.VemdError) as ei:
         s.solve()
     assert ei.value.status == "ERR_STATE"
+
+
+def test_device_setup_limits_checked_on_the_host(vemd):
+    """The GPU element setup rejects k > 1 and polygons of more than 10 edges
+    before touching a device, and without a device it fails loudly."""
+    m = vemd.Mesh.read(golden("square_2x2.msh"))
+    with pytest.raises(vemd.VemdError) as ei:
+        vemd.Solver(m, k=2, setup_device=0)
+    assert ei.value.status == "ERR_ARG"
+    # one regular 11-gon
+    t = np.linspace(0.0, 2.0 * np.pi, 12)[:-1]
+    xy = np.stack([np.cos(t), np.sin(t)], 1)
+    big = vemd.Mesh.from_arrays(xy, np.array([0, 11], np.int32), np.arange(11, dtype=np.int32))
+    with pytest.raises(vemd.VemdError) as ei:
+        vemd.Solver(big, k=1, setup_device=0)
+    assert ei.value.status == "ERR_ARG" and "10 edges" in str(ei.value)
+    if vemd.device_count() == 0:
+        with pytest.raises(vemd.VemdError) as ei:
+            vemd.Solver(m, k=1, setup_device=0)
+        assert ei.value.status == "ERR_DEVICE"

@@ -131,6 +131,23 @@ def test_multigrid_preconditioner(vemd, oracle, cells):
     assert np.linalg.norm(xm - xo) <= 1e-5 * np.linalg.norm(xo)
 
 
+def test_mixed_cycle_with_fp32_finest_level(vemd, oracle, monkeypatch):
+    """The FP32 cycle's finest level in half precision (default, scaled to a
+    unit diagonal) and in FP32 (VEMD_MG_HALF=0) both precondition the mixed
+    solve to the same solution; the half-precision level costs at most a few
+    iterations."""
+    s, op = _gen_pair(vemd, oracle, "annulus", 3000, 1, 1, True)
+    o = vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-10, precond=vemd.PC_MULTIGRID)
+    xh, ih = s.solve(o)
+    monkeypatch.setenv("VEMD_MG_HALF", "0")
+    s32, _ = _gen_pair(vemd, oracle, "annulus", 3000, 1, 1, True)
+    x32, i32 = s32.solve(o)
+    print("iterations: half-precision finest level", ih["iterations"], "FP32", i32["iterations"], flush=True)
+    assert ih["rel_residual"] <= 1e-10 and i32["rel_residual"] <= 1e-10
+    assert np.linalg.norm(xh - x32) <= 1e-8 * np.linalg.norm(x32)
+    assert ih["iterations"] <= 1.1 * i32["iterations"] + 5
+
+
 def test_multigrid_needs_dirichlet(vemd, oracle):
     s, _ = _gen_pair(vemd, oracle, "disk", 1024, 1, 0, False)
     with pytest.raises(vemd.VemdError) as e:

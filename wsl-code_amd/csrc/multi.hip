@@ -10,6 +10,7 @@
 // FP64 iterative refinement per column.
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -847,6 +848,10 @@ void dev_solve_multi(Solver& s, const vemd_solve_opts& o, int ncol, const double
         ++refin;
         std::vector<double> rn = residual_mr(s, m, b, ncol);
         for (int j = 0; j < ncol; ++j) rel[j] = bn[j] > 0 ? rn[j] / bn[j] : 0.0;
+        static const bool verbose = std::getenv("VEMD_VERBOSE") != nullptr;
+        if (verbose)
+            std::fprintf(stderr, "[vemd] multi-RHS refinement %d: %d iterations (column 0), worst true relative "
+                                 "residual %.3e\n", refin, its[0], worst());
     }
     if (x) HIPCHK(hipMemcpyAsync(x, m.x64, (size_t)ncol * n * sizeof(double), hipMemcpyDeviceToDevice, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));

@@ -624,6 +624,10 @@ void down(std::vector<T>& v, const T* d) {
 
 Setup* setup_build_device(const Mesh& m, const vemd_problem_desc& d, int device) {
     if (d.k > 1) throw Error(VEMD_ERR_ARG, "the device element setup supports k = 0 and k = 1");
+    for (int c = 0; c < m.nc; ++c)
+        if (m.npc(c) > NE)
+            throw Error(VEMD_ERR_ARG, "cell " + std::to_string(c) + " has more than " + std::to_string(NE) +
+                                          " edges (the device path's limit)");
     std::unique_ptr<Setup> S(setup_prepare(m, d));
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(VEMD_ERR_DEVICE, "no HIP device");
