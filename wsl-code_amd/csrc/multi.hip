@@ -92,6 +92,7 @@ struct MrArgs {
     float* Y;
     float* Y1;                // second sides of the edge dofs
     int64_t cbase, cpad;
+    int64_t trash;            // a padding row: target of the stores of rows a cell does not have
     int64_t cells_per_wave;
     double* pw;               // per-wave, per-column partials of x.Kx (nullptr: not needed)
     const MState* st;         // per column
@@ -113,44 +114,46 @@ __device__ __forceinline__ float kval(const float* op, int n, int na, int m, int
 
 constexpr int KSTEPS = 13;  // ceil(m / 2) <= 13 for polygons of up to 10 edges (k = 1)
 
-// What a wave needs of one cell: the shape (wave-uniform), two float4 of the
-// packed operator per lane and the lane's row slot (rows < m).
-struct CellMeta {
-    int64_t gpos;    // -1: none
+// The shape of a cell of group g (wave-uniform).
+template <int K>
+struct MrShape {
     int n, m, na, nv4;
-    float s;
-    float4 o0, o1;
-    uint32_t slot;
+    __device__ __forceinline__ MrShape(const MrGroup& g) {
+        constexpr int Q = (K + 1) * (K + 2) / 2;
+        n = g.n;
+        m = n + Q;
+        na = n * (n + 1) / 2;
+        nv4 = (na + Q * n + 3) / 4;
+    }
 };
 
-// Issues the loads of cell gpos of group g (all independent of each other:
-// the group table sits in LDS).
+// Issues the loads of one cell (gpos < 0: none): two float4 of its packed
+// operator per lane, its scale and the lane's row slot (rows < m). They are
+// independent of each other: the group table sits in LDS.
 template <int K>
-__device__ __forceinline__ void cell_meta(const MrArgs& a, const MrGroup& g, int64_t gpos, int l, CellMeta& c) {
+__device__ __forceinline__ void cell_loads(const MrArgs& a, const MrGroup& g, int64_t gpos, int l, float4& o0,
+                                           float4& o1, uint32_t& slot, float& s) {
     constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1));
-    c.gpos = gpos;
-    if (gpos < 0) return;
-    const int ne = g.ne, n = g.n, ned = KK * ne;
-    c.n = n;
-    c.m = n + Q;
-    c.na = n * (n + 1) / 2;
-    c.nv4 = (c.na + Q * n + 3) / 4;
-    const int64_t cg = gpos - g.gpos0;
-    const float4* src = reinterpret_cast<const float4*>(a.op + g.gop) + cg * c.nv4;
     const float4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-    c.o0 = l < c.nv4 ? src[l] : z;
-    c.o1 = l + 64 < c.nv4 ? src[l + 64] : z;
-    c.s = a.scale[gpos];
-    uint32_t sl = 0;
+    o0 = o1 = z;
+    slot = 0;
+    s = 0.0f;
+    if (gpos < 0) return;
+    const MrShape<K> sh(g);
+    const int ne = g.ne, ned = KK * ne;
+    const int64_t cg = gpos - g.gpos0;
+    const float4* src = reinterpret_cast<const float4*>(a.op + g.gop) + cg * sh.nv4;
+    if (l < sh.nv4) o0 = src[l];
+    if (l + 64 < sh.nv4) o1 = src[l + 64];
+    s = a.scale[gpos];
     if (l < ned) {
         const uint32_t id = a.idx[g.i_off + ((cg >> 6) * ne + l % ne) * 64 + (cg & 63)];
-        sl = (id & SIDE1) | ((id & 0x7fffffffu) + (uint32_t)(l / ne));
-    } else if (l < n) {
-        sl = (uint32_t)(a.cbase + (l - ned) * a.cpad + gpos);
-    } else if (l < c.m) {
-        sl = (uint32_t)(a.cbase + (NI + l - n) * a.cpad + gpos);
+        slot = (id & SIDE1) | ((id & 0x7fffffffu) + (uint32_t)(l / ne));
+    } else if (l < sh.n) {
+        slot = (uint32_t)(a.cbase + (l - ned) * a.cpad + gpos);
+    } else if (l < sh.m) {
+        slot = (uint32_t)(a.cbase + (NI + l - sh.n) * a.cpad + gpos);
     }
-    c.slot = sl;
 }
 
 // First real cell at or after store position p and before end, moving the
@@ -167,7 +170,7 @@ __device__ __forceinline__ int64_t next_cell(const MrGroup* lg, int ng, int64_t 
 // loop is software-pipelined: while cell c is multiplied, the rows of cell
 // c + 1 are being gathered and the operator of cell c + 2 is being loaded.
 template <int K>
-__global__ __launch_bounds__(256) void k_apply_mr(MrArgs a) {
+__global__ __launch_bounds__(256, 3) void k_apply_mr(MrArgs a) {
     if (a.all_done && *a.all_done) return;
     __shared__ __attribute__((aligned(16))) float lop[4][2][OPMAX];
     __shared__ uint32_t lslot[4][2][32];
@@ -184,91 +187,117 @@ __global__ __launch_bounds__(256) void k_apply_mr(MrArgs a) {
     const int64_t g0 = (int64_t)wave * a.cells_per_wave;
     const int64_t g1 = g0 + a.cells_per_wave < a.cpad ? g0 + a.cells_per_wave : a.cpad;
     // stage a cell's operator and slots into LDS buffer b and issue its row gathers
-    auto stage = [&](const CellMeta& c, int b, float (&B)[KSTEPS]) {
+    auto stage = [&](int m, int nv4, float4 o0, float4 o1, uint32_t slot, int b, float(&B)[KSTEPS])
+        __attribute__((always_inline)) {
         float4* d = reinterpret_cast<float4*>(lop[w][b]);
-        if (l < c.nv4) d[l] = c.o0;
-        if (l + 64 < c.nv4) d[l + 64] = c.o1;
-        if (l < c.m) lslot[w][b][l] = c.slot;
+        if (l < nv4) d[l] = o0;
+        if (l + 64 < nv4) d[l + 64] = o1;
+        if (l < m) lslot[w][b][l] = slot;
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         __builtin_amdgcn_wave_barrier();
-        const int ks = (c.m + 1) / 2;
+        // every lane loads (rows the cell does not have read the padding row
+        // and are zeroed), so the loads stay in one basic block and the wait
+        // counters track them exactly
 #pragma unroll
         for (int kk = 0; kk < KSTEPS; ++kk) {
-            B[kk] = 0.0f;
             const int cc = 2 * kk + hi;
-            if (kk < ks && cc < c.m) B[kk] = Xj[(int64_t)(lslot[w][b][cc] & 0x7fffffffu) * NR + col];
+            const bool on = cc < m;
+            const uint32_t sl = lslot[w][b][on ? cc : 0];
+            const int64_t row = on ? (int64_t)(sl & 0x7fffffffu) : a.trash;
+            const float v = Xj[row * NR + col];
+            B[kk] = on ? v : 0.0f;
         }
     };
-    CellMeta cur, nxt, nn;
-    float Bc[KSTEPS], Bn[KSTEPS];
+    // Pipeline over the wave's cells: step i multiplies cell i while the rows
+    // of cell i + 1 are gathered and the operator, scale and slots of cell
+    // i + 2 are loaded. Cell j's loads land in register set j % 2 and its rows
+    // in buffer j % 2, and the loop body is written twice with the roles
+    // swapped, so no register that a load is still filling is ever copied
+    // (a copy would make the wave wait for that load).
+    float BX[KSTEPS], BY[KSTEPS];
+#pragma unroll
+    for (int kk = 0; kk < KSTEPS; ++kk) BX[kk] = BY[kk] = 0.0f;
+    float4 oP0, oP1, oQ0, oQ1;
+    uint32_t slP, slQ;
+    float sP, sQ;
+    int64_t gpP, gpQ;
+    int gP, gQ;
     int gi = 0;
-    int64_t p = next_cell(lg, ng, g0, g1, gi);
-    cell_meta<K>(a, lg[gi < ng ? gi : 0], p, l, cur);
-    int b = 0;
-    if (cur.gpos >= 0) {
-        stage(cur, b, Bc);
-        p = next_cell(lg, ng, cur.gpos + 1, g1, gi);
-        cell_meta<K>(a, lg[gi < ng ? gi : 0], p, l, nxt);
-    } else {
-        nxt.gpos = -1;
+    gpP = next_cell(lg, ng, g0, g1, gi);
+    gP = gi < ng ? gi : 0;
+    cell_loads<K>(a, lg[gP], gpP, l, oP0, oP1, slP, sP);
+    if (gpP >= 0) {
+        const MrShape<K> sh(lg[gP]);
+        stage(sh.m, sh.nv4, oP0, oP1, slP, 0, BX);
     }
-    while (cur.gpos >= 0) {
-        if (nxt.gpos >= 0) {
-            stage(nxt, b ^ 1, Bn);
-            p = next_cell(lg, ng, nxt.gpos + 1, g1, gi);
-            cell_meta<K>(a, lg[gi < ng ? gi : 0], p, l, nn);
-        } else {
-            nn.gpos = -1;
+    gpQ = gpP >= 0 ? next_cell(lg, ng, gpP + 1, g1, gi) : -1;
+    gQ = gi < ng ? gi : 0;
+    cell_loads<K>(a, lg[gQ], gpQ, l, oQ0, oQ1, slQ, sQ);
+    // one step: A (set a_*, rows Bm, LDS buffer b) is multiplied, B (set b_*)
+    // is staged into buffer b ^ 1 with its rows gathered into Bg, and the
+    // next cell is loaded into A's set
+    auto step = [&](float4& aO0, float4& aO1, uint32_t& aSl, float& aS, int64_t& aGp, int& aG, const float4& bO0,
+                    const float4& bO1, const uint32_t& bSl, const int64_t& bGp, const int& bG, float(&Bm)[KSTEPS],
+                    float(&Bg)[KSTEPS], int b) __attribute__((always_inline)) {
+        const float curs = aS;
+        const MrShape<K> cur(lg[aG]);
+        const int64_t gpC = bGp >= 0 ? next_cell(lg, ng, bGp + 1, g1, gi) : -1;
+        aGp = gpC;
+        aG = gi < ng ? gi : 0;
+        cell_loads<K>(a, lg[aG], gpC, l, aO0, aO1, aSl, aS);
+        if (bGp >= 0) {
+            const MrShape<K> shB(lg[bG]);
+            stage(shB.m, shB.nv4, bO0, bO1, bSl, b ^ 1, Bg);
         }
-        // multiply cell cur
+        // multiply the cell
         const float* op = lop[w][b];
         const int ks = (cur.m + 1) / 2;
         f16v acc = {};
 #pragma unroll
         for (int kk = 0; kk < KSTEPS; ++kk) {
             if (kk < ks) {
-                const float av = kval(op, cur.n, cur.na, cur.m, col, 2 * kk + hi, cur.s);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Bc[kk], acc, 0, 0, 0);
+                const float av = kval(op, cur.n, cur.na, cur.m, col, 2 * kk + hi, curs);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Bm[kk], acc, 0, 0, 0);
             }
         }
-        // D[row][col]: col = lane % 32, row = 8 (r / 4) + 4 (lane / 32) + r % 4
+        // D[row][col]: col = lane % 32, row = 8 (r / 4) + 4 (lane / 32) + r % 4;
+        // rows the cell does not have hold zeros (their operator rows are
+        // zero) and go to the padding row, without a branch
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int row = 8 * (r >> 2) + 4 * hi + (r & 3);
-            if (row < cur.m) {
-                const uint32_t sl = lslot[w][b][row];
-                const int64_t e = (int64_t)(sl & 0x7fffffffu) * NR + col;
-                if (sl & SIDE1)
-                    a.Y1[e] = acc[r];
-                else
-                    a.Y[e] = acc[r];
-            }
+            const bool on = row < cur.m;
+            const uint32_t sl = lslot[w][b][on ? row : 0];
+            const bool side1 = on && (sl & SIDE1);
+            float* __restrict__ dst = side1 ? a.Y1 : a.Y;
+            const int64_t e = (on ? (int64_t)(sl & 0x7fffffffu) : a.trash) * NR + col;
+            dst[e] = acc[r];
         }
         if (a.pw) {
             // x.Kx of the cell from registers: row 2 kk + h of column col is
-            // Bc[kk] of lane (col, h); the other half-wave's rows come across
+            // Bm[kk] of lane (col, h); the other half-wave's rows come across
             float P[KSTEPS];
 #pragma unroll
-            for (int kk = 0; kk < KSTEPS; ++kk) P[kk] = __shfl_xor(Bc[kk], 32, 64);
+            for (int kk = 0; kk < KSTEPS; ++kk) P[kk] = __shfl_xor(Bm[kk], 32, 64);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = 8 * (r >> 2) + 4 * hi + (r & 3);
                 const int i0 = 4 * (r >> 2) + ((r & 3) >> 1);  // row / 2 on the lower half-wave
                 float xv = 0.0f;
                 if (hi) {
-                    if (i0 + 2 < KSTEPS) xv = (r & 1) ? Bc[i0 + 2 < KSTEPS ? i0 + 2 : 0] : P[i0 + 2 < KSTEPS ? i0 + 2 : 0];
+                    if (i0 + 2 < KSTEPS) xv = (r & 1) ? Bm[i0 + 2 < KSTEPS ? i0 + 2 : 0] : P[i0 + 2 < KSTEPS ? i0 + 2 : 0];
                 } else {
-                    if (i0 < KSTEPS) xv = (r & 1) ? P[i0 < KSTEPS ? i0 : 0] : Bc[i0 < KSTEPS ? i0 : 0];
+                    if (i0 < KSTEPS) xv = (r & 1) ? P[i0 < KSTEPS ? i0 : 0] : Bm[i0 < KSTEPS ? i0 : 0];
                 }
                 if (row < cur.m) dot += (double)xv * (double)acc[r];
             }
         }
         __builtin_amdgcn_wave_barrier();  // buffer b is free for the cell after next
-        cur = nxt;
-        nxt = nn;
-#pragma unroll
-        for (int kk = 0; kk < KSTEPS; ++kk) Bc[kk] = Bn[kk];
-        b ^= 1;
+    };
+    while (gpP >= 0) {
+        step(oP0, oP1, slP, sP, gpP, gP, oQ0, oQ1, slQ, gpQ, gQ, BX, BY, 0);
+        if (gpQ < 0) break;
+        step(oQ0, oQ1, slQ, sQ, gpQ, gQ, oP0, oP1, slP, gpP, gP, BY, BX, 1);
     }
     if (a.pw) {
         dot += __shfl_xor(dot, 32, 64);
@@ -685,6 +714,7 @@ static MrArgs mr_args(Solver& s, MultiState& m, const float* x0, const float* x1
     a.Y = y;
     a.Y1 = y1;
     a.cbase = s.layout.cbase;
+    a.trash = s.layout.trash_idx;
     a.cpad = s.layout.cpad;
     a.cells_per_wave = m.cells_per_wave;
     a.pw = solve ? m.pw : nullptr;
