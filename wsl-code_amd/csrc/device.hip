@@ -834,7 +834,9 @@ static void alloc_krylov(DeviceState& d, DevKrylov<T>& k, int64_t n, int64_t edo
 template <typename T>
 static void destroy_graph(DevKrylov<T>& k) {
     if (k.graph) (void)hipGraphExecDestroy(k.graph);
+    if (k.graph_tail) (void)hipGraphExecDestroy(k.graph_tail);
     k.graph = nullptr;
+    k.graph_tail = nullptr;
 }
 
 static void upload_theta(Solver& s) {
@@ -1145,15 +1147,20 @@ static int minres(Solver& s, DevKrylov<T>& k, const std::vector<double>& rtol, i
     DeviceState& d = *s.dev;
     UpdArgs<T> u = upd_args(s, k, amg);
     check_every = std::max(1, check_every);
+    // Near convergence a shorter graph is launched, so that fewer captured
+    // iterations run (as early-returning kernels) after the stop test fires.
+    const int tail = std::max(1, std::min(check_every, 4));
     if (use_graph && (k.graph_iters != check_every || k.graph_amg != amg || !k.graph)) {
         HIPCHK(hipStreamSynchronize(d.stream));
         destroy_graph(k);
-        hipGraph_t g;
-        HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k, amg);
-        HIPCHK(hipStreamEndCapture(d.stream, &g));
-        HIPCHK(hipGraphInstantiate(&k.graph, g, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(g));
+        for (int pass = 0; pass < 2; ++pass) {
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < (pass == 0 ? check_every : tail); ++i) enqueue_iteration(s, k, amg);
+            HIPCHK(hipStreamEndCapture(d.stream, &g));
+            HIPCHK(hipGraphInstantiate(pass == 0 ? &k.graph : &k.graph_tail, g, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(g));
+        }
         k.graph_iters = check_every;
         k.graph_amg = amg;
     }
@@ -1173,12 +1180,32 @@ static int minres(Solver& s, DevKrylov<T>& k, const std::vector<double>& rtol, i
     HIPCHK(hipGetLastError());
     static const bool verbose = std::getenv("VEMD_VERBOSE") != nullptr;
     auto last = std::chrono::steady_clock::now();
+    // per instance: iteration count and log of the residual estimate at the
+    // previous check (the convergence rate predicts the iterations left)
+    std::vector<int> prev_it(d.ninst, -1);
+    std::vector<double> prev_lg(d.ninst, 0.0);
     while (true) {
         HIPCHK(hipMemcpyAsync(d.st_host, d.st, d.ninst * sizeof(MState), hipMemcpyDeviceToHost, d.stream));
         HIPCHK(hipStreamSynchronize(d.stream));
         bool all = true;
         for (int i = 0; i < d.ninst && all; ++i) all = d.st_host[i].done != 0;
         if (all) break;
+        double left = 0.0;  // predicted iterations until the slowest unfinished instance stops
+        bool known = true;
+        for (int i = 0; i < d.ninst; ++i) {
+            const MState& m = d.st_host[i];
+            if (m.done) continue;
+            const double lg = std::log(std::max(m.phibar, 1e-300));
+            if (prev_it[i] >= 0 && m.itn > prev_it[i] && lg < prev_lg[i]) {
+                const double rate = (lg - prev_lg[i]) / (m.itn - prev_it[i]);
+                left = std::max(left, (std::log(std::max(m.tol, 1e-300)) - lg) / rate);
+            } else {
+                known = false;
+            }
+            prev_it[i] = m.itn;
+            prev_lg[i] = lg;
+        }
+        const bool near_end = known && left < check_every;
         if (verbose && std::chrono::steady_clock::now() - last > std::chrono::seconds(10)) {
             last = std::chrono::steady_clock::now();
             const MState& m = d.st_host[0];
@@ -1186,9 +1213,9 @@ static int minres(Solver& s, DevKrylov<T>& k, const std::vector<double>& rtol, i
                          sizeof(T) == 8 ? "fp64" : "fp32", m.itn, m.phibar / m.beta1, rtol[0], d.ninst);
         }
         if (use_graph)
-            HIPCHK(hipGraphLaunch(k.graph, d.stream));
+            HIPCHK(hipGraphLaunch(near_end ? k.graph_tail : k.graph, d.stream));
         else
-            for (int i = 0; i < check_every; ++i) enqueue_iteration(s, k, amg);
+            for (int i = 0; i < (near_end ? tail : check_every); ++i) enqueue_iteration(s, k, amg);
     }
     hipLaunchKernelGGL((k_flush<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
     HIPCHK(hipGetLastError());

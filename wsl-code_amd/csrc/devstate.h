@@ -33,7 +33,8 @@ struct DevKrylov {
     T* Az = nullptr;
     T* Az1 = nullptr;
     T* b = nullptr;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;   // check_every iterations
+    hipGraphExec_t graph_tail = nullptr;  // a few iterations, launched near convergence
     int graph_iters = 0;
     bool graph_amg = false;
 };
