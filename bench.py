@@ -250,7 +250,8 @@ def batch_bench(pkg, torch, ninst, cmin, cmax, seed):
             "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp32 iterations, fp64 refinement",
             "data": "synthetic: seeded Voronoi disks, per-instance GRF log-permeability, manufactured sin*sin data",
             "config": {"workload": "%d instances, cells log-uniform in [%d, %d], k=1, Dirichlet p" % (ninst, cmin, cmax),
-                       "cells": u.num_cells, "unknowns": s.ntot},
+                       "cells": u.num_cells, "unknowns": s.ntot,
+                       "apply_variants_by_valence": {str(k): v for k, v in s.tuning().items()}},
             "fp64_seconds": round(t6, 4), "speedup_mixed_vs_fp64": round(t6 / tm, 3),
             "iterations": {"max": max(its), "median": int(np.median(its)), "min": min(its)},
             "max_rel_residual": max(p["rel_residual"] for p in pm),
@@ -492,6 +493,7 @@ def main():
         "config": {"workload": "annulus 0.5<=r<=1, %d-cell Lloyd Voronoi (20 iters), k=1, log-normal K "
                                "(sigma 1, ell 0.05), Dirichlet p" % mesh.num_cells,
                    "cells": mesh.num_cells, "unknowns": s.ntot, "valence_groups": s.ngroups,
+                   "apply_variants_by_valence": {str(k): v for k, v in s.tuning().items()},
                    "preconditioner": args.precond, "parallelism": "1 GPU"},
         "fp64_seconds": round(t_fp64, 4), "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 3),
         "mixed": infos[-1], "fp64": i64,

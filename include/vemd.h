@@ -153,6 +153,13 @@ vemd_status vemd_solver_sizes(vemd_solver s, int64_t* sizes);
  * (GPU/src/hdiv_gpu.cu:139-192 does it inside the constructor). */
 vemd_status vemd_solver_upload(vemd_solver s, int device);
 
+/* The apply variant (operator-stream chunk shape of the cell kernel) the
+ * autotuner picked at upload for every polygon valence: out[2 * (ne - 3)] for
+ * FP64 and out[2 * (ne - 3) + 1] for FP32, ne = 3..10, -1 where the mesh has
+ * no such cells (16 ints). No counterpart in the reference, which has one
+ * fixed kernel; VEMD_AUTOTUNE=0 turns the tuner off. */
+vemd_status vemd_solver_tuning(vemd_solver s, int* out);
+
 /* Replaces the per-cell log-permeability; updates the cell scales, the
  * preconditioner and (essential-flux condition) the lifted right-hand side.
  * Replaces MixedGpu::update (GPU/include/equation_gpu.cuh:76-84). */

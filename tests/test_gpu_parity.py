@@ -193,3 +193,22 @@ def test_reference_errors_on_device(vemd, case):
         else:  # half a unit of the 7th digit plus the printing run's rtol 1e-10
             tol = 0.5 * 10.0 ** (np.floor(np.log10(v)) - 6) + 1e-7 * v
         assert abs(e[key] - v) <= tol, (key, e[key], v)
+
+
+def test_apply_autotuner(vemd, oracle, monkeypatch):
+    """The autotuner picks a variant per valence present in the mesh, and the
+    variants only reshape the operator stream's consumption: the apply with
+    the tuned variants equals the apply with variant 0 everywhere, bitwise."""
+    s, _ = _gen_pair(vemd, oracle, "annulus", 12000, 1, 1, True)
+    t = s.tuning()
+    print("apply variants (fp64, fp32) per valence", t, flush=True)
+    a = s.mesh.arrays()
+    present = set(int(v) for v in np.unique(np.diff(a["cell_off"])))
+    assert set(t) == present
+    assert all(0 <= v < 4 for pair in t.values() for v in pair)
+    monkeypatch.setenv("VEMD_AUTOTUNE", "0")
+    s0, _ = _gen_pair(vemd, oracle, "annulus", 12000, 1, 1, True)
+    assert all(pair == (0, 0) for pair in s0.tuning().values())
+    x = np.random.default_rng(4).standard_normal(s.ntot)
+    for prec in (vemd.FP64, vemd.FP32):
+        assert np.array_equal(s.apply(x, prec), s0.apply(x, prec))

@@ -64,6 +64,7 @@ SIGNATURES = {
     "vemd_mesh_destroy": [c_vp],
     "vemd_solver_create": [c_vp, ctypes.POINTER(ProblemDesc), ctypes.POINTER(c_vp)],
     "vemd_solver_create_device": [c_vp, ctypes.POINTER(ProblemDesc), ctypes.c_int, ctypes.POINTER(c_vp)],
+    "vemd_solver_tuning": [c_vp, c_ip],
     "vemd_solver_sizes": [c_vp, c_lp],
     "vemd_solver_upload": [c_vp, ctypes.c_int],
     "vemd_set_theta": [c_vp, c_dp],

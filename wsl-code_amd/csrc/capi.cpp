@@ -288,6 +288,13 @@ vemd_status vemd_solver_upload(vemd_solver sh, int device) {
     API_END
 }
 
+vemd_status vemd_solver_tuning(vemd_solver sh, int* out) {
+    API_BEGIN
+    REQUIRE(sh && out, "null argument");
+    dev_tuning(*sh->s, out);
+    API_END
+}
+
 vemd_status vemd_set_theta(vemd_solver sh, const double* theta) {
     API_BEGIN
     REQUIRE(sh && theta, "null argument");

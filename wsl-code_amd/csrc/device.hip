@@ -60,6 +60,8 @@ struct ApplyArgs {
     T* y1;               // side-1 contributions of edge dofs (cbase values)
     int64_t cbase, cpad, lam_idx, nvec;
     int ntiles, per_xcd;
+    int tile_base;       // first tile of the launch (the tuner launches one group's tiles)
+    int force_var;       // >= 0: every group runs this variant (the tuner's trials)
     double* pdot;        // per-tile partial of x.Kx (nullptr: not needed)
     double* plam;        // per-tile partial of the multiplier row
     const MState* st;
@@ -149,12 +151,28 @@ struct CellShape {
     static constexpr int NV = NAV + NWV;  // 16-byte vectors per cell: packed A, then W
 };
 
-#ifndef VEMD_CHUNK
-#define VEMD_CHUNK 8
-#endif
-#ifndef VEMD_PIPE
-#define VEMD_PIPE 1
-#endif
+// Apply variants: the operator stream of a cell is consumed in chunks of CH
+// 16-byte vectors, with (PIPE) or without the next chunk in flight while one
+// is used. The autotuner picks one per valence.
+template <int V>
+struct Variant;
+template <>
+struct Variant<0> {
+    static constexpr int CH = 8, PIPE = 1;
+};
+template <>
+struct Variant<1> {
+    static constexpr int CH = 4, PIPE = 1;
+};
+template <>
+struct Variant<2> {
+    static constexpr int CH = 2, PIPE = 1;
+};
+template <>
+struct Variant<3> {
+    static constexpr int CH = 8, PIPE = 0;
+};
+constexpr int NVARIANT = 4;
 
 // One cell per lane: gather signed-folded local dofs, y_E = K_E x_E, store the
 // edge-dof results to the slot of the cell's side of each edge.  The cell's
@@ -164,14 +182,14 @@ struct CellShape {
 // consumed.  y_u = s (A u - W^T (p / s)) so that A and W entries can be
 // consumed in stream order; the exact part of W (row 0 from the cell's mask,
 // the -1 entries of the internal columns) is applied after the stream.
-template <typename T, int K, int NE>
+template <typename T, int K, int NE, int CH, int PIPE>
 __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __restrict__ x, const GroupDesc& g, int tau,
                                           int l, double& dot, double& lamp) {
     using SH = CellShape<T, K, NE>;
     using V = typename VT<T>::V;
     constexpr int VW = SH::VW, Q = SH::Q, NI = SH::NI, NED = SH::NED, N = SH::N, NA = SH::NA, NAV = SH::NAV,
                   NW = SH::NW, NV = SH::NV;
-    constexpr int CH = VEMD_CHUNK, NC = (NV + CH - 1) / CH;
+    constexpr int NC = (NV + CH - 1) / CH;
     constexpr uint32_t TS = sizeof(T);
     const int lane = tau * 64 + l;
     const uint32_t gpos = (uint32_t)(g.gpos0 + lane);
@@ -249,7 +267,7 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
             std::make_integer_sequence<int, CH>{});
     };
     V b0[CH], b1[CH];
-#if VEMD_PIPE
+    if constexpr (PIPE) {
     load_chunk(std::integral_constant<int, 0>{}, b0);
     static_for(
         [&](auto cc) {
@@ -270,7 +288,7 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
             __builtin_amdgcn_sched_barrier(0);
         },
         std::make_integer_sequence<int, NC>{});
-#else
+    } else {
     (void)b1;
     static_for(
         [&](auto cc) {
@@ -281,7 +299,7 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
             __builtin_amdgcn_sched_barrier(0);
         },
         std::make_integer_sequence<int, NC>{});
-#endif
+    }
     // exact part of W: row 0 (+-1 or 0 at the first moment of each edge) and
     // W(r + 1, NED + r) = -1
 #pragma unroll
@@ -335,6 +353,18 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// The body of one valence in the variant var (wave-uniform).
+template <typename T, int K, int NE>
+__device__ __forceinline__ void cell_variant(int var, const ApplyArgs<T>& a, const T* __restrict__ x,
+                                             const GroupDesc& g, int tau, int l, double& dot, double& lamp) {
+    switch (var) {
+        case 1: cell_body<T, K, NE, Variant<1>::CH, Variant<1>::PIPE>(a, x, g, tau, l, dot, lamp); break;
+        case 2: cell_body<T, K, NE, Variant<2>::CH, Variant<2>::PIPE>(a, x, g, tau, l, dot, lamp); break;
+        case 3: cell_body<T, K, NE, Variant<3>::CH, Variant<3>::PIPE>(a, x, g, tau, l, dot, lamp); break;
+        default: cell_body<T, K, NE, Variant<0>::CH, Variant<0>::PIPE>(a, x, g, tau, l, dot, lamp); break;
+    }
+}
+
 // One launch for all cells: each wave takes one tile of 64 cells of one
 // valence group and runs the body specialised for that polygon size (a
 // wave-uniform branch).  Consecutive tiles go to one XCD so that neighbouring
@@ -343,8 +373,9 @@ template <typename T, int K>
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs<T> a) {
     const int b = blockIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int tile = ((b & 7) * a.per_xcd + (b >> 3)) * 4 + wave;
-    if (tile >= a.ntiles) return;
+    const int tile_r = ((b & 7) * a.per_xcd + (b >> 3)) * 4 + wave;
+    if (tile_r >= a.ntiles) return;
+    const int tile = a.tile_base + tile_r;
     const GroupDesc g = a.groups[__builtin_amdgcn_readfirstlane(a.tile_group[tile])];
     const MState* st = a.st ? a.st + g.inst : nullptr;  // the Krylov state of the tile's instance
     if (st && st->done) return;
@@ -352,15 +383,16 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs<T> a) {
     const int tau = tile - g.tile0;
     const T* x = st ? a.xs[st->itn & 1] : a.x;
     double dot = 0.0, lamp = 0.0;
+    const int var = a.force_var >= 0 ? a.force_var : g.var;
     switch (g.ne) {
-        case 3: cell_body<T, K, 3>(a, x, g, tau, l, dot, lamp); break;
-        case 4: cell_body<T, K, 4>(a, x, g, tau, l, dot, lamp); break;
-        case 5: cell_body<T, K, 5>(a, x, g, tau, l, dot, lamp); break;
-        case 6: cell_body<T, K, 6>(a, x, g, tau, l, dot, lamp); break;
-        case 7: cell_body<T, K, 7>(a, x, g, tau, l, dot, lamp); break;
-        case 8: cell_body<T, K, 8>(a, x, g, tau, l, dot, lamp); break;
-        case 9: cell_body<T, K, 9>(a, x, g, tau, l, dot, lamp); break;
-        case 10: cell_body<T, K, 10>(a, x, g, tau, l, dot, lamp); break;
+        case 3: cell_variant<T, K, 3>(var, a, x, g, tau, l, dot, lamp); break;
+        case 4: cell_variant<T, K, 4>(var, a, x, g, tau, l, dot, lamp); break;
+        case 5: cell_variant<T, K, 5>(var, a, x, g, tau, l, dot, lamp); break;
+        case 6: cell_variant<T, K, 6>(var, a, x, g, tau, l, dot, lamp); break;
+        case 7: cell_variant<T, K, 7>(var, a, x, g, tau, l, dot, lamp); break;
+        case 8: cell_variant<T, K, 8>(var, a, x, g, tau, l, dot, lamp); break;
+        case 9: cell_variant<T, K, 9>(var, a, x, g, tau, l, dot, lamp); break;
+        case 10: cell_variant<T, K, 10>(var, a, x, g, tau, l, dot, lamp); break;
         default: break;
     }
     if (a.pdot || a.lam_idx >= 0) {
@@ -864,6 +896,9 @@ static void upload_theta(Solver& s) {
     }
 }
 
+template <typename T>
+static void autotune_apply(Solver& s);
+
 void dev_upload(Solver& s, int device) {
     if (device_count() <= device || device < 0)
         throw Error(VEMD_ERR_NO_DEVICE, "no HIP device " + std::to_string(device) + " (the solve path has no CPU fallback)");
@@ -933,6 +968,8 @@ void dev_upload(Solver& s, int device) {
     HIPCHK(hipEventCreate(&d.e0));
     HIPCHK(hipEventCreate(&d.e1));
     upload_theta(s);
+    autotune_apply<double>(s);
+    autotune_apply<float>(s);
 }
 
 void dev_refresh(Solver& s) {
@@ -1012,6 +1049,8 @@ static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, T* y1, const MState* 
     a.st = st;
     a.ntiles = d.ntiles;
     a.per_xcd = (int)(((d.ntiles + 3) / 4 + 7) / 8);
+    a.tile_base = 0;
+    a.force_var = -1;
     return a;
 }
 
@@ -1023,6 +1062,86 @@ static void launch_apply(Solver& s, const ApplyArgs<T>& a) {
         hipLaunchKernelGGL((k_apply<T, 0>), grid, block, 0, s.dev->stream, a);
     else
         hipLaunchKernelGGL((k_apply<T, 1>), grid, block, 0, s.dev->stream, a);
+}
+
+// Autotuner of the apply: for every valence, the variants of the cell body
+// (operator-stream chunk shapes) are timed on the valence's largest group,
+// launched alone, and the fastest is recorded in every group of that valence.
+// VEMD_AUTOTUNE=0 keeps variant 0 everywhere.
+template <typename T>
+static void autotune_apply(Solver& s) {
+    DeviceState& d = *s.dev;
+    Layout& L = s.layout;
+    std::vector<GroupDesc>& G = sizeof(T) == 8 ? L.groups : L.groups32;
+    int* tuned = sizeof(T) == 8 ? d.tuned64 : d.tuned32;
+    float* tuned_ms = sizeof(T) == 8 ? d.tuned_ms64 : d.tuned_ms32;
+    for (int i = 0; i < 16; ++i) {
+        tuned[i] = -1;
+        tuned_ms[i] = 0.0f;
+    }
+    const char* env = std::getenv("VEMD_AUTOTUNE");
+    const bool on = !(env && env[0] == '0');
+    std::vector<int> rep(16, -1);  // largest group of each valence
+    for (size_t gi = 0; gi < G.size(); ++gi) {
+        const int ne = G[gi].ne;
+        if (ne < 0 || ne >= 16) continue;
+        if (rep[ne] < 0 || G[gi].npad > G[rep[ne]].npad) rep[ne] = (int)gi;
+    }
+    T* x = sizeof(T) == 8 ? (T*)d.x64 : (T*)d.r32;
+    T* y = sizeof(T) == 8 ? (T*)d.y64 : (T*)d.k32.Az;
+    T* y1 = sizeof(T) == 8 ? (T*)d.y1_64 : (T*)d.y1_32;
+    HIPCHK(hipMemsetAsync(x, 0, L.nvec * sizeof(T), d.stream));
+    for (int ne = 0; ne < 16; ++ne) {
+        if (rep[ne] < 0) continue;
+        const GroupDesc& g = G[rep[ne]];
+        const int nt = g.npad / TILE;
+        int best = 0;
+        float best_ms = 0.0f;
+        if (on && nt >= 8) {
+            for (int v = 0; v < NVARIANT; ++v) {
+                ApplyArgs<T> a = make_args<T>(s, x, y, y1, nullptr, nullptr, nullptr, false);
+                a.force_var = v;
+                a.tile_base = g.tile0;
+                a.ntiles = nt;
+                a.per_xcd = (nt + 31) / 32;
+                launch_apply(s, a);  // warm
+                HIPCHK(hipEventRecord(d.e0, d.stream));
+                constexpr int R = 5;
+                for (int r = 0; r < R; ++r) launch_apply(s, a);
+                HIPCHK(hipEventRecord(d.e1, d.stream));
+                HIPCHK(hipEventSynchronize(d.e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, d.e0, d.e1));
+                ms /= R;
+                if (v == 0 || ms < best_ms) {
+                    best = v;
+                    best_ms = ms;
+                }
+            }
+        }
+        tuned[ne] = best;
+        tuned_ms[ne] = best_ms;
+        for (GroupDesc& gg : G)
+            if (gg.ne == ne) gg.var = best;
+    }
+    HIPCHK(hipGetLastError());
+    GroupDesc* dg = sizeof(T) == 8 ? d.o64.groups : d.o32.groups;
+    HIPCHK(hipMemcpy(dg, G.data(), G.size() * sizeof(GroupDesc), hipMemcpyHostToDevice));
+    if (std::getenv("VEMD_VERBOSE")) {
+        std::fprintf(stderr, "[vemd] apply autotune (%s): valence:variant(ms)", sizeof(T) == 8 ? "fp64" : "fp32");
+        for (int ne = 0; ne < 16; ++ne)
+            if (tuned[ne] >= 0) std::fprintf(stderr, " %d:%d(%.4f)", ne, tuned[ne], tuned_ms[ne]);
+        std::fprintf(stderr, "\n");
+    }
+}
+
+int dev_tuning(const Solver& s, int* out) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    for (int ne = 3; ne <= 10; ++ne) {
+        out[2 * (ne - 3)] = s.dev->tuned64[ne];
+        out[2 * (ne - 3) + 1] = s.dev->tuned32[ne];
+    }
+    return 16;
 }
 
 // y = K x in the internal layout: cell launch, second sides of the edges, and

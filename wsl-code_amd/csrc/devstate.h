@@ -42,6 +42,9 @@ struct DevKrylov {
 struct DeviceState {
     int device = 0;
     hipStream_t stream = nullptr;
+    // apply variant chosen per valence (index ne, -1: no cells of that valence)
+    int tuned64[16], tuned32[16];
+    float tuned_ms64[16], tuned_ms32[16];  // the chosen variant's time on the valence's largest group
     DevOps<double> o64;
     DevOps<float> o32;
     uint32_t* idx = nullptr;

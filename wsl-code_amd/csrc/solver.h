@@ -43,6 +43,8 @@ int device_count();
 void dev_upload(Solver& s, int device);
 void dev_refresh(Solver& s);  // cell scales and preconditioner after a theta change
 void dev_free(Solver& s);
+// Apply variant chosen by the autotuner for valences 3..10 (FP64, FP32 pairs; -1: none).
+int dev_tuning(const Solver& s, int* out16);
 void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref);
 // y1 == nullptr: complete apply; otherwise y1 receives the edges' second sides.
 void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* y1, void* stream);

@@ -21,6 +21,7 @@ struct GroupDesc {
     int ne, n, npad;          // valence, local flux dofs, cells incl. padding
     int inst;                 // batch instance of the group's cells
     int gpos0, tile0;         // first store position, first global tile
+    int var;                  // apply variant (stream chunk shape) chosen by the autotuner
     int64_t o_off, l_off, i_off;  // element offsets of the operator stream, L and index blocks
 };
 

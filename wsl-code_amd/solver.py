@@ -56,6 +56,13 @@ class Solver:
         check(lib().vemd_solver_upload(self.h, int(device)))
         self.uploaded = True
 
+    def tuning(self):
+        """Apply variant picked by the autotuner per valence: {ne: (fp64, fp32)}."""
+        out = np.zeros(16, np.int32)
+        check(lib().vemd_solver_tuning(self.h, out.ctypes.data_as(c_ip)))
+        return {ne: (int(out[2 * (ne - 3)]), int(out[2 * (ne - 3) + 1])) for ne in range(3, 11)
+                if out[2 * (ne - 3)] >= 0}
+
     def set_theta(self, theta):
         self._theta = np.ascontiguousarray(theta, np.float64)
         check(lib().vemd_set_theta(self.h, _d(self._theta)))
