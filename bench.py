@@ -81,14 +81,15 @@ def time_applies(pkg, s, torch, precision, reps, split=False):
     g = torch.Generator(device="cuda").manual_seed(5)
     x = torch.randn(s.nvec, dtype=dt, device="cuda", generator=g)
     y = torch.empty_like(x)
-    y1 = torch.zeros_like(x)
+    nside, _ = s.edge_sides()
+    h = torch.zeros(nside * (s.k + 1), dtype=dt, device="cuda")
     st = torch.cuda.current_stream()
     L = pkg.lib()
-    px, py, py1, pst = (ctypes.c_void_p(t) for t in (x.data_ptr(), y.data_ptr(), y1.data_ptr(), st.cuda_stream))
+    px, py, ph, pst = (ctypes.c_void_p(t) for t in (x.data_ptr(), y.data_ptr(), h.data_ptr(), st.cuda_stream))
 
     def one():
         if split:
-            rc = L.vemd_apply_device_split(s.h, precision, px, py, py1, pst)
+            rc = L.vemd_apply_device_split(s.h, precision, px, py, ph, pst)
         else:
             rc = L.vemd_apply_device(s.h, precision, px, py, pst)
         if rc:

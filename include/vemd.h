@@ -227,15 +227,25 @@ vemd_status vemd_apply(vemd_solver s, int precision, const double* x, double* y)
 vemd_status vemd_apply_device(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* stream);
 
 /* The same apply without its final edge sum, for consumers that fuse it: y_dev
- * receives the first-side value of every edge dof and all cell-owned and
- * multiplier rows, y1_dev (sizes[3] values, of which only the leading edge-dof
- * block is used) the second-side values, so that K x = y + y1 on that block.
- * Edge dofs with one cell have no second side and are never written in y1_dev:
- * zero it once before the first call.  This is the form the solver's own
- * iteration uses (a single kernel launch plus, with the multiplier, one
- * reduction).  No reference counterpart. */
-vemd_status vemd_apply_device_split(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* y1_dev,
+ * receives all cell-owned and multiplier rows (its leading edge-dof block is
+ * not written), h_dev (nside * (k + 1) values, nside from vemd_edge_sides)
+ * the edge results of every cell side: side slot p holds values
+ * [p * (k + 1), (p + 1) * (k + 1)).  K x at edge dof (k + 1) * e + r (internal
+ * edge e) is h[t[2e] * (k + 1) + r] + h[t[2e + 1] * (k + 1) + r] with t the
+ * table of vemd_edge_sides.  The last slot (second side of boundary edges) is
+ * never written: zero h_dev once before the first call.  This is the form the
+ * solver's own iteration uses (one kernel launch, whose stores cover whole
+ * cache lines, plus, with the multiplier, one reduction).  No reference
+ * counterpart. */
+vemd_status vemd_apply_device_split(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* h_dev,
                                     void* stream);
+
+/* The cell sides of the edges for vemd_apply_device_split: *nside receives
+ * the number of side slots, and table (if not NULL; 2 * number of mesh edges
+ * int32 values) the slots of side 0 and side 1 of every internal edge (edge e
+ * owns internal dofs (k + 1) * e .. (k + 1) * e + k).  A boundary edge's
+ * side 1 is the zero slot nside - 1.  No reference counterpart. */
+vemd_status vemd_edge_sides(vemd_solver s, int64_t* nside, int32_t* table);
 
 /* Simultaneous solves with 1..32 right-hand sides: b_dev and x_dev hold nrhs
  * columns of sizes[3] doubles each (column-major, internal layout, as

@@ -157,23 +157,41 @@ def test_multigrid_needs_dirichlet(vemd, oracle):
 
 @pytest.mark.parametrize("case", GEN[1:], ids=GEN_IDS[1:])
 def test_split_apply_sums_to_apply(vemd, oracle, case):
-    """The solver's one-launch form: first sides plus second sides of the edge
-    dofs equal the complete apply, bitwise (same single addition)."""
+    """The solver's one-launch form: the two cell sides of every edge dof,
+    gathered through the side table, sum to the complete apply bitwise (same
+    single addition); every cell side of an edge appears in exactly one slot."""
     import torch
     s, _ = _gen_pair(vemd, oracle, *case)
     L = vemd.lib()
     import ctypes
+    nside, t = s.edge_sides()
+    nedge, kk = s.mesh.num_edges, s.k + 1
+    # side 0 of every edge is a real slot, slots are distinct, the zero slot is last
+    assert t.min() >= 0 and t.max() == nside - 1 and (t[0::2] < nside - 1).all()
+    real = t[t < nside - 1]
+    assert len(np.unique(real)) == len(real)
+    tt = torch.from_numpy(t.astype(np.int64)).cuda()
     for prec, dt in ((vemd.FP64, torch.float64), (vemd.FP32, torch.float32)):
         g = torch.Generator(device="cuda").manual_seed(3)
         x = torch.randn(s.nvec, dtype=dt, device="cuda", generator=g)
-        y, y0, y1 = torch.empty_like(x), torch.empty_like(x), torch.zeros_like(x)
-        p = [ctypes.c_void_p(t.data_ptr()) for t in (x, y, y0, y1)]
+        y, y0 = torch.empty_like(x), torch.full_like(x, 7.0)
+        h = torch.zeros(nside * kk, dtype=dt, device="cuda")
+        p = [ctypes.c_void_p(v.data_ptr()) for v in (x, y, y0, h)]
         assert L.vemd_apply_device(s.h, prec, p[0], p[1], None) == 0
         assert L.vemd_apply_device_split(s.h, prec, p[0], p[2], p[3], None) == 0
         torch.cuda.synchronize()
-        ne = (s.k + 1) * s.mesh.num_edges
-        assert torch.equal(y[:ne], y0[:ne] + y1[:ne])
-        assert torch.equal(y[ne:], y0[ne:])
+        hv = h.view(nside, kk)
+        edge = (hv[tt[0::2]] + hv[tt[1::2]]).reshape(-1)
+        ne = kk * nedge
+        assert torch.equal(y[:ne], edge)
+        assert torch.equal(y0[:ne], torch.full_like(y0[:ne], 7.0))  # edge block untouched
+        # cell-owned and multiplier rows; the padding lanes' slot of k + 1
+        # entries before the multiplier is no row of K
+        tr = s.nvec - kk - (1 if s.bc == vemd.BC_FLUX_MEANZERO else 0)
+        keep = torch.ones(s.nvec, dtype=torch.bool, device="cuda")
+        keep[:ne] = False
+        keep[tr:tr + kk] = False
+        assert torch.equal(y[keep], y0[keep])
 
 
 GOLD = [c for c in json.load(open(golden("golden_errors.json")))["cases"] if c["k"] <= 1]

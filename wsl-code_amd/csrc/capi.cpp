@@ -358,11 +358,20 @@ vemd_status vemd_apply_device(vemd_solver sh, int precision, const void* x, void
     API_END
 }
 
-vemd_status vemd_apply_device_split(vemd_solver sh, int precision, const void* x, void* y, void* y1, void* stream) {
+vemd_status vemd_apply_device_split(vemd_solver sh, int precision, const void* x, void* y, void* h, void* stream) {
     API_BEGIN
-    REQUIRE(sh && x && y && y1 && x != y && x != y1 && y != y1, "null or aliased argument");
+    REQUIRE(sh && x && y && h && x != y && x != h && y != h, "null or aliased argument");
     REQUIRE(precision == VEMD_FP64 || precision == VEMD_FP32, "precision must be VEMD_FP64 or VEMD_FP32");
-    dev_apply_int(*sh->s, precision, x, y, y1, stream);
+    dev_apply_int(*sh->s, precision, x, y, h, stream);
+    API_END
+}
+
+vemd_status vemd_edge_sides(vemd_solver sh, int64_t* nside, int32_t* table) {
+    API_BEGIN
+    REQUIRE(sh && nside, "null argument");
+    const Layout& L = sh->s->layout;
+    *nside = L.nside;
+    if (table) std::copy(L.hside.begin(), L.hside.end(), table);
     API_END
 }
 

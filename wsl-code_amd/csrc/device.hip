@@ -56,8 +56,8 @@ struct ApplyArgs {
     const uint32_t* wmask;  // row 0 of W per store position (bit j: -1, bit 16 + j: 0)
     const T* x;          // input when st == nullptr
     const T* xs[2];      // inputs selected by the Krylov iteration parity when st != nullptr
-    T* y;
-    T* y1;               // side-1 contributions of edge dofs (cbase values)
+    T* y;                // cell-owned rows (internal dofs and pressure moments)
+    T* h;                // edge results per cell side: k + 1 values per entry of idx
     int64_t cbase, cpad, lam_idx, nvec;
     int ntiles, per_xcd;
     int tile_base;       // first tile of the launch (the tuner launches one group's tiles)
@@ -191,15 +191,16 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
                   NW = SH::NW, NV = SH::NV;
     constexpr int NC = (NV + CH - 1) / CH;
     constexpr uint32_t TS = sizeof(T);
+    constexpr int KK = SH::KK;
     const int lane = tau * 64 + l;
     const uint32_t gpos = (uint32_t)(g.gpos0 + lane);
     const Rsrc rx = make_rsrc(x, a.nvec * (int64_t)TS);
     const Rsrc ry = make_rsrc(a.y, a.nvec * (int64_t)TS);
-    const Rsrc ry1 = make_rsrc(a.y1, a.cbase * (int64_t)TS);
+    // the tile's side slots are contiguous: edge j of lane l at (j * 64 + l)
+    const Rsrc rh = make_rsrc(a.h + (g.i_off + (int64_t)tau * NE * 64) * KK, (int64_t)NE * 64 * KK * TS);
     const Rsrc ri = make_rsrc(a.idx + g.i_off + (int64_t)tau * NE * 64, (int64_t)NE * 64 * 4);
     const Rsrc rO = make_rsrc(a.O + g.o_off + (int64_t)tau * NV * 64 * VW, (int64_t)NV * 64 * 16);
     const uint32_t lv = (uint32_t)l * 16u;
-    constexpr int KK = SH::KK;
     uint32_t id[NE];
     T u[N], ps[Q], acc[N], yp[Q];
 #pragma unroll
@@ -330,16 +331,14 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
 #pragma unroll
         for (int r = 0; r < Q; ++r) dot += (double)p[r] * (double)yp[r];
     }
+    // edge results to the cell's own side slots: each store covers 64
+    // consecutive slots, so every written line is written whole
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
-        const uint32_t off = (id[j] & 0x7fffffffu) * TS;
         T v[KK];
 #pragma unroll
         for (int r = 0; r < KK; ++r) v[r] = acc[r * NE + j];
-        if (id[j] & SIDE1)
-            store_edge<T, KK>(ry1, off, v);
-        else
-            store_edge<T, KK>(ry, off, v);
+        store_edge<T, KK>(rh, (uint32_t)((j * 64 + l) * KK) * TS, v);
     }
 #pragma unroll
     for (int j = 0; j < NI; ++j) bstore1(acc[NED + j], ry, gpos * TS, (uint32_t)((a.cbase + j * a.cpad) * TS));
@@ -405,10 +404,22 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs<T> a) {
     }
 }
 
-// y[i] += y1[i] over the edge dofs: the second side of every edge.
+// Entry i of the edge-dof block: the sum of the edge's two cell sides (a
+// boundary edge's second slot is the zero slot).
 template <typename T>
-__global__ __launch_bounds__(256) void k_combine(T* __restrict__ y, const T* __restrict__ y1, int64_t n) {
-    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] += y1[i];
+__device__ __forceinline__ T side_sum(const T* __restrict__ h, const int* __restrict__ hside, int kk, int64_t i) {
+    const int64_t e = kk == 2 ? i >> 1 : i;
+    const int r = (int)(i - e * kk);
+    return h[(int64_t)hside[2 * e] * kk + r] + h[(int64_t)hside[2 * e + 1] * kk + r];
+}
+
+// y[i] = side sum over the edge dofs (i < n); the padding lanes' slot of the
+// vector (k + 1 entries at trash) is zeroed.
+template <typename T>
+__global__ __launch_bounds__(256) void k_combine(T* __restrict__ y, const T* __restrict__ h, const int* __restrict__ hside,
+                                                 int kk, int64_t n, int64_t trash) {
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] = side_sum(h, hside, kk, i);
+    if (blockIdx.x == 0 && (int)threadIdx.x < kk) y[trash + threadIdx.x] = T(0);
 }
 
 // Sums n partials in a fixed order with one block of 256 threads.
@@ -493,8 +504,10 @@ struct UpdArgs {
     T* Z[2];
     T* WB[2];
     T* x;
-    const T* Az;
-    const T* Az1;        // side-1 half of the edge dofs of Az
+    const T* Az;         // cell-owned and multiplier rows of the apply
+    const T* H;          // cell sides of the edge dofs
+    const int* hside;
+    int kk;
     const T* minv;
     int64_t edof;
     const VecChunk* chunks;
@@ -558,8 +571,7 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
     double d1 = 0.0, d2 = 0.0;
     for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
         T r2i = r2[i];
-        T az = a.Az[i];
-        if (i < a.edof) az += a.Az1[i];
+        const T az = i < a.edof ? side_sum(a.H, a.hside, a.kk, i) : a.Az[i];
         T y = caz * az - cr1 * r1[i] - cr2 * r2i;
         if (pend) {
             T w = uv * zo[i] - uw1 * wn[i] - uw2 * wp[i];
@@ -724,16 +736,13 @@ __global__ __launch_bounds__(256) void k_flush(UpdArgs<T> a) {
     }
 }
 
-// r = b - y over one chunk with the partial of |r|^2 (FP64); y's edge dofs
-// are the sum of both sides.
-__global__ __launch_bounds__(256) void k_residual(const VecChunk* chunks, const double* b, const double* y,
-                                                  const double* y1, int64_t edof, double* r, double* p) {
+// r = b - y over one chunk with the partial of |r|^2 (FP64).
+__global__ __launch_bounds__(256) void k_residual(const VecChunk* chunks, const double* b, const double* y, double* r,
+                                                  double* p) {
     const VecChunk ch = chunks[blockIdx.x];
     double d = 0.0;
     for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
-        double yi = y[i];
-        if (i < edof) yi += y1[i];
-        double v = b[i] - yi;
+        double v = b[i] - y[i];
         r[i] = v;
         d += v * v;
     }
@@ -843,7 +852,7 @@ static void upload_ops(DeviceState& d, const Layout& L, DevOps<T>& o, const std:
 }
 
 template <typename T>
-static void alloc_krylov(DeviceState& d, DevKrylov<T>& k, int64_t n, int64_t edof) {
+static void alloc_krylov(DeviceState& d, DevKrylov<T>& k, int64_t n, int64_t nh) {
     for (int i = 0; i < 2; ++i) {
         k.R[i] = (T*)dalloc(d, n * sizeof(T));
         k.Z[i] = (T*)dalloc(d, n * sizeof(T));
@@ -854,10 +863,10 @@ static void alloc_krylov(DeviceState& d, DevKrylov<T>& k, int64_t n, int64_t edo
     }
     k.x = (T*)dalloc(d, n * sizeof(T));
     k.Az = (T*)dalloc(d, n * sizeof(T));
-    k.Az1 = (T*)dalloc(d, edof * sizeof(T));
+    k.H = (T*)dalloc(d, nh * sizeof(T));
     k.b = (T*)dalloc(d, n * sizeof(T));
-    // side-1 slots of boundary edges are never written and stay zero
-    HIPCHK(hipMemset(k.Az1, 0, std::max<int64_t>(edof, 1) * sizeof(T)));
+    // the zero slot (second side of boundary edges) is never written
+    HIPCHK(hipMemset(k.H, 0, nh * sizeof(T)));
     HIPCHK(hipMemset(k.x, 0, n * sizeof(T)));
     HIPCHK(hipMemset(k.Az, 0, n * sizeof(T)));
     HIPCHK(hipMemset(k.b, 0, n * sizeof(T)));
@@ -920,6 +929,7 @@ void dev_upload(Solver& s, int device) {
         d.wmask = dupload(d, wmask);
     }
     d.idx = dupload(d, L.idx);
+    d.hside = dupload(d, L.hside);
     d.tile_group = dupload(d, L.tile_group);
     d.cell_of_gpos = dupload(d, L.cell_of_gpos);
     d.ntiles = (int)L.tile_group.size();
@@ -951,12 +961,13 @@ void dev_upload(Solver& s, int device) {
     HIPCHK(hipHostMalloc((void**)&d.rtol_host, d.ninst * sizeof(double), hipHostMallocDefault));
     d.dsum = (double*)dalloc(d, d.ninst * sizeof(double));
     HIPCHK(hipHostMalloc((void**)&d.hsum, d.ninst * sizeof(double), hipHostMallocDefault));
-    alloc_krylov(d, d.k64, L.nvec, L.cbase);
-    alloc_krylov(d, d.k32, L.nvec, L.cbase);
-    d.y1_64 = (double*)dalloc(d, L.cbase * sizeof(double));
-    d.y1_32 = (float*)dalloc(d, L.cbase * sizeof(float));
-    HIPCHK(hipMemset(d.y1_64, 0, std::max<int64_t>(L.cbase, 1) * sizeof(double)));
-    HIPCHK(hipMemset(d.y1_32, 0, std::max<int64_t>(L.cbase, 1) * sizeof(float)));
+    const int64_t nh = L.nside * (L.k + 1);
+    alloc_krylov(d, d.k64, L.nvec, nh);
+    alloc_krylov(d, d.k32, L.nvec, nh);
+    d.h64 = (double*)dalloc(d, nh * sizeof(double));
+    d.h32 = (float*)dalloc(d, nh * sizeof(float));
+    HIPCHK(hipMemset(d.h64, 0, nh * sizeof(double)));
+    HIPCHK(hipMemset(d.h32, 0, nh * sizeof(float)));
     d.x64 = (double*)dalloc(d, L.nvec * sizeof(double));
     d.r64 = (double*)dalloc(d, L.nvec * sizeof(double));
     d.y64 = (double*)dalloc(d, L.nvec * sizeof(double));
@@ -1011,18 +1022,18 @@ const DevOps<float>& ops<float>(DeviceState& d) {
 }
 
 template <typename T>
-static T* side1_scratch(DeviceState& d);
+static T* side_scratch(DeviceState& d);
 template <>
-double* side1_scratch<double>(DeviceState& d) {
-    return d.y1_64;
+double* side_scratch<double>(DeviceState& d) {
+    return d.h64;
 }
 template <>
-float* side1_scratch<float>(DeviceState& d) {
-    return d.y1_32;
+float* side_scratch<float>(DeviceState& d) {
+    return d.h32;
 }
 
 template <typename T>
-static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, T* y1, const MState* st, const T* x0, const T* x1,
+static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, T* h, const MState* st, const T* x0, const T* x1,
                               bool dot) {
     DeviceState& d = *s.dev;
     const DevOps<T>& o = ops<T>(d);
@@ -1039,7 +1050,7 @@ static ApplyArgs<T> make_args(Solver& s, const T* x, T* y, T* y1, const MState* 
     a.xs[0] = x0;
     a.xs[1] = x1;
     a.y = y;
-    a.y1 = y1;
+    a.h = h;
     a.cbase = L.cbase;
     a.cpad = L.cpad;
     a.lam_idx = L.lam ? L.lam_idx : -1;
@@ -1089,7 +1100,7 @@ static void autotune_apply(Solver& s) {
     }
     T* x = sizeof(T) == 8 ? (T*)d.x64 : (T*)d.r32;
     T* y = sizeof(T) == 8 ? (T*)d.y64 : (T*)d.k32.Az;
-    T* y1 = sizeof(T) == 8 ? (T*)d.y1_64 : (T*)d.y1_32;
+    T* h = side_scratch<T>(d);
     HIPCHK(hipMemsetAsync(x, 0, L.nvec * sizeof(T), d.stream));
     for (int ne = 0; ne < 16; ++ne) {
         if (rep[ne] < 0) continue;
@@ -1099,7 +1110,7 @@ static void autotune_apply(Solver& s) {
         float best_ms = 0.0f;
         if (on && nt >= 8) {
             for (int v = 0; v < NVARIANT; ++v) {
-                ApplyArgs<T> a = make_args<T>(s, x, y, y1, nullptr, nullptr, nullptr, false);
+                ApplyArgs<T> a = make_args<T>(s, x, y, h, nullptr, nullptr, nullptr, false);
                 a.force_var = v;
                 a.tile_base = g.tile0;
                 a.ntiles = nt;
@@ -1144,22 +1155,25 @@ int dev_tuning(const Solver& s, int* out) {
     return 16;
 }
 
-// y = K x in the internal layout: cell launch, second sides of the edges, and
-// the multiplier row.
+// y = K x in the internal layout: cell launch, the edge sums of the two cell
+// sides, and the multiplier row.  With h_out the edge sums are left to the
+// caller: h_out receives the cell sides and y's edge-dof block is untouched.
 template <typename T>
-static void apply_plain(Solver& s, const T* x, T* y, T* y1_out = nullptr) {
+static void apply_plain(Solver& s, const T* x, T* y, T* h_out = nullptr) {
     DeviceState& d = *s.dev;
-    T* y1 = y1_out ? y1_out : side1_scratch<T>(d);
-    ApplyArgs<T> a = make_args<T>(s, x, y, y1, nullptr, nullptr, nullptr, false);
+    const Layout& L = s.layout;
+    T* h = h_out ? h_out : side_scratch<T>(d);
+    ApplyArgs<T> a = make_args<T>(s, x, y, h, nullptr, nullptr, nullptr, false);
     launch_apply(s, a);
-    if (!y1_out)
-        hipLaunchKernelGGL((k_combine<T>), dim3(d.nblk), dim3(256), 0, d.stream, y, (const T*)y1, s.layout.cbase);
+    if (!h_out)
+        hipLaunchKernelGGL((k_combine<T>), dim3(d.nblk), dim3(256), 0, d.stream, y, (const T*)h, (const int*)d.hside,
+                           L.k + 1, L.cbase, L.trash_idx);
     if (s.layout.lam)
         hipLaunchKernelGGL((k_lam_row<T>), dim3(1), dim3(256), 0, d.stream, d.plam, d.ntiles, y, s.layout.lam_idx);
     HIPCHK(hipGetLastError());
 }
 
-void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* y1, void* stream) {
+void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* h, void* stream) {
     if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
     HIPCHK(hipSetDevice(s.dev->device));
     // the caller's stream; NULL is the null (default) stream, as documented
@@ -1167,9 +1181,9 @@ void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* y1, v
     s.dev->stream = (hipStream_t)stream;
     try {
         if (precision == VEMD_FP64)
-            apply_plain<double>(s, (const double*)x, (double*)y, (double*)y1);
+            apply_plain<double>(s, (const double*)x, (double*)y, (double*)h);
         else
-            apply_plain<float>(s, (const float*)x, (float*)y, (float*)y1);
+            apply_plain<float>(s, (const float*)x, (float*)y, (float*)h);
     } catch (...) {
         s.dev->stream = saved;
         throw;
@@ -1217,8 +1231,8 @@ template <typename T>
 static UpdArgs<T> upd_args(Solver& s, DevKrylov<T>& k, bool amg) {
     DeviceState& d = *s.dev;
     const Layout& L = s.layout;
-    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x,  k.Az,      k.Az1, ops<T>(d).minv,
-                 L.cbase,          d.chunks,          d.p1,              d.p2, d.st, amg ? 1 : 0, 0, nullptr,
+    UpdArgs<T> u{{k.R[0], k.R[1]}, {k.Z[0], k.Z[1]}, {k.WB[0], k.WB[1]}, k.x,  k.Az,      k.H, d.hside, L.k + 1,
+                 ops<T>(d).minv,   L.cbase,          d.chunks,          d.p1,    d.p2, d.st, amg ? 1 : 0, 0, nullptr,
                  L.cpad,           L.q,               nullptr,           nullptr, T(0)};
     if (amg) {
         const AmgDev<T>& M = *amg_of<T>(d);
@@ -1239,7 +1253,7 @@ template <typename T>
 static void enqueue_iteration(Solver& s, DevKrylov<T>& k, bool amg) {
     DeviceState& d = *s.dev;
     const Layout& L = s.layout;
-    ApplyArgs<T> a = make_args<T>(s, nullptr, k.Az, k.Az1, d.st, k.Z[0], k.Z[1], true);
+    ApplyArgs<T> a = make_args<T>(s, nullptr, k.Az, k.H, d.st, k.Z[0], k.Z[1], true);
     launch_apply(s, a);
     hipLaunchKernelGGL((k_finA<T>), dim3(d.ninst), dim3(FIN), 0, d.stream, d.st, d.pdot, d.plam, d.inst_tile0, k.Az,
                        k.Z[0], k.Z[1], L.lam ? L.lam_idx : (int64_t)-1);
@@ -1350,10 +1364,9 @@ static int minres(Solver& s, DevKrylov<T>& k, const std::vector<double>& rtol, i
 // r64 = b64 - K x64; returns the 2-norm of every instance's block of r64.
 static std::vector<double> residual64(Solver& s) {
     DeviceState& d = *s.dev;
-    apply_plain<double>(s, d.x64, d.y64, d.y1_64);
+    apply_plain<double>(s, d.x64, d.y64);
     hipLaunchKernelGGL(k_residual, dim3(d.nchunks), dim3(256), 0, d.stream, (const VecChunk*)d.chunks,
-                       (const double*)d.b64, (const double*)d.y64, (const double*)d.y1_64, s.layout.cbase, d.r64,
-                       d.p1);
+                       (const double*)d.b64, (const double*)d.y64, d.r64, d.p1);
     hipLaunchKernelGGL(k_inst_sum, dim3(d.ninst), dim3(FIN), 0, d.stream, (const double*)d.p1,
                        (const int*)d.inst_chunk0, d.dsum);
     HIPCHK(hipMemcpyAsync(d.hsum, d.dsum, d.ninst * sizeof(double), hipMemcpyDeviceToHost, d.stream));

@@ -30,8 +30,8 @@ struct DevKrylov {
     T* Z[2] = {nullptr, nullptr};
     T* WB[2] = {nullptr, nullptr};
     T* x = nullptr;
-    T* Az = nullptr;
-    T* Az1 = nullptr;
+    T* Az = nullptr;  // cell-owned and multiplier rows of the apply
+    T* H = nullptr;   // edge results per cell side (summed in the Krylov sweep)
     T* b = nullptr;
     hipGraphExec_t graph = nullptr;   // check_every iterations
     hipGraphExec_t graph_tail = nullptr;  // a few iterations, launched near convergence
@@ -49,6 +49,7 @@ struct DeviceState {
     DevOps<float> o32;
     uint32_t* idx = nullptr;
     uint32_t* wmask = nullptr;
+    int* hside = nullptr;  // cell-side slots of every edge (two per edge)
     int* tile_group = nullptr;
     int* cell_of_gpos = nullptr;
     int ntiles = 0;
@@ -79,8 +80,8 @@ struct DeviceState {
     double* b64 = nullptr;
     double* bprob = nullptr;  // the problem's own right-hand side, internal order
     float* r32 = nullptr;
-    double* y1_64 = nullptr;  // side-1 scratch of the plain applies
-    float* y1_32 = nullptr;
+    double* h64 = nullptr;  // cell-side scratch of the plain applies
+    float* h32 = nullptr;
     AmgDev<double>* amg64 = nullptr;  // multigrid pressure preconditioner (when built)
     AmgDev<float>* amg32 = nullptr;
     double* q1 = nullptr;  // per-chunk partials of the pressure-block dots (zero outside it)

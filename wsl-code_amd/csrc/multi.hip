@@ -39,7 +39,7 @@ struct MultiState {
     int64_t cells_per_wave = 0;
     float *R[2] = {nullptr, nullptr}, *Z[2] = {nullptr, nullptr}, *WB[2] = {nullptr, nullptr};
     float *x = nullptr, *Az = nullptr, *Az1 = nullptr, *b = nullptr;  // [entry][NR]
-    double *x64 = nullptr, *r64 = nullptr, *y64 = nullptr, *y1_64 = nullptr;  // [NR][nvec]
+    double *x64 = nullptr, *r64 = nullptr, *y64 = nullptr;  // [NR][nvec]
     MState* st = nullptr;
     MState* st_host = nullptr;
     double* rtol = nullptr;
@@ -563,14 +563,11 @@ __global__ __launch_bounds__(256) void k_il_add_cm(double* x, const float* d, in
 }
 
 // r = b - y (y's edge dofs: both sides) and block partials of |r|^2.
-__global__ __launch_bounds__(256) void k_resid_col(const double* b, const double* y, const double* y1, int64_t edof,
-                                                   int64_t n, double* r, double* part) {
+__global__ __launch_bounds__(256) void k_resid_col(const double* b, const double* y, int64_t n, double* r, double* part) {
     __shared__ double sh[256];
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        double yi = y[i];
-        if (i < edof) yi += y1[i];
-        double v = b[i] - yi;
+        double v = b[i] - y[i];
         r[i] = v;
         s += v * v;
     }
@@ -679,8 +676,6 @@ static MultiState& multi_setup(Solver& s) {
     m.x64 = (double*)malloc_m(m, dv);
     m.r64 = (double*)malloc_m(m, dv);
     m.y64 = (double*)malloc_m(m, (size_t)L.nvec * sizeof(double));
-    m.y1_64 = (double*)malloc_m(m, (size_t)std::max<int64_t>(L.cbase, 1) * sizeof(double));
-    HIPCHK(hipMemset(m.y1_64, 0, (size_t)std::max<int64_t>(L.cbase, 1) * sizeof(double)));
     m.st = (MState*)malloc_m(m, NR * sizeof(MState));
     HIPCHK(hipMemset(m.st, 0, NR * sizeof(MState)));
     HIPCHK(hipHostMalloc((void**)&m.st_host, NR * sizeof(MState), hipHostMallocDefault));
@@ -829,9 +824,9 @@ static std::vector<double> residual_mr(Solver& s, MultiState& m, const double* b
     DeviceState& d = *s.dev;
     const int64_t n = s.layout.nvec;
     for (int j = 0; j < ncol; ++j) {
-        dev_apply_int(s, VEMD_FP64, m.x64 + (int64_t)j * n, m.y64, m.y1_64, d.stream);
-        hipLaunchKernelGGL(k_resid_col, dim3(d.nblk), dim3(256), 0, d.stream, b + (int64_t)j * n, (const double*)m.y64,
-                           (const double*)m.y1_64, s.layout.cbase, n, m.r64 + (int64_t)j * n, m.part);
+        dev_apply_int(s, VEMD_FP64, m.x64 + (int64_t)j * n, m.y64, nullptr, d.stream);
+        hipLaunchKernelGGL(k_resid_col, dim3(d.nblk), dim3(256), 0, d.stream, b + (int64_t)j * n, (const double*)m.y64, n,
+                           m.r64 + (int64_t)j * n, m.part);
         hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, d.stream, (const double*)m.part, d.nblk, m.dsum + j);
     }
     HIPCHK(hipMemcpyAsync(m.hsum, m.dsum, ncol * sizeof(double), hipMemcpyDeviceToHost, d.stream));

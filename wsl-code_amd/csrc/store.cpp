@@ -198,6 +198,9 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     L.l_len = loff;
     L.idx.assign(ioff, (uint32_t)L.trash_idx);
     if (L.nvec >= (int64_t)SIDE1) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit dof indices");
+    if (ioff + 1 >= (int64_t)INT32_MAX) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit side slots");
+    L.nside = ioff + 1;
+    L.hside.assign(2 * (size_t)m.ne, (int32_t)ioff);
     // one index per local edge: the edge's first moment (its k + 1 moments are
     // adjacent), flagged when the cell is the edge's second side
     for (auto& g : L.groups) {
@@ -207,11 +210,16 @@ Layout layout_build(const Mesh& m, const Setup& S) {
             for (int j = 0; j < g.ne; ++j) {
                 int e = m.cell_edges[m.off[c] + j];
                 uint32_t id = (uint32_t)(kk * L.edge_int[e]);
-                if (L.gpos_of_cell[c] != side0[e]) id |= SIDE1;
-                L.idx[g.i_off + ((int64_t)(a / TILE) * g.ne + j) * TILE + a % TILE] = id;
+                const bool second = L.gpos_of_cell[c] != side0[e];
+                if (second) id |= SIDE1;
+                const int64_t slot = g.i_off + ((int64_t)(a / TILE) * g.ne + j) * TILE + a % TILE;
+                L.idx[slot] = id;
+                L.hside[2 * (size_t)L.edge_int[e] + (second ? 1 : 0)] = (int32_t)slot;
             }
         }
     }
+    for (int e = 0; e < m.ne; ++e)
+        if (L.hside[2 * (size_t)e] == (int32_t)ioff) throw Error(VEMD_ERR_MESH, "edge without a first side");
     return L;
 }
 

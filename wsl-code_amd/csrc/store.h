@@ -38,7 +38,7 @@ struct Layout {
     int ngroups = 0, max_ne = 0;
     bool lam = false;
     int64_t nedge = 0, ncell = 0, cpad = 0;  // edges, real cells, padded cell slots
-    int64_t cbase = 0;        // first cell-owned slot (= edge dofs = length of the side-1 array)
+    int64_t cbase = 0;        // first cell-owned slot (= number of edge-dof entries)
     int64_t lam_idx = -1, trash_idx = 0, nvec = 0;
     std::vector<GroupDesc> groups;      // offsets of the FP64 store (vector width 2)
     std::vector<GroupDesc> groups32;    // the same groups with FP32 store offsets (width 4)
@@ -49,6 +49,13 @@ struct Layout {
     std::vector<int> edge_int;          // internal number of each mesh edge
     std::vector<int64_t> int_of_ref;    // reference dof -> internal slot
     std::vector<uint32_t> idx;          // first moment of every local edge (tile-major per group), SIDE1 flag
+    // Cell sides of the edges: the apply writes the k + 1 edge results of local
+    // edge j of a cell to side slot p (the position of that edge's entry in
+    // idx), values [p * (k + 1), (p + 1) * (k + 1)).  hside[2 e + s] is the
+    // slot of side s of internal edge e; a boundary edge's side 1 is the zero
+    // slot nside - 1, which no cell writes.
+    std::vector<int32_t> hside;
+    int64_t nside = 0;
     int64_t o_len2 = 0, o_len4 = 0, l_len = 0;  // packed sizes (elements) of the FP64 / FP32 streams
     // Batch instances (one for a plain mesh): each owns a contiguous range of
     // tiles, store positions and internal edges.

@@ -63,6 +63,15 @@ class Solver:
         return {ne: (int(out[2 * (ne - 3)]), int(out[2 * (ne - 3) + 1])) for ne in range(3, 11)
                 if out[2 * (ne - 3)] >= 0}
 
+    def edge_sides(self):
+        """(nside, table): the number of cell-side slots of the split apply and
+        the slots of side 0 and side 1 of every internal edge (2 per edge)."""
+        n = ctypes.c_int64()
+        check(lib().vemd_edge_sides(self.h, ctypes.byref(n), None))
+        t = np.zeros(2 * self.mesh.num_edges, np.int32)
+        check(lib().vemd_edge_sides(self.h, ctypes.byref(n), t.ctypes.data_as(c_ip)))
+        return int(n.value), t
+
     def set_theta(self, theta):
         self._theta = np.ascontiguousarray(theta, np.float64)
         check(lib().vemd_set_theta(self.h, _d(self._theta)))
