@@ -47,31 +47,58 @@ __device__ __forceinline__ void sell_acc(const int* __restrict__ col, const V* _
                 for (int cc = 0; cc < B; ++cc) out[r] += (T)v[(r * B + cc) * 64] * xc[u][cc];
         }
     }
-    for (; k < w; ++k) {
-        const int64_t c = (int64_t)col[o + (int64_t)k * 64] * B;
-        const V* v = vs + (int64_t)k * 64 * (B * B);
-        T xc[B];
+    if constexpr (B == 1) {
+        // the tail as one more group, loads issued together: entries past w
+        // re-read the row's last entry (in bounds) and contribute zero
+        if (k < w) {
+            int kk[U];
+            int64_t c[U];
 #pragma unroll
-        for (int cc = 0; cc < B; ++cc) {
-            const int64_t xi = (c + cc) * NC + j;
-            T xv = y ? x[xi] + y[xi] : x[xi];
-            if (cs) xv *= cs[c + cc];
-            xc[cc] = xv;
+            for (int u = 0; u < U; ++u) {
+                kk[u] = k + u < w ? k + u : w - 1;
+                c[u] = (int64_t)col[o + (int64_t)kk[u] * 64];
+            }
+            T xc[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t xi = c[u] * NC + j;
+                T xv = y ? x[xi] + y[xi] : x[xi];
+                if (cs) xv *= cs[c[u]];
+                xc[u] = k + u < w ? xv : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) out[0] += (T)vs[(int64_t)kk[u] * 64] * xc[u];
         }
+    } else {
+        for (; k < w; ++k) {
+            const int64_t c = (int64_t)col[o + (int64_t)k * 64] * B;
+            const V* v = vs + (int64_t)k * 64 * (B * B);
+            T xc[B];
 #pragma unroll
-        for (int r = 0; r < B; ++r)
+            for (int cc = 0; cc < B; ++cc) {
+                const int64_t xi = (c + cc) * NC + j;
+                T xv = y ? x[xi] + y[xi] : x[xi];
+                if (cs) xv *= cs[c + cc];
+                xc[cc] = xv;
+            }
 #pragma unroll
-            for (int cc = 0; cc < B; ++cc) out[r] += (T)v[(r * B + cc) * 64] * xc[cc];
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int cc = 0; cc < B; ++cc) out[r] += (T)v[(r * B + cc) * 64] * xc[cc];
+        }
     }
 }
 
 // out = (A (x + y))_brow, column j of NC interleaved columns (vector entry
 // (row, j) at row * NC + j), for one block row of bs = B rows (y may be null);
 // with ds non-null the gathered entries are scaled by ds[row] first.
-template <typename T, int B, int NC>
+// UG: gathers in flight per lane for scalar rows (0: the default of the level
+// operators; the prolongation's short rows pass 4).
+template <typename T, int B, int NC, int UG = 0>
 __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x, T (&out)[B],
                                          const T* __restrict__ y = nullptr, const T* __restrict__ ds = nullptr) {
-    constexpr int U = B == 1 ? 4 : 2;
+    // scalar (coarse) levels: few rows, so each lane keeps many gathers in flight
+    constexpr int U = UG > 0 ? UG : (B == 1 ? 16 : 2);
     const int64_t s = brow >> 6;
     const int w = A.width[s];
     const int64_t o = A.off[s] + (brow & 63);
@@ -237,7 +264,7 @@ __global__ __launch_bounds__(256) void k_prol(Sell<T> P, const T* xc, const T* d
     const int64_t n = P.nrows;
     AMG_ROW
     T px[1];
-    sell_row<T, 1, NC>(P, row, (int)(i % NC), xc, px, dc);
+    sell_row<T, 1, NC, 4>(P, row, (int)(i % NC), xc, px, dc);
     x[i] += d ? d[i] + px[0] : px[0];
 }
 
