@@ -273,6 +273,14 @@ vemd_status vemd_rhs(vemd_solver s, double* rhs);
  * 942-1131) and [pressure, flux] in the physical measure.  Host FP64. */
 vemd_status vemd_errors(vemd_solver s, const double* solution, int nq2d, double* out4);
 
+/* The four errors of vemd_errors, then the physical-domain divergence error
+ * ||div u_h - div u|| (div u_h = (H^-1 W u) . m / det J, H the monomial mass
+ * matrix of the computational cell) and the mass int p_h det J: the fields
+ * scalar_comp, flux_comp, scalar_phys, flux_phys, div_phys and mass of the
+ * reference GPU driver's report.  Replaces error_kernel / reduce_errors
+ * (GPU/src/equation_gpu.cu:385-479).  Host FP64. */
+vemd_status vemd_errors_full(vemd_solver s, const double* solution, int nq2d, double* out6);
+
 /* Adjoint gradient of a loss L(x) with respect to theta: solves K lambda =
  * dL/dx (dldx in reference order) and returns dL/dtheta_E =
  * lambda_E^T (exp(-theta_E) A_E) x_E per cell.  No reference counterpart. */

@@ -1450,29 +1450,42 @@ static int gmres(const Csr& A, const Ilu& M, const std::vector<double>& b, std::
 
 // L2 errors in the computational measure (CPU/solver/DarcySolver.cpp:942-1131)
 // plus physical-domain variants (det J weight, forward Piola u = J u_hat / det J).
+// out: pressure and flux errors in the computational measure, the same in the
+// physical measure, the physical divergence error and the physical mass
+// int p_h (GPU/src/equation_gpu.cu:385-479: the divergence is the
+// polynomial H^{-1} W u of the computational cell, divided by det J).
 static void errors(const Ctx& X, int nq, double* out) {
     const Mesh& m = *X.m;
     const Basis& bs = X.basis;
     TriRule rule = triangle_rule(nq);
-    double ep = 0, eu = 0, epp = 0, eup = 0;
+    double ep = 0, eu = 0, epp = 0, eup = 0, ed = 0, mass = 0;
     for (int c = 0; c < m.nc; ++c) {
         Elem e = element(X, c);
         std::vector<int> g;
         std::vector<double> dir;
         dofmap(X, c, g, dir);
         int n = e.n, p = bs.p, q = X.q;
-        std::vector<double> uf(n), coef(p, 0.0), pc(q);
+        std::vector<double> uf(n), coef(p, 0.0), pc(q), wu(q, 0.0), dc(q, 0.0);
         for (int i = 0; i < n; ++i) uf[i] = dir[i] * X.sol[g[i]];
         for (int r = 0; r < q; ++r) pc[r] = X.sol[g[n + r]];
         for (int a = 0; a < p; ++a)
             for (int i = 0; i < n; ++i) coef[a] += e.P[a * n + i] * uf[i];
+        for (int r = 0; r < q; ++r)
+            for (int i = 0; i < n; ++i) wu[r] += e.W[r * n + i] * uf[i];
+        Mat Hi = inverse(e.H, q);
+        for (int r = 0; r < q; ++r)
+            for (int t = 0; t < q; ++t) dc[r] += Hi[r * q + t] * wu[t];
         double xc = m.cx[c], yc = m.cy[c], h = m.diam[c];
         std::vector<QP> pts;
         for (auto& t : e.tris) {
             map_triangle(rule, t.v, pts);
             for (auto& qp : pts) {
-                double ph = 0.0, vx = 0.0, vy = 0.0;
-                for (int r = 0; r < q; ++r) ph += pc[r] * bs.mono(r, xc, yc, h, qp.x, qp.y);
+                double ph = 0.0, vx = 0.0, vy = 0.0, dv = 0.0;
+                for (int r = 0; r < q; ++r) {
+                    const double mr = bs.mono(r, xc, yc, h, qp.x, qp.y);
+                    ph += pc[r] * mr;
+                    dv += dc[r] * mr;
+                }
                 for (int a = 0; a < p; ++a) {
                     double bx, by;
                     bs.hdiv(a, xc, yc, h, qp.x, qp.y, bx, by);
@@ -1488,6 +1501,9 @@ static void errors(const Ctx& X, int nq, double* out) {
                 epp += dp * dp * qp.w * G.det;
                 double fx = (G.j00 * dx + G.j01 * dy) / G.det, fy = (G.j10 * dx + G.j11 * dy) / G.det;
                 eup += (fx * fx + fy * fy) * qp.w * G.det;
+                const double dd = (dv - gh) / G.det;  // div_x u_h - g, with g_hat = det J g
+                ed += dd * dd * qp.w * G.det;
+                mass += ph * qp.w * G.det;
             }
         }
     }
@@ -1495,6 +1511,8 @@ static void errors(const Ctx& X, int nq, double* out) {
     out[1] = std::sqrt(eu);
     out[2] = std::sqrt(epp);
     out[3] = std::sqrt(eup);
+    out[4] = std::sqrt(ed);
+    out[5] = mass;
 }
 
 }  // namespace orc

@@ -182,9 +182,9 @@ class Problem:
         return x
 
     def errors(self, nq=9):
-        out = np.zeros(4)
+        out = np.zeros(6)
         _check(lib().orc_errors(self.h, nq, _d(out)))
-        return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3])
+        return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3], div_phys=out[4], mass=out[5])
 
     def map(self, cell, xi, eta):
         out = np.zeros(7)

@@ -1,4 +1,5 @@
 import importlib.util
+import json
 import os
 import sys
 
@@ -41,3 +42,17 @@ def oracle():
 
 def golden(name):
     return os.path.join(GOLDEN, name)
+
+
+# Golden cases carrying the reference GPU driver's physical-domain values.
+PHYS = [c for c in json.load(open(golden("golden_errors.json")))["cases"] if "phys" in c]
+
+
+def check_phys(e, ph):
+    """Physical-domain pressure, flux and divergence errors and the mass
+    int p_h det J against a reference report (16 digits, runs solved to
+    ~1e-11): relative 5e-10; the mean-zero multiplier makes the mass zero to
+    round-off."""
+    for key in ("p_phys", "u_phys", "div_phys"):
+        assert abs(e[key] - ph[key]) <= 5e-10 * ph[key], (key, e[key], ph[key])
+    assert abs(e["mass"]) <= 1e-12 and abs(ph["mass"]) <= 1e-12

@@ -4,7 +4,7 @@ import json
 import numpy as np
 import pytest
 
-from conftest import golden
+from conftest import check_phys, golden
 
 pytestmark = pytest.mark.gpu
 
@@ -211,6 +211,8 @@ def test_reference_errors_on_device(vemd, case):
         else:  # half a unit of the 7th digit plus the printing run's rtol 1e-10
             tol = 0.5 * 10.0 ** (np.floor(np.log10(v)) - 6) + 1e-7 * v
         assert abs(e[key] - v) <= tol, (key, e[key], v)
+    if "phys" in case:
+        check_phys(e, case["phys"])
 
 
 def test_apply_autotuner(vemd, oracle, monkeypatch):

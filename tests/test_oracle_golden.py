@@ -6,7 +6,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import golden
+from conftest import PHYS, check_phys, golden
 
 CASES = json.load(open(golden("golden_errors.json")))["cases"]
 
@@ -33,6 +33,36 @@ def test_golden_errors(oracle, case):
     e = p.errors(case["nq2"])
     assert abs(e["p_comp"] - case["p_comp"]) <= _tol(case, case["p_comp"]), (e, case)
     assert abs(e["u_comp"] - case["u_comp"]) <= _tol(case, case["u_comp"]), (e, case)
+
+
+@pytest.mark.parametrize("case", PHYS, ids=lambda c: "%s-ex%d-%s" % (c["mesh"][:-4], c["example"],
+                                                                   c["phys"]["source"].split("/")[-2][:18]))
+def test_golden_physical_errors(oracle, case):
+    m = oracle.Mesh.read(golden(case["mesh"]))
+    p = oracle.Problem(m, k=case["k"], nq2=case["nq2"], nq1=case["nq1"],
+                       eps=0.05 if case["example"] == 1 else 0.0)
+    p.solve_direct()
+    check_phys(p.errors(case["nq2"]), case["phys"])
+
+
+def test_divergence_error_depends_on_flux_only(oracle):
+    """div u_h = H^{-1} W u uses the flux dofs alone: moving the pressure and
+    the multiplier changes the pressure errors and the mass but leaves the
+    divergence error and the flux errors as they were."""
+    m = oracle.Mesh.read(golden("square_2x2.msh"))
+    p = oracle.Problem(m, k=1, nq2=9, nq1=9, eps=0.05)
+    p.solve_direct()
+    e0 = p.errors(9)
+    x = p.solution().copy()
+    nf = p.sizes()["nf"]
+    rng = np.random.default_rng(4)
+    x[nf:] += rng.standard_normal(len(x) - nf)  # pressure moments and multiplier
+    p.set_solution(x)
+    e1 = p.errors(9)
+    for key in ("div_phys", "u_phys", "u_comp"):
+        assert e1[key] == e0[key], key
+    assert abs(e1["p_phys"] - e0["p_phys"]) > 1e-3 * e0["p_phys"]
+    assert abs(e1["mass"]) > 1e-6
 
 
 @pytest.mark.parametrize("k", [0, 1, 2, 3])

@@ -80,6 +80,7 @@ SIGNATURES = {
     "vemd_apply_multi_device": [c_vp, c_vp, c_vp, c_vp, c_vp],
     "vemd_rhs": [c_vp, c_dp],
     "vemd_errors": [c_vp, c_dp, ctypes.c_int, c_dp],
+    "vemd_errors_full": [c_vp, c_dp, ctypes.c_int, c_dp],
     "vemd_adjoint_theta": [c_vp, ctypes.POINTER(SolveOptsC), c_dp, c_dp, c_dp, ctypes.POINTER(SolveInfo)],
     "vemd_element_matrices": [c_vp, ctypes.c_int, c_ip, c_dp, c_dp, c_dp, c_dp, c_dp, c_lp],
     "vemd_mesh_concat": [ctypes.c_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)],

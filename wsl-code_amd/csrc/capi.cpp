@@ -401,6 +401,15 @@ vemd_status vemd_rhs(vemd_solver sh, double* rhs) {
 vemd_status vemd_errors(vemd_solver sh, const double* sol, int nq, double* out) {
     API_BEGIN
     REQUIRE(sh && sol && out, "null argument");
+    double all[6];
+    errors_host(*sh->s, sol, nq, all);
+    std::copy(all, all + 4, out);
+    API_END
+}
+
+vemd_status vemd_errors_full(vemd_solver sh, const double* sol, int nq, double* out) {
+    API_BEGIN
+    REQUIRE(sh && sol && out, "null argument");
     errors_host(*sh->s, sol, nq, out);
     API_END
 }

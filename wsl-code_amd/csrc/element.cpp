@@ -794,7 +794,10 @@ Setup* setup_build(const Mesh& m, const vemd_problem_desc& d) {
 
 // L2 errors against the manufactured solution (CPU/solver/DarcySolver.cpp:942-1131):
 // pressure and projected flux in the computational measure, then in the
-// physical measure (det J weight, forward Piola u = J u_hat / det J).
+// physical measure (det J weight, forward Piola u = J u_hat / det J), then the
+// physical divergence error and the mass int p_h det J
+// (GPU/src/equation_gpu.cu:385-479): div_x u_h = (H^{-1} W u) . m / det J with
+// H the monomial mass matrix of the computational cell.
 void errors_host(const struct Solver& s, const double* sol, int nq, double* out);
 
 }  // namespace vemd
@@ -809,10 +812,10 @@ void errors_host(const Solver& s, const double* sol, int nq, double* out) {
     const Shape& sh = S.sh;
     const vemd_problem_desc& d = S.desc;
     Rule rule = tri_rule(nq);
-    std::vector<double> e(4 * (size_t)m.nc, 0.0);
+    std::vector<double> e(6 * (size_t)m.nc, 0.0);
     parallel_for(m.nc, [&](int64_t b, int64_t en) {
         std::vector<Tri> tris;
-        std::vector<double> u, coef(sh.p), pc(sh.q);
+        std::vector<double> u, coef(sh.p), pc(sh.q), wu(sh.q), dc(sh.q), H(sh.q * sh.q), mr(sh.q);
         for (int64_t c = b; c < en; ++c) {
             int ne = m.npc((int)c), n = sh.nflux(ne);
             int64_t v0 = S.ofs.v_off[c];
@@ -829,7 +832,44 @@ void errors_host(const Solver& s, const double* sol, int nq, double* out) {
             double ta;
             triangulate(m, (int)c, tris, ta);
             double xc = m.cx[c], yc = m.cy[c], h = m.diam[c];
-            double ep = 0, eu = 0, epp = 0, eup = 0;
+            // divergence coefficients: H dc = W u (W stored with the direction
+            // signs in its columns, so W sol[gdof] = W_unsigned u)
+            const double* Wc = &S.W[S.ofs.w_off[c]];
+            for (int r = 0; r < sh.q; ++r) {
+                double v = 0;
+                for (int i = 0; i < n; ++i) v += Wc[r * n + i] * sol[S.gdof[v0 + i]];
+                wu[r] = v;
+            }
+            std::fill(H.begin(), H.end(), 0.0);
+            for (auto& t : tris) {
+                const double* v = t.v;
+                double J = std::fabs((v[2] - v[0]) * (v[5] - v[1]) - (v[4] - v[0]) * (v[3] - v[1]));
+                for (size_t i = 0; i < rule.w.size(); ++i) {
+                    double x = v[0] + (v[2] - v[0]) * rule.a[i] + (v[4] - v[0]) * rule.b[i];
+                    double y = v[1] + (v[3] - v[1]) * rule.a[i] + (v[5] - v[1]) * rule.b[i];
+                    double xm = (x - xc) / h, ym = (y - yc) / h;
+                    for (int r = 0; r < sh.q; ++r) {
+                        int ea, eb;
+                        mono_index(r, ea, eb);
+                        mr[r] = ipow(xm, ea) * ipow(ym, eb);
+                    }
+                    for (int r = 0; r < sh.q; ++r)
+                        for (int t2 = 0; t2 < sh.q; ++t2) H[r * sh.q + t2] += mr[r] * mr[t2] * rule.w[i] * J;
+                }
+            }
+            {
+                // H / |E| is O(1) for the scaled monomials
+                const double s0 = H[0] > 0 ? 1.0 / H[0] : 1.0;
+                std::vector<double> Hs(H.size()), Hi(H.size());
+                for (size_t i = 0; i < H.size(); ++i) Hs[i] = H[i] * s0;
+                inverse(Hs.data(), sh.q, Hi.data(), (int)c);
+                for (int r = 0; r < sh.q; ++r) {
+                    double v = 0;
+                    for (int t2 = 0; t2 < sh.q; ++t2) v += Hi[r * sh.q + t2] * wu[t2] * s0;
+                    dc[r] = v;
+                }
+            }
+            double ep = 0, eu = 0, epp = 0, eup = 0, ed = 0, ms = 0;
             for (auto& t : tris) {
                 const double* v = t.v;
                 double J = std::fabs((v[2] - v[0]) * (v[5] - v[1]) - (v[4] - v[0]) * (v[3] - v[1]));
@@ -838,11 +878,13 @@ void errors_host(const Solver& s, const double* sol, int nq, double* out) {
                     double y = v[1] + (v[3] - v[1]) * rule.a[i] + (v[5] - v[1]) * rule.b[i];
                     double w = rule.w[i] * J;
                     double xm = (x - xc) / h, ym = (y - yc) / h;
-                    double ph = 0, vx = 0, vy = 0;
+                    double ph = 0, vx = 0, vy = 0, dv = 0;
                     for (int r = 0; r < sh.q; ++r) {
                         int ea, eb;
                         mono_index(r, ea, eb);
-                        ph += pc[r] * ipow(xm, ea) * ipow(ym, eb);
+                        const double mv = ipow(xm, ea) * ipow(ym, eb);
+                        ph += pc[r] * mv;
+                        dv += dc[r] * mv;
                     }
                     for (int a = 0; a < sh.p; ++a) {
                         double bx, by;
@@ -860,18 +902,25 @@ void errors_host(const Solver& s, const double* sol, int nq, double* out) {
                     epp += dp * dp * w * g.det;
                     double fx = (g.j00 * dx + g.j01 * dy) / g.det, fy = (g.j10 * dx + g.j11 * dy) / g.det;
                     eup += (fx * fx + fy * fy) * w * g.det;
+                    const double dd = dv / g.det - gg;  // div_x u_h - div u (= g)
+                    ed += dd * dd * w * g.det;
+                    ms += ph * w * g.det;
                 }
             }
-            e[4 * c] = ep;
-            e[4 * c + 1] = eu;
-            e[4 * c + 2] = epp;
-            e[4 * c + 3] = eup;
+            double* ec = &e[6 * c];
+            ec[0] = ep;
+            ec[1] = eu;
+            ec[2] = epp;
+            ec[3] = eup;
+            ec[4] = ed;
+            ec[5] = ms;
         }
     });
-    double t[4] = {0, 0, 0, 0};
+    double t[6] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < m.nc; ++c)
-        for (int j = 0; j < 4; ++j) t[j] += e[4 * (size_t)c + j];
-    for (int j = 0; j < 4; ++j) out[j] = std::sqrt(t[j]);
+        for (int j = 0; j < 6; ++j) t[j] += e[6 * (size_t)c + j];
+    for (int j = 0; j < 5; ++j) out[j] = std::sqrt(t[j]);
+    out[5] = t[5];
 }
 
 }  // namespace vemd

@@ -141,10 +141,13 @@ class Solver:
         return grad, dict(iterations=info.iterations, rel_residual=info.rel_residual)
 
     def errors(self, x, nq=9):
+        """L2 errors of a reference-order solution: pressure and flux in the
+        computational and the physical measure, the physical divergence error
+        and the mass int p_h det J."""
         x = np.ascontiguousarray(x, np.float64)
-        out = np.zeros(4)
-        check(lib().vemd_errors(self.h, _d(x), nq, _d(out)))
-        return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3])
+        out = np.zeros(6)
+        check(lib().vemd_errors_full(self.h, _d(x), nq, _d(out)))
+        return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3], div_phys=out[4], mass=out[5])
 
     def element(self, cell):
         """FP64 element operator of one cell: dict with A (n x n), W (q x n), L, f, sign, gdof."""
