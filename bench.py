@@ -375,8 +375,9 @@ def main():
                     help="where the FP64 element operators are built (not part of the timed solve)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; a figure at %d GPUs is not produced (one process per GPU "
-              "would need the halo exchange described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: this solver runs on one GPU; a figure at %d GPUs is not produced (one process per GPU "
+              "would need a per-apply halo exchange of shared-edge flux values, which is not built)" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
     import torch
     pkg = load_pkg()

@@ -830,7 +830,7 @@ static Geo eval_map(const Ctx& X, int cell, const Tri* tri, double xi, double et
 }
 
 // Physical exact solution (CPU/examples/darcy_problem.cpp:116-131; the
-// sin*sin variant is the BASELINE manufactured solution).
+// sin*sin variant is the manufactured solution of the benchmark workloads).
 static void exact(int sol, double x, double y, double* p, double* ux, double* uy, double* g) {
     if (sol == SOL_SINCOS) {
         *p = std::sin(M_PI * x) * std::cos(M_PI * y);

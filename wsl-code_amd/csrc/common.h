@@ -20,8 +20,8 @@ struct Error : std::runtime_error {
 
 void set_last_error(const std::string& msg);
 
-// Number of worker threads for host setup (bounded by the CPU share of a
-// one-GPU box).
+// Number of worker threads for host setup (bounded by the CPU share that a
+// machine gives one GPU).
 int host_threads();
 
 // Runs f(begin, end) over [0, n) on host_threads() threads.
