@@ -445,29 +445,61 @@ __global__ void k_lam_row(const double* plam, int ntiles, T* y, int64_t lam_idx)
     if (threadIdx.x == 0) y[lam_idx] = (T)s;
 }
 
-// Sums of the partials a[b..e) and c[b..e) (c may be null) in a fixed order
-// by one block of FIN threads; every thread gets the results.
+// Sums of the partials src[k][b..e) (a null array sums to zero) in a fixed
+// order by one block of FIN threads; every thread gets the results. The
+// partials were written by other kernels, mostly on other XCDs, so each load
+// misses this L2: every thread issues all its loads (32 / NA per array) before
+// adding, and the block sum is two shuffle trees around one LDS exchange.
 constexpr int FIN = 1024;
-__device__ void block_sums(const double* a, const double* c, int b, int e, double& sa, double& sc) {
-    __shared__ double sh[2][FIN];
-    double x = 0.0, y = 0.0;
-    for (int i = b + (int)threadIdx.x; i < e; i += FIN) {
-        x += a[i];
-        if (c) y += c[i];
+template <int NA>
+__device__ void block_sums_n(const double* const (&src)[NA], int b, int e, double (&out)[NA]) {
+    constexpr int U = 32 / NA, NW = FIN / 64;
+    __shared__ double sh[NA][NW];
+    double acc[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    for (int i0 = b + (int)threadIdx.x; i0 < e; i0 += FIN * U) {
+        double v[NA][U];
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * FIN;
+                v[k][u] = src[k] && i < e ? src[k][i] : 0.0;
+            }
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[k] += v[k][u];
     }
-    sh[0][threadIdx.x] = x;
-    sh[1][threadIdx.x] = y;
+    const int w = (int)threadIdx.x >> 6, l = (int)threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        double x = acc[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (l == 0) sh[k][w] = x;
+    }
     __syncthreads();
-    for (int o = FIN / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+    if (w == 0) {
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            double x = l < NW ? sh[k][l] : 0.0;
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+            if (l == 0) sh[k][0] = x;
         }
-        __syncthreads();
     }
-    sa = sh[0][0];
-    sc = sh[1][0];
     __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NA; ++k) out[k] = sh[k][0];
+    __syncthreads();
+}
+
+__device__ void block_sums(const double* a, const double* c, int b, int e, double& sa, double& sc) {
+    const double* src[2] = {a, c};
+    double out[2];
+    block_sums_n<2>(src, b, e, out);
+    sa = out[0];
+    sc = out[1];
 }
 
 // The Krylov recurrences run per batch instance: every instance has its own
@@ -626,9 +658,16 @@ __global__ void k_finB(MState* sts, const double* p1, const double* p2, const do
     MState* st = sts + blockIdx.x;
     if (st->done) return;
     const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
-    double b2, rho, qb2 = 0.0, qrho = 0.0;
-    block_sums(p1, p2, c0, c1, b2, rho);
-    if (useq) block_sums(q1, q2, c0, c1, qb2, qrho);
+    double b2, rho, qb2, qrho;
+    {
+        const double* src[4] = {p1, p2, useq ? q1 : nullptr, useq ? q2 : nullptr};
+        double out[4];
+        block_sums_n<4>(src, c0, c1, out);
+        b2 = out[0];
+        rho = out[1];
+        qb2 = out[2];
+        qrho = out[3];
+    }
     b2 += qb2;
     rho += qrho;
     if (threadIdx.x == 0) {
