@@ -92,11 +92,70 @@ __device__ __forceinline__ void sell_acc(const int* __restrict__ col, const V* _
 // out = (A (x + y))_brow, column j of NC interleaved columns (vector entry
 // (row, j) at row * NC + j), for one block row of bs = B rows (y may be null);
 // with ds non-null the gathered entries are scaled by ds[row] first.
+// Lanes per row: small single-column scalar levels (below 32k rows, ~50
+// entries per row) spread each row over 8 consecutive lanes (entries
+// k = sub, sub + 8, ...) summed with shuffles, which shortens each lane's chain
+// of dependent loads; larger levels (scattered slice loads would multiply),
+// block levels and the 32-column cycle keep one lane per row. The host and the
+// kernels derive it from the same row count.
+constexpr int LANES_SMALL = 8;
+template <int B, int NC>
+__host__ __device__ inline int lanes_per_row(int64_t nrows) {
+    return B == 1 && NC == 1 && nrows < 32768 ? LANES_SMALL : 1;
+}
+
+// Row brow of a scalar SELL operator by the LP lanes of its group (lane sub):
+// every lane's share as one group of U gathers (loads issued together; entries
+// past w re-read entry w - 1 in bounds and contribute zero), then the sum over
+// the group. Every lane of the group gets the row's value.
+template <typename T, typename V, int U>
+__device__ __forceinline__ T sell_row_lanes(const int* __restrict__ col, const V* __restrict__ vs, int64_t o, int w,
+                                            int sub, const T* __restrict__ x, const T* __restrict__ y,
+                                            const T* __restrict__ cs) {
+    constexpr int LP = LANES_SMALL;
+    T acc = T(0);
+    for (int k0 = sub; k0 < w; k0 += LP * U) {
+        int kk[U];
+        int64_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * LP;
+            kk[u] = k < w ? k : w - 1;
+            c[u] = (int64_t)col[o + (int64_t)kk[u] * 64];
+        }
+        T xc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            T xv = y ? x[c[u]] + y[c[u]] : x[c[u]];
+            if (cs) xv *= cs[c[u]];
+            xc[u] = k0 + u * LP < w ? xv : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += (T)vs[(int64_t)kk[u] * 64] * xc[u];
+    }
+#pragma unroll
+    for (int m = LP / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
+    return acc;
+}
+
 // UG: gathers in flight per lane for scalar rows (0: the default of the level
 // operators; the prolongation's short rows pass 4).
 template <typename T, int B, int NC, int UG = 0>
 __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x, T (&out)[B],
-                                         const T* __restrict__ y = nullptr, const T* __restrict__ ds = nullptr) {
+                                         const T* __restrict__ y = nullptr, const T* __restrict__ ds = nullptr,
+                                         int sub = 0) {
+    if constexpr (UG == 0 && B == 1 && NC == 1) {
+        if (lanes_per_row<B, NC>(A.nrows) > 1) {
+            const int64_t s = brow >> 6;
+            const int w = A.width[s];
+            const int64_t o = A.off[s] + (brow & 63);
+            if (A.hval)
+                out[0] = sell_row_lanes<T, _Float16, 4>(A.col, A.hval + A.off[s] + (brow & 63), o, w, sub, x, y, ds);
+            else
+                out[0] = sell_row_lanes<T, T, 4>(A.col, A.val + A.off[s] + (brow & 63), o, w, sub, x, y, ds);
+            return;
+        }
+    }
     // scalar (coarse) levels: few rows, so each lane keeps many gathers in flight
     constexpr int U = UG > 0 ? UG : (B == 1 ? 16 : 2);
     const int64_t s = brow >> 6;
@@ -118,11 +177,13 @@ __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, 
     const int64_t row = i / NC;
 
 // Thread t of a block-row sweep: block row t / NC, column t % NC.
-#define AMG_BROW                                                 \
-    if (done && *done) return;                                   \
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;   \
-    const int64_t bi = t / NC;                                   \
-    const int j = (int)(t % NC);                                 \
+#define AMG_BROW                                                          \
+    if (done && *done) return;                                            \
+    const int LPR = lanes_per_row<B, NC>(A.nrows);                        \
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;            \
+    const int64_t bi = t / (NC * LPR);                                    \
+    const int j = (int)((t / LPR) % NC);                                  \
+    const int sub = (int)(t % LPR);                                       \
     if (bi >= A.nrows) return;
 
 // Diagonal entry of D^-1 (null: unit diagonal) and the right-hand side of a
@@ -158,7 +219,7 @@ __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, co
                                             int last, const T* xs, const T* b0, const T* es, const int* done) {
     AMG_BROW
     T ad[B];
-    sell_row<T, B, NC>(A, bi, j, din, ad);
+    sell_row<T, B, NC>(A, bi, j, din, ad, nullptr, nullptr, sub);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
@@ -184,7 +245,7 @@ __global__ __launch_bounds__(256) void k_step0(Sell<T> A, const T* b, const T* d
                                              T c2, int last, const int* done) {
     AMG_BROW
     T ad[B];
-    sell_row<T, B, NC>(A, bi, j, b, ad, nullptr, dinv);
+    sell_row<T, B, NC>(A, bi, j, b, ad, nullptr, dinv, sub);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
@@ -208,7 +269,7 @@ __global__ __launch_bounds__(256) void k_resid(Sell<T> A, const T* b, const T* e
                                              const int* done) {
     AMG_BROW
     T ax[B];
-    sell_row<T, B, NC>(A, bi, j, x, ax, d);
+    sell_row<T, B, NC>(A, bi, j, x, ax, d, nullptr, sub);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
@@ -222,7 +283,7 @@ __global__ __launch_bounds__(256) void k_post0(Sell<T> A, const T* b, const T* e
                                              T* d, T c, const int* done) {
     AMG_BROW
     T ax[B];
-    sell_row<T, B, NC>(A, bi, j, x, ax);
+    sell_row<T, B, NC>(A, bi, j, x, ax, nullptr, nullptr, sub);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
@@ -473,12 +534,14 @@ static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStre
         const double rn = 1.0 / (2.0 * sigma - rho);
         const int last = k + 1 == L.degree;
         if (fused && k == 1) {
-            hipLaunchKernelGGL((k_step0<T, B, NC>), grid_of(L.A.nrows * NC), dim3(256), 0, st, L.A, (const T*)v.b,
+            hipLaunchKernelGGL((k_step0<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A.nrows)), dim3(256), 0, st,
+                               L.A, (const T*)v.b,
                                (const T*)L.dinv, v.r, din, v.x, (T)(1.0 / L.centre), (T)(rn * rho),
                                (T)(2.0 * rn / L.half), last, ms);
         } else {
             const bool b0 = from_b && k == 1;
-            hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC), dim3(256), 0, st, L.A, v.r,
+            hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A.nrows)), dim3(256), 0, st,
+                               L.A, v.r,
                                (const T*)L.dinv, (const T*)din, dout, v.x, (T)(rn * rho), (T)(2.0 * rn / L.half),
                                last, last ? xs : nullptr, b0 ? (const T*)v.b : nullptr, b0 ? (const T*)L.sc : nullptr,
                                ms);
@@ -502,7 +565,7 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
     const AmgLevelDev<T>& L = D.lv[l];
     const AmgLevelDev<T>& C = D.lv[l + 1];
     const LevelVecs<T> v = vecs<T, NC>(L), vc = vecs<T, NC>(C);
-    const dim3 g = grid_of(L.n * NC), gb = grid_of(L.A.nrows * NC), b(256);
+    const dim3 g = grid_of(L.n * NC), gb = grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A.nrows)), b(256);
     const dim3 gr = NC == 1 ? grid_of(C.n * 32) : grid_of(C.n * NC);
     const T c0 = (T)(1.0 / L.centre);
     T* din = v.d0;
