@@ -343,8 +343,20 @@ __global__ __launch_bounds__(64) void k_dense(int64_t n, const T* cinv, const T*
     if (done && *done) return;
     const int64_t i = blockIdx.x;
     if constexpr (NC == 1) {
+        // groups of U loads issued together (the inverse is cold in the cache)
+        constexpr int U = 8;
         double acc = 0.0;
-        for (int64_t j = threadIdx.x; j < n; j += 64) acc += (double)cinv[i * n + j] * (double)b[j];
+        for (int64_t j0 = threadIdx.x; j0 < n; j0 += 64 * U) {
+            T cv[U], bv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t j = j0 + 64 * u;
+                cv[u] = j < n ? cinv[i * n + j] : T(0);
+                bv[u] = j < n ? b[j] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += (double)cv[u] * (double)bv[u];
+        }
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
         if (threadIdx.x == 0) x[i] = (T)acc;
     } else {
