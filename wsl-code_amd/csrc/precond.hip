@@ -19,7 +19,9 @@ namespace vemd {
 // gathers depend on the indices; batching keeps several in flight per lane).
 // Gathered entries are x (+ y) times cs[column] when cs is non-null; the sum
 // runs in entry order.
-template <typename T, typename V, int B, int NC, int U>
+// HY, HC: the pending update y and the column scaling cs are present (known at
+// compile time, so no gather waits behind a branch).
+template <typename T, typename V, int B, int NC, int U, bool HY, bool HC>
 __device__ __forceinline__ void sell_acc(const int* __restrict__ col, const V* __restrict__ vs, int64_t o, int w, int j,
                                          const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ cs,
                                          T (&out)[B]) {
@@ -34,8 +36,9 @@ __device__ __forceinline__ void sell_acc(const int* __restrict__ col, const V* _
 #pragma unroll
             for (int cc = 0; cc < B; ++cc) {
                 const int64_t xi = (c[u] + cc) * NC + j;
-                T xv = y ? x[xi] + y[xi] : x[xi];
-                if (cs) xv *= cs[c[u] + cc];
+                T xv = x[xi];
+                if constexpr (HY) xv += y[xi];
+                if constexpr (HC) xv *= cs[c[u] + cc];
                 xc[u][cc] = xv;
             }
 #pragma unroll
@@ -62,8 +65,9 @@ __device__ __forceinline__ void sell_acc(const int* __restrict__ col, const V* _
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int64_t xi = c[u] * NC + j;
-                T xv = y ? x[xi] + y[xi] : x[xi];
-                if (cs) xv *= cs[c[u]];
+                T xv = x[xi];
+                if constexpr (HY) xv += y[xi];
+                if constexpr (HC) xv *= cs[c[u]];
                 xc[u] = k + u < w ? xv : T(0);
             }
 #pragma unroll
@@ -77,8 +81,9 @@ __device__ __forceinline__ void sell_acc(const int* __restrict__ col, const V* _
 #pragma unroll
             for (int cc = 0; cc < B; ++cc) {
                 const int64_t xi = (c + cc) * NC + j;
-                T xv = y ? x[xi] + y[xi] : x[xi];
-                if (cs) xv *= cs[c + cc];
+                T xv = x[xi];
+                if constexpr (HY) xv += y[xi];
+                if constexpr (HC) xv *= cs[c + cc];
                 xc[cc] = xv;
             }
 #pragma unroll
@@ -108,7 +113,7 @@ __host__ __device__ inline int lanes_per_row(int64_t nrows) {
 // every lane's share as one group of U gathers (loads issued together; entries
 // past w re-read entry w - 1 in bounds and contribute zero), then the sum over
 // the group. Every lane of the group gets the row's value.
-template <typename T, typename V, int U>
+template <typename T, typename V, int U, bool HY, bool HC>
 __device__ __forceinline__ T sell_row_lanes(const int* __restrict__ col, const V* __restrict__ vs, int64_t o, int w,
                                             int sub, const T* __restrict__ x, const T* __restrict__ y,
                                             const T* __restrict__ cs) {
@@ -126,8 +131,9 @@ __device__ __forceinline__ T sell_row_lanes(const int* __restrict__ col, const V
         T xc[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            T xv = y ? x[c[u]] + y[c[u]] : x[c[u]];
-            if (cs) xv *= cs[c[u]];
+            T xv = x[c[u]];
+            if constexpr (HY) xv += y[c[u]];
+            if constexpr (HC) xv *= cs[c[u]];
             xc[u] = k0 + u * LP < w ? xv : T(0);
         }
 #pragma unroll
@@ -140,19 +146,20 @@ __device__ __forceinline__ T sell_row_lanes(const int* __restrict__ col, const V
 
 // UG: gathers in flight per lane for scalar rows (0: the default of the level
 // operators; the prolongation's short rows pass 4).
-template <typename T, int B, int NC, int UG = 0>
-__device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x, T (&out)[B],
-                                         const T* __restrict__ y = nullptr, const T* __restrict__ ds = nullptr,
-                                         int sub = 0) {
+template <typename T, int B, int NC, int UG, bool HY, bool HC>
+__device__ __forceinline__ void sell_row_impl(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x,
+                                              T (&out)[B], const T* __restrict__ y, const T* __restrict__ ds,
+                                              int sub) {
     if constexpr (UG == 0 && B == 1 && NC == 1) {
         if (lanes_per_row<B, NC>(A.nrows) > 1) {
             const int64_t s = brow >> 6;
             const int w = A.width[s];
             const int64_t o = A.off[s] + (brow & 63);
             if (A.hval)
-                out[0] = sell_row_lanes<T, _Float16, 4>(A.col, A.hval + A.off[s] + (brow & 63), o, w, sub, x, y, ds);
+                out[0] = sell_row_lanes<T, _Float16, 4, HY, HC>(A.col, A.hval + A.off[s] + (brow & 63), o, w, sub, x, y,
+                                                                ds);
             else
-                out[0] = sell_row_lanes<T, T, 4>(A.col, A.val + A.off[s] + (brow & 63), o, w, sub, x, y, ds);
+                out[0] = sell_row_lanes<T, T, 4, HY, HC>(A.col, A.val + A.off[s] + (brow & 63), o, w, sub, x, y, ds);
             return;
         }
     }
@@ -164,9 +171,29 @@ __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, 
 #pragma unroll
     for (int r = 0; r < B; ++r) out[r] = T(0);
     if (A.hval)
-        sell_acc<T, _Float16, B, NC, U>(A.col, A.hval + A.off[s] * (B * B) + (brow & 63), o, w, j, x, y, ds, out);
+        sell_acc<T, _Float16, B, NC, U, HY, HC>(A.col, A.hval + A.off[s] * (B * B) + (brow & 63), o, w, j, x, y, ds,
+                                                out);
     else
-        sell_acc<T, T, B, NC, U>(A.col, A.val + A.off[s] * (B * B) + (brow & 63), o, w, j, x, y, ds, out);
+        sell_acc<T, T, B, NC, U, HY, HC>(A.col, A.val + A.off[s] * (B * B) + (brow & 63), o, w, j, x, y, ds, out);
+}
+
+// Row brow of A times x (+ y) (scaled by ds per column): one uniform branch
+// on which of y and ds are given, then loops without per-gather tests.
+template <typename T, int B, int NC, int UG = 0>
+__device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x, T (&out)[B],
+                                         const T* __restrict__ y = nullptr, const T* __restrict__ ds = nullptr,
+                                         int sub = 0) {
+    if (y) {
+        if (ds)
+            sell_row_impl<T, B, NC, UG, true, true>(A, brow, j, x, out, y, ds, sub);
+        else
+            sell_row_impl<T, B, NC, UG, true, false>(A, brow, j, x, out, y, ds, sub);
+    } else {
+        if (ds)
+            sell_row_impl<T, B, NC, UG, false, true>(A, brow, j, x, out, y, ds, sub);
+        else
+            sell_row_impl<T, B, NC, UG, false, false>(A, brow, j, x, out, y, ds, sub);
+    }
 }
 
 // Thread t of an elementwise sweep over n rows x NC columns: entry t, row t / NC.
@@ -350,12 +377,14 @@ __global__ __launch_bounds__(64) void k_dense(int64_t n, const T* cinv, const T*
             T cv[U], bv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int64_t j = j0 + 64 * u;
-                cv[u] = j < n ? cinv[i * n + j] : T(0);
-                bv[u] = j < n ? b[j] : T(0);
+                // unconditional loads (a clamped index, in bounds), so none
+                // waits for another; entries past n contribute zero
+                const int64_t j = j0 + 64 * u < n ? j0 + 64 * u : n - 1;
+                cv[u] = cinv[i * n + j];
+                bv[u] = b[j];
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc += (double)cv[u] * (double)bv[u];
+            for (int u = 0; u < U; ++u) acc += j0 + 64 * u < n ? (double)cv[u] * (double)bv[u] : 0.0;
         }
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
         if (threadIdx.x == 0) x[i] = (T)acc;
