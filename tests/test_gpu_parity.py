@@ -131,6 +131,26 @@ def test_multigrid_preconditioner(vemd, oracle, cells):
     assert np.linalg.norm(xm - xo) <= 1e-5 * np.linalg.norm(xo)
 
 
+def test_multigrid_scalar_level_paths(vemd, oracle, monkeypatch):
+    """The scalar multigrid levels run either 8 lanes per row (levels below
+    VEMD_MG_LANE_ROWS rows, the default for small levels) or one lane per row
+    with 16 gathers in flight (larger levels): with every scalar level forced
+    to each form, the FP64 and mixed solves match the oracle's direct solve,
+    and the two forms need the same iterations (they differ by rounding)."""
+    res = {}
+    for lane_rows in ("1000000000", "0"):
+        monkeypatch.setenv("VEMD_MG_LANE_ROWS", lane_rows)
+        s, op = _gen_pair(vemd, oracle, "annulus", 6000, 1, 1, True)
+        xo = op.solve_direct()
+        x, info = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12, precond=vemd.PC_MULTIGRID))
+        assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo), (lane_rows, info)
+        xm, im = s.solve(vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-10, precond=vemd.PC_MULTIGRID))
+        assert im["rel_residual"] <= 1e-10 and np.linalg.norm(xm - xo) <= 1e-7 * np.linalg.norm(xo), (lane_rows, im)
+        res[lane_rows] = (info["iterations"], im["iterations"])
+    print("iterations (fp64, mixed): 8 lanes per row", res["1000000000"], "one lane", res["0"], flush=True)
+    assert abs(res["0"][0] - res["1000000000"][0]) <= 2
+
+
 def test_mixed_cycle_with_fp32_finest_level(vemd, oracle, monkeypatch):
     """The FP32 cycle's finest level in half precision (default, scaled to a
     unit diagonal) and in FP32 (VEMD_MG_HALF=0) both precondition the mixed

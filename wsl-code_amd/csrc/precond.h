@@ -47,6 +47,7 @@ struct Sell {
     const int* col = nullptr;
     const T* val = nullptr;
     const _Float16* hval = nullptr;
+    int lanes = 1;  // lanes per row in the single-column kernels of a scalar level (1 or 8)
 };
 
 // Compressed sparse rows (restriction: few long rows).

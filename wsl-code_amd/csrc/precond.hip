@@ -97,16 +97,17 @@ __device__ __forceinline__ void sell_acc(const int* __restrict__ col, const V* _
 // out = (A (x + y))_brow, column j of NC interleaved columns (vector entry
 // (row, j) at row * NC + j), for one block row of bs = B rows (y may be null);
 // with ds non-null the gathered entries are scaled by ds[row] first.
-// Lanes per row: small single-column scalar levels (below 32k rows, ~50
-// entries per row) spread each row over 8 consecutive lanes (entries
+// Lanes per row: small single-column scalar levels (below 32k rows by default,
+// ~50 entries per row) spread each row over 8 consecutive lanes (entries
 // k = sub, sub + 8, ...) summed with shuffles, which shortens each lane's chain
 // of dependent loads; larger levels (scattered slice loads would multiply),
-// block levels and the 32-column cycle keep one lane per row. The host and the
-// kernels derive it from the same row count.
+// block levels and the 32-column cycle keep one lane per row. The upload
+// records the choice in the level's operator (Sell::lanes); the launch grids
+// and the kernels read it there.
 constexpr int LANES_SMALL = 8;
-template <int B, int NC>
-__host__ __device__ inline int lanes_per_row(int64_t nrows) {
-    return B == 1 && NC == 1 && nrows < 32768 ? LANES_SMALL : 1;
+template <int B, int NC, typename T>
+__host__ __device__ inline int lanes_per_row(const Sell<T>& A) {
+    return B == 1 && NC == 1 ? A.lanes : 1;
 }
 
 // Row brow of a scalar SELL operator by the LP lanes of its group (lane sub):
@@ -151,7 +152,7 @@ __device__ __forceinline__ void sell_row_impl(const Sell<T>& A, int64_t brow, in
                                               T (&out)[B], const T* __restrict__ y, const T* __restrict__ ds,
                                               int sub) {
     if constexpr (UG == 0 && B == 1 && NC == 1) {
-        if (lanes_per_row<B, NC>(A.nrows) > 1) {
+        if (lanes_per_row<B, NC>(A) > 1) {
             const int64_t s = brow >> 6;
             const int w = A.width[s];
             const int64_t o = A.off[s] + (brow & 63);
@@ -206,7 +207,7 @@ __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, 
 // Thread t of a block-row sweep: block row t / NC, column t % NC.
 #define AMG_BROW                                                          \
     if (done && *done) return;                                            \
-    const int LPR = lanes_per_row<B, NC>(A.nrows);                        \
+    const int LPR = lanes_per_row<B, NC>(A);                              \
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;            \
     const int64_t bi = t / (NC * LPR);                                    \
     const int j = (int)((t / LPR) % NC);                                  \
@@ -427,6 +428,10 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<dou
     s.bs = bs;
     s.nrows = A.nrows / bs;
     s.nslices = (int)((s.nrows + 63) / 64);
+    // rows below VEMD_MG_LANE_ROWS (default 32768) of a scalar level: 8 lanes each
+    const char* lr = std::getenv("VEMD_MG_LANE_ROWS");
+    const int64_t lane_rows = lr ? (int64_t)std::atoll(lr) : (int64_t)32768;
+    s.lanes = bs == 1 && s.nrows < lane_rows ? LANES_SMALL : 1;
     // block columns of every block row
     std::vector<std::vector<int>> bc(s.nrows);
     for (int64_t br = 0; br < s.nrows; ++br) {
@@ -575,13 +580,13 @@ static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStre
         const double rn = 1.0 / (2.0 * sigma - rho);
         const int last = k + 1 == L.degree;
         if (fused && k == 1) {
-            hipLaunchKernelGGL((k_step0<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A.nrows)), dim3(256), 0, st,
+            hipLaunchKernelGGL((k_step0<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A)), dim3(256), 0, st,
                                L.A, (const T*)v.b,
                                (const T*)L.dinv, v.r, din, v.x, (T)(1.0 / L.centre), (T)(rn * rho),
                                (T)(2.0 * rn / L.half), last, ms);
         } else {
             const bool b0 = from_b && k == 1;
-            hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A.nrows)), dim3(256), 0, st,
+            hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A)), dim3(256), 0, st,
                                L.A, v.r,
                                (const T*)L.dinv, (const T*)din, dout, v.x, (T)(rn * rho), (T)(2.0 * rn / L.half),
                                last, last ? xs : nullptr, b0 ? (const T*)v.b : nullptr, b0 ? (const T*)L.sc : nullptr,
@@ -606,7 +611,7 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
     const AmgLevelDev<T>& L = D.lv[l];
     const AmgLevelDev<T>& C = D.lv[l + 1];
     const LevelVecs<T> v = vecs<T, NC>(L), vc = vecs<T, NC>(C);
-    const dim3 g = grid_of(L.n * NC), gb = grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A.nrows)), b(256);
+    const dim3 g = grid_of(L.n * NC), gb = grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A)), b(256);
     const dim3 gr = NC == 1 ? grid_of(C.n * 32) : grid_of(C.n * NC);
     const T c0 = (T)(1.0 / L.centre);
     T* din = v.d0;
