@@ -175,7 +175,7 @@ def test_multigrid_needs_dirichlet(vemd, oracle):
     assert e.value.status == "ERR_ARG"
 
 
-@pytest.mark.parametrize("case", GEN[1:], ids=GEN_IDS[1:])
+@pytest.mark.parametrize("case", GEN, ids=GEN_IDS)
 def test_split_apply_sums_to_apply(vemd, oracle, case):
     """The solver's one-launch form: the two cell sides of every edge dof,
     gathered through the side table, sum to the complete apply bitwise (same
