@@ -94,6 +94,30 @@ def test_physical_errors_match_reference_reports(vemd, oracle, case):
     check_phys(s.errors(op.solve_direct(), case["nq2"]), case["phys"])
 
 
+@pytest.mark.parametrize("k", [0, 1])
+def test_errors_on_curved_cells_match_oracle(vemd, oracle, k):
+    """All six error functionals (both measures, physical divergence, mass)
+    of the product's host routine against the oracle's, on a generated annulus
+    with exact boundary arcs, Dirichlet pressure and a cellwise permeability,
+    for the oracle's direct solution and for a perturbed one (so that the
+    errors are not small by construction)."""
+    from wsl_code_amd.fields import grf_theta
+    m = vemd.generate_mesh(400, domain=vemd.DOMAIN_ANNULUS, r_inner=0.5, lloyd_iters=20, seed=21,
+                           min_edge_frac=0.05)
+    a = m.arrays()
+    th = grf_theta(a["cx"], a["cy"], sigma=1.0, ell=0.1, seed=22)
+    s = vemd.Solver(m, k=k, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P, theta=th)
+    om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+    op = oracle.Problem(om, k=k, map_kind=oracle.MAP_ARCS, sol=oracle.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P, theta=th)
+    x = op.solve_direct()
+    for y in (x, x + 1e-3 * np.random.default_rng(23).standard_normal(len(x))):
+        op.set_solution(y)
+        eo, ep = op.errors(9), s.errors(y, 9)
+        for key in eo:
+            assert abs(ep[key] - eo[key]) <= 1e-12 * abs(eo[key]) + 1e-15, (key, ep[key], eo[key])
+    assert eo["div_phys"] > 0 and abs(eo["mass"]) > 0
+
+
 def test_errors_reported_not_raised(vemd):
     with pytest.raises(vemd.VemdError) as ei:
         vemd.Mesh.read("/nonexistent/mesh.msh")
