@@ -47,6 +47,7 @@ struct Sell {
     const int* col = nullptr;
     const T* val = nullptr;
     const _Float16* hval = nullptr;
+    float hscale = 1.0f;  // hval holds the values divided by hscale
     int lanes = 1;  // lanes per row in the single-column kernels of a scalar level (1 or 8)
 };
 
@@ -57,6 +58,8 @@ struct CsrDev {
     const int64_t* rp = nullptr;
     const int* ci = nullptr;
     const T* v = nullptr;
+    const _Float16* hv = nullptr;  // values in half precision (divided by hscale) instead of v
+    float hscale = 1.0f;
 };
 
 template <typename T>

@@ -8,6 +8,7 @@
 // vector that other rows of the same kernel read.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 
 #include "precond.h"
@@ -330,8 +331,13 @@ __global__ __launch_bounds__(256) void k_restrict(CsrDev<T> R, const T* v, T* y,
         const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 5;
         const int lane = threadIdx.x & 31;
         T acc = T(0);
-        if (row < R.nrows)
-            for (int64_t p = R.rp[row] + lane; p < R.rp[row + 1]; p += 32) acc += R.v[p] * v[R.ci[p]];
+        if (row < R.nrows) {
+            if (R.hv) {
+                for (int64_t p = R.rp[row] + lane; p < R.rp[row + 1]; p += 32) acc += (T)R.hv[p] * v[R.ci[p]];
+                acc *= (T)R.hscale;
+            } else
+                for (int64_t p = R.rp[row] + lane; p < R.rp[row + 1]; p += 32) acc += R.v[p] * v[R.ci[p]];
+        }
         for (int o = 16; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 32);
         if (row < R.nrows && lane == 0) y[row] = acc;
     } else {
@@ -340,7 +346,11 @@ __global__ __launch_bounds__(256) void k_restrict(CsrDev<T> R, const T* v, T* y,
         const int j = (int)(t % NC);
         if (row >= R.nrows) return;
         T acc = T(0);
-        for (int64_t p = R.rp[row]; p < R.rp[row + 1]; ++p) acc += R.v[p] * v[(int64_t)R.ci[p] * NC + j];
+        if (R.hv) {
+            for (int64_t p = R.rp[row]; p < R.rp[row + 1]; ++p) acc += (T)R.hv[p] * v[(int64_t)R.ci[p] * NC + j];
+            acc *= (T)R.hscale;
+        } else
+            for (int64_t p = R.rp[row]; p < R.rp[row + 1]; ++p) acc += R.v[p] * v[(int64_t)R.ci[p] * NC + j];
         y[t] = acc;
     }
 }
@@ -354,6 +364,7 @@ __global__ __launch_bounds__(256) void k_prol(Sell<T> P, const T* xc, const T* d
     AMG_ROW
     T px[1];
     sell_row<T, 1, NC, 4>(P, row, (int)(i % NC), xc, px, dc);
+    px[0] *= (T)P.hscale;  // 1 unless the values are stored scaled in half precision
     x[i] += d ? d[i] + px[0] : px[0];
 }
 
@@ -423,7 +434,10 @@ static S* amg_copy(AmgDev<T>& D, const std::vector<S>& v) {
 
 // dsq non-null: half-precision values of D^-1/2 A D^-1/2 with dsq = D^1/2.
 template <typename T>
-static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<double>* dsq = nullptr) {
+// dsq: store D^-1/2 A D^-1/2 in half precision; half: store A's values
+// divided by their largest magnitude in half precision (hscale keeps it).
+static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<double>* dsq = nullptr,
+                       bool half = false) {
     Sell<T> s;
     s.bs = bs;
     s.nrows = A.nrows / bs;
@@ -450,8 +464,14 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<dou
         off[sl + 1] = off[sl] + (int64_t)w * 64;
     }
     std::vector<int> col(off[s.nslices], 0);
-    std::vector<T> val(dsq ? 0 : off[s.nslices] * bs * bs, T(0));
-    std::vector<_Float16> hval(dsq ? off[s.nslices] * bs * bs : 0, (_Float16)0.0f);
+    const bool h16 = dsq || half;
+    double amax = 0.0;
+    if (half)
+        for (double v : A.v) amax = std::max(amax, std::fabs(v));
+    if (half && amax > 0.0) s.hscale = (float)amax;
+    const double hinv = half && amax > 0.0 ? 1.0 / amax : 1.0;
+    std::vector<T> val(h16 ? 0 : off[s.nslices] * bs * bs, T(0));
+    std::vector<_Float16> hval(h16 ? off[s.nslices] * bs * bs : 0, (_Float16)0.0f);
     for (int64_t br = 0; br < s.nrows; ++br) {
         const int64_t sl = br >> 6, l = br & 63;
         const std::vector<int>& v = bc[br];
@@ -462,6 +482,8 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<dou
                 const int64_t at = (off[sl] + j * 64) * bs * bs + (k * bs + A.ci[p] % bs) * 64 + l;
                 if (dsq)
                     hval[at] = (_Float16)(A.v[p] / ((*dsq)[br * bs + k] * (*dsq)[A.ci[p]]));
+                else if (half)
+                    hval[at] = (_Float16)(A.v[p] * hinv);
                 else
                     val[at] = (T)A.v[p];
             }
@@ -469,7 +491,7 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<dou
     s.off = amg_copy(D, off);
     s.width = amg_copy(D, width);
     s.col = amg_copy(D, col);
-    if (dsq)
+    if (h16)
         s.hval = amg_copy(D, hval);
     else
         s.val = amg_copy(D, val);
@@ -477,14 +499,34 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<dou
 }
 
 template <typename T>
-static CsrDev<T> to_csr(AmgDev<T>& D, const Csr& A) {
+static CsrDev<T> to_csr(AmgDev<T>& D, const Csr& A, bool half = false) {
     CsrDev<T> c;
     c.nrows = A.nrows;
     c.rp = amg_copy(D, A.rp);
     c.ci = amg_copy(D, A.ci);
-    std::vector<T> v(A.v.begin(), A.v.end());
-    c.v = amg_copy(D, v);
+    if (half) {
+        double amax = 0.0;
+        for (double x : A.v) amax = std::max(amax, std::fabs(x));
+        const double hinv = amax > 0.0 ? 1.0 / amax : 1.0;
+        c.hscale = amax > 0.0 ? (float)amax : 1.0f;
+        std::vector<_Float16> v(A.v.size());
+        for (size_t i = 0; i < v.size(); ++i) v[i] = (_Float16)(A.v[i] * hinv);
+        c.hv = amg_copy(D, v);
+    } else {
+        std::vector<T> v(A.v.begin(), A.v.end());
+        c.v = amg_copy(D, v);
+    }
     return c;
+}
+
+// Half-precision storage (values divided by the largest magnitude) needs
+// finite values; entries below 6e-5 of the largest lose relative precision,
+// which changes the preconditioner by less than that.
+static bool half_safe(const std::vector<const Csr*>& ms) {
+    for (const Csr* m : ms)
+        for (double v : m->v)
+            if (!std::isfinite(v)) return false;
+    return true;
 }
 
 template <typename T>
@@ -513,8 +555,15 @@ AmgDev<T>* amg_upload(const Amg& H) {
                 for (int64_t i = 0; i < P.nrows; ++i)
                     for (int64_t p = P.rp[i]; p < P.rp[i + 1]; ++p) P.v[p] *= dsq[i];
                 for (int64_t p = 0; p < R.nnz(); ++p) R.v[p] *= dsq[R.ci[p]];
-                d.P = to_sell(*D, P, 1);
-                d.R = to_csr(*D, R);
+                // the transfers in half precision too where their values fit
+                // (VEMD_MG_HALF_TRANSFER=0 keeps them in FP32)
+                const char* ht = std::getenv("VEMD_MG_HALF_TRANSFER");
+                const bool htr = !(ht && ht[0] == '0') && half_safe({&P, &R});
+                d.P = to_sell(*D, P, 1, nullptr, htr);
+                d.R = to_csr(*D, R, htr);
+                if (std::getenv("VEMD_VERBOSE"))
+                    std::fprintf(stderr, "[vemd] finest-level transfers of the FP32 cycle in %s\n",
+                                 htr ? "half precision" : "FP32 (values outside half precision's range)");
                 std::vector<T> sc(dsq.size());
                 for (size_t i = 0; i < dsq.size(); ++i) sc[i] = (T)(1.0 / dsq[i]);
                 d.sc = amg_copy(*D, sc);
