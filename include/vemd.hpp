@@ -134,6 +134,16 @@ public:
     const vemd_solve_info& info() const { return info_; }
     double computeL2ErrorPressure(int nq = 9) { return errors(nq)[0]; }
     double computeL2ErrorFlux(int nq = 9) { return errors(nq)[1]; }
+    // The error fields of the reference GPU driver's report.json
+    // (GPU/src/solver_driver.cpp:340-351): scalar_comp, flux_comp, scalar_phys,
+    // flux_phys, div_phys, mass.
+    struct Report {
+        double scalar_comp, flux_comp, scalar_phys, flux_phys, div_phys, mass;
+    };
+    Report computeErrorReport(int nq = 9) {
+        const std::vector<double> e = errors(nq);
+        return Report{e[0], e[1], e[2], e[3], e[4], e[5]};
+    }
 
 private:
     vemd_solver s_ = nullptr;
@@ -145,8 +155,8 @@ private:
     std::vector<double> solution_;
     std::vector<double> errors(int nq) {
         if (solution_.empty()) throw std::runtime_error("errors require a completed numerical solution");
-        std::vector<double> e(4);
-        check(vemd_errors(s_, solution_.data(), nq, e.data()));
+        std::vector<double> e(6);
+        check(vemd_errors_full(s_, solution_.data(), nq, e.data()));
         return e;
     }
 };
