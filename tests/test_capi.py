@@ -118,6 +118,25 @@ def test_errors_on_curved_cells_match_oracle(vemd, oracle, k):
     assert eo["div_phys"] > 0 and abs(eo["mass"]) > 0
 
 
+@pytest.mark.parametrize("k", [0, 1])
+def test_edge_side_table(vemd, k):
+    """The side table of the one-launch apply (built with the layout, on the
+    host): every edge has a real first side, exactly the boundary edges have
+    the zero slot as second side, and the real slots are distinct and as many
+    as the cells' edges."""
+    m = vemd.generate_mesh(500, domain=vemd.DOMAIN_ANNULUS, r_inner=0.5, lloyd_iters=10, seed=31,
+                           min_edge_frac=0.05)
+    occ = m.arrays()["edge_occ"]
+    s = vemd.Solver(m, k=k, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P)
+    nside, t = s.edge_sides()
+    zero = nside - 1
+    assert t.shape == (2 * m.num_edges,) and t.min() >= 0 and t.max() <= zero
+    assert (t[0::2] < zero).all()
+    assert (t[1::2] == zero).sum() == (occ == 1).sum()
+    real = t[t < zero]
+    assert len(real) == occ.sum() == len(np.unique(real))
+
+
 def test_errors_reported_not_raised(vemd):
     with pytest.raises(vemd.VemdError) as ei:
         vemd.Mesh.read("/nonexistent/mesh.msh")
