@@ -512,20 +512,25 @@ template <typename T>
 __global__ void k_finA(MState* sts, const double* pdot, const double* plam, const int* inst_tile0, T* Az,
                        const T* Z0, const T* Z1, int64_t lam_idx) {
     MState* st = sts + blockIdx.x;
-    if (st->done) return;
+    // the state, the stop flag and the partials are loaded together (one
+    // round trip); a finished instance leaves after the sums (a uniform exit)
+    MState sv;
+    if (threadIdx.x == 0) sv = *st;
+    const int done = st->done;
     double zaz, yl;
     block_sums(pdot, lam_idx >= 0 ? plam : nullptr, inst_tile0[blockIdx.x], inst_tile0[blockIdx.x + 1], zaz, yl);
+    if (done) return;
     if (threadIdx.x == 0) {
         if (lam_idx >= 0) {
-            const T* z = (st->itn & 1) ? Z1 : Z0;
+            const T* z = (sv.itn & 1) ? Z1 : Z0;
             Az[lam_idx] = (T)yl;
             zaz += (double)z[lam_idx] * yl;
         }
-        double beta = st->beta;
-        double alfa = zaz / (beta * beta) - (st->itn > 0 ? st->rho / st->oldb : 0.0);
+        double beta = sv.beta;
+        double alfa = zaz / (beta * beta) - (sv.itn > 0 ? sv.rho / sv.oldb : 0.0);
         st->alfa = alfa;
         st->c_az = 1.0 / beta;
-        st->c_r1 = st->itn > 0 ? beta / st->oldb : 0.0;
+        st->c_r1 = sv.itn > 0 ? beta / sv.oldb : 0.0;
         st->c_r2 = alfa / beta;
     }
 }
@@ -656,7 +661,10 @@ __global__ __launch_bounds__(256) void k_pdot(UpdArgs<T> a, const int* pchunks, 
 __global__ void k_finB(MState* sts, const double* p1, const double* p2, const double* q1, const double* q2,
                        const int* inst_chunk0, int useq, int* all_done) {
     MState* st = sts + blockIdx.x;
-    if (st->done) return;
+    // as in k_finA: state, stop flag and partials in one round trip
+    MState sv;
+    if (threadIdx.x == 0) sv = *st;
+    const int done = st->done;
     const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
     double b2, rho, qb2, qrho;
     {
@@ -668,32 +676,35 @@ __global__ void k_finB(MState* sts, const double* p1, const double* p2, const do
         qb2 = out[2];
         qrho = out[3];
     }
+    if (done) return;
     b2 += qb2;
     rho += qrho;
     if (threadIdx.x == 0) {
+        MState& s = sv;
         double beta_new = std::sqrt(b2 > 0.0 ? b2 : 0.0);
-        st->rho = rho;
-        st->oldb = st->beta;
-        st->beta = beta_new;
-        double oldeps = st->epsln;
-        double delta = st->cs * st->dbar + st->sn * st->alfa;
-        double gbar = st->sn * st->dbar - st->cs * st->alfa;
-        st->epsln = st->sn * beta_new;
-        st->dbar = -st->cs * beta_new;
+        s.rho = rho;
+        s.oldb = s.beta;
+        s.beta = beta_new;
+        double oldeps = s.epsln;
+        double delta = s.cs * s.dbar + s.sn * s.alfa;
+        double gbar = s.sn * s.dbar - s.cs * s.alfa;
+        s.epsln = s.sn * beta_new;
+        s.dbar = -s.cs * beta_new;
         double gamma = std::sqrt(gbar * gbar + beta_new * beta_new);
         if (gamma < 1e-300) gamma = 1e-300;
-        st->cs = gbar / gamma;
-        st->sn = beta_new / gamma;
-        double phi = st->cs * st->phibar;
-        st->phibar = st->sn * st->phibar;
-        st->u_v = 1.0 / (st->oldb * gamma);
-        st->u_w1 = oldeps / gamma;
-        st->u_w2 = delta / gamma;
-        st->u_x = phi;
-        st->pending = 1;
-        st->itn += 1;
-        if (st->phibar <= st->tol || st->itn >= st->maxit || !(beta_new > 0.0)) st->done = 1;
-        if (all_done) *all_done = st->done;
+        s.cs = gbar / gamma;
+        s.sn = beta_new / gamma;
+        double phi = s.cs * s.phibar;
+        s.phibar = s.sn * s.phibar;
+        s.u_v = 1.0 / (s.oldb * gamma);
+        s.u_w1 = oldeps / gamma;
+        s.u_w2 = delta / gamma;
+        s.u_x = phi;
+        s.pending = 1;
+        s.itn += 1;
+        if (s.phibar <= s.tol || s.itn >= s.maxit || !(beta_new > 0.0)) s.done = 1;
+        *st = s;
+        if (all_done) *all_done = s.done;
     }
 }
 
