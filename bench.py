@@ -355,7 +355,7 @@ def multi_bench(pkg, torch, cells, seed, nrhs):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=1000000)
     ap.add_argument("--seed", type=int, default=2)
@@ -453,9 +453,13 @@ def main():
         return dict(iterations=info.iterations, refinements=info.refinements, rel_residual=info.rel_residual,
                     seconds=info.seconds)
 
+    # both precisions warmed up (each captures its own iteration graphs on its
+    # first solve) and timed over the same number of solves
     for _ in range(args.warmup):
         i = solve(pkg.MIXED, xm)
         log("warmup mixed: %s" % i)
+        i = solve(pkg.FP64, x64)
+        log("warmup fp64: %s" % i)
     torch.cuda.synchronize()
     t0 = time.time()
     infos = [solve(pkg.MIXED, xm) for _ in range(args.steps)]
@@ -464,10 +468,11 @@ def main():
     log("mixed: %.3f s per solve, %s" % (t_mixed, infos[-1]))
     torch.cuda.synchronize()
     t0 = time.time()
-    i64 = solve(pkg.FP64, x64)
+    i64s = [solve(pkg.FP64, x64) for _ in range(args.steps)]
     torch.cuda.synchronize()
-    t_fp64 = time.time() - t0
-    log("fp64: %.3f s, %s" % (t_fp64, i64))
+    t_fp64 = (time.time() - t0) / args.steps
+    i64 = i64s[-1]
+    log("fp64: %.3f s per solve, %s" % (t_fp64, i64))
     # accuracy of the mixed solution against the FP64 one
     xr_m, xr_64 = np.zeros(s.ntot), np.zeros(s.ntot)
     hm, h64 = xm.cpu().numpy(), x64.cpu().numpy()
