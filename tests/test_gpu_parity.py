@@ -4,7 +4,7 @@ import json
 import numpy as np
 import pytest
 
-from conftest import check_phys, golden
+from conftest import PHYS, ROOT, check_phys, golden
 
 pytestmark = pytest.mark.gpu
 
@@ -252,3 +252,32 @@ def test_apply_autotuner(vemd, oracle, monkeypatch):
     x = np.random.default_rng(4).standard_normal(s.ntot)
     for prec in (vemd.FP64, vemd.FP32):
         assert np.array_equal(s.apply(x, prec), s0.apply(x, prec))
+
+
+def test_cpp_driver_reports_reference_errors():
+    """The C++ drop-in (include/vemd.hpp: the reference's class names and call
+    order) through its driver vemd_darcy, example 1, k = 1, on square_2x2 and
+    square_4x4: the computational and physical errors, the divergence error
+    and the mass it prints against the reference reports' 16-digit values."""
+    import os
+    import subprocess
+    cases = {}
+    for c in PHYS:
+        if c["example"] == 1 and c["mesh"] in ("square_2x2.msh", "square_4x4.msh"):
+            cases.setdefault(c["mesh"], c)
+    meshes = sorted(cases)
+    exe = os.path.join(ROOT, "wsl-code_amd", "vemd_darcy")
+    print("running", exe, meshes, flush=True)
+    out = subprocess.run([exe, "--example", "1", "--k", "1", "--report"] + [golden(m) for m in meshes],
+                         capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr
+    reports = [ln.split() for ln in out.stdout.splitlines() if ln.startswith("report ")]
+    assert len(reports) == len(meshes), out.stdout
+    for m, r in zip(meshes, reports):
+        f = dict(zip(r[1::2], map(float, r[2::2])))
+        c = cases[m]
+        e = dict(p_comp=f["scalar_comp"], u_comp=f["flux_comp"], p_phys=f["scalar_phys"], u_phys=f["flux_phys"],
+                 div_phys=f["div_phys"], mass=f["mass"])
+        for key in ("p_comp", "u_comp"):
+            assert abs(e[key] - c[key]) <= 5e-10 * c[key], (m, key, e[key], c[key])
+        check_phys(e, c["phys"])

@@ -2,11 +2,13 @@
 // following the reference's CPU/main/Darcy_curved_main.cpp:48-93 (read mesh,
 // set up the problem, construct the solver, solve, report L2 errors and orders).
 //
-//   vemd_darcy [--example 1|2] [--k K] [--nq2 N] [--nq1 N] mesh1.msh [mesh2.msh ...]
+//   vemd_darcy [--example 1|2] [--k K] [--nq2 N] [--nq1 N] [--report] mesh1.msh [mesh2.msh ...]
 //
 // Prints one line per mesh: cells, dofs, pressure and flux L2 errors (in the
 // computational measure, as the reference), observed orders, iterations and
-// solve seconds.
+// solve seconds. With --report, one more line per mesh with the error fields
+// of the reference GPU driver's report (physical-domain pressure, flux and
+// divergence errors, mass).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +20,7 @@
 
 int main(int argc, char** argv) {
     int example = 1, k = 1, nq2 = 9, nq1 = 9;
+    bool report = false;
     std::vector<std::string> meshes;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -29,11 +32,13 @@ int main(int argc, char** argv) {
             nq2 = std::atoi(argv[++i]);
         else if (a == "--nq1" && i + 1 < argc)
             nq1 = std::atoi(argv[++i]);
+        else if (a == "--report")
+            report = true;
         else
             meshes.push_back(a);
     }
     if (meshes.empty()) {
-        std::fprintf(stderr, "usage: vemd_darcy [--example 1|2] [--k K] [--nq2 N] [--nq1 N] mesh.msh ...\n");
+        std::fprintf(stderr, "usage: vemd_darcy [--example 1|2] [--k K] [--nq2 N] [--nq1 N] [--report] mesh.msh ...\n");
         return 2;
     }
     try {
@@ -64,6 +69,12 @@ int main(int argc, char** argv) {
             std::printf("%-28s %8lld %10lld %16.10e %7s %16.10e %7s %7d %9.4f\n", path.substr(path.rfind('/') + 1).c_str(),
                         (long long)counts[1], (long long)(solver.getTotalDof() + 1), ep, op.c_str(), eu, ou.c_str(),
                         solver.info().iterations, solver.info().seconds);
+            if (report) {
+                const vemd::DarcySolver::Report r = solver.computeErrorReport(nq2);
+                std::printf("report scalar_comp %.16e flux_comp %.16e scalar_phys %.16e flux_phys %.16e div_phys %.16e "
+                            "mass %.3e\n",
+                            r.scalar_comp, r.flux_comp, r.scalar_phys, r.flux_phys, r.div_phys, r.mass);
+            }
             prev_h = h;
             prev_p = ep;
             prev_u = eu;
