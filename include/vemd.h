@@ -55,11 +55,15 @@ enum { VEMD_MESH_MSH = 0, VEMD_MESH_VTK = 1 };
  * conventions (CCW cells, quad centroid = bounding-box centre, others = vertex
  * average, diameter = max vertex distance, edges numbered by sorted node pair).
  * Replaces vem::StraightMeshReader::read_mesh / read_mesh_vtk
- * (CPU/mesh/straight_mesh.cpp:10-41,64-90). */
+ * (CPU/mesh/straight_mesh.cpp:10-41,64-90).  Cells with a non-finite vertex,
+ * a repeated vertex or zero area are rejected with VEMD_ERR_MESH (the
+ * reference's reader does not check for them). */
 vemd_status vemd_mesh_read(const char* path, int format, vemd_mesh* out);
 
 /* Builds a mesh from arrays: cell c has nodes cell_nodes[cell_off[c] ..
- * cell_off[c+1]).  Optional circular boundary arcs (centre at the origin):
+ * cell_off[c+1]); cell_off[0] = 0, offsets do not decrease, and cell_nodes
+ * holds cell_off[num_cells] entries.  Cells are checked as in vemd_mesh_read
+ * and clockwise cells reordered.  Optional circular boundary arcs (centre at the origin):
  * arc_nodes holds (n0, n1) per arc, arc_radius its radius; the straight edge
  * n0-n1 is the chord.  Replaces the same reader's post-processing chain
  * (CPU/mesh/straight_mesh.cpp:504-660) for meshes made in memory. */

@@ -150,6 +150,56 @@ def test_errors_reported_not_raised(vemd):
     assert ei.value.status == "ERR_ARG"
 
 
+def test_malformed_inputs_rejected(vemd, tmp_path):
+    """Inputs the element setup cannot integrate, and arrays whose lengths do
+    not match what the C ABI reads, fail with a status instead of reading out
+    of bounds or producing NaN operators."""
+    xy = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+    one = np.array([0, 4], np.int32)
+
+    def status(f):
+        with pytest.raises(vemd.VemdError) as ei:
+            f()
+        return ei.value.status
+
+    assert status(lambda: vemd.Mesh.from_arrays(xy, one, np.array([0, 1, 2, 7], np.int32))) == "ERR_MESH"
+    assert status(lambda: vemd.Mesh.from_arrays(xy, one, np.array([0, 1, 2, -1], np.int32))) == "ERR_MESH"
+    assert status(lambda: vemd.Mesh.from_arrays(xy, np.array([0, 2], np.int32), np.array([0, 1], np.int32))) == "ERR_MESH"
+    assert status(lambda: vemd.Mesh.from_arrays(xy, one, np.array([0, 1, 1, 3], np.int32))) == "ERR_MESH"
+    assert status(lambda: vemd.Mesh.from_arrays(np.array([[0, 0], [1, 0], [2, 0], [3, 0]], float), one,
+                                                np.arange(4, dtype=np.int32))) == "ERR_MESH"  # zero area
+    bad = xy.copy()
+    bad[2, 1] = np.nan
+    assert status(lambda: vemd.Mesh.from_arrays(bad, one, np.arange(4, dtype=np.int32))) == "ERR_MESH"
+    # offsets past the end of cell_nodes, and decreasing offsets
+    assert status(lambda: vemd.Mesh.from_arrays(xy, np.array([0, 5], np.int32), np.arange(4, dtype=np.int32))) == "ERR_ARG"
+    assert status(lambda: vemd.Mesh.from_arrays(xy, np.array([0, 4, 2], np.int32), np.arange(4, dtype=np.int32))) == "ERR_ARG"
+    # a clockwise cell is reordered, as the reference's readers do
+    cw = vemd.Mesh.from_arrays(xy, one, np.array([3, 2, 1, 0], np.int32))
+    ccw = vemd.Mesh.from_arrays(xy, one, np.arange(4, dtype=np.int32))
+    assert np.allclose(cw.arrays()["area"], ccw.arrays()["area"])
+    e = vemd.Solver(ccw, k=1).element(0)
+    assert np.allclose(vemd.Solver(cw, k=1).element(0)["A"], e["A"])
+    # Gmsh 4.1 file with one quad: valid, then with a repeated node
+    def msh(quad):
+        return ("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Nodes\n1 4 1 4\n2 1 0 4\n1\n2\n3\n4\n0 0 0\n1 0 0\n"
+                "1 1 0\n0 1 0\n$EndNodes\n$Elements\n1 1 1 1\n2 1 3 1\n1 %s\n$EndElements\n" % quad)
+    (tmp_path / "ok.msh").write_text(msh("1 2 3 4"))
+    assert vemd.Mesh.read(tmp_path / "ok.msh").num_cells == 1
+    (tmp_path / "rep.msh").write_text(msh("1 2 2 4"))
+    assert status(lambda: vemd.Mesh.read(tmp_path / "rep.msh")) == "ERR_MESH"
+    m = vemd.Mesh.read(golden("square_2x2.msh"))
+    assert status(lambda: vemd.Solver(m, k=1, theta=np.zeros(3))) == "ERR_ARG"
+    theta = np.zeros(m.num_cells)
+    theta[1] = np.inf
+    assert status(lambda: vemd.Solver(m, k=1, theta=theta, bc=vemd.BC_DIRICHLET_P)) == "ERR_ARG"
+    s = vemd.Solver(m, k=1, bc=vemd.BC_DIRICHLET_P)
+    assert status(lambda: s.set_theta(np.zeros(m.num_cells + 1))) == "ERR_ARG"
+    assert status(lambda: s.set_theta(theta)) == "ERR_ARG"
+    assert status(lambda: s.apply(np.zeros(s.ntot - 1))) == "ERR_ARG"
+    assert status(lambda: s.errors(np.zeros(s.ntot + 1))) == "ERR_ARG"
+
+
 def test_solve_without_device_fails_loudly(vemd):
     if vemd.device_count() > 0:
         pytest.skip("a HIP device is present")

@@ -704,6 +704,9 @@ Setup* setup_prepare(const Mesh& m, const vemd_problem_desc& d) {
     if (d.map_kind != VEMD_MAP_SIN && d.map_kind != VEMD_MAP_ARCS) throw Error(VEMD_ERR_ARG, "map_kind");
     if (d.bc != VEMD_BC_FLUX_MEANZERO && d.bc != VEMD_BC_DIRICHLET_P) throw Error(VEMD_ERR_ARG, "bc");
     (void)tri_rule(d.nq2d);  // validates nq2d
+    if (d.theta)
+        for (int c = 0; c < m.nc; ++c)
+            if (!std::isfinite(d.theta[c])) throw Error(VEMD_ERR_ARG, "theta must be finite");
     std::vector<double> glx, glw;
     gauss_legendre(d.nq1d, glx, glw);
     Setup* S = new Setup();

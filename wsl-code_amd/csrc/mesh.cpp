@@ -5,20 +5,26 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <memory>
 #include <sstream>
 
 #include "common.h"
 
 namespace vemd {
 
-static void make_ccw(std::vector<int>& v, const std::vector<double>& xy) {
+// Twice the signed (shoelace) area of the polygon v.
+static double shoelace2(const std::vector<int>& v, const std::vector<double>& xy) {
     double a = 0.0;
     int n = (int)v.size();
     for (int i = 0; i < n; ++i) {
         int j = (i + 1) % n;
         a += xy[2 * v[i]] * xy[2 * v[j] + 1] - xy[2 * v[j]] * xy[2 * v[i] + 1];
     }
-    if (a < 0) std::reverse(v.begin(), v.end());
+    return a;
+}
+
+static void make_ccw(std::vector<int>& v, const std::vector<double>& xy) {
+    if (shoelace2(v, xy) < 0) std::reverse(v.begin(), v.end());
 }
 
 void mesh_finalize(Mesh& m) {
@@ -97,7 +103,20 @@ void mesh_finalize(Mesh& m) {
         }
 }
 
+// Appends one cell (vertex indices already checked against the node count),
+// counter-clockwise as the reference's readers store it
+// (CPU/mesh/straight_mesh.cpp:462-482).  Rejects what the element setup
+// cannot integrate: non-finite coordinates, a repeated vertex (a zero-length
+// edge or a pinched polygon) and zero area.
 static void push_cell(Mesh& m, std::vector<int> v) {
+    const std::string cell = "cell " + std::to_string(m.off.size() - 1);
+    for (int a : v)
+        if (!std::isfinite(m.xy[2 * a]) || !std::isfinite(m.xy[2 * a + 1]))
+            throw Error(VEMD_ERR_MESH, cell + " has a non-finite vertex coordinate");
+    std::vector<int> u(v);
+    std::sort(u.begin(), u.end());
+    if (std::adjacent_find(u.begin(), u.end()) != u.end()) throw Error(VEMD_ERR_MESH, cell + " repeats a vertex");
+    if (shoelace2(v, m.xy) == 0.0) throw Error(VEMD_ERR_MESH, cell + " has zero area");
     make_ccw(v, m.xy);
     m.nodes.insert(m.nodes.end(), v.begin(), v.end());
     m.off.push_back((int)m.nodes.size());
@@ -106,7 +125,7 @@ static void push_cell(Mesh& m, std::vector<int> v) {
 Mesh* mesh_read_msh(const char* path) {
     std::ifstream f(path);
     if (!f) throw Error(VEMD_ERR_IO, std::string("cannot open ") + path);
-    Mesh* m = new Mesh();
+    std::unique_ptr<Mesh> m(new Mesh());
     std::string line;
     bool found = false;
     while (std::getline(f, line))
@@ -115,14 +134,12 @@ Mesh* mesh_read_msh(const char* path) {
             break;
         }
     if (!found) {
-        delete m;
         throw Error(VEMD_ERR_IO, "$Nodes section not found");
     }
     int nblocks = 0, nn = 0;
     std::getline(f, line);
     std::istringstream(line) >> nblocks >> nn;
     if (nn <= 0) {
-        delete m;
         throw Error(VEMD_ERR_IO, "invalid number of nodes");
     }
     m->nn = nn;
@@ -156,7 +173,6 @@ Mesh* mesh_read_msh(const char* path) {
             break;
         }
     if (!found) {
-        delete m;
         throw Error(VEMD_ERR_IO, "$Elements section not found");
     }
     std::getline(f, line);
@@ -180,7 +196,6 @@ Mesh* mesh_read_msh(const char* path) {
                 es >> v[j];
                 v[j] -= 1;
                 if (v[j] < 0 || v[j] >= nn) {
-                    delete m;
                     throw Error(VEMD_ERR_IO, "element references a missing node");
                 }
             }
@@ -188,11 +203,10 @@ Mesh* mesh_read_msh(const char* path) {
         }
     }
     if (m->off.size() < 2) {
-        delete m;
         throw Error(VEMD_ERR_IO, "no polygon elements");
     }
     mesh_finalize(*m);
-    return m;
+    return m.release();
 }
 
 Mesh* mesh_read_vtk(const char* path) {
@@ -204,23 +218,20 @@ Mesh* mesh_read_vtk(const char* path) {
             if (t == k) return true;
         return false;
     };
-    Mesh* m = new Mesh();
+    std::unique_ptr<Mesh> m(new Mesh());
     std::string dt;
     if (!seek("POINTS") || !(f >> m->nn >> dt) || m->nn <= 0) {
-        delete m;
         throw Error(VEMD_ERR_IO, "invalid POINTS section");
     }
     m->xy.assign(2 * (size_t)m->nn, 0.0);
     for (int i = 0; i < m->nn; ++i) {
         double z;
         if (!(f >> m->xy[2 * i] >> m->xy[2 * i + 1] >> z)) {
-            delete m;
             throw Error(VEMD_ERR_IO, "truncated POINTS section");
         }
     }
     int nc = 0, tot = 0;
     if (!seek("CELLS") || !(f >> nc >> tot) || nc <= 0) {
-        delete m;
         throw Error(VEMD_ERR_IO, "invalid CELLS section");
     }
     m->off.push_back(0);
@@ -228,14 +239,12 @@ Mesh* mesh_read_vtk(const char* path) {
         int nv = 0;
         f >> nv;
         if (nv < 3) {
-            delete m;
             throw Error(VEMD_ERR_IO, "cell with fewer than 3 vertices");
         }
         std::vector<int> v(nv);
         for (int i = 0; i < nv; ++i) {
             f >> v[i];
             if (v[i] < 0 || v[i] >= m->nn) {
-                delete m;
                 throw Error(VEMD_ERR_IO, "cell references a missing node");
             }
         }
@@ -243,19 +252,17 @@ Mesh* mesh_read_vtk(const char* path) {
     }
     int nt = 0;
     if (!seek("CELL_TYPES") || !(f >> nt) || nt != nc) {
-        delete m;
         throw Error(VEMD_ERR_IO, "CELL_TYPES missing or count mismatch");
     }
     for (int c = 0; c < nc; ++c) {
         int t = 0;
         f >> t;
         if (t != 5 && t != 7 && t != 9) {
-            delete m;
             throw Error(VEMD_ERR_IO, "unsupported VTK cell type");
         }
     }
     mesh_finalize(*m);
-    return m;
+    return m.release();
 }
 
 Mesh* mesh_concat(const std::vector<const Mesh*>& parts) {
@@ -291,27 +298,28 @@ Mesh* mesh_concat(const std::vector<const Mesh*>& parts) {
 Mesh* mesh_from_arrays(int nn, const double* xy, int nc, const int* off, const int* nodes, int narc,
                        const int* arc_nodes, const double* arc_r) {
     if (nn <= 0 || nc <= 0 || !xy || !off || !nodes) throw Error(VEMD_ERR_ARG, "empty mesh arrays");
-    Mesh* m = new Mesh();
+    if (off[0] != 0) throw Error(VEMD_ERR_ARG, "cell offsets must start at 0");
+    if (narc < 0 || (narc > 0 && (!arc_nodes || !arc_r))) throw Error(VEMD_ERR_ARG, "arc arrays");
+    std::unique_ptr<Mesh> m(new Mesh());
     m->nn = nn;
     m->xy.assign(xy, xy + 2 * (size_t)nn);
     m->off.push_back(0);
     for (int c = 0; c < nc; ++c) {
         int n = off[c + 1] - off[c];
-        if (n < 3) {
-            delete m;
-            throw Error(VEMD_ERR_MESH, "cell with fewer than 3 vertices");
-        }
+        if (n < 3) throw Error(VEMD_ERR_MESH, "cell with fewer than 3 vertices");
         std::vector<int> v(nodes + off[c], nodes + off[c + 1]);
         for (int a : v)
-            if (a < 0 || a >= nn) {
-                delete m;
-                throw Error(VEMD_ERR_MESH, "cell references a missing node");
-            }
+            if (a < 0 || a >= nn) throw Error(VEMD_ERR_MESH, "cell references a missing node");
         push_cell(*m, v);
     }
     mesh_finalize(*m);
-    for (int a = 0; a < narc; ++a) m->arcs[Mesh::key(arc_nodes[2 * a], arc_nodes[2 * a + 1])] = arc_r[a];
-    return m;
+    for (int a = 0; a < narc; ++a) {
+        const int i = arc_nodes[2 * a], j = arc_nodes[2 * a + 1];
+        if (i < 0 || i >= nn || j < 0 || j >= nn) throw Error(VEMD_ERR_MESH, "arc references a missing node");
+        if (!std::isfinite(arc_r[a])) throw Error(VEMD_ERR_MESH, "non-finite arc radius");
+        m->arcs[Mesh::key(i, j)] = arc_r[a];
+    }
+    return m.release();
 }
 
 }  // namespace vemd

@@ -4,7 +4,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import MeshOpts, c_dp, c_ip, c_vp, check, lib, DOMAIN_DISK
+from ._lib import MeshOpts, VemdError, c_dp, c_ip, c_vp, check, lib, DOMAIN_DISK
 
 
 def _d(a):
@@ -50,7 +50,13 @@ class Mesh:
         if arc_nodes is None:
             arc_nodes, arc_radius = np.zeros(0, np.int32), np.zeros(0)
         an = np.ascontiguousarray(arc_nodes, np.int32).reshape(-1)
-        ar = np.ascontiguousarray(arc_radius, np.float64)
+        ar = np.ascontiguousarray(arc_radius, np.float64).reshape(-1)
+        # the C ABI reads cell_nodes up to cell_off[-1] and two nodes per arc
+        if len(xy) % 2 or off.ndim != 1 or len(off) < 2 or off[0] != 0 or np.any(np.diff(off) < 0) \
+                or off[-1] != len(nodes) or len(an) != 2 * len(ar):
+            raise VemdError(1, "inconsistent mesh arrays: %d coordinates, %d offsets ending at %s, %d cell nodes, "
+                               "%d arc nodes for %d radii" % (len(xy), len(off), off[-1] if len(off) else None,
+                                                               len(nodes), len(an), len(ar)))
         h = c_vp()
         check(lib().vemd_mesh_from_arrays(len(xy) // 2, _d(xy), len(off) - 1, _i(off), _i(nodes), len(ar), _i(an),
                                           _d(ar), ctypes.byref(h)))

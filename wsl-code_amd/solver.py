@@ -13,6 +13,15 @@ def _d(a):
     return a.ctypes.data_as(c_dp)
 
 
+def _vec(a, n, what):
+    """Contiguous float64 copy of a 1-D array of exactly n values: the C ABI
+    takes bare pointers and reads n values."""
+    a = np.ascontiguousarray(a, np.float64)
+    if a.shape != (n,):
+        raise VemdError(1, "%s: expected %d values, got shape %s" % (what, n, a.shape))
+    return a
+
+
 @dataclass
 class SolveOptions:
     precision: int = FP64
@@ -36,7 +45,7 @@ class Solver:
         """setup_device: build the element operators on that GPU instead of
         the host cores (k <= 1, polygons of up to 10 edges)."""
         self.mesh = mesh
-        self._theta = None if theta is None else np.ascontiguousarray(theta, np.float64)
+        self._theta = None if theta is None else _vec(theta, mesh.num_cells, "theta")
         d = ProblemDesc(k, nq2d, nq1d, map_kind, eps, solution, bc,
                         None if self._theta is None else _d(self._theta))
         self.h = c_vp()
@@ -73,7 +82,7 @@ class Solver:
         return int(n.value), t
 
     def set_theta(self, theta):
-        self._theta = np.ascontiguousarray(theta, np.float64)
+        self._theta = _vec(theta, self.mesh.num_cells, "theta")
         check(lib().vemd_set_theta(self.h, _d(self._theta)))
 
     def rhs(self):
@@ -82,7 +91,7 @@ class Solver:
         return b
 
     def apply(self, x, precision=FP64):
-        x = np.ascontiguousarray(x, np.float64)
+        x = _vec(x, self.ntot, "x")
         y = np.zeros(self.ntot)
         check(lib().vemd_apply(self.h, precision, _d(x), _d(y)))
         return y
@@ -93,7 +102,7 @@ class Solver:
         info = SolveInfo()
         rb = None
         if rhs is not None:
-            rhs = np.ascontiguousarray(rhs, np.float64)
+            rhs = _vec(rhs, self.ntot, "rhs")
             rb = _d(rhs)
         oc = opts.c()
         check(lib().vemd_solve(self.h, ctypes.byref(oc), rb, _d(x), ctypes.byref(info)))
@@ -132,8 +141,8 @@ class Solver:
 
     def adjoint_theta(self, x, dldx, opts=None):
         opts = opts or SolveOptions()
-        x = np.ascontiguousarray(x, np.float64)
-        g = np.ascontiguousarray(dldx, np.float64)
+        x = _vec(x, self.ntot, "x")
+        g = _vec(dldx, self.ntot, "dldx")
         grad = np.zeros(self.mesh.num_cells)
         info = SolveInfo()
         oc = opts.c()
@@ -144,7 +153,7 @@ class Solver:
         """L2 errors of a reference-order solution: pressure and flux in the
         computational and the physical measure, the physical divergence error
         and the mass int p_h det J."""
-        x = np.ascontiguousarray(x, np.float64)
+        x = _vec(x, self.ntot, "x")
         out = np.zeros(6)
         check(lib().vemd_errors_full(self.h, _d(x), nq, _d(out)))
         return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3], div_phys=out[4], mass=out[5])
