@@ -56,3 +56,19 @@ def check_phys(e, ph):
     for key in ("p_phys", "u_phys", "div_phys"):
         assert abs(e[key] - ph[key]) <= 5e-10 * ph[key], (key, e[key], ph[key])
     assert abs(e["mass"]) <= 1e-12 and abs(ph["mass"]) <= 1e-12
+
+
+def rosette():
+    """A regular 10-gon (the largest polygon of the device path) ringed by 20
+    triangles, tiling a regular 10-gon of radius 0.45 centred at (0.5, 0.5):
+    (xy, cell_off, cell_nodes)."""
+    import numpy as np
+    t = 2 * np.pi * np.arange(10) / 10
+    inner = np.stack([0.5 + 0.2 * np.cos(t), 0.5 + 0.2 * np.sin(t)], 1)
+    outer = np.stack([0.5 + 0.45 * np.cos(t + np.pi / 10), 0.5 + 0.45 * np.sin(t + np.pi / 10)], 1)
+    cells = [list(range(10))]
+    for i in range(10):
+        j = (i + 1) % 10
+        cells += [[i, j, 10 + i], [10 + i, j, 10 + j]]
+    off = np.cumsum([0] + [len(c) for c in cells]).astype(np.int32)
+    return np.concatenate([inner, outer]), off, np.concatenate(cells).astype(np.int32)

@@ -7,7 +7,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import PHYS, ROOT, check_phys, golden
+from conftest import rosette, PHYS, ROOT, check_phys, golden
 
 
 def declared_symbols():
@@ -148,6 +148,35 @@ def test_errors_reported_not_raised(vemd):
     with pytest.raises(vemd.VemdError) as ei:
         vemd.Solver(m, k=3, nq2d=9)
     assert ei.value.status == "ERR_ARG"
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_largest_polygon_setup_matches_oracle(vemd, oracle, k):
+    """A 10-gon ringed by triangles (both ends of the valence range): element
+    operators, lifted right-hand side and the direct solution's error
+    functionals of the product's setup against the oracle."""
+    xy, off, nodes = rosette()
+    m = vemd.Mesh.from_arrays(xy, off, nodes)
+    a = m.arrays()
+    assert abs(a["area"].sum() - 5 * 0.45 ** 2 * np.sin(np.pi / 5)) <= 1e-14
+    s = vemd.Solver(m, k=k, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P)
+    om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+    op = oracle.Problem(om, k=k, map_kind=oracle.MAP_ARCS, sol=oracle.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P)
+    for c in range(m.num_cells):
+        e = s.element(c)
+        n = e["n"]
+        assert n == (k + 1) * (10 if c == 0 else 3) + ((k + 1) * (k + 2) // 2 - 1) + (k * (k + 1) // 2)
+        d = op.element(c, 13)[:n]
+        Ao = op.element(c, 7, (n, n)) * np.outer(d, d)
+        assert np.abs(e["A"] - Ao).max() <= 1e-13 * np.abs(Ao).max()
+        assert np.abs(e["W"] - op.element(c, 2).reshape(-1, n) * d).max() <= 1e-13
+    A, rr = op.csr(True)
+    fi, _ = op.free()
+    assert np.abs(s.rhs()[fi] - rr).max() <= 1e-13 * np.abs(rr).max()
+    x = op.solve_direct()
+    eo, ep = op.errors(9), s.errors(x, 9)
+    for key in ("p_comp", "u_comp"):
+        assert abs(ep[key] - eo[key]) <= 1e-12 * abs(eo[key]), key
 
 
 def test_malformed_inputs_rejected(vemd, tmp_path):

@@ -4,7 +4,7 @@ import json
 import numpy as np
 import pytest
 
-from conftest import PHYS, ROOT, check_phys, golden
+from conftest import PHYS, ROOT, check_phys, golden, rosette
 
 pytestmark = pytest.mark.gpu
 
@@ -109,6 +109,39 @@ def test_generated_apply_and_solve(vemd, oracle, case):
     xo = op.solve_direct()
     xs, info = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12))
     assert np.linalg.norm(xs - xo) <= 1e-10 * np.linalg.norm(xo), info
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_largest_polygon_apply_and_solve(vemd, oracle, k):
+    """A 10-gon (the largest polygon of the device path, alone in its
+    valence group: one live cell in a 64-cell tile) ringed by 20 triangles:
+    the apply against the oracle's CSR, the FP64 solve against its direct
+    solve, and the mixed solve to the same true residual."""
+    print("rosette k", k, flush=True)
+    assert vemd.device_count() > 0, "no HIP device"
+    xy, off, nodes = rosette()
+    m = vemd.Mesh.from_arrays(xy, off, nodes)
+    a = m.arrays()
+    s = vemd.Solver(m, k=k, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P, device=0)
+    assert s.ngroups == 2 and s.max_ne == 10
+    om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+    op = oracle.Problem(om, k=k, map_kind=oracle.MAP_ARCS, sol=oracle.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P)
+    A, _ = op.csr(False)
+    fi, _ = op.free()
+    x = np.zeros(s.ntot)
+    x[fi] = np.random.default_rng(9).standard_normal(len(fi))
+    y_ref = A @ x
+    y = s.apply(x, vemd.FP64)
+    scale = np.abs(y_ref[fi]).max()
+    assert np.abs(y[fi] - y_ref[fi]).max() <= 1e-13 * scale
+    assert np.abs(s.apply(x, vemd.FP32)[fi] - y_ref[fi]).max() <= 2e-5 * scale
+    assert np.array_equal(s.apply(x, vemd.FP64), y)
+    xo = op.solve_direct()
+    xs, info = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12))
+    assert np.linalg.norm(xs - xo) <= 1e-10 * np.linalg.norm(xo), info
+    xm, im = s.solve(vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-8))
+    assert im["rel_residual"] <= 1e-8
+    assert np.linalg.norm(xm - xo) <= 1e-5 * np.linalg.norm(xo), im
 
 
 @pytest.mark.parametrize("cells", [3000, 12000])
