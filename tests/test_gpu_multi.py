@@ -12,13 +12,20 @@ pytestmark = pytest.mark.gpu
 NR = 32
 
 
-@pytest.fixture(scope="module")
-def setup(vemd):
+@pytest.fixture(scope="module", params=["annulus", "rosette"])
+def setup(vemd, request):
+    """A 3000-cell Voronoi annulus with a permeability field, and the 10-gon
+    rosette (the largest cell block of the MFMA apply: 26 rows, 348 floats)."""
     import torch
+    from conftest import rosette
     from wsl_code_amd.fields import grf_theta
     assert vemd.device_count() > 0, "no HIP device"
-    print("multi-rhs annulus 3000 cells", flush=True)
-    m = vemd.generate_mesh(3000, domain=vemd.DOMAIN_ANNULUS, r_inner=0.5, lloyd_iters=20, seed=41, min_edge_frac=0.05)
+    print("multi-rhs", request.param, flush=True)
+    if request.param == "annulus":
+        m = vemd.generate_mesh(3000, domain=vemd.DOMAIN_ANNULUS, r_inner=0.5, lloyd_iters=20, seed=41,
+                               min_edge_frac=0.05)
+    else:
+        m = vemd.Mesh.from_arrays(*rosette())
     a = m.arrays()
     th = grf_theta(a["cx"], a["cy"], sigma=1.0, ell=0.1, seed=42)
     s = vemd.Solver(m, k=1, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P, theta=th,
