@@ -8,7 +8,7 @@ library: relative 1e-11 per cell."""
 import numpy as np
 import pytest
 
-from conftest import golden
+from conftest import golden, rosette
 
 pytestmark = pytest.mark.gpu
 
@@ -19,11 +19,15 @@ CASES = [
     ("disk1500", 0, "arcs", 0.0, "sinsin", "dirichlet"),
     ("annulus2500", 1, "arcs", 0.0, "sinsin", "dirichlet"),
     ("annulus2500", 1, "arcs", 0.0, "sincos", "flux"),
+    ("rosette", 1, "sin", 0.05, "sincos", "flux"),  # a 10-gon: the largest polygon of the GPU setup
+    ("rosette", 0, "arcs", 0.0, "sinsin", "dirichlet"),
 ]
 IDS = ["%s-k%d-%s-%s" % (c[0], c[1], c[2], c[5]) for c in CASES]
 
 
 def _mesh(vemd, name):
+    if name == "rosette":
+        return vemd.Mesh.from_arrays(*rosette())
     if name.endswith(".msh"):
         return vemd.Mesh.read(golden(name))
     if name.startswith("disk"):
