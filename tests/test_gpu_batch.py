@@ -14,7 +14,7 @@ SIZES, SEEDS = [400, 1500, 3000, 6000], [31, 32, 33, 34]
 def batch(vemd):
     from wsl_code_amd.fields import grf_theta
     assert vemd.device_count() > 0, "no HIP device"
-    print("batch of", SIZES, flush=True)
+    print("batch of", SIZES, "and the 10-gon rosette", flush=True)
     meshes, thetas = [], []
     for n, sd in zip(SIZES, SEEDS):
         m = vemd.generate_mesh(n, domain=vemd.DOMAIN_DISK, lloyd_iters=20, seed=sd, min_edge_frac=0.05)
@@ -22,6 +22,12 @@ def batch(vemd):
         sigma, ell = 0.5 + 1.5 * (sd % 4) / 3.0, 0.02 + 0.08 * (sd % 3) / 2.0
         meshes.append(m)
         thetas.append(grf_theta(a["cx"], a["cy"], sigma=sigma, ell=ell, seed=sd + 100))
+    # a 10-gon ringed by triangles: a valence group of its own in the shared store
+    from conftest import rosette
+    m = vemd.Mesh.from_arrays(*rosette())
+    a = m.arrays()
+    meshes.append(m)
+    thetas.append(grf_theta(a["cx"], a["cy"], sigma=1.0, ell=0.1, seed=135))
     u = vemd.Mesh.concat(meshes)
     kw = dict(k=1, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P)
     sb = vemd.Solver(u, theta=np.concatenate(thetas), device=0, **kw)
