@@ -1454,7 +1454,9 @@ static int gmres(const Csr& A, const Ilu& M, const std::vector<double>& b, std::
 // physical measure, the physical divergence error and the physical mass
 // int p_h (GPU/src/equation_gpu.cu:385-479: the divergence is the
 // polynomial H^{-1} W u of the computational cell, divided by det J).
-static void errors(const Ctx& X, int nq, double* out) {
+// cell_sq (nullable, 2 per cell): each cell's squared pressure and flux errors
+// in the computational measure, so that tests can split the error by region.
+static void errors(const Ctx& X, int nq, double* out, double* cell_sq = nullptr) {
     const Mesh& m = *X.m;
     const Basis& bs = X.basis;
     TriRule rule = triangle_rule(nq);
@@ -1476,6 +1478,7 @@ static void errors(const Ctx& X, int nq, double* out) {
         for (int r = 0; r < q; ++r)
             for (int t = 0; t < q; ++t) dc[r] += Hi[r * q + t] * wu[t];
         double xc = m.cx[c], yc = m.cy[c], h = m.diam[c];
+        const double ep0 = ep, eu0 = eu;
         std::vector<QP> pts;
         for (auto& t : e.tris) {
             map_triangle(rule, t.v, pts);
@@ -1505,6 +1508,10 @@ static void errors(const Ctx& X, int nq, double* out) {
                 ed += dd * dd * qp.w * G.det;
                 mass += ph * qp.w * G.det;
             }
+        }
+        if (cell_sq) {
+            cell_sq[2 * c] = ep - ep0;
+            cell_sq[2 * c + 1] = eu - eu0;
         }
     }
     out[0] = std::sqrt(ep);
@@ -1775,6 +1782,40 @@ int orc_errors(void* cp, int nq, double* out) {
         Ctx* X = (Ctx*)cp;
         if (!X->have_sol) throw std::runtime_error("errors require a solution");
         errors(*X, nq, out);
+        return 0;
+    })
+}
+
+// The six error values of orc_errors plus each cell's squared pressure and
+// flux errors in the computational measure (2 per cell).
+int orc_errors_cells(void* cp, int nq, double* out, double* cell_sq) {
+    ORC_TRY({
+        Ctx* X = (Ctx*)cp;
+        if (!X->have_sol) throw std::runtime_error("errors require a solution");
+        errors(*X, nq, out, cell_sq);
+        return 0;
+    })
+}
+
+// Pulled-back problem data at a computational point of a cell, in the order
+// of Darcy::DarcyProblem's computational_* accessors
+// (CPU/examples/darcy_problem.cpp:37-83): [p_hat, u_hat (2), g_hat, K_hat
+// (xx, xy, yx, yy), det J].  K_hat is the inverse of the tensor the element
+// integrals use (kinv_hat), so this also checks that inversion.
+int orc_problem_data(void* cp, int cell, double xi, double eta, double* out) {
+    ORC_TRY({
+        Ctx* X = (Ctx*)cp;
+        if (cell < 0 || cell >= X->m->nc) throw std::out_of_range("cell out of range");
+        Geo G = eval_map(*X, cell, nullptr, xi, eta);
+        comp_data(*X, cell, G, &out[0], &out[1], &out[2], &out[3]);
+        double T[4];
+        kinv_hat(*X, cell, G, T);
+        const double d = T[0] * T[3] - T[1] * T[2];
+        out[4] = T[3] / d;
+        out[5] = -T[1] / d;
+        out[6] = -T[2] / d;
+        out[7] = T[0] / d;
+        out[8] = G.det;
         return 0;
     })
 }

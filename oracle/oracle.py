@@ -186,6 +186,19 @@ class Problem:
         _check(lib().orc_errors(self.h, nq, _d(out)))
         return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3], div_phys=out[4], mass=out[5])
 
+    def cell_errors(self, nq=9):
+        """Each cell's squared pressure and flux errors (computational measure), shape (nc, 2)."""
+        out = np.zeros(6)
+        sq = np.zeros(2 * self.mesh.nc)
+        _check(lib().orc_errors_cells(self.h, nq, _d(out), _d(sq)))
+        return sq.reshape(-1, 2)
+
+    def problem_data(self, cell, xi, eta):
+        """[p_hat, u_hat_x, u_hat_y, g_hat, K_hat xx, xy, yx, yy, det J] at a computational point."""
+        out = np.zeros(9)
+        _check(lib().orc_problem_data(self.h, int(cell), ctypes.c_double(xi), ctypes.c_double(eta), _d(out)))
+        return out
+
     def map(self, cell, xi, eta):
         out = np.zeros(7)
         _check(lib().orc_map(self.h, int(cell), ctypes.c_double(xi), ctypes.c_double(eta), _d(out)))

@@ -126,13 +126,22 @@ def test_basis(ref, oracle, k):
 
 @pytest.mark.parametrize("example,eps", [(1, 0.05), (2, 0.0)])
 def test_problem_data(ref, oracle, example, eps):
-    """Pulled-back p, u, g and the map of reference examples 1 and 2."""
+    """Pulled-back p, u, g, the computational permeability tensor and det J of
+    reference examples 1 and 2 (Darcy::DarcyProblem::computational_*), each
+    against the reference's own accessor at the same computational point."""
     m = oracle.Mesh.read(golden("square_1x1.msh"))
     prob = oracle.Problem(m, k=1, eps=eps)
     rng = np.random.default_rng(example)
     out = np.zeros(9)
+    names = ("p", "u_x", "u_y", "g", "K_xx", "K_xy", "K_yx", "K_yy", "det J")
     for _ in range(30):
         xi, eta = rng.uniform(0, 1, 2)
         assert ref.ref_problem_data(example, ctypes.c_double(xi), ctypes.c_double(eta), _d(out)) == 0
+        mine = prob.problem_data(0, xi, eta)
+        for i, name in enumerate(names):
+            assert abs(mine[i] - out[i]) <= 1e-14 * max(1.0, abs(out[i])), (name, mine[i], out[i])
         g = prob.map(0, xi, eta)
         assert abs(g[6] - out[8]) <= 1e-15
+    if eps == 0.0:
+        # example 2 is the identity map: K_hat = K = I exactly
+        assert np.array_equal(out[4:8], [1.0, 0.0, 0.0, 1.0])
