@@ -186,6 +186,13 @@ typedef struct {
                            flux block and one smoothed-aggregation V-cycle on the pressure Schur
                            approximation W diag(A)^-1 W^T (Dirichlet pressure problems);
                            VEMD_PC_AUTO (0): multigrid where built, else Jacobi */
+    void* stream;       /* the caller's HIP stream (hipStream_t; NULL = the null stream).  The
+                           device-resident solves (vemd_solve_device, vemd_solve_multi_device)
+                           wait for work queued on it before they read b_dev, and work queued
+                           on it after the call is ordered after their writes of x_dev.  The
+                           solver's own stream runs the iteration (the reference synchronises
+                           the device at every borrow and restore instead,
+                           GPU/include/petsc_gpu.h:62-214). */
 } vemd_solve_opts;
 
 typedef struct {
@@ -227,7 +234,11 @@ vemd_status vemd_apply(vemd_solver s, int precision, const double* x, double* y)
 
 /* Device-resident apply in the internal (store-ordered) layout, on the given
  * HIP stream (NULL = default); x_dev and y_dev must not alias.  Used by
- * benchmarks; vector length is sizes[3], element type double or float. */
+ * benchmarks; vector length is sizes[3], element type double or float.
+ * Asynchronous: it returns once the kernels are queued.  The solver's device
+ * scratch (side slots, partial sums) is shared by every apply and solve of
+ * one solver, so each such call is ordered after the previous one, whatever
+ * stream either ran on. */
 vemd_status vemd_apply_device(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* stream);
 
 /* The same apply without its final edge sum, for consumers that fuse it: y_dev

@@ -104,7 +104,7 @@ public:
         nflux_ = z[0];
         npres_ = z[1];
         ntot_ = z[2];
-        opts_ = vemd_solve_opts{VEMD_FP64, 1e-10, 1000000, 1e-4, 32, 1, VEMD_PC_AUTO};
+        opts_ = vemd_solve_opts{VEMD_FP64, 1e-10, 1000000, 1e-4, 32, 1, VEMD_PC_AUTO, nullptr};
     }
     ~DarcySolver() {
         if (s_) vemd_solver_destroy(s_);

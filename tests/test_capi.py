@@ -259,3 +259,21 @@ def test_device_setup_limits_checked_on_the_host(vemd):
         with pytest.raises(vemd.VemdError) as ei:
             vemd.Solver(m, k=1, setup_device=0)
         assert ei.value.status == "ERR_DEVICE"
+
+
+def test_vector_range_limit_fails_loudly(vemd, monkeypatch):
+    """The apply addresses the solve's vectors with 32-bit byte offsets (a
+    record past the range reads as zero), so a layout whose FP64 vector would
+    not fit is rejected at setup with VEMD_ERR_MESH instead of solving with
+    silently zeroed entries.  The real limit (2^31 bytes, 268M entries) needs
+    a mesh of ~20M cells; the environment lowers it here to force the error."""
+    m = vemd.Mesh.read(golden("square_2x2.msh"))
+    nbytes = 8 * vemd.Solver(m, k=1).nvec
+    monkeypatch.setenv("VEMD_MAX_VECTOR_BYTES", str(nbytes - 1))
+    with pytest.raises(vemd.VemdError) as e:
+        vemd.Solver(m, k=1)
+    assert e.value.code == 3 and "32-bit offsets" in str(e.value)
+    monkeypatch.setenv("VEMD_MAX_VECTOR_BYTES", str(nbytes))
+    assert vemd.Solver(m, k=1).nvec * 8 == nbytes  # exactly at the limit
+    monkeypatch.setenv("VEMD_MAX_VECTOR_BYTES", str(1 << 40))  # the variable cannot raise the limit
+    assert vemd.Solver(m, k=1).nvec * 8 == nbytes

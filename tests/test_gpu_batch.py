@@ -1,7 +1,8 @@
 """Batched solves: several independent problems on meshes of uneven size in
 one operator store and one captured Krylov loop, each with its own
-recurrences, stopping test and refinement.  Every instance must agree with a
-solve of that instance alone."""
+recurrences, stopping test and refinement.  Every instance must agree with the
+oracle's direct solve of that instance's mesh (FP64 to 1e-10; mixed to 1e-5
+with error functionals within 1e-9) and with a solve of that instance alone."""
 import numpy as np
 import pytest
 
@@ -32,12 +33,44 @@ def batch(vemd):
     kw = dict(k=1, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P)
     sb = vemd.Solver(u, theta=np.concatenate(thetas), device=0, **kw)
     singles = [vemd.Solver(m, theta=t, device=0, **kw) for m, t in zip(meshes, thetas)]
-    return sb, singles
+    return sb, singles, thetas
+
+
+@pytest.fixture(scope="module")
+def direct(batch, oracle):
+    """The oracle's direct solution of every instance on its own mesh."""
+    _, singles, thetas = batch
+    out = []
+    for s, th in zip(singles, thetas):
+        a = s.mesh.arrays()
+        om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+        op = oracle.Problem(om, k=1, map_kind=oracle.MAP_ARCS, sol=oracle.SOL_SINSIN, bc=oracle.BC_DIRICHLET_P,
+                            theta=th)
+        out.append((op.solve_direct(), op.errors(9)))
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_matches_oracle_direct_solves(vemd, batch, direct, prec):
+    sb, singles, _ = batch
+    p = vemd.FP64 if prec == "fp64" else vemd.MIXED
+    xb, info = sb.solve(vemd.SolveOptions(precision=p, rtol=1e-12 if prec == "fp64" else 1e-10))
+    print("batch vs oracle", prec, info, flush=True)
+    for xi, s, (xo, eo) in zip(sb.instance_vectors(xb), singles, direct):
+        rel = np.linalg.norm(xi - xo) / np.linalg.norm(xo)
+        e = s.errors(xi)
+        print("instance", s.mesh.num_cells, "cells: |x - x_oracle| / |x_oracle| =", rel, flush=True)
+        if prec == "fp64":
+            assert rel <= 1e-10
+        else:
+            assert rel <= 1e-5
+        for key in ("p_comp", "u_comp", "p_phys", "u_phys", "div_phys"):
+            assert abs(e[key] - eo[key]) <= 1e-9 * max(1.0, eo[key]), (key, e[key], eo[key])
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_batch_matches_single_solves(vemd, batch, prec):
-    sb, singles = batch
+    sb, singles, _ = batch
     p = vemd.FP64 if prec == "fp64" else vemd.MIXED
     xb, info = sb.solve(vemd.SolveOptions(precision=p, rtol=1e-10))
     per = sb.batch_info()
@@ -56,7 +89,7 @@ def test_batch_matches_single_solves(vemd, batch, prec):
 
 
 def test_batch_of_one_equals_plain_solve(vemd, batch):
-    _, singles = batch
+    _, singles, _ = batch
     s = singles[1]
     x1, i1 = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-10))
     x2, i2 = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-10))

@@ -1,7 +1,8 @@
 """Multi-right-hand-side path: the MFMA element apply on 32 interleaved FP32
-columns against the single-column FP32 apply, and simultaneous solves of 32
-right-hand sides (FP32 MFMA MINRES inside FP64 refinement to 1e-10) against
-one FP64 solve per right-hand side."""
+columns against the oracle's assembled operator and the single-column FP32
+apply, and simultaneous solves of 32 right-hand sides (FP32 MFMA MINRES inside
+FP64 refinement to 1e-10) against a sparse LU solve of the oracle's system
+and one FP64 solve per right-hand side."""
 import ctypes
 
 import numpy as np
@@ -13,7 +14,7 @@ NR = 32
 
 
 @pytest.fixture(scope="module", params=["annulus", "rosette"])
-def setup(vemd, request):
+def setup(vemd, oracle, request):
     """A 3000-cell Voronoi annulus with a permeability field, and the 10-gon
     rosette (the largest cell block of the MFMA apply: 26 rows, 348 floats)."""
     import torch
@@ -30,15 +31,50 @@ def setup(vemd, request):
     th = grf_theta(a["cx"], a["cy"], sigma=1.0, ell=0.1, seed=42)
     s = vemd.Solver(m, k=1, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P, theta=th,
                     device=0)
-    return s, torch
+    om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+    op = oracle.Problem(om, k=1, map_kind=oracle.MAP_ARCS, sol=oracle.SOL_SINSIN, bc=oracle.BC_DIRICHLET_P,
+                        theta=th)
+    A, _ = op.csr(False)  # Dirichlet pressure: nothing eliminated, reference-form rows
+    return s, torch, A
 
 
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def test_mfma_apply_matches_oracle_operator(vemd, setup):
+    """Every column of the MFMA apply, taken back to the reference order and
+    row form, against the oracle's assembled operator times the same column:
+    FP32 products with FP32 accumulation, so 5e-5 of the column's largest
+    entry (the single-column FP32 apply meets 2e-5 against the same matrix)."""
+    s, torch, A = setup
+    L = vemd.lib()
+    g = torch.Generator(device="cuda").manual_seed(19)
+    X = torch.randn(s.nvec, NR, dtype=torch.float32, device="cuda", generator=g)
+    Y, Y1 = torch.empty_like(X), torch.zeros_like(X)
+    assert L.vemd_apply_multi_device(s.h, _ptr(X), _ptr(Y), _ptr(Y1), None) == 0
+    torch.cuda.synchronize()
+    ne = (s.k + 1) * s.mesh.num_edges
+    Y[:ne] += Y1[:ne]
+    Xh, Yh = X.double().cpu().numpy(), Y.double().cpu().numpy()
+    worst = 0.0
+    for j in range(NR):
+        xr, yr = np.zeros(s.ntot), np.zeros(s.ntot)
+        L.vemd_to_reference(s.h, _dp(np.ascontiguousarray(Xh[:, j])), _dp(xr), 0)
+        L.vemd_to_reference(s.h, _dp(np.ascontiguousarray(Yh[:, j])), _dp(yr), 1)
+        y_ref = A @ xr
+        err = np.abs(yr - y_ref).max() / np.abs(y_ref).max()
+        worst = max(worst, err)
+        assert err <= 5e-5, (j, err)
+    print("MFMA apply vs oracle operator: largest relative difference", worst, flush=True)
+
+
 def test_mfma_apply_matches_single_column_apply(vemd, setup):
-    s, torch = setup
+    s, torch, _ = setup
     L = vemd.lib()
     g = torch.Generator(device="cuda").manual_seed(9)
     X = torch.randn(s.nvec, NR, dtype=torch.float32, device="cuda", generator=g)
@@ -74,7 +110,12 @@ def test_mfma_apply_matches_single_column_apply(vemd, setup):
 
 
 def test_multi_rhs_solve_matches_single_solves(vemd, setup):
-    s, torch = setup
+    """Every column against a sparse LU solve of the oracle's system with the
+    same right-hand side (1e-7: the columns stop at a 1e-10 true residual)
+    and against the product's FP64 single solve."""
+    import scipy.sparse.linalg as sla
+    s, torch, A = setup
+    lu = sla.splu(A.tocsc())
     L = vemd.lib()
     rng = np.random.default_rng(43)
     base = s.rhs()
@@ -104,13 +145,16 @@ def test_multi_rhs_solve_matches_single_solves(vemd, setup):
         xr = np.zeros(s.ntot)
         L.vemd_to_reference(s.h, np.ascontiguousarray(Xi[j]).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                             xr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0)
+        xo = lu.solve(B_ref[j])
+        rel_o = np.linalg.norm(xr - xo) / np.linalg.norm(xo)
+        assert rel_o <= 1e-7, (j, rel_o)
         xs, _ = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12), rhs=B_ref[j])
         rel = np.linalg.norm(xr - xs) / np.linalg.norm(xs)
         assert rel <= 1e-7, (j, rel)
 
 
 def test_multi_rhs_needs_mixed(vemd, setup):
-    s, torch = setup
+    s, torch, _ = setup
     L = vemd.lib()
     Bd = torch.zeros(1, s.nvec, dtype=torch.float64, device="cuda")
     o = vemd.SolveOptions(precision=vemd.FP64).c()

@@ -1,10 +1,11 @@
-"""Element setup on the GPU (vemd_solver_create_device) against the host
-setup: every cell's signed A_E, W_E, L_E, f_E, the assembled right-hand side
-(boundary data included) and the projector (through the flux error), on
-straight, sinusoidally mapped and curved-arc meshes, k = 0 and 1, both
-boundary conditions.  The device threads follow the host arithmetic step by
-step, so the two agree to the rounding of fused multiply-adds and the math
-library: relative 1e-11 per cell."""
+"""Element setup on the GPU (vemd_solver_create_device) against the oracle
+and the host setup: every cell's signed A_E, W_E, L_E, f_E, the assembled
+right-hand side (boundary data included) and the projector (through the flux
+error), on straight, sinusoidally mapped and curved-arc meshes, k = 0 and 1,
+both boundary conditions.  The device threads follow the host arithmetic step
+by step, so the two agree to the rounding of fused multiply-adds and the math
+library: relative 1e-11 per cell; the oracle computes the same quantities
+in its own order, and the device values meet it to 1e-11 too."""
 import numpy as np
 import pytest
 
@@ -48,6 +49,42 @@ def _pair(vemd, case):
               bc=vemd.BC_FLUX_MEANZERO if bc == "flux" else vemd.BC_DIRICHLET_P,
               theta=rng.standard_normal(m.num_cells))
     return m, vemd.Solver(m, **kw), vemd.Solver(m, setup_device=0, **kw)
+
+
+def _oracle_problem(oracle, m, case, theta):
+    name, k, mk, eps, sol, bc = case
+    if name.endswith(".msh"):
+        om = oracle.Mesh.read(golden(name))
+    else:
+        a = m.arrays()
+        om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+    return oracle.Problem(om, k=k, map_kind=oracle.MAP_SIN if mk == "sin" else oracle.MAP_ARCS, eps=eps,
+                          sol=oracle.SOL_SINCOS if sol == "sincos" else oracle.SOL_SINSIN,
+                          bc=oracle.BC_FLUX_MEANZERO if bc == "flux" else oracle.BC_DIRICHLET_P, theta=theta)
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_device_setup_matches_oracle(vemd, oracle, case):
+    m, _, sd = _pair(vemd, case)
+    theta = np.random.default_rng(3).standard_normal(m.num_cells)  # the permeability _pair used
+    op = _oracle_problem(oracle, m, case, theta)
+    worst = 0.0
+    for c in range(m.num_cells):
+        ed = sd.element(c)
+        n = ed["n"]
+        d = op.element(c, 13)[:n]
+        ref = dict(A=op.element(c, 7, (n, n)) * np.outer(d, d) * np.exp(theta[c]),  # oracle A carries exp(-theta)
+                   W=op.element(c, 2).reshape(-1, n) * d, L=op.element(c, 8), f=op.element(c, 9))
+        for key, v in ref.items():
+            scale = max(np.abs(v).max(), 1e-300)
+            worst = max(worst, np.abs(ed[key] - v).max() / scale)
+        assert np.array_equal(ed["sign"], d) and np.array_equal(ed["gdof"], op.element(c, 12)[:n].astype(int))
+    print("largest relative difference from the oracle's element data", worst, flush=True)
+    assert worst <= 1e-11
+    _, rr = op.csr(True)
+    fi, ub = op.free()
+    b = sd.rhs()
+    assert np.abs(b[fi] - rr).max() <= 1e-11 * np.abs(rr).max()
 
 
 @pytest.mark.parametrize("case", CASES, ids=IDS)

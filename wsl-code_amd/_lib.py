@@ -42,7 +42,7 @@ class ProblemDesc(ctypes.Structure):
 class SolveOptsC(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iters", ctypes.c_int),
                 ("inner_rtol", ctypes.c_double), ("check_every", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("precond", ctypes.c_int)]
+                ("precond", ctypes.c_int), ("stream", ctypes.c_void_p)]
 
 
 class SolveInfo(ctypes.Structure):

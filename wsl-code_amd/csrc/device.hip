@@ -932,6 +932,8 @@ static void destroy_graph(DevKrylov<T>& k) {
 
 static void upload_theta(Solver& s) {
     DeviceState& d = *s.dev;
+    // the scales, diagonals and hierarchy below are read by queued applies
+    HIPCHK(hipEventSynchronize(d.ev_order));
     std::vector<float> f(s.scale.begin(), s.scale.end());
     HIPCHK(hipMemcpy(d.o64.scale, s.scale.data(), s.scale.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.o32.scale, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -969,6 +971,9 @@ void dev_upload(Solver& s, int device) {
     s.device = device;
     const Layout& L = s.layout;
     HIPCHK(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&d.ev_order, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&d.ev_caller, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(d.ev_order, d.stream));
     {
         std::vector<double> O64, L64;
         std::vector<float> O32, L32;
@@ -1055,6 +1060,8 @@ void dev_free(Solver& s) {
     if (d.rtol_host) (void)hipHostFree(d.rtol_host);
     if (d.e0) (void)hipEventDestroy(d.e0);
     if (d.e1) (void)hipEventDestroy(d.e1);
+    if (d.ev_order) (void)hipEventDestroy(d.ev_order);
+    if (d.ev_caller) (void)hipEventDestroy(d.ev_caller);
     if (d.stream) (void)hipStreamDestroy(d.stream);
     delete s.dev;
     s.dev = nullptr;
@@ -1230,6 +1237,7 @@ void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* h, vo
     hipStream_t saved = s.dev->stream;
     s.dev->stream = (hipStream_t)stream;
     try {
+        ScratchUse use(*s.dev, (hipStream_t)stream);
         if (precision == VEMD_FP64)
             apply_plain<double>(s, (const double*)x, (double*)y, (double*)h);
         else
@@ -1245,6 +1253,7 @@ void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref)
     if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
     DeviceState& d = *s.dev;
     HIPCHK(hipSetDevice(d.device));
+    ScratchUse use(d, d.stream);
     const Layout& L = s.layout;
     std::vector<double> xi(L.nvec, 0.0), yi(L.nvec);
     s.to_internal(x_ref, xi.data(), false);
@@ -1497,6 +1506,7 @@ void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, doubl
     if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
     DeviceState& d = *s.dev;
     HIPCHK(hipSetDevice(d.device));
+    ScratchUse use(d, d.stream);
     const Layout& L = s.layout;
     std::vector<double> bi(L.nvec, 0.0);
     s.to_internal(rhs_ref ? rhs_ref : s.rhs_ref.data(), bi.data(), true);
@@ -1530,6 +1540,8 @@ void dev_solve_int(Solver& s, const vemd_solve_opts& o, const void* b_dev, void*
     HIPCHK(hipSetDevice(d.device));
     const int64_t n = s.layout.nvec;
     auto t0 = std::chrono::steady_clock::now();
+    // b_dev may still be being written on the caller's stream
+    ScratchUse use(d, d.stream, true, (hipStream_t)o.stream);
     HIPCHK(hipMemcpyAsync(d.b64, b_dev ? b_dev : d.bprob, n * sizeof(double), hipMemcpyDeviceToDevice, d.stream));
     vemd_solve_info tmp{};
     std::string err;
@@ -1556,6 +1568,7 @@ void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, con
     if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
     DeviceState& d = *s.dev;
     HIPCHK(hipSetDevice(d.device));
+    ScratchUse use(d, d.stream);
     const Layout& L = s.layout;
     // adjoint rhs in internal form: the internal operator is S K~ with S the
     // row scaling, symmetric; K~^T lambda~ = g  <=>  (S K~) (S^{-1}... ) -- we

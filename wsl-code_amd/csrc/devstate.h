@@ -88,7 +88,38 @@ struct DeviceState {
     double* q2 = nullptr;
     std::vector<void*> allocs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // Order of the calls that use the solver's shared device scratch (side
+    // slots, partial sums, Krylov and multi-right-hand-side buffers): recorded
+    // at the end of every apply or solve on whatever stream it ran; the next
+    // call's stream waits for it.  ev_caller marks a caller's stream at the
+    // start of a device-resident solve.
+    hipEvent_t ev_order = nullptr, ev_caller = nullptr;
     MultiState* multi = nullptr;  // multi-right-hand-side solver state (allocated on first use)
+};
+
+// One apply or solve: its stream (work) waits for the previous user of the
+// scratch and, given a caller stream, for the work queued there; on leaving,
+// even by an exception, it records itself as the last user, and the caller's
+// stream waits for it.  Calls that overlap on one solver therefore run one
+// after the other on the device, whichever streams they were given.
+struct ScratchUse {
+    DeviceState& d;
+    hipStream_t work, caller;
+    bool with_caller;
+    ScratchUse(DeviceState& dd, hipStream_t w, bool wc = false, hipStream_t c = nullptr)
+        : d(dd), work(w), caller(c), with_caller(wc) {
+        if (with_caller) {
+            HIPCHK(hipEventRecord(d.ev_caller, caller));
+            HIPCHK(hipStreamWaitEvent(work, d.ev_caller, 0));
+        }
+        HIPCHK(hipStreamWaitEvent(work, d.ev_order, 0));
+    }
+    ~ScratchUse() {
+        (void)hipEventRecord(d.ev_order, work);
+        if (with_caller) (void)hipStreamWaitEvent(caller, d.ev_order, 0);
+    }
+    ScratchUse(const ScratchUse&) = delete;
+    ScratchUse& operator=(const ScratchUse&) = delete;
 };
 
 inline void* dalloc(DeviceState& d, size_t bytes) {

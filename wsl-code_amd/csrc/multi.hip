@@ -66,15 +66,18 @@ void multi_invalidate(DeviceState& d) {
     d.multi->graph = nullptr;
 }
 
+static void multi_release(MultiState* m) {
+    if (!m) return;
+    if (m->graph) (void)hipGraphExecDestroy(m->graph);
+    for (void* p : m->allocs) (void)hipFree(p);
+    if (m->st_host) (void)hipHostFree(m->st_host);
+    if (m->rtol_host) (void)hipHostFree(m->rtol_host);
+    if (m->hsum) (void)hipHostFree(m->hsum);
+    delete m;
+}
+
 void multi_free(DeviceState& d) {
-    if (!d.multi) return;
-    MultiState& m = *d.multi;
-    if (m.graph) (void)hipGraphExecDestroy(m.graph);
-    for (void* p : m.allocs) (void)hipFree(p);
-    if (m.st_host) (void)hipHostFree(m.st_host);
-    if (m.rtol_host) (void)hipHostFree(m.rtol_host);
-    if (m.hsum) (void)hipHostFree(m.hsum);
-    delete d.multi;
+    multi_release(d.multi);
     d.multi = nullptr;
 }
 
@@ -597,14 +600,33 @@ __global__ void k_sum_to(const double* part, int n, double* out) {
 // Host side
 // ---------------------------------------------------------------------------
 
+static void multi_build(Solver& s, MultiState& m);
+
+// The multi-right-hand-side state is built aside and published in d.multi only
+// once every allocation and check has succeeded: a failure on the way (device
+// out of memory, too many valence groups) frees what was allocated and leaves
+// d.multi null, so a later call reports the error again or builds afresh
+// instead of launching kernels on a half-built state.
 static MultiState& multi_setup(Solver& s) {
     DeviceState& d = *s.dev;
     if (d.multi) return *d.multi;
+    if (s.layout.lam || s.layout.ninst != 1)
+        throw Error(VEMD_ERR_ARG, "multi-right-hand-side solves need one Dirichlet-pressure mesh");
+    MultiState* m = new MultiState();
+    try {
+        multi_build(s, *m);
+    } catch (...) {
+        multi_release(m);
+        throw;
+    }
+    d.multi = m;
+    return *m;
+}
+
+static void multi_build(Solver& s, MultiState& m) {
+    DeviceState& d = *s.dev;
     const Layout& L = s.layout;
     const Setup& S = *s.setup;
-    if (L.lam || L.ninst != 1) throw Error(VEMD_ERR_ARG, "multi-right-hand-side solves need one Dirichlet-pressure mesh");
-    d.multi = new MultiState();
-    MultiState& m = *d.multi;
     // FP32 cell blocks: packed A then W, per group in store order
     std::vector<int64_t> gop(L.groups.size());
     int64_t tot = 0;
@@ -632,7 +654,13 @@ static MultiState& multi_setup(Solver& s) {
     });
     m.op = (float*)malloc_m(m, op.size() * sizeof(float));
     HIPCHK(hipMemcpy(m.op, op.data(), op.size() * sizeof(float), hipMemcpyHostToDevice));
-    if ((int)L.groups.size() > MR_MAXG) throw Error(VEMD_ERR_ARG, "too many valence groups for the multi-RHS apply");
+    // the group table lives in LDS: at most MR_MAXG groups (VEMD_MR_MAX_GROUPS
+    // lowers the cap; tests use it to reach this failure after allocations)
+    int max_groups = MR_MAXG;
+    if (const char* e = std::getenv("VEMD_MR_MAX_GROUPS")) max_groups = std::min(max_groups, std::max(0, std::atoi(e)));
+    if ((int)L.groups.size() > max_groups)
+        throw Error(VEMD_ERR_ARG, "too many valence groups for the multi-RHS apply: " + std::to_string(L.groups.size()) +
+                                      " > " + std::to_string(max_groups));
     std::vector<MrGroup> mg(L.groups.size());
     for (size_t gi = 0; gi < L.groups.size(); ++gi) {
         const GroupDesc& g = L.groups[gi];
@@ -693,7 +721,6 @@ static MultiState& multi_setup(Solver& s) {
     m.all_done = (int*)malloc_m(m, sizeof(int));
     HIPCHK(hipMemset(m.all_done, 0, sizeof(int)));
     if (d.amg32) amg_multi_alloc(*d.amg32);
-    return m;
 }
 
 static MrArgs mr_args(Solver& s, MultiState& m, const float* x0, const float* x1, float* y, float* y1, bool solve) {
@@ -730,6 +757,7 @@ void dev_apply_multi(Solver& s, const void* x, void* y, void* y1, void* stream) 
     if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
     HIPCHK(hipSetDevice(s.dev->device));
     MultiState& m = multi_setup(s);
+    ScratchUse use(*s.dev, (hipStream_t)stream);
     MrArgs a = mr_args(s, m, (const float*)x, (const float*)x, (float*)y, (float*)y1, false);
     launch_apply_mr(s, m, a, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
@@ -852,6 +880,8 @@ void dev_solve_multi(Solver& s, const vemd_solve_opts& o, int ncol, const double
     const bool amg = o.precond != VEMD_PC_JACOBI && d.amg32 != nullptr;
     const int64_t n = s.layout.nvec;
     auto t0 = std::chrono::steady_clock::now();
+    // b may still be being written on the caller's stream
+    ScratchUse use(d, d.stream, true, (hipStream_t)o.stream);
     HIPCHK(hipMemsetAsync(m.x64, 0, (size_t)ncol * n * sizeof(double), d.stream));
     std::vector<double> bn = residual_mr(s, m, b, ncol), rel(ncol);
     for (int j = 0; j < ncol; ++j) rel[j] = bn[j] > 0 ? 1.0 : 0.0;

@@ -3,7 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
+#include <string>
 
 namespace vemd {
 
@@ -198,6 +200,17 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     L.l_len = loff;
     L.idx.assign(ioff, (uint32_t)L.trash_idx);
     if (L.nvec >= (int64_t)SIDE1) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit dof indices");
+    // The apply reads and writes the vectors through buffer resources with
+    // 32-bit byte offsets (records beyond the range read as zero and writes are
+    // dropped), so a vector of the solve must stay below 2^31 bytes in FP64:
+    // at most 268,435,455 internal entries.  VEMD_MAX_VECTOR_BYTES lowers the
+    // limit (tests force the error with it); it cannot raise it.
+    int64_t vec_limit = (int64_t)INT32_MAX;
+    if (const char* e = std::getenv("VEMD_MAX_VECTOR_BYTES")) vec_limit = std::min(vec_limit, (int64_t)std::max(0LL, std::atoll(e)));
+    if (L.nvec * (int64_t)sizeof(double) > vec_limit)
+        throw Error(VEMD_ERR_MESH, "mesh too large: " + std::to_string(L.nvec) + " internal vector entries (" +
+                                       std::to_string(L.nvec * (int64_t)sizeof(double)) + " bytes in FP64) exceed the " +
+                                       std::to_string(vec_limit) + "-byte range of the apply's 32-bit offsets");
     if (ioff + 1 >= (int64_t)INT32_MAX) throw Error(VEMD_ERR_MESH, "mesh too large for 31-bit side slots");
     L.nside = ioff + 1;
     L.hside.assign(2 * (size_t)m.ne, (int32_t)ioff);

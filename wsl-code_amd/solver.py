@@ -31,10 +31,11 @@ class SolveOptions:
     check_every: int = 32
     use_graph: int = 1
     precond: int = 0  # PC_AUTO: multigrid where built (Dirichlet pressure), else Jacobi
+    stream: int = 0   # caller's HIP stream handle for the device-resident solves (0: the null stream)
 
     def c(self):
         return SolveOptsC(self.precision, self.rtol, self.max_iters, self.inner_rtol, self.check_every,
-                          self.use_graph, self.precond)
+                          self.use_graph, self.precond, self.stream or None)
 
 
 class Solver:
