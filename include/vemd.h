@@ -241,6 +241,21 @@ vemd_status vemd_apply(vemd_solver s, int precision, const double* x, double* y)
  * stream either ran on. */
 vemd_status vemd_apply_device(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* stream);
 
+/* The apply of vemd_apply_device, allocating nothing: its cell-side slots and
+ * multiplier-row partials live in a static device array of the library
+ * (64 MiB on each GPU it runs on, held until the process ends) instead of the
+ * solver's scratch, and the call only launches kernels on the given stream.
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  The mesh must fit: (k + 1) * nside side values (nside from
+ * vemd_edge_sides) of the precision plus 8 bytes per 64-cell tile, at most
+ * 64 MiB (about 650k k = 1 cells in FP64, twice that in FP32); a larger one
+ * fails with VEMD_ERR_ARG.  Every call on a GPU shares that GPU's array and
+ * leaves its contents behind: calls that overlap on one GPU race, and the
+ * caller orders them (one stream, or events).  Same results as
+ * vemd_apply_device, bit for bit. */
+vemd_status vemd_apply_device_static(vemd_solver s, int precision, const void* x_dev, void* y_dev, void* stream);
+
 /* The same apply without its final edge sum, for consumers that fuse it: y_dev
  * receives all cell-owned and multiplier rows (its leading edge-dof block is
  * not written), h_dev (nside * (k + 1) values, nside from vemd_edge_sides)

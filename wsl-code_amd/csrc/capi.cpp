@@ -358,6 +358,14 @@ vemd_status vemd_apply_device(vemd_solver sh, int precision, const void* x, void
     API_END
 }
 
+vemd_status vemd_apply_device_static(vemd_solver sh, int precision, const void* x, void* y, void* stream) {
+    API_BEGIN
+    REQUIRE(sh && x && y && x != y, "null or aliased argument");
+    REQUIRE(precision == VEMD_FP64 || precision == VEMD_FP32, "precision must be VEMD_FP64 or VEMD_FP32");
+    dev_apply_static(*sh->s, precision, x, y, stream);
+    API_END
+}
+
 vemd_status vemd_apply_device_split(vemd_solver sh, int precision, const void* x, void* y, void* h, void* stream) {
     API_BEGIN
     REQUIRE(sh && x && y && h && x != y && x != h && y != h, "null or aliased argument");

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -930,10 +931,37 @@ static void destroy_graph(DevKrylov<T>& k) {
     k.graph_tail = nullptr;
 }
 
+// Static scratch of the allocation-free apply (vemd_apply_device_static, below).
+constexpr size_t STATIC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Records the device address of the static array on `device`, which must be
+// current (each GPU holds its own copy at its own address).  In host code
+// `scratch` names a host-side stand-in: only this address may reach kernels.
+static void static_scratch_init(int device) {
+    if (device < 0 || device >= MAX_DEVICES) throw Error(VEMD_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+        throw Error(VEMD_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+}
+
+static void* static_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < MAX_DEVICES ? g_scratch_ptr[device] : nullptr;
+}
+
 static void upload_theta(Solver& s) {
     DeviceState& d = *s.dev;
-    // the scales, diagonals and hierarchy below are read by queued applies
-    HIPCHK(hipEventSynchronize(d.ev_order));
+    // the scales, diagonals and hierarchy below are read by queued applies,
+    // the allocation-free ones included (those run on any stream, unordered)
+    HIPCHK(hipDeviceSynchronize());
     std::vector<float> f(s.scale.begin(), s.scale.end());
     HIPCHK(hipMemcpy(d.o64.scale, s.scale.data(), s.scale.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.o32.scale, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -974,6 +1002,7 @@ void dev_upload(Solver& s, int device) {
     HIPCHK(hipEventCreateWithFlags(&d.ev_order, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&d.ev_caller, hipEventDisableTiming));
     HIPCHK(hipEventRecord(d.ev_order, d.stream));
+    static_scratch_init(device);
     {
         std::vector<double> O64, L64;
         std::vector<float> O32, L32;
@@ -1247,6 +1276,79 @@ void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* h, vo
         throw;
     }
     s.dev->stream = saved;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free apply.  The side slots and the multiplier-row partials of
+// the apply live in one static device array per GPU instead of the solver's
+// scratch, for callers that cannot allocate device memory (code under HIP
+// graph capture, serving loops that forbid hipMalloc).  Each GPU the library
+// runs on holds STATIC_SCRATCH_BYTES for it until the process ends; every call
+// on a GPU shares that GPU's array and leaves its contents behind, so calls
+// that overlap on one GPU race (the caller orders them).
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_fill(T* p, int n, T v) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+// The apply of apply_plain with h (side slots) and plam (multiplier-row
+// partials) in the static array; kernel launches only.
+template <typename T>
+static void apply_static(Solver& s, const T* x, T* y, T* h, double* plam) {
+    DeviceState& d = *s.dev;
+    const Layout& L = s.layout;
+    const int kk = L.k + 1;
+    // the zero slot (second side of every boundary edge) must read zero, and
+    // the array still holds what the last call on this GPU left there
+    hipLaunchKernelGGL((k_fill<T>), dim3(1), dim3(64), 0, d.stream, h + (L.nside - 1) * kk, kk, T(0));
+    ApplyArgs<T> a = make_args<T>(s, x, y, h, nullptr, nullptr, nullptr, false);
+    a.plam = plam;
+    launch_apply(s, a);
+    hipLaunchKernelGGL((k_combine<T>), dim3(d.nblk), dim3(256), 0, d.stream, y, (const T*)h, (const int*)d.hside, kk,
+                       L.cbase, L.trash_idx);
+    if (L.lam)
+        hipLaunchKernelGGL((k_lam_row<T>), dim3(1), dim3(256), 0, d.stream, (const double*)plam, d.ntiles, y,
+                           L.lam_idx);
+    HIPCHK(hipGetLastError());
+}
+
+size_t dev_static_scratch_need(const Solver& s, int precision) {
+    const Layout& L = s.layout;
+    const size_t w = precision == VEMD_FP64 ? sizeof(double) : sizeof(float);
+    const size_t hbytes = ((size_t)L.nside * (L.k + 1) * w + 255) / 256 * 256;
+    return hbytes + L.tile_group.size() * sizeof(double);
+}
+
+void dev_apply_static(Solver& s, int precision, const void* x, void* y, void* stream) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "solver not uploaded");
+    DeviceState& d = *s.dev;
+    HIPCHK(hipSetDevice(d.device));
+    void* base = static_scratch(d.device);
+    if (!base) throw Error(VEMD_ERR_STATE, "static scratch not set up on device " + std::to_string(d.device));
+    const size_t need = dev_static_scratch_need(s, precision);
+    size_t limit = STATIC_SCRATCH_BYTES;
+    if (const char* e = std::getenv("VEMD_STATIC_SCRATCH_LIMIT"))  // lowers the limit (tests); cannot raise it
+        limit = std::min(limit, (size_t)std::max(0LL, std::atoll(e)));
+    if (need > limit)
+        throw Error(VEMD_ERR_ARG, "the allocation-free apply needs " + std::to_string(need) +
+                                      " bytes of static scratch for this mesh and precision, more than its limit of " +
+                                      std::to_string(limit) + " bytes; use vemd_apply_device");
+    const size_t hbytes = need - (size_t)d.ntiles * sizeof(double);
+    double* plam = (double*)((char*)base + hbytes);
+    hipStream_t saved = d.stream;
+    d.stream = (hipStream_t)stream;
+    try {
+        if (precision == VEMD_FP64)
+            apply_static<double>(s, (const double*)x, (double*)y, (double*)base, plam);
+        else
+            apply_static<float>(s, (const float*)x, (float*)y, (float*)base, plam);
+    } catch (...) {
+        d.stream = saved;
+        throw;
+    }
+    d.stream = saved;
 }
 
 void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref) {

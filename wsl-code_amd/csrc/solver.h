@@ -48,6 +48,10 @@ int dev_tuning(const Solver& s, int* out16);
 void dev_apply_ref(Solver& s, int precision, const double* x_ref, double* y_ref);
 // y1 == nullptr: complete apply; otherwise y1 receives the edges' second sides.
 void dev_apply_int(Solver& s, int precision, const void* x, void* y, void* h, void* stream);
+// The same complete apply with its side slots and partials in the library's
+// static device array (no allocation); the bytes of that array it needs.
+void dev_apply_static(Solver& s, int precision, const void* x, void* y, void* stream);
+size_t dev_static_scratch_need(const Solver& s, int precision);
 void dev_solve(Solver& s, const vemd_solve_opts& o, const double* rhs_ref, double* sol_ref, vemd_solve_info* info);
 void dev_solve_int(Solver& s, const vemd_solve_opts& o, const void* b_dev, void* x_dev, vemd_solve_info* info);
 // Up to 32 right-hand sides at once (FP32 MFMA iterations, FP64 refinement):
