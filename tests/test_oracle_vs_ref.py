@@ -145,3 +145,44 @@ def test_problem_data(ref, oracle, example, eps):
     if eps == 0.0:
         # example 2 is the identity map: K_hat = K = I exactly
         assert np.array_equal(out[4:8], [1.0, 0.0, 0.0, 1.0])
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_vtk_reader(ref, oracle, vemd, tmp_path, seed):
+    """A generated Voronoi disk (mostly 5- to 7-gons after Lloyd relaxation,
+    quads from the edge collapse) written by vemd_mesh_write_vtk and read
+    back by the reference's read_mesh_vtk (CPU/mesh/straight_mesh.cpp:92-214),
+    the oracle's and the product's VTK readers: coordinates, centroids,
+    diameters, areas, node lists, edge numbering and the ear-clipping
+    triangulation of every cell agree bitwise."""
+    m = vemd.generate_mesh(300, lloyd_iters=20, seed=seed, min_edge_frac=0.05)
+    path = str(tmp_path / "disk.vtk")
+    m.write_vtk(path)
+    r = _ref_mesh_vtk(ref, path)
+    o = oracle.Mesh.read(path, vtk=True)
+    p = vemd.Mesh.read(path, vtk=True).arrays()
+    valences = np.diff(r["off"])
+    assert valences.min() <= 4 and valences.max() >= 7
+    for key in ("cx", "cy", "area", "diam"):
+        assert np.array_equal(getattr(o, key), r[key]), key
+        assert np.array_equal(p[key], r[key]), key
+    for key, pk in (("off", "cell_off"), ("nodes", "cell_nodes"), ("cell_edges", "cell_edges"),
+                    ("edge_nodes", "edge_nodes"), ("edge_occ", "edge_occ")):
+        assert np.array_equal(getattr(o, key), r[key]), key
+        assert np.array_equal(p[pk], r[key]), key
+    for cell in range(o.nc):
+        buf = np.zeros(6 * 16)
+        nt = ref.ref_triangulate(cell, _d(buf), 16)
+        assert np.array_equal(o.triangulate(cell), buf[: 6 * nt].reshape(nt, 6)), cell
+
+
+def _ref_mesh_vtk(ref, path):
+    cnt = np.zeros(4, np.int32)
+    assert ref.ref_read(path.encode(), 1, _i(cnt)) == 0
+    nn, nc, ne, nt = (int(v) for v in cnt)
+    out = dict(cx=np.zeros(nc), cy=np.zeros(nc), area=np.zeros(nc), diam=np.zeros(nc),
+               off=np.zeros(nc + 1, np.int32), nodes=np.zeros(nt, np.int32), cell_edges=np.zeros(nt, np.int32),
+               edge_nodes=np.zeros(2 * ne, np.int32), edge_occ=np.zeros(ne, np.int32))
+    ref.ref_mesh_get(_d(out["cx"]), _d(out["cy"]), _d(out["area"]), _d(out["diam"]), _i(out["off"]),
+                     _i(out["nodes"]), _i(out["cell_edges"]), _i(out["edge_nodes"]), _i(out["edge_occ"]))
+    return out

@@ -61,9 +61,14 @@ void mesh_finalize(Mesh& m) {
                 m.cy[c] = sy / n;
                 m.area[c] = std::fabs(a) / 2.0;
             }
+            // largest vertex distance, rounded as the reference does
+            // (sqrt of the sum of squares, CPU/mesh/straight_mesh.cpp:562-571)
             double md = 0.0;
             for (int i = 0; i < n; ++i)
-                for (int j = i + 1; j < n; ++j) md = std::max(md, std::hypot(x[i] - x[j], y[i] - y[j]));
+                for (int j = i + 1; j < n; ++j) {
+                    const double dx = x[i] - x[j], dy = y[i] - y[j];
+                    md = std::max(md, std::sqrt(dx * dx + dy * dy));
+                }
             if (!(md > 0.0)) throw Error(VEMD_ERR_MESH, "cell " + std::to_string(c) + " is degenerate");
             m.diam[c] = md;
         }
