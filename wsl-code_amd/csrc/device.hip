@@ -793,7 +793,7 @@ __global__ __launch_bounds__(256) void k_residual(const VecChunk* chunks, const 
     const VecChunk ch = chunks[blockIdx.x];
     double d = 0.0;
     for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
-        double v = b[i] - y[i];
+        double v = y ? b[i] - y[i] : b[i];  // y null: the residual of x = 0
         r[i] = v;
         d += v * v;
     }
@@ -1522,12 +1522,13 @@ static int minres(Solver& s, DevKrylov<T>& k, const std::vector<double>& rtol, i
     return mx;
 }
 
-// r64 = b64 - K x64; returns the 2-norm of every instance's block of r64.
-static std::vector<double> residual64(Solver& s) {
+// r64 = b64 - K x64 (x64 = 0: r64 = b64, no apply); returns the 2-norm of
+// every instance's block of r64.
+static std::vector<double> residual64(Solver& s, bool x_zero = false) {
     DeviceState& d = *s.dev;
-    apply_plain<double>(s, d.x64, d.y64);
+    if (!x_zero) apply_plain<double>(s, d.x64, d.y64);
     hipLaunchKernelGGL(k_residual, dim3(d.nchunks), dim3(256), 0, d.stream, (const VecChunk*)d.chunks,
-                       (const double*)d.b64, (const double*)d.y64, d.r64, d.p1);
+                       (const double*)d.b64, x_zero ? (const double*)nullptr : (const double*)d.y64, d.r64, d.p1);
     hipLaunchKernelGGL(k_inst_sum, dim3(d.ninst), dim3(FIN), 0, d.stream, (const double*)d.p1,
                        (const int*)d.inst_chunk0, d.dsum);
     HIPCHK(hipMemcpyAsync(d.hsum, d.dsum, d.ninst * sizeof(double), hipMemcpyDeviceToHost, d.stream));
@@ -1546,7 +1547,7 @@ static void solve_device(Solver& s, const vemd_solve_opts& o, vemd_solve_info* i
     const int64_t n = s.layout.nvec;
     const int ni = d.ninst;
     HIPCHK(hipMemsetAsync(d.x64, 0, n * sizeof(double), d.stream));
-    std::vector<double> bn = residual64(s), rel(ni);  // r = b
+    std::vector<double> bn = residual64(s, true), rel(ni);  // r = b
     for (int i = 0; i < ni; ++i) rel[i] = bn[i] > 0 ? 1.0 : 0.0;
     std::vector<int> it_tot(ni, 0), its;
     int iters = 0, refin = 0;

@@ -570,7 +570,7 @@ __global__ __launch_bounds__(256) void k_resid_col(const double* b, const double
     __shared__ double sh[256];
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        double v = b[i] - y[i];
+        double v = y ? b[i] - y[i] : b[i];  // y null: the residual of x = 0
         r[i] = v;
         s += v * v;
     }
@@ -848,13 +848,14 @@ static int minres_mr(Solver& s, MultiState& m, const std::vector<double>& rtol, 
 }
 
 // r_j = b_j - K x_j (FP64, column-major) for j < ncol; returns |r_j|.
-static std::vector<double> residual_mr(Solver& s, MultiState& m, const double* b, int ncol) {
+// Columns of r64 = b - K x64 (x_zero: x64 = 0, r64 = b, no applies) and their 2-norms.
+static std::vector<double> residual_mr(Solver& s, MultiState& m, const double* b, int ncol, bool x_zero = false) {
     DeviceState& d = *s.dev;
     const int64_t n = s.layout.nvec;
     for (int j = 0; j < ncol; ++j) {
-        dev_apply_int(s, VEMD_FP64, m.x64 + (int64_t)j * n, m.y64, nullptr, d.stream);
-        hipLaunchKernelGGL(k_resid_col, dim3(d.nblk), dim3(256), 0, d.stream, b + (int64_t)j * n, (const double*)m.y64, n,
-                           m.r64 + (int64_t)j * n, m.part);
+        if (!x_zero) dev_apply_int(s, VEMD_FP64, m.x64 + (int64_t)j * n, m.y64, nullptr, d.stream);
+        hipLaunchKernelGGL(k_resid_col, dim3(d.nblk), dim3(256), 0, d.stream, b + (int64_t)j * n,
+                           x_zero ? (const double*)nullptr : (const double*)m.y64, n, m.r64 + (int64_t)j * n, m.part);
         hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, d.stream, (const double*)m.part, d.nblk, m.dsum + j);
     }
     HIPCHK(hipMemcpyAsync(m.hsum, m.dsum, ncol * sizeof(double), hipMemcpyDeviceToHost, d.stream));
@@ -883,7 +884,7 @@ void dev_solve_multi(Solver& s, const vemd_solve_opts& o, int ncol, const double
     // b may still be being written on the caller's stream
     ScratchUse use(d, d.stream, true, (hipStream_t)o.stream);
     HIPCHK(hipMemsetAsync(m.x64, 0, (size_t)ncol * n * sizeof(double), d.stream));
-    std::vector<double> bn = residual_mr(s, m, b, ncol), rel(ncol);
+    std::vector<double> bn = residual_mr(s, m, b, ncol, true), rel(ncol);
     for (int j = 0; j < ncol; ++j) rel[j] = bn[j] > 0 ? 1.0 : 0.0;
     std::vector<int> tot(ncol, 0), its;
     int iters = 0, refin = 0;

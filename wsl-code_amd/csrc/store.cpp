@@ -115,18 +115,45 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     L.spatial_gpos.resize(nc);
     for (int i = 0; i < nc; ++i) L.spatial_gpos[i] = L.gpos_of_cell[order[i]];
     L.cpad = gpos;
-    // edges numbered by the spatial rank of the first cell that owns them
-    std::vector<int> efirst(m.ne, INT32_MAX);
+    // An interior edge's first side is the adjacent cell with the lower store
+    // position.  Edges are numbered in the order of their first side's slot
+    // in the apply's side array (group, tile, local edge, lane): for each
+    // local edge j of a tile, the lanes that are the first side of their j-th
+    // edge read consecutive edges of x in the apply, and the Krylov sweep's
+    // edge sums read those side slots in order.  VEMD_EDGE_ORDER=hilbert
+    // numbers edges by the spatial rank of their first owner instead.
+    std::vector<int> side0(m.ne, INT32_MAX);
     for (int c = 0; c < nc; ++c)
-        for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
-            int e = m.cell_edges[j];
-            efirst[e] = std::min(efirst[e], rank[c]);
-        }
-    std::vector<int> eo(m.ne);
-    std::iota(eo.begin(), eo.end(), 0);
-    std::sort(eo.begin(), eo.end(), [&](int a, int b) { return efirst[a] < efirst[b] || (efirst[a] == efirst[b] && a < b); });
-    L.edge_int.assign(m.ne, 0);
-    for (int i = 0; i < m.ne; ++i) L.edge_int[eo[i]] = i;
+        for (int j = m.off[c]; j < m.off[c + 1]; ++j)
+            side0[m.cell_edges[j]] = std::min(side0[m.cell_edges[j]], L.gpos_of_cell[c]);
+    L.edge_int.assign(m.ne, -1);
+    const char* eord = std::getenv("VEMD_EDGE_ORDER");
+    if (eord && std::string(eord) == "hilbert") {
+        std::vector<int> efirst(m.ne, INT32_MAX);
+        for (int c = 0; c < nc; ++c)
+            for (int j = m.off[c]; j < m.off[c + 1]; ++j) {
+                int e = m.cell_edges[j];
+                efirst[e] = std::min(efirst[e], rank[c]);
+            }
+        std::vector<int> eo(m.ne);
+        std::iota(eo.begin(), eo.end(), 0);
+        std::sort(eo.begin(), eo.end(),
+                  [&](int a, int b) { return efirst[a] < efirst[b] || (efirst[a] == efirst[b] && a < b); });
+        for (int i = 0; i < m.ne; ++i) L.edge_int[eo[i]] = i;
+    } else {
+        int next = 0;
+        for (const GroupDesc& g : L.groups)
+            for (int t = 0; t < g.npad / TILE; ++t)
+                for (int j = 0; j < g.ne; ++j)
+                    for (int l = 0; l < TILE; ++l) {
+                        const int gp = g.gpos0 + t * TILE + l;
+                        const int c = L.cell_of_gpos[gp];
+                        if (c < 0) continue;
+                        const int e = m.cell_edges[m.off[c] + j];
+                        if (side0[e] == gp) L.edge_int[e] = next++;
+                    }
+        if (next != m.ne) throw Error(VEMD_ERR_INTERNAL, "edge numbering missed edges");
+    }
     // edges of an instance are contiguous (owners' ranks are instance-major)
     L.inst_edge0.assign(L.ninst + 1, 0);
     for (int e = 0; e < m.ne; ++e) L.inst_edge0[inst[m.edge_cells[2 * e]] + 1]++;
@@ -169,12 +196,7 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         for (int r = 0; r < L.q; ++r) L.int_of_ref[S.nflux_total + (int64_t)c * L.q + r] = L.cbase + (L.ni + r) * L.cpad + gp;
     }
     if (L.lam) L.int_of_ref[S.ntot - 1] = L.lam_idx;
-    // packed edge-dof indices; an interior edge's first side is the adjacent
-    // cell with the lower store position
-    std::vector<int> side0(m.ne, INT32_MAX);
-    for (int c = 0; c < nc; ++c)
-        for (int j = m.off[c]; j < m.off[c + 1]; ++j)
-            side0[m.cell_edges[j]] = std::min(side0[m.cell_edges[j]], L.gpos_of_cell[c]);
+    // packed edge-dof indices (side0: the edges' first sides, above)
     // per-cell stream lengths in 16-byte vectors: packed A, then the rows
     // 1..q-1 of W at the edge columns, each padded
     auto nvec_of = [&](const GroupDesc& g, int vw) {
