@@ -35,8 +35,6 @@ struct MultiState {
     float* op = nullptr;            // per cell: packed A (lower triangle) then W (q x n), padded to 4 floats
     MrGroup* mg = nullptr;          // group table of the apply
     int ngroups = 0;
-    int nwaves = 0;
-    int64_t cells_per_wave = 0;
     float *R[2] = {nullptr, nullptr}, *Z[2] = {nullptr, nullptr}, *WB[2] = {nullptr, nullptr};
     float *x = nullptr, *Az = nullptr, *Az1 = nullptr, *b = nullptr;  // [entry][NR]
     double *x64 = nullptr, *r64 = nullptr, *y64 = nullptr;  // [NR][nvec]
@@ -50,6 +48,10 @@ struct MultiState {
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
     bool graph_amg = false;
+    int pw_waves = 0;               // waves writing x.Kx partials (entries of pw per column)
+    std::vector<MrGroup> mg_host;   // group table (per-valence launches)
+    std::vector<int64_t> mr_cpw;   // per group: cells per wave, blocks, first wave
+    std::vector<int> mr_blocks, mr_wave0;
     std::vector<void*> allocs;
 };
 
@@ -95,218 +97,269 @@ struct MrArgs {
     float* Y;
     float* Y1;                // second sides of the edge dofs
     int64_t cbase, cpad;
-    int64_t trash;            // a padding row: target of the stores of rows a cell does not have
-    int64_t cells_per_wave;
+    int64_t trash;            // the padding row (zeroed by each apply; no cell stores there)
     double* pw;               // per-wave, per-column partials of x.Kx (nullptr: not needed)
     const MState* st;         // per column
     const int* all_done;
 };
 
-// Entry (i, c) of the cell block [[s A, -W^T], [-W, 0]] (n flux rows, m rows
-// in all) from the packed operator in LDS.
-__device__ __forceinline__ float kval(const float* op, int n, int na, int m, int i, int c, float s) {
-    if (i >= m || c >= m) return 0.0f;
-    if (i < n && c < n) {
-        const int a = i > c ? i : c, b = i > c ? c : i;
-        return s * op[a * (a + 1) / 2 + b];
-    }
-    if (i < n) return -op[na + (c - n) * n + i];
-    if (c < n) return -op[na + (i - n) * n + c];
-    return 0.0f;
-}
+// The MFMA element apply on 32 interleaved columns: y = [[s A, -W^T], [-W,
+// 0]] x per cell, one wavefront per cell, one launch per valence group with
+// the valence a template argument (the MFMA step count, the row count and
+// each lane's offsets into the staged operator are constants or loop
+// invariants of a run of cells, and the kernel's registers are those of one
+// valence body: 76-126 VGPRs, 4-5 waves per SIMD).
+//
+// v_mfma_f32_32x32x2f32 with the gathered rows as the A operand (lane l: row
+// 2 kk + l / 32 of column l % 32) and the cell block as the B operand (lane
+// l: entry (l % 32, 2 kk + l / 32)), so D = (block X)^T: lane l holds cell
+// row l % 32 for the 16 columns 8 g + 4 (l / 32) + e (g, e < 4), and its
+// results go out as four 16-byte stores to one row address.  A block of m
+// rows fills m of the tile's 32 rows (a hexagon at k = 1: 18).
+// ---------------------------------------------------------------------------
 
-constexpr int KSTEPS = 13;  // ceil(m / 2) <= 13 for polygons of up to 10 edges (k = 1)
+constexpr int MR_LDX = 33;  // row pitch of the per-wave x.Kx transpose tile (bank-conflict free)
+#ifndef MR_WAVES
+#define MR_WAVES 4  // occupancy bound of k_apply_mr (waves per SIMD)
+#endif
 
-// The shape of a cell of group g (wave-uniform).
-template <int K>
-struct MrShape {
-    int n, m, na, nv4;
-    __device__ __forceinline__ MrShape(const MrGroup& g) {
-        constexpr int Q = (K + 1) * (K + 2) / 2;
-        n = g.n;
-        m = n + Q;
-        na = n * (n + 1) / 2;
-        nv4 = (na + Q * n + 3) / 4;
-    }
+template <int K, int NE>
+struct MrC {
+    static constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1));
+    static constexpr int NED = KK * NE, N = NED + NI, M = N + Q, NA = N * (N + 1) / 2;
+    static constexpr int NV4 = (NA + Q * N + 3) / 4, ZERO = NV4 * 4, KS = (M + 1) / 2;
+    static_assert(ZERO < OPMAX, "staged operator and its zero entry must fit the LDS buffer");
+    static_assert(M <= 32, "a cell block must fit the 32 rows of the MFMA tile");
 };
 
-// Issues the loads of one cell (gpos < 0: none): two float4 of its packed
-// operator per lane, its scale and the lane's row slot (rows < m). They are
-// independent of each other: the group table sits in LDS.
-template <int K>
-__device__ __forceinline__ void cell_loads(const MrArgs& a, const MrGroup& g, int64_t gpos, int l, float4& o0,
-                                           float4& o1, uint32_t& slot, float& s) {
-    constexpr int KK = K + 1, Q = (K + 1) * (K + 2) / 2, NI = (Q - 1) + (Q - (K + 1));
+// Loads of one cell (gp < 0: none): its packed operator (float4 per lane,
+// two rounds), its scale and the lane's row slot (lanes < M).
+template <int K, int NE>
+__device__ __forceinline__ void mr_loads(const MrArgs& a, const MrGroup& g, int64_t gp, int l, float4& o0, float4& o1,
+                                          uint32_t& slot, float& s) {
+    using C = MrC<K, NE>;
     const float4 z = {0.0f, 0.0f, 0.0f, 0.0f};
     o0 = o1 = z;
     slot = 0;
     s = 0.0f;
-    if (gpos < 0) return;
-    const MrShape<K> sh(g);
-    const int ne = g.ne, ned = KK * ne;
-    const int64_t cg = gpos - g.gpos0;
-    const float4* src = reinterpret_cast<const float4*>(a.op + g.gop) + cg * sh.nv4;
-    if (l < sh.nv4) o0 = src[l];
-    if (l + 64 < sh.nv4) o1 = src[l + 64];
-    s = a.scale[gpos];
-    if (l < ned) {
-        const uint32_t id = a.idx[g.i_off + ((cg >> 6) * ne + l % ne) * 64 + (cg & 63)];
-        slot = (id & SIDE1) | ((id & 0x7fffffffu) + (uint32_t)(l / ne));
-    } else if (l < sh.n) {
-        slot = (uint32_t)(a.cbase + (l - ned) * a.cpad + gpos);
-    } else if (l < sh.m) {
-        slot = (uint32_t)(a.cbase + (NI + l - sh.n) * a.cpad + gpos);
+    if (gp < 0) return;
+    const int64_t cg = gp - g.gpos0;
+    const float4* src = reinterpret_cast<const float4*>(a.op + g.gop) + cg * C::NV4;
+    if (l < C::NV4) o0 = src[l];
+    if (C::NV4 > 64 && l + 64 < C::NV4) o1 = src[l + 64];
+    s = a.scale[gp];
+    if (l < C::NED) {
+        const uint32_t id = a.idx[g.i_off + ((cg >> 6) * NE + l % NE) * 64 + (cg & 63)];
+        slot = (id & SIDE1) | ((id & 0x7fffffffu) + (uint32_t)(l / NE));
+    } else if (l < C::N) {
+        slot = (uint32_t)(a.cbase + (l - C::NED) * a.cpad + gp);
+    } else if (l < C::M) {
+        slot = (uint32_t)(a.cbase + (C::NI + l - C::N) * a.cpad + gp);
     }
 }
 
-// First real cell at or after store position p and before end, moving the
-// group cursor gi along (wave-uniform); -1 when there is none.
-__device__ __forceinline__ int64_t next_cell(const MrGroup* lg, int ng, int64_t p, int64_t end, int& gi) {
-    while (gi < ng && p >= (int64_t)lg[gi].gpos0 + lg[gi].count) {
-        ++gi;
-        if (gi < ng && p < lg[gi].gpos0) p = lg[gi].gpos0;
+// The cells of one group at store positions [p0, p1) (all real), pipelined as
+// in k_apply_mr: while cell i is multiplied the rows of cell i + 1 are
+// gathered and the operator of cell i + 2 is loaded.
+template <int K, int NE>
+__device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_t p0, int64_t p1, int l,
+                                        const float* __restrict__ Xj, float (*lop)[OPMAX], uint32_t (*lsl)[32],
+                                        float* lkx, double& dot) {
+    using C = MrC<K, NE>;
+    const int hi = l >> 5, j = l & 31;
+    // this lane's entry of the cell block at step kk: staged operator offset,
+    // and whether it belongs to s A (else -W or zero)
+    // two 16-bit offsets per register (the operator has at most 348 entries)
+    uint32_t off2[(C::KS + 1) / 2] = {};
+    uint32_t isA = 0;
+#pragma unroll
+    for (int kk = 0; kk < C::KS; ++kk) {
+        const int c = 2 * kk + hi;
+        int o = C::ZERO;
+        if (j < C::M && c < C::M) {
+            if (j < C::N && c < C::N) {
+                const int r = j > c ? j : c, q = j > c ? c : j;
+                o = r * (r + 1) / 2 + q;
+                isA |= 1u << kk;
+            } else if (j < C::N) {
+                o = C::NA + (c - C::N) * C::N + j;
+            } else if (c < C::N) {
+                o = C::NA + (j - C::N) * C::N + c;
+            }
+        }
+        off2[kk >> 1] |= (uint32_t)o << (16 * (kk & 1));
     }
-    return gi < ng && p < end ? p : -1;
+    auto stage = [&](const float4& o0, const float4& o1, uint32_t slot, int b, float (&B)[C::KS])
+        __attribute__((always_inline)) {
+        float4* d = reinterpret_cast<float4*>(lop[b]);
+        if (l < C::NV4) d[l] = o0;
+        if (C::NV4 > 64 && l + 64 < C::NV4) d[l + 64] = o1;
+        if (l == 0) lop[b][C::ZERO] = 0.0f;
+        if (l < C::M) lsl[b][l] = slot;
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int kk = 0; kk < C::KS; ++kk) {
+            const int c = 2 * kk + hi;
+            if (c < C::M) {
+                const uint32_t sl = lsl[b][c];
+                B[kk] = Xj[(int64_t)(sl & 0x7fffffffu) * NR + (l & 31)];
+            } else {
+                B[kk] = 0.0f;  // the odd row count's last step, upper half-wave
+            }
+        }
+    };
+    auto multiply = [&](int b, float s, const float (&B)[C::KS]) __attribute__((always_inline)) {
+        f16v acc = {};
+#pragma unroll
+        for (int kk = 0; kk < C::KS; ++kk) {
+            const float v = lop[b][(off2[kk >> 1] >> (16 * (kk & 1))) & 0xffffu];
+            const float av = (isA >> kk) & 1u ? s * v : -v;
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(B[kk], av, acc, 0, 0, 0);
+        }
+        if (j < C::M) {
+            const uint32_t sl = lsl[b][j];
+            float* __restrict__ dst = (sl & SIDE1) ? a.Y1 : a.Y;
+            float4* row = reinterpret_cast<float4*>(dst + (int64_t)(sl & 0x7fffffffu) * NR);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) row[2 * q + hi] = float4{acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+        }
+        if (a.pw) {
+            // x.Kx: (K x) goes through LDS to the lanes that hold x
+            if (j < C::M) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) lkx[j * MR_LDX + 8 * (r >> 2) + 4 * hi + (r & 3)] = acc[r];
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int kk = 0; kk < C::KS; ++kk) {
+                const int c = 2 * kk + hi;
+                if (c < C::M) dot += (double)B[kk] * (double)lkx[c * MR_LDX + (l & 31)];
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    };
+    // Three cells in flight: while cell c is multiplied, the rows of c + 1 and
+    // c + 2 are being gathered and the operator of c + 3 is being loaded.
+    // Cell j's registers, LDS buffer and rows are set j % 3; the loop body is
+    // written three times with the roles rotated, so register sets are never
+    // copied (a copy would wait for the loads still filling them).
+    struct Set {
+        float4 o0, o1;
+        uint32_t sl;
+        float s;
+        int64_t gp;
+    };
+    Set S0, S1, S2;
+    float B0[C::KS], B1[C::KS], B2[C::KS];
+    auto load = [&](Set& t, int64_t gp) __attribute__((always_inline)) {
+        t.gp = gp;
+        mr_loads<K, NE>(a, g, gp, l, t.o0, t.o1, t.sl, t.s);
+    };
+    auto nxt = [&](int64_t gp) { return gp >= 0 && gp + 1 < p1 ? gp + 1 : (int64_t)-1; };
+    load(S0, p0);
+    load(S1, nxt(p0));
+    stage(S0.o0, S0.o1, S0.sl, 0, B0);
+    load(S2, nxt(S1.gp));
+    if (S1.gp >= 0) stage(S1.o0, S1.o1, S1.sl, 1, B1);
+    // step: multiply cell c (set c, buffer bc), stage cell n2 (two behind it)
+    // into its buffer with its rows gathered, and load the cell after n2 into
+    // c's registers
+    auto step = [&](Set& c, Set& n2, float (&Bc)[C::KS], float (&Bn2)[C::KS], int bc, int bn2)
+        __attribute__((always_inline)) {
+        const float sc = c.s;
+        load(c, nxt(n2.gp));
+        if (n2.gp >= 0) stage(n2.o0, n2.o1, n2.sl, bn2, Bn2);
+        multiply(bc, sc, Bc);
+        __builtin_amdgcn_wave_barrier();  // buffer bc is free for the cell loaded now
+    };
+    while (S0.gp >= 0) {
+        step(S0, S2, B0, B2, 0, 2);
+        if (S1.gp < 0) break;
+        step(S1, S0, B1, B0, 1, 0);
+        if (S2.gp < 0) break;
+        step(S2, S1, B2, B1, 2, 1);
+    }
 }
 
-// Persistent waves, each over a contiguous range of store positions.  The cell
-// loop is software-pipelined: while cell c is multiplied, the rows of cell
-// c + 1 are being gathered and the operator of cell c + 2 is being loaded.
-template <int K>
-__global__ __launch_bounds__(256, 3) void k_apply_mr(MrArgs a) {
+// One launch per valence group: the waves of the launch share the group's
+// cells in contiguous runs; wave0 is the launch's first wave in the x.Kx
+// partials (pw).  A single valence body per kernel keeps the register budget
+// to that body's needs (about 85), so more waves fit on a SIMD and more row
+// gathers are in flight.
+template <int K, int NE>
+__global__ __launch_bounds__(256, MR_WAVES) void k_apply_mr(MrArgs a, MrGroup g, int64_t cpw, int wave0, int zero_trash) {
     if (a.all_done && *a.all_done) return;
-    __shared__ __attribute__((aligned(16))) float lop[4][2][OPMAX];
-    __shared__ uint32_t lslot[4][2][32];
-    __shared__ MrGroup lg[MR_MAXG];
-    if ((int)threadIdx.x < a.ngroups) lg[threadIdx.x] = a.mg[threadIdx.x];
-    __syncthreads();
-    const int ng = a.ngroups;
+    __shared__ __attribute__((aligned(16))) float lop[4][3][OPMAX];
+    __shared__ uint32_t lsl[4][3][32];
+    __shared__ float lkx[4][32 * MR_LDX];
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int wave = blockIdx.x * 4 + w;
     const int l = threadIdx.x & 63;
-    const int col = l & 31, hi = l >> 5;
+    const int col = l & 31;
     const float* Xj = a.st ? a.X[a.st[col].itn & 1] : a.X[0];
+    // rows a cell does not have are not stored; the padding row reads zero
+    if (zero_trash && wave == 0 && l < NR) a.Y[a.trash * NR + l] = 0.0f;
     double dot = 0.0;
-    const int64_t g0 = (int64_t)wave * a.cells_per_wave;
-    const int64_t g1 = g0 + a.cells_per_wave < a.cpad ? g0 + a.cells_per_wave : a.cpad;
-    // stage a cell's operator and slots into LDS buffer b and issue its row gathers
-    auto stage = [&](int m, int nv4, float4 o0, float4 o1, uint32_t slot, int b, float(&B)[KSTEPS])
-        __attribute__((always_inline)) {
-        float4* d = reinterpret_cast<float4*>(lop[w][b]);
-        if (l < nv4) d[l] = o0;
-        if (l + 64 < nv4) d[l + 64] = o1;
-        if (l < m) lslot[w][b][l] = slot;
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        __builtin_amdgcn_wave_barrier();
-        // every lane loads (rows the cell does not have read the padding row
-        // and are zeroed), so the loads stay in one basic block and the wait
-        // counters track them exactly
-#pragma unroll
-        for (int kk = 0; kk < KSTEPS; ++kk) {
-            const int cc = 2 * kk + hi;
-            const bool on = cc < m;
-            const uint32_t sl = lslot[w][b][on ? cc : 0];
-            const int64_t row = on ? (int64_t)(sl & 0x7fffffffu) : a.trash;
-            const float v = Xj[row * NR + col];
-            B[kk] = on ? v : 0.0f;
-        }
-    };
-    // Pipeline over the wave's cells: step i multiplies cell i while the rows
-    // of cell i + 1 are gathered and the operator, scale and slots of cell
-    // i + 2 are loaded. Cell j's loads land in register set j % 2 and its rows
-    // in buffer j % 2, and the loop body is written twice with the roles
-    // swapped, so no register that a load is still filling is ever copied
-    // (a copy would make the wave wait for that load).
-    float BX[KSTEPS], BY[KSTEPS];
-#pragma unroll
-    for (int kk = 0; kk < KSTEPS; ++kk) BX[kk] = BY[kk] = 0.0f;
-    float4 oP0, oP1, oQ0, oQ1;
-    uint32_t slP, slQ;
-    float sP, sQ;
-    int64_t gpP, gpQ;
-    int gP, gQ;
-    int gi = 0;
-    gpP = next_cell(lg, ng, g0, g1, gi);
-    gP = gi < ng ? gi : 0;
-    cell_loads<K>(a, lg[gP], gpP, l, oP0, oP1, slP, sP);
-    if (gpP >= 0) {
-        const MrShape<K> sh(lg[gP]);
-        stage(sh.m, sh.nv4, oP0, oP1, slP, 0, BX);
-    }
-    gpQ = gpP >= 0 ? next_cell(lg, ng, gpP + 1, g1, gi) : -1;
-    gQ = gi < ng ? gi : 0;
-    cell_loads<K>(a, lg[gQ], gpQ, l, oQ0, oQ1, slQ, sQ);
-    // one step: A (set a_*, rows Bm, LDS buffer b) is multiplied, B (set b_*)
-    // is staged into buffer b ^ 1 with its rows gathered into Bg, and the
-    // next cell is loaded into A's set
-    auto step = [&](float4& aO0, float4& aO1, uint32_t& aSl, float& aS, int64_t& aGp, int& aG, const float4& bO0,
-                    const float4& bO1, const uint32_t& bSl, const int64_t& bGp, const int& bG, float(&Bm)[KSTEPS],
-                    float(&Bg)[KSTEPS], int b) __attribute__((always_inline)) {
-        const float curs = aS;
-        const MrShape<K> cur(lg[aG]);
-        const int64_t gpC = bGp >= 0 ? next_cell(lg, ng, bGp + 1, g1, gi) : -1;
-        aGp = gpC;
-        aG = gi < ng ? gi : 0;
-        cell_loads<K>(a, lg[aG], gpC, l, aO0, aO1, aSl, aS);
-        if (bGp >= 0) {
-            const MrShape<K> shB(lg[bG]);
-            stage(shB.m, shB.nv4, bO0, bO1, bSl, b ^ 1, Bg);
-        }
-        // multiply the cell
-        const float* op = lop[w][b];
-        const int ks = (cur.m + 1) / 2;
-        f16v acc = {};
-#pragma unroll
-        for (int kk = 0; kk < KSTEPS; ++kk) {
-            if (kk < ks) {
-                const float av = kval(op, cur.n, cur.na, cur.m, col, 2 * kk + hi, curs);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Bm[kk], acc, 0, 0, 0);
-            }
-        }
-        // D[row][col]: col = lane % 32, row = 8 (r / 4) + 4 (lane / 32) + r % 4;
-        // rows the cell does not have hold zeros (their operator rows are
-        // zero) and go to the padding row, without a branch
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = 8 * (r >> 2) + 4 * hi + (r & 3);
-            const bool on = row < cur.m;
-            const uint32_t sl = lslot[w][b][on ? row : 0];
-            const bool side1 = on && (sl & SIDE1);
-            float* __restrict__ dst = side1 ? a.Y1 : a.Y;
-            const int64_t e = (on ? (int64_t)(sl & 0x7fffffffu) : a.trash) * NR + col;
-            dst[e] = acc[r];
-        }
-        if (a.pw) {
-            // x.Kx of the cell from registers: row 2 kk + h of column col is
-            // Bm[kk] of lane (col, h); the other half-wave's rows come across
-            float P[KSTEPS];
-#pragma unroll
-            for (int kk = 0; kk < KSTEPS; ++kk) P[kk] = __shfl_xor(Bm[kk], 32, 64);
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = 8 * (r >> 2) + 4 * hi + (r & 3);
-                const int i0 = 4 * (r >> 2) + ((r & 3) >> 1);  // row / 2 on the lower half-wave
-                float xv = 0.0f;
-                if (hi) {
-                    if (i0 + 2 < KSTEPS) xv = (r & 1) ? Bm[i0 + 2 < KSTEPS ? i0 + 2 : 0] : P[i0 + 2 < KSTEPS ? i0 + 2 : 0];
-                } else {
-                    if (i0 < KSTEPS) xv = (r & 1) ? P[i0 < KSTEPS ? i0 : 0] : Bm[i0 < KSTEPS ? i0 : 0];
-                }
-                if (row < cur.m) dot += (double)xv * (double)acc[r];
-            }
-        }
-        __builtin_amdgcn_wave_barrier();  // buffer b is free for the cell after next
-    };
-    while (gpP >= 0) {
-        step(oP0, oP1, slP, sP, gpP, gP, oQ0, oQ1, slQ, gpQ, gQ, BX, BY, 0);
-        if (gpQ < 0) break;
-        step(oQ0, oQ1, slQ, sQ, gpQ, gQ, oP0, oP1, slP, gpP, gP, BY, BX, 1);
-    }
+    const int64_t end = (int64_t)g.gpos0 + g.count;
+    const int64_t p0 = (int64_t)g.gpos0 + (int64_t)wave * cpw;
+    const int64_t p1 = p0 + cpw < end ? p0 + cpw : end;
+    if (p0 < p1) mr_run<K, NE>(a, g, p0, p1, l, Xj, lop[w], lsl[w], lkx[w], dot);
     if (a.pw) {
         dot += __shfl_xor(dot, 32, 64);
-        if (l < 32) a.pw[(int64_t)wave * NR + col] = dot;
+        if (l < 32) a.pw[(int64_t)(wave0 + wave) * NR + col] = dot;
     }
 }
+
+constexpr int MR_MIN_RUN = 1;  // fewest cells a wave of k_apply_mr runs through
+
+template <int K>
+static void mr_launch(int ne, dim3 grid, hipStream_t st, const MrArgs& a, const MrGroup& g, int64_t cpw, int wave0,
+                       int zero_trash) {
+    switch (ne) {
+#define MR_CASE(NE_)                                                                                    \
+    case NE_:                                                                                            \
+        hipLaunchKernelGGL((k_apply_mr<K, NE_>), grid, dim3(256), 0, st, a, g, cpw, wave0, zero_trash); \
+        break;
+        MR_CASE(3)
+        MR_CASE(4)
+        MR_CASE(5)
+        MR_CASE(6)
+        MR_CASE(7)
+        MR_CASE(8)
+        MR_CASE(9)
+        MR_CASE(10)
+#undef MR_CASE
+        default:
+            throw Error(VEMD_ERR_MESH, "the multi-RHS apply handles polygons of 3 to 10 edges");
+    }
+}
+
+template <int K>
+static int mr_occupancy_k(int ne) {
+    int bpc = 0;
+    switch (ne) {
+#define MR_OCC(NE_)                                                                         \
+    case NE_:                                                                                \
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_apply_mr<K, NE_>, 256, 0)); \
+        break;
+        MR_OCC(3)
+        MR_OCC(4)
+        MR_OCC(5)
+        MR_OCC(6)
+        MR_OCC(7)
+        MR_OCC(8)
+        MR_OCC(9)
+        MR_OCC(10)
+#undef MR_OCC
+        default:
+            throw Error(VEMD_ERR_MESH, "the multi-RHS apply handles polygons of 3 to 10 edges");
+    }
+    return bpc;
+}
+
+static int mr_occupancy(int k, int ne) { return k == 0 ? mr_occupancy_k<0>(ne) : mr_occupancy_k<1>(ne); }
 
 // ---------------------------------------------------------------------------
 // Simultaneous MINRES over NR interleaved columns (one MState per column)
@@ -674,20 +727,27 @@ static void multi_build(Solver& s, MultiState& m) {
     m.ngroups = (int)mg.size();
     m.mg = (MrGroup*)malloc_m(m, mg.size() * sizeof(MrGroup));
     HIPCHK(hipMemcpy(m.mg, mg.data(), mg.size() * sizeof(MrGroup), hipMemcpyHostToDevice));
-    // persistent waves: about 8 per CU of the chip
+    // one launch per valence group: as many waves as fit on the chip at once
+    // (the occupancy of that valence's kernel), each over a contiguous run of
+    // the group's cells
     int dev = 0, ncu = 256;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    // exactly the waves that fit on the chip at once (a second round of
-    // persistent waves would run after the first and double the time)
-    int bpc = 0;
-    if (s.layout.k == 0)
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_apply_mr<0>, 256, 0));
-    else
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_apply_mr<1>, 256, 0));
-    if (const char* e = std::getenv("VEMD_MR_BLOCKS_PER_CU")) bpc = std::max(1, std::atoi(e));
-    m.nwaves = std::max(1, ncu * std::max(bpc, 1)) * 4;
-    m.cells_per_wave = (L.cpad + m.nwaves - 1) / m.nwaves;
+    int wave0 = 0;
+    m.mg_host = mg;
+    for (const MrGroup& g : mg) {
+        int occ = mr_occupancy(s.layout.k, g.ne);
+        if (const char* e = std::getenv("VEMD_MR_BLOCKS_PER_CU")) occ = std::max(1, std::atoi(e));
+        const int64_t fit = (int64_t)ncu * std::max(occ, 1) * 4;
+        const int64_t cpw = std::max<int64_t>(MR_MIN_RUN, (g.count + fit - 1) / fit);
+        const int64_t waves = (g.count + cpw - 1) / cpw;
+        const int blocks = (int)std::max<int64_t>(1, (waves + 3) / 4);
+        m.mr_cpw.push_back(cpw);
+        m.mr_blocks.push_back(blocks);
+        m.mr_wave0.push_back(wave0);
+        wave0 += 4 * blocks;
+    }
+    m.pw_waves = std::max(wave0, 1);
     const size_t fv = (size_t)L.nvec * NR * sizeof(float);
     for (int i = 0; i < 2; ++i) {
         m.R[i] = (float*)malloc_m(m, fv);
@@ -710,7 +770,7 @@ static void multi_build(Solver& s, MultiState& m) {
     m.rtol = (double*)malloc_m(m, NR * sizeof(double));
     HIPCHK(hipHostMalloc((void**)&m.rtol_host, NR * sizeof(double), hipHostMallocDefault));
     const int nch = (int)L.chunks.size();
-    m.pw = (double*)malloc_m(m, (size_t)m.nwaves * NR * sizeof(double));
+    m.pw = (double*)malloc_m(m, (size_t)m.pw_waves * NR * sizeof(double));
     for (double** p : {&m.p1, &m.p2, &m.q1, &m.q2}) {
         *p = (double*)malloc_m(m, (size_t)nch * NR * sizeof(double));
         HIPCHK(hipMemset(*p, 0, (size_t)nch * NR * sizeof(double)));
@@ -738,7 +798,6 @@ static MrArgs mr_args(Solver& s, MultiState& m, const float* x0, const float* x1
     a.cbase = s.layout.cbase;
     a.trash = s.layout.trash_idx;
     a.cpad = s.layout.cpad;
-    a.cells_per_wave = m.cells_per_wave;
     a.pw = solve ? m.pw : nullptr;
     a.st = solve ? m.st : nullptr;
     a.all_done = solve ? m.all_done : nullptr;
@@ -746,11 +805,16 @@ static MrArgs mr_args(Solver& s, MultiState& m, const float* x0, const float* x1
 }
 
 static void launch_apply_mr(Solver& s, MultiState& m, const MrArgs& a, hipStream_t st) {
-    const dim3 grid(m.nwaves / 4), block(256);
-    if (s.layout.k == 0)
-        hipLaunchKernelGGL((k_apply_mr<0>), grid, block, 0, st, a);
-    else
-        hipLaunchKernelGGL((k_apply_mr<1>), grid, block, 0, st, a);
+    int first = 1;  // the first launch zeroes the padding row
+    for (size_t gi = 0; gi < m.mg_host.size(); ++gi) {
+        const MrGroup& g = m.mg_host[gi];
+        if (g.count == 0) continue;
+        if (s.layout.k == 0)
+            mr_launch<0>(g.ne, dim3(m.mr_blocks[gi]), st, a, g, m.mr_cpw[gi], m.mr_wave0[gi], first);
+        else
+            mr_launch<1>(g.ne, dim3(m.mr_blocks[gi]), st, a, g, m.mr_cpw[gi], m.mr_wave0[gi], first);
+        first = 0;
+    }
 }
 
 void dev_apply_multi(Solver& s, const void* x, void* y, void* y1, void* stream) {
@@ -780,7 +844,7 @@ static void enqueue_iteration_mr(Solver& s, MultiState& m, bool amg) {
     DeviceState& d = *s.dev;
     MrArgs a = mr_args(s, m, m.Z[0], m.Z[1], m.Az, m.Az1, true);
     launch_apply_mr(s, m, a, d.stream);
-    hipLaunchKernelGGL(k_finA_mr, dim3(NR), dim3(256), 0, d.stream, m.st, (const double*)m.pw, m.nwaves);
+    hipLaunchKernelGGL(k_finA_mr, dim3(NR), dim3(256), 0, d.stream, m.st, (const double*)m.pw, m.pw_waves);
     MrUpd u = mr_upd(s, m, amg);
     hipLaunchKernelGGL(k_update_mr, dim3(d.nchunks), dim3(256), 0, d.stream, u);
     if (amg) {
