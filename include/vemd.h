@@ -184,8 +184,9 @@ typedef struct {
                            diagonal pressure/multiplier Schur approximations (the reference's
                            choice, GPU/src/equation_gpu.cu:282-315); VEMD_PC_MULTIGRID: the same
                            flux block and one smoothed-aggregation V-cycle on the pressure Schur
-                           approximation W diag(A)^-1 W^T (Dirichlet pressure problems);
-                           VEMD_PC_AUTO (0): multigrid where built, else Jacobi */
+                           approximation S = W diag(A)^-1 W^T (with the mean-zero multiplier, on
+                           S + rho L L^T, the multiplier row then preconditioned by 1 / rho);
+                           VEMD_PC_AUTO (0): multigrid where built (k = 0 and 1), else Jacobi */
     void* stream;       /* the caller's HIP stream (hipStream_t; NULL = the null stream).  The
                            device-resident solves (vemd_solve_device, vemd_solve_multi_device)
                            wait for work queued on it before they read b_dev, and work queued

@@ -296,9 +296,15 @@ static void dense_inverse(std::vector<double>& a, int64_t n) {
 }
 
 Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const std::vector<double>& b00,
-              const std::vector<int>& node_order0, const AmgOptions& o) {
+              const std::vector<int>& node_order0, const AmgOptions& o, const std::vector<double>* l0, double rho) {
     Amg H;
     H.opt = o;
+    std::vector<double> lvec;
+    if (l0) {
+        if ((int64_t)l0->size() != A.nrows) throw Error(VEMD_ERR_INTERNAL, "rank-one term of the wrong length");
+        lvec = *l0;
+        H.rho = rho;
+    }
     std::vector<int> nor = node_of_row0, order = node_order0;
     std::vector<double> b0 = b00;
     int nn = nnodes0;
@@ -401,6 +407,11 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         AP = Csr();
         Csr R = csr_transpose(P);
         Csr Ac = csr_mult(R, csr_mult(A, P));
+        if (!lvec.empty()) {
+            std::vector<double> lc(R.nrows);
+            csr_spmv(R, lvec.data(), lc.data());
+            lvec.swap(lc);
+        }
         L.A = std::move(A);
         L.P = std::move(P);
         L.R = std::move(R);
@@ -422,6 +433,9 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
     H.coarse_inv.assign(H.ncoarse * H.ncoarse, 0.0);
     for (int64_t i = 0; i < H.ncoarse; ++i)
         for (int64_t p = C.A.rp[i]; p < C.A.rp[i + 1]; ++p) H.coarse_inv[i * H.ncoarse + C.A.ci[p]] = C.A.v[p];
+    if (!lvec.empty())
+        for (int64_t i = 0; i < H.ncoarse; ++i)
+            for (int64_t j = 0; j < H.ncoarse; ++j) H.coarse_inv[i * H.ncoarse + j] += rho * lvec[i] * lvec[j];
     dense_inverse(H.coarse_inv, H.ncoarse);
     return H;
 }

@@ -42,6 +42,7 @@ struct Amg {
     std::vector<AmgLevel> lv;         // lv[0] finest
     std::vector<double> coarse_inv;   // dense inverse of the coarsest operator, row-major
     int64_t ncoarse = 0;
+    double rho = 0.0;                 // weight of the rank-one coarse term (0: none)
 };
 
 // Pressure Schur approximation on the cells' pressure moments, cell-major:
@@ -52,9 +53,14 @@ Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::ve
 
 // Builds the hierarchy.  Rows are grouped into nodes (node_of_row, -1 = row
 // left to the smoother), b0 is the near-null vector (one value per row) and
-// node_order the order in which aggregates are grown.
+// node_order the order in which aggregates are grown.  With l0 (one value per
+// row) the operator may be singular: the coarsest operator becomes A_c + rho
+// l_c l_c^T, l_c = R ... R l0 (the restrictions of the levels), so that the
+// V-cycle is symmetric positive definite and approximates (A + rho l0 l0^T)^-1
+// (a null vector of A with a component along l0 is moved off zero).
 Amg amg_build(Csr A0, const std::vector<int>& node_of_row, int nnodes, const std::vector<double>& b0,
-              const std::vector<int>& node_order, const AmgOptions& o);
+              const std::vector<int>& node_order, const AmgOptions& o, const std::vector<double>* l0 = nullptr,
+              double rho = 0.0);
 
 // Helpers exposed for tests: y = A x, C = A B, transpose.
 void csr_spmv(const Csr& A, const double* x, double* y);

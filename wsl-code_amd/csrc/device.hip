@@ -1555,8 +1555,16 @@ static void solve_device(Solver& s, const vemd_solve_opts& o, vemd_solve_info* i
     bool graph = o.use_graph != 0;
     int chk = o.check_every > 0 ? o.check_every : 32;
     if (o.precond == VEMD_PC_MULTIGRID && !d.amg64)
-        throw Error(VEMD_ERR_ARG, "no multigrid hierarchy: it is built for Dirichlet pressure problems only");
+        throw Error(VEMD_ERR_ARG, "no multigrid hierarchy on this solver");
     const bool amg = o.precond != VEMD_PC_JACOBI && d.amg64 != nullptr;
+    if (s.layout.lam) {
+        // the multiplier row's preconditioner depends on the pressure block's
+        const double lm = amg ? s.amg_lam_minv : s.minv[s.layout.lam_idx];
+        const float lf = (float)lm;
+        HIPCHK(hipMemcpyAsync(d.o64.minv + s.layout.lam_idx, &lm, sizeof(double), hipMemcpyHostToDevice, d.stream));
+        HIPCHK(hipMemcpyAsync(d.o32.minv + s.layout.lam_idx, &lf, sizeof(float), hipMemcpyHostToDevice, d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));  // the sources are on this stack frame
+    }
     auto worst = [&]() { return *std::max_element(rel.begin(), rel.end()); };
     std::vector<double> target(ni);
     // Inner targets are relative to each pass's right-hand side and are met in

@@ -939,7 +939,7 @@ void dev_solve_multi(Solver& s, const vemd_solve_opts& o, int ncol, const double
         throw Error(VEMD_ERR_ARG, "multi-right-hand-side solves run FP32 MFMA iterations inside FP64 refinement "
                                   "(VEMD_MIXED)");
     if (o.precond == VEMD_PC_MULTIGRID && !d.amg32)
-        throw Error(VEMD_ERR_ARG, "no multigrid hierarchy: it is built for Dirichlet pressure problems only");
+        throw Error(VEMD_ERR_ARG, "no multigrid hierarchy on this solver");
     MultiState& m = multi_setup(s);
     if (d.amg32) amg_multi_alloc(*d.amg32);  // the hierarchy is rebuilt when theta changes
     const bool amg = o.precond != VEMD_PC_JACOBI && d.amg32 != nullptr;

@@ -19,11 +19,12 @@ struct Solver {
     std::vector<double> minv;      // preconditioner inverse, internal order
     std::vector<double> rowscale;  // internal = rowscale * reference-form rows
     std::vector<double> rhs_ref;   // reference-form right-hand side
-    // multigrid hierarchy of the pressure Schur approximation (Dirichlet
-    // pressure problems; the mean-zero multiplier problem keeps the diagonal
-    // preconditioner because its pressure block is singular)
+    // multigrid hierarchy of the pressure Schur approximation (with the
+    // mean-zero multiplier: of that approximation plus a rank-one term along
+    // the multiplier row, whose inverse weight preconditions the multiplier)
     std::unique_ptr<Amg> amg;
     AmgOptions amg_opt;
+    double amg_lam_minv = 0.0;     // multiplier entry of M^-1 with the multigrid cycle (0: no multiplier)
     double amg_setup_s = 0.0;
     DeviceState* dev = nullptr;
     int device = -1;
