@@ -406,7 +406,8 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         }
         AP = Csr();
         Csr R = csr_transpose(P);
-        Csr Ac = csr_mult(R, csr_mult(A, P));
+        Csr M = csr_mult(R, A);
+        Csr Ac = csr_mult(M, P);
         if (!lvec.empty()) {
             std::vector<double> lc(R.nrows);
             csr_spmv(R, lvec.data(), lc.data());
@@ -415,6 +416,7 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         L.A = std::move(A);
         L.P = std::move(P);
         L.R = std::move(R);
+        L.M = std::move(M);
         H.lv.push_back(std::move(L));
         A = std::move(Ac);
         // the next level: one row per aggregate, ordered as created (spatially)

@@ -34,6 +34,7 @@ struct AmgLevel {
     std::vector<double> dinv;  // inverse of its diagonal
     double lmax = 0.0;         // upper estimate of the spectral radius of D^-1 A
     Csr P, R;                  // prolongation from the next coarser level and R = P^T (not on the coarsest)
+    Csr M;                     // R A: the next level's right-hand side R (b - A x) = R b - M x (not on the coarsest)
 };
 
 struct Amg {

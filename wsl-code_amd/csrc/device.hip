@@ -568,30 +568,34 @@ __device__ __forceinline__ int64_t cyc_index(const UpdArgs<T>& a, int64_t i, int
     return (i - a.pb - (int64_t)r * a.cpad) * a.q + r;
 }
 
-// Block partials of two sums (256 threads), written by thread 0.
+// Block partials of two sums (256 threads), written by thread 0: a shuffle
+// tree per wavefront, then the four wavefronts' sums in order.
 __device__ __forceinline__ void block_pair(double d1, double d2, double* o1, double* o2) {
-    __shared__ double s1[256], s2[256];
-    s1[threadIdx.x] = d1;
-    s2[threadIdx.x] = d2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            s1[threadIdx.x] += s1[threadIdx.x + o];
-            s2[threadIdx.x] += s2[threadIdx.x + o];
-        }
-        __syncthreads();
+    __shared__ double sh[2][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        d1 += __shfl_xor(d1, o, 64);
+        d2 += __shfl_xor(d2, o, 64);
     }
+    const int w = (int)threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][w] = d1;
+        sh[1][w] = d2;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        *o1 = s1[0];
-        if (o2) *o2 = s2[0];
+        *o1 = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        if (o2) *o2 = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
     }
 }
 
 // Lanczos step, preconditioner, and the solution update of the previous
 // iteration (delayed by one so that it needs no extra pass), in one sweep of
-// one chunk per block.
+// one chunk per block.  Each thread takes the chunk's entries t, t + 256, ...
+// (CHUNK / 256 of them) and issues every load of them -- the edge sides
+// through the side table first -- before any arithmetic or store.
 template <typename T>
 __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
+    constexpr int U = CHUNK / 256;
     const VecChunk ch = a.chunks[blockIdx.x];
     const MState* st = a.st + ch.inst;
     if (st->done) return;
@@ -606,26 +610,75 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
     const T uv = (T)st->u_v, uw1 = (T)st->u_w1, uw2 = (T)st->u_w2, ux = (T)st->u_x;
     const bool cyc = a.amg && ch.pslot;
     const int pr = ch.pslot - 1;
+    const int64_t end = ch.start + ch.len;
+    int64_t ii[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t i = ch.start + threadIdx.x + 256 * u;
+        ok[u] = i < end;
+        ii[u] = ok[u] ? i : ch.start;  // in bounds; the result is not stored
+    }
+    T az[U], r1v[U], r2v[U];
+    if (ch.start < a.edof) {  // a chunk of edge dofs: the sum of the edge's two cell sides
+        int2 sd[U];
+        int rr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e = a.kk == 2 ? ii[u] >> 1 : ii[u];
+            rr[u] = (int)(ii[u] - e * a.kk);
+            sd[u] = reinterpret_cast<const int2*>(a.hside)[e];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            az[u] = a.H[(int64_t)sd[u].x * a.kk + rr[u]] + a.H[(int64_t)sd[u].y * a.kk + rr[u]];
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) az[u] = a.Az[ii[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        r1v[u] = r1[ii[u]];
+        r2v[u] = r2[ii[u]];
+    }
+    T zv[U], wnv[U], wpv[U], xv[U], mv[U], sv[U];
+    if (pend) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            zv[u] = zo[ii[u]];
+            wnv[u] = wn[ii[u]];
+            wpv[u] = wp[ii[u]];
+            xv[u] = a.x[ii[u]];
+        }
+    }
+    if (!cyc) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) mv[u] = a.minv[ii[u]];
+    } else if (a.bd0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) sv[u] = a.bsc[cyc_index(a, ii[u], pr)];
+    }
     double d1 = 0.0, d2 = 0.0;
-    for (int64_t i = ch.start + threadIdx.x; i < ch.start + ch.len; i += 256) {
-        T r2i = r2[i];
-        const T az = i < a.edof ? side_sum(a.H, a.hside, a.kk, i) : a.Az[i];
-        T y = caz * az - cr1 * r1[i] - cr2 * r2i;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        const int64_t i = ii[u];
+        const T y = caz * az[u] - cr1 * r1v[u] - cr2 * r2v[u];
         if (pend) {
-            T w = uv * zo[i] - uw1 * wn[i] - uw2 * wp[i];
+            const T w = uv * zv[u] - uw1 * wnv[u] - uw2 * wpv[u];
             wn[i] = w;
-            a.x[i] += ux * w;
+            a.x[i] = xv[u] + ux * w;
         }
         r1[i] = y;
         if (cyc) {
             const int64_t j = cyc_index(a, i, pr);
-            a.bp[j] = y;  // preconditioned by the cycle, dots in k_pdot
-            if (a.bd0) a.bd0[j] = a.c0 * a.bsc[j] * y;
+            a.bp[j] = y;  // preconditioned by the multigrid cycle, which also forms the dots
+            if (a.bd0) a.bd0[j] = a.c0 * sv[u] * y;
         } else {
-            T zn = a.minv[i] * y;
+            const T zn = mv[u] * y;
             zo[i] = zn;
             d1 += (double)y * (double)zn;
-            d2 += (double)zn * (double)r2i;
+            d2 += (double)zn * (double)r2v[u];
         }
     }
     block_pair(d1, d2, a.p1 + blockIdx.x, a.p2 + blockIdx.x);
@@ -659,16 +712,18 @@ __global__ __launch_bounds__(256) void k_pdot(UpdArgs<T> a, const int* pchunks, 
 // block per instance.
 // With all_done non-null (a single instance) the block also publishes the
 // instance's stop flag as the all-done flag.
+// The pressure-block partials q1, q2 are per chunk (inst_q0 = inst_chunk0)
+// or per tile (inst_q0 = inst_tile0) when the multigrid cycle wrote them.
 __global__ void k_finB(MState* sts, const double* p1, const double* p2, const double* q1, const double* q2,
-                       const int* inst_chunk0, int useq, int* all_done) {
+                       const int* inst_chunk0, int useq, int* all_done, const int* inst_q0) {
     MState* st = sts + blockIdx.x;
     // as in k_finA: state, stop flag and partials in one round trip
     MState sv;
     if (threadIdx.x == 0) sv = *st;
     const int done = st->done;
     const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
-    double b2, rho, qb2, qrho;
-    {
+    double b2, rho, qb2 = 0.0, qrho = 0.0;
+    if (inst_q0 == inst_chunk0) {
         const double* src[4] = {p1, p2, useq ? q1 : nullptr, useq ? q2 : nullptr};
         double out[4];
         block_sums_n<4>(src, c0, c1, out);
@@ -676,6 +731,9 @@ __global__ void k_finB(MState* sts, const double* p1, const double* p2, const do
         rho = out[1];
         qb2 = out[2];
         qrho = out[3];
+    } else {
+        block_sums(p1, p2, c0, c1, b2, rho);
+        if (useq) block_sums(q1, q2, inst_q0[blockIdx.x], inst_q0[blockIdx.x + 1], qb2, qrho);
     }
     if (done) return;
     b2 += qb2;
@@ -749,11 +807,11 @@ __global__ __launch_bounds__(256) void k_init(UpdArgs<T> a, const T* b) {
 // Initial state per instance: beta1 = sqrt(b.z), tolerance rtol_i * beta1;
 // an instance with rtol_i >= 1 or a zero right-hand side starts done.
 __global__ void k_finInit(MState* sts, const double* p1, const double* q1, const int* inst_chunk0, int useq,
-                          const double* rtol, int maxit) {
+                          const double* rtol, int maxit, const int* inst_q0) {
     const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
     double b2, qb2 = 0.0, unused;
     block_sums(p1, nullptr, c0, c1, b2, unused);
-    if (useq) block_sums(q1, nullptr, c0, c1, qb2, unused);
+    if (useq) block_sums(q1, nullptr, inst_q0[blockIdx.x], inst_q0[blockIdx.x + 1], qb2, unused);
     b2 += qb2;
     if (threadIdx.x == 0) {
         double beta1 = std::sqrt(b2 > 0.0 ? b2 : 0.0);
@@ -1033,9 +1091,17 @@ void dev_upload(Solver& s, int device) {
     d.inst_chunk0 = dupload(d, L.inst_chunk0);
     d.p1 = (double*)dalloc(d, d.nchunks * sizeof(double));
     d.p2 = (double*)dalloc(d, d.nchunks * sizeof(double));
-    d.q1 = (double*)dalloc(d, d.nchunks * sizeof(double));
-    d.q2 = (double*)dalloc(d, d.nchunks * sizeof(double));
-    for (double* p : {d.p1, d.p2, d.q1, d.q2}) HIPCHK(hipMemset(p, 0, d.nchunks * sizeof(double)));
+    // pressure-block partials: per chunk (k_pdot) or per tile (the cycle's last kernel)
+    const int nq = std::max(d.nchunks, d.ntiles);
+    d.q1 = (double*)dalloc(d, nq * sizeof(double));
+    d.q2 = (double*)dalloc(d, nq * sizeof(double));
+    for (double* p : {d.p1, d.p2}) HIPCHK(hipMemset(p, 0, d.nchunks * sizeof(double)));
+    for (double* p : {d.q1, d.q2}) HIPCHK(hipMemset(p, 0, nq * sizeof(double)));
+    {
+        std::vector<int> ti(d.ntiles);
+        for (int t = 0; t < d.ntiles; ++t) ti[t] = L.groups[L.tile_group[t]].inst;
+        d.tile_inst = dupload(d, ti);
+    }
     d.st = (MState*)dalloc(d, d.ninst * sizeof(MState));
     HIPCHK(hipMemset(d.st, 0, d.ninst * sizeof(MState)));
     HIPCHK(hipHostMalloc((void**)&d.st_host, d.ninst * sizeof(MState), hipHostMallocDefault));
@@ -1408,6 +1474,27 @@ static UpdArgs<T> upd_args(Solver& s, DevKrylov<T>& k, bool amg) {
     return u;
 }
 
+// The Krylov output of the multigrid cycle's last kernel: the pressure block
+// of z and the per-tile partials of the pressure-block dots.
+template <typename T>
+static KOut<T> kout(Solver& s, DevKrylov<T>& k, bool init) {
+    DeviceState& d = *s.dev;
+    const Layout& L = s.layout;
+    KOut<T> o;
+    o.Z[0] = k.Z[0];
+    o.Z[1] = k.Z[1];
+    o.R[0] = k.R[0];
+    o.R[1] = k.R[1];
+    o.st = d.st;
+    o.tile_inst = d.ninst > 1 ? d.tile_inst : nullptr;
+    o.pb = L.cbase + (int64_t)L.ni * L.cpad;
+    o.cpad = L.cpad;
+    o.q1 = d.q1;
+    o.q2 = d.q2;
+    o.init = init ? 1 : 0;
+    return o;
+}
+
 // One MINRES iteration of every unfinished instance: apply, alfa,
 // Lanczos/precondition/update, [pressure cycle], Givens.
 template <typename T>
@@ -1420,14 +1507,21 @@ static void enqueue_iteration(Solver& s, DevKrylov<T>& k, bool amg) {
                        k.Z[0], k.Z[1], L.lam ? L.lam_idx : (int64_t)-1);
     UpdArgs<T> u = upd_args(s, k, amg);
     hipLaunchKernelGGL((k_update<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
+    const int* q0 = d.inst_chunk0;
     if (amg) {
         const AmgDev<T>& M = *amg_of<T>(d);
-        amg_vcycle(M, d.stream, d.all_done, u.bd0 != nullptr);
-        hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
-                           (const T*)M.lv[0].x, d.q1, d.q2, 0);
+        if (amg_kout_fusable(M)) {  // z and its dots from the cycle's last kernel
+            const KOut<T> ko = kout(s, k, false);
+            amg_vcycle(M, d.stream, d.all_done, u.bd0 != nullptr, &ko);
+            q0 = d.inst_tile0;
+        } else {
+            amg_vcycle(M, d.stream, d.all_done, u.bd0 != nullptr);
+            hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
+                               (const T*)M.lv[0].x, d.q1, d.q2, 0);
+        }
     }
     hipLaunchKernelGGL(k_finB, dim3(d.ninst), dim3(FIN), 0, d.stream, d.st, d.p1, d.p2, d.q1, d.q2, d.inst_chunk0,
-                       amg ? 1 : 0, d.ninst == 1 ? d.all_done : nullptr);
+                       amg ? 1 : 0, d.ninst == 1 ? d.all_done : nullptr, q0);
     if (d.ninst > 1) hipLaunchKernelGGL(k_alldone, dim3(1), dim3(256), 0, d.stream, d.st, d.ninst, d.all_done);
 }
 
@@ -1462,14 +1556,21 @@ static int minres(Solver& s, DevKrylov<T>& k, const std::vector<double>& rtol, i
     for (int i = 0; i < d.ninst; ++i) d.rtol_host[i] = rtol[i];
     HIPCHK(hipMemcpyAsync(d.rtol, d.rtol_host, d.ninst * sizeof(double), hipMemcpyHostToDevice, d.stream));
     hipLaunchKernelGGL((k_init<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u, (const T*)k.b);
+    const int* q0 = d.inst_chunk0;
     if (amg) {
         const AmgDev<T>& M = *amg_of<T>(d);
-        amg_vcycle(M, d.stream, nullptr);
-        hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
-                           (const T*)M.lv[0].x, d.q1, d.q2, 1);
+        if (amg_kout_fusable(M)) {
+            const KOut<T> ko = kout(s, k, true);
+            amg_vcycle(M, d.stream, nullptr, false, &ko);
+            q0 = d.inst_tile0;
+        } else {
+            amg_vcycle(M, d.stream, nullptr);
+            hipLaunchKernelGGL((k_pdot<T>), dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
+                               (const T*)M.lv[0].x, d.q1, d.q2, 1);
+        }
     }
     hipLaunchKernelGGL(k_finInit, dim3(d.ninst), dim3(FIN), 0, d.stream, d.st, d.p1, d.q1, d.inst_chunk0, amg ? 1 : 0,
-                       (const double*)d.rtol, maxit);
+                       (const double*)d.rtol, maxit, q0);
     hipLaunchKernelGGL(k_alldone, dim3(1), dim3(256), 0, d.stream, d.st, d.ninst, d.all_done);
     HIPCHK(hipGetLastError());
     static const bool verbose = std::getenv("VEMD_VERBOSE") != nullptr;

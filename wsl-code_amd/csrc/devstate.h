@@ -51,6 +51,7 @@ struct DeviceState {
     uint32_t* wmask = nullptr;
     int* hside = nullptr;  // cell-side slots of every edge (two per edge)
     int* tile_group = nullptr;
+    int* tile_inst = nullptr;  // batch instance of every tile
     int* cell_of_gpos = nullptr;
     int ntiles = 0;
     double* pdot = nullptr;

@@ -62,11 +62,31 @@ struct CsrDev {
     float hscale = 1.0f;
 };
 
+// Rows of 1 x bs blocks (bs = 1: plain compressed rows): row i holds the
+// entries [rp[i], rp[i + 1]); entry e has block column bc[e] (columns
+// bc[e] * bs .. bc[e] * bs + bs - 1) and values v[e * bs + c].
+template <typename T>
+struct BcsrDev {
+    int64_t nrows = 0;
+    int bs = 1;
+    const int64_t* rp = nullptr;
+    const int* bc = nullptr;
+    const T* v = nullptr;
+    const _Float16* hv = nullptr;  // values in half precision (divided by hscale) instead of v
+    float hscale = 1.0f;
+};
+
 template <typename T>
 struct AmgLevelDev {
     int64_t n = 0;
     Sell<T> A, P;             // P: prolongation from the next level
     CsrDev<T> R;              // R = P^T
+    // Restriction of the residual in one pass (single column): the next
+    // level's right-hand side Rb b - M x with M = R A (scaled level: M carries
+    // the scaling, M = R D^1/2 (D^-1/2 A D^-1/2), and Rb the plain R, which
+    // acts on the unscaled b).  M.nrows = 0: residual and restriction apart.
+    BcsrDev<T> M;
+    CsrDev<T> Rb;
     T* dinv = nullptr;        // null: unit diagonal
     // Scaled level (sc non-null): the operator is D^-1/2 A D^-1/2 (unit
     // diagonal, entries in [-1, 1], stored in half precision) acting on
@@ -96,16 +116,39 @@ struct AmgDev {
 template <typename T>
 AmgDev<T>* amg_upload(const Amg& H);
 
+// Krylov side of the cycle's last kernel (single column): instead of lv[0].x
+// the finest level's last smoothing step writes the pressure block of the
+// MINRES vector z (entry pb + r * cpad + g for moment r of store position g)
+// and per-tile (64 positions) partials q1 = b.z and q2 = z.r2 (b the cycle's
+// input), selecting z and r2 by the iteration parity of the tile's instance
+// as the Krylov sweeps do; tiles of a finished instance are left alone.  init:
+// z = Z[0] and no q2.
+template <typename T>
+struct KOut {
+    T* Z[2];
+    const T* R[2];
+    const MState* st;
+    const int* tile_inst;  // instance of every tile (null: one instance)
+    int64_t pb, cpad;
+    double *q1, *q2;
+    int init;
+};
+
 // Enqueues one V-cycle on the stream: x = B b with b = lv[0].b, x = lv[0].x.
 // With done non-null every kernel returns at once while *done is set (work
 // skipped inside a captured chunk of iterations after convergence).  With
 // pre_done the producer of b has also written lv[0].d0 = b / centre times
 // lv[0].sc (scaled level) or lv[0].dinv, the cycle's first update, which
 // saves the cycle's first sweep over the level (only if amg_pre_fusable).
+// With ko non-null and amg_kout_fusable(D) the cycle ends in the Krylov
+// output described by KOut (lv[0].x is not written).
 template <typename T>
-void amg_vcycle(const AmgDev<T>& D, hipStream_t stream, const int* done, bool pre_done = false);
+void amg_vcycle(const AmgDev<T>& D, hipStream_t stream, const int* done, bool pre_done = false,
+                const KOut<T>* ko = nullptr);
 template <typename T>
 bool amg_pre_fusable(const AmgDev<T>& D);
+template <typename T>
+bool amg_kout_fusable(const AmgDev<T>& D);
 
 // The same cycle applied to MCOLS interleaved columns (entry (row, j) at
 // row * MCOLS + j): b = lv[0].mb, x = lv[0].mx.  amg_multi_alloc first.

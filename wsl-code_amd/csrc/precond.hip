@@ -355,6 +355,121 @@ __global__ __launch_bounds__(256) void k_restrict(CsrDev<T> R, const T* v, T* y,
     }
 }
 
+// The next level's right-hand side straight from this level's iterate:
+// y = Rb b - M (x + d) (d null: x complete), with M = R A, so the residual
+// b - A (x + d) is never formed.  32 lanes per row; each lane takes groups of
+// U entries with their column indices and gathers issued together (entries
+// past the row re-read its last entry, in bounds, and add zero); the lanes'
+// sums combine in a fixed butterfly order.  RH, MH: values in half precision.
+template <typename T, int B, bool RH, bool MH, bool HY>
+__global__ __launch_bounds__(256) void k_rres(CsrDev<T> R, BcsrDev<T> M, const T* __restrict__ b,
+                                            const T* __restrict__ x, const T* __restrict__ d, T* __restrict__ y,
+                                            const int* done) {
+    if (done && *done) return;
+    constexpr int U = 4;
+    const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 5;
+    const int lane = threadIdx.x & 31;
+    T acc = T(0);
+    if (row < M.nrows) {
+        T rb = T(0), mx = T(0);
+        const int64_t r0 = R.rp[row], r1 = R.rp[row + 1], m0 = M.rp[row], m1 = M.rp[row + 1];
+        for (int64_t p0 = r0 + lane; p0 < r1; p0 += 32 * U) {
+            int64_t pp[U];
+            int c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                pp[u] = p0 + 32 * u < r1 ? p0 + 32 * u : r1 - 1;
+                c[u] = R.ci[pp[u]];
+            }
+            T bv[U], rv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                bv[u] = b[c[u]];
+                if constexpr (RH)
+                    rv[u] = (T)R.hv[pp[u]];
+                else
+                    rv[u] = R.v[pp[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) rb += p0 + 32 * u < r1 ? rv[u] * bv[u] : T(0);
+        }
+        if constexpr (RH) rb *= (T)R.hscale;
+        for (int64_t e0 = m0 + lane; e0 < m1; e0 += 32 * U) {
+            int64_t ee[U], c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ee[u] = e0 + 32 * u < m1 ? e0 + 32 * u : m1 - 1;
+                c[u] = (int64_t)M.bc[ee[u]] * B;
+            }
+            T xv[U][B], mv[U][B];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int cc = 0; cc < B; ++cc) {
+                    T v = x[c[u] + cc];
+                    if constexpr (HY) v += d[c[u] + cc];
+                    xv[u][cc] = v;
+                    if constexpr (MH)
+                        mv[u][cc] = (T)M.hv[ee[u] * B + cc];
+                    else
+                        mv[u][cc] = M.v[ee[u] * B + cc];
+                }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                T s = T(0);
+#pragma unroll
+                for (int cc = 0; cc < B; ++cc) s += mv[u][cc] * xv[u][cc];
+                mx += e0 + 32 * u < m1 ? s : T(0);
+            }
+        }
+        if constexpr (MH) mx *= (T)M.hscale;
+        acc = rb - mx;
+    }
+    for (int o = 16; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 32);
+    if (row < M.nrows && lane == 0) y[row] = acc;
+}
+
+// Last Chebyshev step of the finest level (single column, one lane per block
+// row, so a wavefront is one 64-position tile) with the Krylov output of
+// KOut: x + d + d' (times xs on a scaled level) goes to the pressure block
+// of z, with the tile's partials of b.z and z.r2.
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_step_z(Sell<T> A, const T* r, const T* dinv, const T* din, const T* x, T c1,
+                                              T c2, const T* xs, const T* bcyc, KOut<T> ko, const int* done) {
+    if (done && *done) return;
+    const int64_t bi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (bi >= A.nrows) return;  // whole wavefronts: the level has a multiple of 64 block rows
+    const int64_t tile = bi >> 6;
+    const MState* st = ko.st + (ko.tile_inst ? ko.tile_inst[tile] : 0);
+    if (!ko.init && st->done) return;  // uniform over the tile
+    const int it = ko.init ? 0 : st->itn;
+    T* __restrict__ zo = ko.init ? ko.Z[0] : ko.Z[(it + 1) & 1];
+    const T* __restrict__ r2 = ko.init ? nullptr : ko.R[it & 1];
+    T ad[B];
+    sell_row<T, B, 1>(A, bi, 0, din, ad);
+    double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+        const int64_t i = bi * B + k;
+        const T di = din[i];
+        const T dn = c1 * di + c2 * dval(dinv, i) * (r[i] - ad[k]);
+        const T xn = x[i] + di + dn;
+        const T z = xs ? xn * xs[i] : xn;
+        const int64_t kv = ko.pb + k * ko.cpad + bi;
+        zo[kv] = z;
+        d1 += (double)bcyc[i] * (double)z;
+        if (r2) d2 += (double)z * (double)r2[kv];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        d1 += __shfl_xor(d1, o, 64);
+        d2 += __shfl_xor(d2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        ko.q1[tile] = d1;
+        ko.q2[tile] = d2;
+    }
+}
+
 // Coarse correction of the lagged iterate: x += d + P (xc + dc), with d this
 // level's and dc the coarse level's pending Chebyshev update (null where the
 // iterate is complete).
@@ -519,6 +634,50 @@ static CsrDev<T> to_csr(AmgDev<T>& D, const Csr& A, bool half = false) {
     return c;
 }
 
+// Rows of 1 x bs blocks from compressed rows with sorted columns (missing
+// components of a block are zero); with csc each column c is divided by
+// csc[c] first; half: values divided by their largest magnitude in half
+// precision.
+template <typename T>
+static BcsrDev<T> to_bcsr(AmgDev<T>& D, const Csr& A, int bs, const std::vector<double>* csc, bool half) {
+    BcsrDev<T> m;
+    m.nrows = A.nrows;
+    m.bs = bs;
+    std::vector<int64_t> rp(A.nrows + 1, 0);
+    std::vector<int> bc;
+    std::vector<double> v;
+    bc.reserve(A.nnz() / bs + A.nrows);
+    v.reserve(A.nnz() + A.nrows * bs);
+    for (int64_t i = 0; i < A.nrows; ++i) {
+        int last = -1;
+        for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+            const int b = A.ci[p] / bs;
+            if (b != last) {
+                bc.push_back(b);
+                v.resize(v.size() + bs, 0.0);
+                last = b;
+            }
+            v[v.size() - bs + A.ci[p] % bs] = csc ? A.v[p] / (*csc)[A.ci[p]] : A.v[p];
+        }
+        rp[i + 1] = (int64_t)bc.size();
+    }
+    m.rp = amg_copy(D, rp);
+    m.bc = amg_copy(D, bc);
+    if (half) {
+        double amax = 0.0;
+        for (double x : v) amax = std::max(amax, std::fabs(x));
+        const double hinv = amax > 0.0 ? 1.0 / amax : 1.0;
+        m.hscale = amax > 0.0 ? (float)amax : 1.0f;
+        std::vector<_Float16> hv(v.size());
+        for (size_t i = 0; i < v.size(); ++i) hv[i] = (_Float16)(v[i] * hinv);
+        m.hv = amg_copy(D, hv);
+    } else {
+        std::vector<T> tv(v.begin(), v.end());
+        m.v = amg_copy(D, tv);
+    }
+    return m;
+}
+
 // Half-precision storage (values divided by the largest magnitude) needs
 // finite values; entries below 6e-5 of the largest lose relative precision,
 // which changes the preconditioner by less than that.
@@ -527,6 +686,12 @@ static bool half_safe(const std::vector<const Csr*>& ms) {
         for (double v : m->v)
             if (!std::isfinite(v)) return false;
     return true;
+}
+
+// The residual restricted in one pass (k_rres) unless VEMD_MG_FUSED_RESTRICT=0.
+static bool fused_restriction() {
+    const char* e = std::getenv("VEMD_MG_FUSED_RESTRICT");
+    return !(e && e[0] == '0');
 }
 
 template <typename T>
@@ -561,6 +726,12 @@ AmgDev<T>* amg_upload(const Amg& H) {
                 const bool htr = !(ht && ht[0] == '0') && half_safe({&P, &R});
                 d.P = to_sell(*D, P, 1, nullptr, htr);
                 d.R = to_csr(*D, R, htr);
+                if (fused_restriction()) {
+                    // M = R D^1/2 (D^-1/2 A D^-1/2) = (R A) D^-1/2 on the scaled
+                    // iterate; the plain R on the unscaled right-hand side
+                    d.M = to_bcsr(*D, h.M, bs, &dsq, htr && half_safe({&h.M}));
+                    d.Rb = to_csr(*D, h.R, htr && half_safe({&h.R}));
+                }
                 if (std::getenv("VEMD_VERBOSE"))
                     std::fprintf(stderr, "[vemd] finest-level transfers of the FP32 cycle in %s\n",
                                  htr ? "half precision" : "FP32 (values outside half precision's range)");
@@ -573,6 +744,10 @@ AmgDev<T>* amg_upload(const Amg& H) {
                 if (l + 1 < H.lv.size()) {
                     d.P = to_sell(*D, h.P, 1);
                     d.R = to_csr(*D, h.R);
+                    if (fused_restriction()) {
+                        d.M = to_bcsr(*D, h.M, bs, nullptr, false);
+                        d.Rb = d.R;
+                    }
                 }
                 std::vector<T> di(h.dinv.begin(), h.dinv.end());
                 d.dinv = amg_copy(*D, di);
@@ -621,14 +796,18 @@ static dim3 grid_of(int64_t threads) { return dim3((unsigned)((threads + 255) / 
 // step and no fused start the update stays pending in din.
 template <typename T, int B, int NC>
 static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStream_t st, const int* ms, T*& din,
-                         bool fused, const T* xs, bool from_b = false) {
+                         bool fused, const T* xs, bool from_b = false, const KOut<T>* ko = nullptr) {
     const double sigma = L.centre / L.half;
     double rho = 1.0 / sigma;
     T* dout = din == v.d0 ? v.d1 : v.d0;
     for (int k = 1; k < L.degree; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
         const int last = k + 1 == L.degree;
-        if (fused && k == 1) {
+        if (last && ko && NC == 1) {
+            hipLaunchKernelGGL((k_step_z<T, B>), grid_of(L.A.nrows), dim3(256), 0, st, L.A, (const T*)v.r,
+                               (const T*)L.dinv, (const T*)din, (const T*)v.x, (T)(rn * rho), (T)(2.0 * rn / L.half),
+                               xs, (const T*)v.b, *ko, ms);
+        } else if (fused && k == 1) {
             hipLaunchKernelGGL((k_step0<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A)), dim3(256), 0, st,
                                L.A, (const T*)v.b,
                                (const T*)L.dinv, v.r, din, v.x, (T)(1.0 / L.centre), (T)(rn * rho),
@@ -648,15 +827,40 @@ static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStre
 }
 
 template <typename T, int NC>
-static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done = false);
+static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done = false,
+                             const KOut<T>* ko = nullptr);
 
 // Pre-smoothing, residual, restriction, coarse cycle, prolongation and
 // post-smoothing of one level whose operator has blocks of B rows.  The
 // result is x + d with d the returned pending update: null unless the level
 // smooths with a single step (then coarser levels leave the sum to the
 // prolongation that reads them and the finest level adds it).
+// y = Rb b - M (x + d) of the level (k_rres), specialised on which values
+// are in half precision and on the pending update.
+template <typename T, int B>
+static void launch_rres(const AmgLevelDev<T>& L, const T* b, const T* x, const T* d, T* y, hipStream_t st,
+                        const int* ms) {
+    const dim3 g = grid_of(L.M.nrows * 32), bl(256);
+    const bool rh = L.Rb.hv != nullptr, mh = L.M.hv != nullptr;
+#define VEMD_RRES(RH, MH, HY) \
+    hipLaunchKernelGGL((k_rres<T, B, RH, MH, HY>), g, bl, 0, st, L.Rb, L.M, b, x, d, y, ms)
+    if (d) {
+        if (rh && mh) VEMD_RRES(true, true, true);
+        else if (rh) VEMD_RRES(true, false, true);
+        else if (mh) VEMD_RRES(false, true, true);
+        else VEMD_RRES(false, false, true);
+    } else {
+        if (rh && mh) VEMD_RRES(true, true, false);
+        else if (rh) VEMD_RRES(true, false, false);
+        else if (mh) VEMD_RRES(false, true, false);
+        else VEMD_RRES(false, false, false);
+    }
+#undef VEMD_RRES
+}
+
 template <typename T, int B, int NC>
-static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done) {
+static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done,
+                            const KOut<T>* ko) {
     const AmgLevelDev<T>& L = D.lv[l];
     const AmgLevelDev<T>& C = D.lv[l + 1];
     const LevelVecs<T> v = vecs<T, NC>(L), vc = vecs<T, NC>(C);
@@ -673,22 +877,27 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
         hipLaunchKernelGGL((k_pre0<T, NC>), g, b, 0, st, L.n, (const T*)v.b, sc, (const T*)L.dinv, v.r, v.d0, v.x, c0,
                            ms);
     smooth_steps<T, B, NC>(L, v, st, ms, din, fused, (const T*)nullptr, pre_done);
-    hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)din, v.r,
-                       ms);
-    hipLaunchKernelGGL((k_restrict<T, NC>), gr, b, 0, st, L.R, (const T*)v.r, vc.b, ms);
+    if (NC == 1 && L.M.nrows > 0) {
+        launch_rres<T, B>(L, (const T*)v.b, (const T*)v.x, (const T*)din, vc.b, st, ms);
+    } else {
+        hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)din,
+                           v.r, ms);
+        hipLaunchKernelGGL((k_restrict<T, NC>), gr, b, 0, st, L.R, (const T*)v.r, vc.b, ms);
+    }
     const T* dc = vcycle_level<T, NC>(D, l + 1, st, ms);
     hipLaunchKernelGGL((k_prol<T, NC>), g, b, 0, st, L.P, (const T*)vc.x, dc, v.x, (const T*)din, ms);
     din = v.d0;
     hipLaunchKernelGGL((k_post0<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)L.dinv,
                        v.r, v.d0, c0, ms);
-    smooth_steps<T, B, NC>(L, v, st, ms, din, false, sc);
+    smooth_steps<T, B, NC>(L, v, st, ms, din, false, sc, false, ko);
     if (l > 0 || !din) return din;
     hipLaunchKernelGGL((k_addto<T, NC>), g, b, 0, st, L.n, v.x, (const T*)din, sc, ms);
     return nullptr;
 }
 
 template <typename T, int NC>
-static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done) {
+static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const int* ms, bool pre_done,
+                             const KOut<T>* ko) {
     const AmgLevelDev<T>& L = D.lv[l];
     if (l + 1 == D.lv.size()) {
         const LevelVecs<T> v = vecs<T, NC>(L);
@@ -697,8 +906,8 @@ static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const
         return nullptr;
     }
     switch (L.A.bs) {
-        case 1: return level_cycle<T, 1, NC>(D, l, st, ms, pre_done);
-        case 3: return level_cycle<T, 3, NC>(D, l, st, ms, pre_done);
+        case 1: return level_cycle<T, 1, NC>(D, l, st, ms, pre_done, ko);
+        case 3: return level_cycle<T, 3, NC>(D, l, st, ms, pre_done, ko);
         default: throw Error(VEMD_ERR_INTERNAL, "unsupported multigrid block size");
     }
 }
@@ -708,10 +917,23 @@ bool amg_pre_fusable(const AmgDev<T>& D) {
     return D.lv.size() > 1 && D.lv[0].degree >= 2 && !(D.lv[0].A.bs == 1 && !D.lv[0].sc);
 }
 
+// The Krylov output needs the finest level's last step to be a k_step_z
+// launch: a Chebyshev degree of at least 2, one lane per block row, whole
+// 64-row tiles, and a coarser level (a single-level hierarchy ends in the
+// dense solve).
 template <typename T>
-void amg_vcycle(const AmgDev<T>& D, hipStream_t st, const int* ms, bool pre_done) {
+bool amg_kout_fusable(const AmgDev<T>& D) {
+    const char* e = std::getenv("VEMD_MG_FUSED_DOTS");
+    if (e && e[0] == '0') return false;
+    const AmgLevelDev<T>& L = D.lv[0];
+    return D.lv.size() > 1 && L.degree >= 2 && lanes_per_row<1, 1>(L.A) == 1 && L.A.nrows % 64 == 0;
+}
+
+template <typename T>
+void amg_vcycle(const AmgDev<T>& D, hipStream_t st, const int* ms, bool pre_done, const KOut<T>* ko) {
     if (pre_done && !amg_pre_fusable(D)) throw Error(VEMD_ERR_INTERNAL, "multigrid cycle cannot take a formed start");
-    vcycle_level<T, 1>(D, 0, st, ms, pre_done);
+    if (ko && !amg_kout_fusable(D)) throw Error(VEMD_ERR_INTERNAL, "multigrid cycle cannot end in the Krylov output");
+    vcycle_level<T, 1>(D, 0, st, ms, pre_done, ko);
     HIPCHK(hipGetLastError());
 }
 
@@ -741,10 +963,12 @@ template struct AmgDev<double>;
 template struct AmgDev<float>;
 template AmgDev<double>* amg_upload<double>(const Amg&);
 template AmgDev<float>* amg_upload<float>(const Amg&);
-template void amg_vcycle<double>(const AmgDev<double>&, hipStream_t, const int*, bool);
-template void amg_vcycle<float>(const AmgDev<float>&, hipStream_t, const int*, bool);
+template void amg_vcycle<double>(const AmgDev<double>&, hipStream_t, const int*, bool, const KOut<double>*);
+template void amg_vcycle<float>(const AmgDev<float>&, hipStream_t, const int*, bool, const KOut<float>*);
 template bool amg_pre_fusable<double>(const AmgDev<double>&);
 template bool amg_pre_fusable<float>(const AmgDev<float>&);
+template bool amg_kout_fusable<double>(const AmgDev<double>&);
+template bool amg_kout_fusable<float>(const AmgDev<float>&);
 template void amg_multi_alloc<double>(AmgDev<double>&);
 template void amg_multi_alloc<float>(AmgDev<float>&);
 template void amg_vcycle_multi<double>(const AmgDev<double>&, hipStream_t, const int*);
