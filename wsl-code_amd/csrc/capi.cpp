@@ -116,7 +116,8 @@ void Solver::build_amg() {
     if (std::getenv("VEMD_VERBOSE")) {
         std::fprintf(stderr, "[vemd] multigrid setup %.2fs:", amg_setup_s);
         for (const AmgLevel& l : amg->lv)
-            std::fprintf(stderr, " %lld rows/%lld nnz", (long long)l.A.nrows, (long long)l.A.nnz());
+            std::fprintf(stderr, " %lld rows/%lld nnz (R %lld, RA %lld)", (long long)l.A.nrows, (long long)l.A.nnz(),
+                         (long long)l.R.nnz(), (long long)l.M.nnz());
         std::fprintf(stderr, "\n");
     }
 }

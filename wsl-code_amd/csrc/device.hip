@@ -452,21 +452,27 @@ __global__ void k_lam_row(const double* plam, int ntiles, T* y, int64_t lam_idx)
 // misses this L2: every thread issues all its loads (32 / NA per array) before
 // adding, and the block sum is two shuffle trees around one LDS exchange.
 constexpr int FIN = 1024;
+// Array k sums its range [b[k], e[k]).
 template <int NA>
-__device__ void block_sums_n(const double* const (&src)[NA], int b, int e, double (&out)[NA]) {
+__device__ void block_sums_r(const double* const (&src)[NA], const int (&b)[NA], const int (&e)[NA],
+                             double (&out)[NA]) {
     constexpr int U = 32 / NA, NW = FIN / 64;
     __shared__ double sh[NA][NW];
     double acc[NA];
+    int len = 0;
 #pragma unroll
-    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
-    for (int i0 = b + (int)threadIdx.x; i0 < e; i0 += FIN * U) {
+    for (int k = 0; k < NA; ++k) {
+        acc[k] = 0.0;
+        len = e[k] - b[k] > len ? e[k] - b[k] : len;
+    }
+    for (int i0 = (int)threadIdx.x; i0 < len; i0 += FIN * U) {
         double v[NA][U];
 #pragma unroll
         for (int k = 0; k < NA; ++k)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = i0 + u * FIN;
-                v[k][u] = src[k] && i < e ? src[k][i] : 0.0;
+                v[k][u] = src[k] && i < e[k] - b[k] ? src[k][b[k] + i] : 0.0;
             }
 #pragma unroll
         for (int k = 0; k < NA; ++k)
@@ -493,6 +499,17 @@ __device__ void block_sums_n(const double* const (&src)[NA], int b, int e, doubl
 #pragma unroll
     for (int k = 0; k < NA; ++k) out[k] = sh[k][0];
     __syncthreads();
+}
+
+template <int NA>
+__device__ void block_sums_n(const double* const (&src)[NA], int b, int e, double (&out)[NA]) {
+    int bb[NA], ee[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        bb[k] = b;
+        ee[k] = e;
+    }
+    block_sums_r<NA>(src, bb, ee, out);
 }
 
 __device__ void block_sums(const double* a, const double* c, int b, int e, double& sa, double& sc) {
@@ -722,18 +739,17 @@ __global__ void k_finB(MState* sts, const double* p1, const double* p2, const do
     if (threadIdx.x == 0) sv = *st;
     const int done = st->done;
     const int c0 = inst_chunk0[blockIdx.x], c1 = inst_chunk0[blockIdx.x + 1];
-    double b2, rho, qb2 = 0.0, qrho = 0.0;
-    if (inst_q0 == inst_chunk0) {
+    double b2, rho, qb2, qrho;
+    {
+        const int t0 = inst_q0[blockIdx.x], t1 = inst_q0[blockIdx.x + 1];
         const double* src[4] = {p1, p2, useq ? q1 : nullptr, useq ? q2 : nullptr};
+        const int bb[4] = {c0, c0, t0, t0}, ee[4] = {c1, c1, t1, t1};
         double out[4];
-        block_sums_n<4>(src, c0, c1, out);
+        block_sums_r<4>(src, bb, ee, out);
         b2 = out[0];
         rho = out[1];
         qb2 = out[2];
         qrho = out[3];
-    } else {
-        block_sums(p1, p2, c0, c1, b2, rho);
-        if (useq) block_sums(q1, q2, inst_q0[blockIdx.x], inst_q0[blockIdx.x + 1], qb2, qrho);
     }
     if (done) return;
     b2 += qb2;

@@ -84,9 +84,9 @@ struct AmgLevelDev {
     // Restriction of the residual in one pass (single column): the next
     // level's right-hand side Rb b - M x with M = R A (scaled level: M carries
     // the scaling, M = R D^1/2 (D^-1/2 A D^-1/2), and Rb the plain R, which
-    // acts on the unscaled b).  M.nrows = 0: residual and restriction apart.
-    BcsrDev<T> M;
-    CsrDev<T> Rb;
+    // acts on the unscaled b), both in rows of 1 x bs blocks.  M.nrows = 0:
+    // residual and restriction apart.
+    BcsrDev<T> M, Rb;
     T* dinv = nullptr;        // null: unit diagonal
     // Scaled level (sc non-null): the operator is D^-1/2 A D^-1/2 (unit
     // diagonal, entries in [-1, 1], stored in half precision) acting on

@@ -357,57 +357,48 @@ __global__ __launch_bounds__(256) void k_restrict(CsrDev<T> R, const T* v, T* y,
 
 // The next level's right-hand side straight from this level's iterate:
 // y = Rb b - M (x + d) (d null: x complete), with M = R A, so the residual
-// b - A (x + d) is never formed.  32 lanes per row; each lane takes groups of
-// U entries with their column indices and gathers issued together (entries
-// past the row re-read its last entry, in bounds, and add zero); the lanes'
-// sums combine in a fixed butterfly order.  RH, MH: values in half precision.
-template <typename T, int B, bool RH, bool MH, bool HY>
-__global__ __launch_bounds__(256) void k_rres(CsrDev<T> R, BcsrDev<T> M, const T* __restrict__ b,
+// b - A (x + d) is never formed.  Both matrices are rows of 1 x B blocks.
+// LPR lanes per row; each lane takes groups of U entries of both matrices,
+// issuing the block columns of both, then the values and gathers of both,
+// before adding (entries past a row re-read its last entry, in bounds, and
+// add zero; rows are never empty); the lanes' sums combine in a fixed
+// butterfly order.  RH, MH: values in half precision.
+template <typename T, int B, int LPR, int U, bool RH, bool MH, bool HY>
+__global__ __launch_bounds__(256) void k_rres(BcsrDev<T> R, BcsrDev<T> M, const T* __restrict__ b,
                                             const T* __restrict__ x, const T* __restrict__ d, T* __restrict__ y,
                                             const int* done) {
     if (done && *done) return;
-    constexpr int U = 4;
-    const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 5;
-    const int lane = threadIdx.x & 31;
+    const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) / LPR;
+    const int lane = threadIdx.x % LPR;
     T acc = T(0);
     if (row < M.nrows) {
         T rb = T(0), mx = T(0);
         const int64_t r0 = R.rp[row], r1 = R.rp[row + 1], m0 = M.rp[row], m1 = M.rp[row + 1];
-        for (int64_t p0 = r0 + lane; p0 < r1; p0 += 32 * U) {
-            int64_t pp[U];
-            int c[U];
+        for (int64_t p0 = r0 + lane, e0 = m0 + lane; p0 < r1 || e0 < m1; p0 += LPR * U, e0 += LPR * U) {
+            int64_t pp[U], ee[U], cr[U], cm[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                pp[u] = p0 + 32 * u < r1 ? p0 + 32 * u : r1 - 1;
-                c[u] = R.ci[pp[u]];
+                const int64_t p = p0 + LPR * u, e = e0 + LPR * u;
+                pp[u] = p < r1 ? p : r1 - 1;
+                ee[u] = e < m1 ? e : m1 - 1;
             }
-            T bv[U], rv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                bv[u] = b[c[u]];
-                if constexpr (RH)
-                    rv[u] = (T)R.hv[pp[u]];
-                else
-                    rv[u] = R.v[pp[u]];
+                cr[u] = (int64_t)R.bc[pp[u]] * B;
+                cm[u] = (int64_t)M.bc[ee[u]] * B;
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) rb += p0 + 32 * u < r1 ? rv[u] * bv[u] : T(0);
-        }
-        if constexpr (RH) rb *= (T)R.hscale;
-        for (int64_t e0 = m0 + lane; e0 < m1; e0 += 32 * U) {
-            int64_t ee[U], c[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                ee[u] = e0 + 32 * u < m1 ? e0 + 32 * u : m1 - 1;
-                c[u] = (int64_t)M.bc[ee[u]] * B;
-            }
-            T xv[U][B], mv[U][B];
+            T bv[U][B], rv[U][B], xv[U][B], mv[U][B];
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
                 for (int cc = 0; cc < B; ++cc) {
-                    T v = x[c[u] + cc];
-                    if constexpr (HY) v += d[c[u] + cc];
+                    bv[u][cc] = b[cr[u] + cc];
+                    if constexpr (RH)
+                        rv[u][cc] = (T)R.hv[pp[u] * B + cc];
+                    else
+                        rv[u][cc] = R.v[pp[u] * B + cc];
+                    T v = x[cm[u] + cc];
+                    if constexpr (HY) v += d[cm[u] + cc];
                     xv[u][cc] = v;
                     if constexpr (MH)
                         mv[u][cc] = (T)M.hv[ee[u] * B + cc];
@@ -416,16 +407,21 @@ __global__ __launch_bounds__(256) void k_rres(CsrDev<T> R, BcsrDev<T> M, const T
                 }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                T s = T(0);
+                T sr = T(0), sm = T(0);
 #pragma unroll
-                for (int cc = 0; cc < B; ++cc) s += mv[u][cc] * xv[u][cc];
-                mx += e0 + 32 * u < m1 ? s : T(0);
+                for (int cc = 0; cc < B; ++cc) {
+                    sr += rv[u][cc] * bv[u][cc];
+                    sm += mv[u][cc] * xv[u][cc];
+                }
+                rb += p0 + LPR * u < r1 ? sr : T(0);
+                mx += e0 + LPR * u < m1 ? sm : T(0);
             }
         }
+        if constexpr (RH) rb *= (T)R.hscale;
         if constexpr (MH) mx *= (T)M.hscale;
         acc = rb - mx;
     }
-    for (int o = 16; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 32);
+    for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
     if (row < M.nrows && lane == 0) y[row] = acc;
 }
 
@@ -694,6 +690,13 @@ static bool fused_restriction() {
     return !(e && e[0] == '0');
 }
 
+// k_rres reads at least one entry of every row of R and M.
+static bool rows_nonempty(const Csr& A) {
+    for (int64_t i = 0; i < A.nrows; ++i)
+        if (A.rp[i + 1] == A.rp[i]) return false;
+    return true;
+}
+
 template <typename T>
 AmgDev<T>* amg_upload(const Amg& H) {
     AmgDev<T>* D = new AmgDev<T>();
@@ -726,11 +729,11 @@ AmgDev<T>* amg_upload(const Amg& H) {
                 const bool htr = !(ht && ht[0] == '0') && half_safe({&P, &R});
                 d.P = to_sell(*D, P, 1, nullptr, htr);
                 d.R = to_csr(*D, R, htr);
-                if (fused_restriction()) {
+                if (fused_restriction() && rows_nonempty(h.M) && rows_nonempty(h.R)) {
                     // M = R D^1/2 (D^-1/2 A D^-1/2) = (R A) D^-1/2 on the scaled
                     // iterate; the plain R on the unscaled right-hand side
                     d.M = to_bcsr(*D, h.M, bs, &dsq, htr && half_safe({&h.M}));
-                    d.Rb = to_csr(*D, h.R, htr && half_safe({&h.R}));
+                    d.Rb = to_bcsr(*D, h.R, bs, nullptr, htr && half_safe({&h.R}));
                 }
                 if (std::getenv("VEMD_VERBOSE"))
                     std::fprintf(stderr, "[vemd] finest-level transfers of the FP32 cycle in %s\n",
@@ -744,9 +747,9 @@ AmgDev<T>* amg_upload(const Amg& H) {
                 if (l + 1 < H.lv.size()) {
                     d.P = to_sell(*D, h.P, 1);
                     d.R = to_csr(*D, h.R);
-                    if (fused_restriction()) {
+                    if (fused_restriction() && rows_nonempty(h.M) && rows_nonempty(h.R)) {
                         d.M = to_bcsr(*D, h.M, bs, nullptr, false);
-                        d.Rb = d.R;
+                        d.Rb = to_bcsr(*D, h.R, bs, nullptr, false);
                     }
                 }
                 std::vector<T> di(h.dinv.begin(), h.dinv.end());
@@ -837,13 +840,16 @@ static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const
 // prolongation that reads them and the finest level adds it).
 // y = Rb b - M (x + d) of the level (k_rres), specialised on which values
 // are in half precision and on the pending update.
-template <typename T, int B>
-static void launch_rres(const AmgLevelDev<T>& L, const T* b, const T* x, const T* d, T* y, hipStream_t st,
-                        const int* ms) {
-    const dim3 g = grid_of(L.M.nrows * 32), bl(256);
+// Block levels (the finest, most rows): 16 lanes per row with 8 entries each in
+// flight, so that fewer rounds of wavefronts pass the chain of dependent
+// loads; scalar levels: 32 lanes, 4 entries each.
+template <typename T, int B, int LPR, int U>
+static void launch_rres_(const AmgLevelDev<T>& L, const T* b, const T* x, const T* d, T* y, hipStream_t st,
+                         const int* ms) {
+    const dim3 g = grid_of(L.M.nrows * LPR), bl(256);
     const bool rh = L.Rb.hv != nullptr, mh = L.M.hv != nullptr;
 #define VEMD_RRES(RH, MH, HY) \
-    hipLaunchKernelGGL((k_rres<T, B, RH, MH, HY>), g, bl, 0, st, L.Rb, L.M, b, x, d, y, ms)
+    hipLaunchKernelGGL((k_rres<T, B, LPR, U, RH, MH, HY>), g, bl, 0, st, L.Rb, L.M, b, x, d, y, ms)
     if (d) {
         if (rh && mh) VEMD_RRES(true, true, true);
         else if (rh) VEMD_RRES(true, false, true);
@@ -856,6 +862,14 @@ static void launch_rres(const AmgLevelDev<T>& L, const T* b, const T* x, const T
         else VEMD_RRES(false, false, false);
     }
 #undef VEMD_RRES
+}
+
+// 32 lanes per row, 2 entries of each matrix in flight per lane (measured at
+// 1M cells against 16 and 64 lanes and 1, 4 or 8 entries).
+template <typename T, int B>
+static void launch_rres(const AmgLevelDev<T>& L, const T* b, const T* x, const T* d, T* y, hipStream_t st,
+                        const int* ms) {
+    launch_rres_<T, B, 32, 2>(L, b, x, d, y, st, ms);
 }
 
 template <typename T, int B, int NC>
