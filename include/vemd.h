@@ -119,6 +119,40 @@ enum { VEMD_MAP_SIN = 0, VEMD_MAP_ARCS = 1 };
 enum { VEMD_SOL_SINCOS = 0, VEMD_SOL_SINSIN = 1 };
 enum { VEMD_BC_FLUX_MEANZERO = 0, VEMD_BC_DIRICHLET_P = 1 };
 
+/* Problem data as callbacks: the fields of the reference's DarcyPdeData and
+ * its IsoparametricMapping (CPU/examples/darcy_problem.h:39-63,162-180), so a
+ * caller of DarcyProblem can pass its own permeability, solution, source,
+ * boundary data and map.  Points are physical (x, y) except for map, which
+ * takes computational (xi, eta).  The host element setup and the error
+ * functionals call them, from several library threads at once (they must be
+ * thread-safe); the library keeps a copy of this struct, the caller keeps ctx
+ * valid while the solver exists.  A NULL member keeps the built-in choice of
+ * vemd_problem_desc (K = I, the manufactured solution `solution`, the map
+ * `map_kind`).  Only vemd_solver_create evaluates callbacks; the GPU element
+ * setup rejects them (VEMD_ERR_ARG). */
+typedef struct {
+    void* ctx;
+    /* permeability tensor K(x, y), row-major [xx, xy, yx, yy], symmetric
+       positive definite (the reference's kappa); per cell, exp(theta_E)
+       multiplies it */
+    void (*kappa)(void* ctx, double x, double y, double* k4);
+    /* exact pressure (the errors; Dirichlet data unless dirichlet_p is set) */
+    double (*solution_p)(void* ctx, double x, double y);
+    /* exact flux u = -K grad p as [ux, uy] (the errors; the essential
+       normal-flux data unless neumann_un is set) */
+    void (*solution_u)(void* ctx, double x, double y, double* u2);
+    /* source g = div u */
+    double (*source_g)(void* ctx, double x, double y);
+    /* boundary pressure (VEMD_BC_DIRICHLET_P) */
+    double (*dirichlet_p)(void* ctx, double x, double y);
+    /* boundary normal flux u . n for the outward unit normal (nx, ny)
+       (VEMD_BC_FLUX_MEANZERO) */
+    double (*neumann_un)(void* ctx, double x, double y, double nx, double ny);
+    /* computational -> physical map: xy = F(xi, eta) and its Jacobian
+       j4 = [dx/dxi, dx/deta, dy/dxi, dy/deta] (det > 0) */
+    void (*map)(void* ctx, double xi, double eta, double* xy, double* j4);
+} vemd_problem_funcs;
+
 typedef struct {
     int k;              /* polynomial degree: 0 or 1 on the device (0..3 in setup) */
     int nq2d;           /* triangle rule points: 1, 3, 4, 7, 9, 12 */
@@ -131,6 +165,7 @@ typedef struct {
     int bc;             /* VEMD_BC_FLUX_MEANZERO (reference: essential normal flux and a
                            mean-zero pressure multiplier) or VEMD_BC_DIRICHLET_P */
     const double* theta;/* per-cell log-permeability (K = exp(theta) I), or NULL for K = I */
+    const vemd_problem_funcs* funcs; /* callbacks replacing the built-in data, or NULL */
 } vemd_problem_desc;
 
 /* Builds every element operator in FP64 on the host (projector, stabilisation,

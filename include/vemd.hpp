@@ -10,6 +10,10 @@
 #ifndef VEMD_HPP
 #define VEMD_HPP
 
+#include <exception>
+#include <functional>
+#include <limits>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -55,8 +59,34 @@ private:
     }
 };
 
+// The reference's data types and its DarcyPdeData fields
+// (CPU/examples/darcy_problem.h:17-33,162-180), and its mapping as the pair
+// physical_coords / jacobian of IsoparametricMapping (:39-63).
+struct Vector2D {
+    double x = 0.0, y = 0.0;
+};
+struct Tensor2D {
+    double xx = 1.0, xy = 0.0, yx = 0.0, yy = 1.0;
+};
+struct DarcyPdeData {
+    std::function<Tensor2D(double x, double y)> kappa;
+    std::function<double(double x, double y)> dirichlet_p;
+    std::function<double(double x, double y, double nx, double ny)> neumann_un;
+    std::function<double(double x, double y)> solution_p;
+    std::function<Vector2D(double x, double y)> solution_u;
+    std::function<double(double x, double y)> source_g;
+};
+struct IsoparametricMapping {
+    std::function<Vector2D(double xi, double eta)> physical_coords;
+    std::function<Tensor2D(double xi, double eta)> jacobian;  // xx = dx/dxi, xy = dx/deta, ...
+};
+
 class DarcyProblem {
 public:
+    DarcyProblem() = default;
+    DarcyProblem(const DarcyProblem&) = delete;  // the solver keeps a pointer to it, as the reference does
+    DarcyProblem& operator=(const DarcyProblem&) = delete;
+
     void set_problem_index(int i) { index_ = i; }
     bool init_problem() {
         desc_.map_kind = VEMD_MAP_SIN;
@@ -77,28 +107,126 @@ public:
     void set_solution(int s) { desc_.solution = s; }
     void set_bc(int bc) { desc_.bc = bc; }
     void set_theta(std::vector<double> theta) { theta_ = std::move(theta); }
+    // Caller-defined data and map (any empty member keeps the built-in one).
+    // The problem must outlive the solvers built on it.
+    void set_pde_data(DarcyPdeData data) { pde_ = std::move(data); }
+    void set_mapping(IsoparametricMapping map) { map_ = std::move(map); }
+    const DarcyPdeData& get_pde_data() const { return pde_; }
+
     vemd_problem_desc desc(int k, int nq2d, int nq1d) const {
         vemd_problem_desc d = desc_;
         d.k = k;
         d.nq2d = nq2d;
         d.nq1d = nq1d;
         d.theta = theta_.empty() ? nullptr : theta_.data();
+        funcs_ = vemd_problem_funcs{};
+        funcs_.ctx = const_cast<DarcyProblem*>(this);
+        if (pde_.kappa) funcs_.kappa = &DarcyProblem::t_kappa;
+        if (pde_.solution_p) funcs_.solution_p = &DarcyProblem::t_solution_p;
+        if (pde_.solution_u) funcs_.solution_u = &DarcyProblem::t_solution_u;
+        if (pde_.source_g) funcs_.source_g = &DarcyProblem::t_source_g;
+        if (pde_.dirichlet_p) funcs_.dirichlet_p = &DarcyProblem::t_dirichlet_p;
+        if (pde_.neumann_un) funcs_.neumann_un = &DarcyProblem::t_neumann_un;
+        if (map_.physical_coords && map_.jacobian) funcs_.map = &DarcyProblem::t_map;
+        const bool any = funcs_.kappa || funcs_.solution_p || funcs_.solution_u || funcs_.source_g ||
+                         funcs_.dirichlet_p || funcs_.neumann_un || funcs_.map;
+        d.funcs = any ? &funcs_ : nullptr;
         return d;
+    }
+    // The first exception thrown by a callback since the last call (the
+    // library sees NaN; the exception must not cross the C ABI).
+    void rethrow_pending() const {
+        std::exception_ptr e;
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            std::swap(e, err_);
+        }
+        if (e) std::rethrow_exception(e);
     }
 
 private:
     int index_ = 0;
-    vemd_problem_desc desc_{1, 9, 9, VEMD_MAP_SIN, 0.05, VEMD_SOL_SINCOS, VEMD_BC_FLUX_MEANZERO, nullptr};
+    vemd_problem_desc desc_{1, 9, 9, VEMD_MAP_SIN, 0.05, VEMD_SOL_SINCOS, VEMD_BC_FLUX_MEANZERO, nullptr, nullptr};
     std::vector<double> theta_;
+    DarcyPdeData pde_;
+    IsoparametricMapping map_;
+    mutable vemd_problem_funcs funcs_{};
+    mutable std::mutex mu_;
+    mutable std::exception_ptr err_;
+
+    static constexpr double nan_() { return std::numeric_limits<double>::quiet_NaN(); }
+    template <class F>
+    static void guarded(void* ctx, F&& f) {
+        const DarcyProblem* p = static_cast<const DarcyProblem*>(ctx);
+        try {
+            f(*p);
+        } catch (...) {
+            std::lock_guard<std::mutex> l(p->mu_);
+            if (!p->err_) p->err_ = std::current_exception();
+        }
+    }
+    static void t_kappa(void* c, double x, double y, double* k) {
+        k[0] = k[1] = k[2] = k[3] = nan_();
+        guarded(c, [&](const DarcyProblem& p) {
+            const Tensor2D t = p.pde_.kappa(x, y);
+            k[0] = t.xx;
+            k[1] = t.xy;
+            k[2] = t.yx;
+            k[3] = t.yy;
+        });
+    }
+    static double t_solution_p(void* c, double x, double y) {
+        double v = nan_();
+        guarded(c, [&](const DarcyProblem& p) { v = p.pde_.solution_p(x, y); });
+        return v;
+    }
+    static void t_solution_u(void* c, double x, double y, double* u) {
+        u[0] = u[1] = nan_();
+        guarded(c, [&](const DarcyProblem& p) {
+            const Vector2D v = p.pde_.solution_u(x, y);
+            u[0] = v.x;
+            u[1] = v.y;
+        });
+    }
+    static double t_source_g(void* c, double x, double y) {
+        double v = nan_();
+        guarded(c, [&](const DarcyProblem& p) { v = p.pde_.source_g(x, y); });
+        return v;
+    }
+    static double t_dirichlet_p(void* c, double x, double y) {
+        double v = nan_();
+        guarded(c, [&](const DarcyProblem& p) { v = p.pde_.dirichlet_p(x, y); });
+        return v;
+    }
+    static double t_neumann_un(void* c, double x, double y, double nx, double ny) {
+        double v = nan_();
+        guarded(c, [&](const DarcyProblem& p) { v = p.pde_.neumann_un(x, y, nx, ny); });
+        return v;
+    }
+    static void t_map(void* c, double xi, double eta, double* xy, double* j) {
+        xy[0] = xy[1] = j[0] = j[1] = j[2] = j[3] = nan_();
+        guarded(c, [&](const DarcyProblem& p) {
+            const Vector2D f = p.map_.physical_coords(xi, eta);
+            const Tensor2D t = p.map_.jacobian(xi, eta);
+            xy[0] = f.x;
+            xy[1] = f.y;
+            j[0] = t.xx;
+            j[1] = t.xy;
+            j[2] = t.yx;
+            j[3] = t.yy;
+        });
+    }
 };
 
 class DarcySolver {
 public:
     DarcySolver(const StraightMeshReader& reader, const DarcyProblem& problem, int gauss_point_num = 9,
                 int gauss_point_num_1d = 9, int k = 1, int device = 0)
-        : device_(device) {
+        : problem_(&problem), device_(device) {
         vemd_problem_desc d = problem.desc(k, gauss_point_num, gauss_point_num_1d);
-        check(vemd_solver_create(reader.handle(), &d, &s_));
+        const vemd_status st = vemd_solver_create(reader.handle(), &d, &s_);
+        problem.rethrow_pending();
+        check(st);
         int64_t z[6];
         check(vemd_solver_sizes(s_, z));
         nflux_ = z[0];
@@ -146,6 +274,7 @@ public:
     }
 
 private:
+    const DarcyProblem* problem_;
     vemd_solver s_ = nullptr;
     int device_;
     bool uploaded_ = false;
@@ -156,7 +285,9 @@ private:
     std::vector<double> errors(int nq) {
         if (solution_.empty()) throw std::runtime_error("errors require a completed numerical solution");
         std::vector<double> e(6);
-        check(vemd_errors_full(s_, solution_.data(), nq, e.data()));
+        const vemd_status st = vemd_errors_full(s_, solution_.data(), nq, e.data());
+        problem_->rethrow_pending();
+        check(st);
         return e;
     }
 };

@@ -14,7 +14,11 @@
 //   * a per-cell scalar log-permeability theta_E (K = exp(theta_E) I);
 //   * the manufactured pressure sin(pi x) sin(pi y);
 //   * a Dirichlet-pressure (natural) boundary condition, which the reference
-//     declares but never defines (CPU/solver/DarcySolver.h:88).
+//     declares but never defines (CPU/solver/DarcySolver.h:88);
+//   * the caller-defined data (OrcFuncs) also feed the reference's declared
+//     but unused dirichlet_p and neumann_un fields into the boundary
+//     conditions; kappa, solution_p/u, source_g and the map enter as in the
+//     reference (computational_kappa etc., CPU/examples/darcy_problem.cpp:37-83).
 //
 // Parity status of each part is recorded in the test files that pin it.
 
@@ -581,12 +585,27 @@ enum { MAP_SIN = 0, MAP_ARCS = 1 };
 enum { SOL_SINCOS = 0, SOL_SINSIN = 1 };
 enum { BC_FLUX_MEANZERO = 0, BC_DIRICHLET_P = 1 };
 
+// Caller-defined data: the DarcyPdeData fields (CPU/examples/darcy_problem.h:
+// 162-180) and the mapping's physical_coords + jacobian (:39-63) as C
+// callbacks; a null member keeps the built-in choice.
+struct OrcFuncs {
+    void* ctx;
+    void (*kappa)(void*, double, double, double*);           // [xx, xy, yx, yy]
+    double (*solution_p)(void*, double, double);
+    void (*solution_u)(void*, double, double, double*);
+    double (*source_g)(void*, double, double);
+    double (*dirichlet_p)(void*, double, double);
+    double (*neumann_un)(void*, double, double, double, double);
+    void (*map)(void*, double, double, double*, double*);   // F, J = [dx/dxi, dx/deta, dy/dxi, dy/deta]
+};
+
 struct Problem {
     int map_kind = MAP_SIN;
     double eps = 0.05;  // SinPerturbationMapping amplitude (darcy_problem.h:55-96)
     int sol = SOL_SINCOS;
     int bc = BC_FLUX_MEANZERO;
     std::vector<double> theta;  // EXTENSION: per-cell log-permeability
+    OrcFuncs f{};               // caller-defined data (all null: built-in)
 };
 
 struct Geo {  // map value at a computational point
@@ -803,6 +822,19 @@ struct Ctx {
 };
 
 static Geo eval_map(const Ctx& X, int cell, const Tri* tri, double xi, double eta) {
+    if (X.pb.f.map) {  // IsoparametricMapping::physical_coords and jacobian
+        double F[2], J[4];
+        X.pb.f.map(X.pb.f.ctx, xi, eta, F, J);
+        Geo g;
+        g.x = F[0];
+        g.y = F[1];
+        g.j00 = J[0];
+        g.j01 = J[1];
+        g.j10 = J[2];
+        g.j11 = J[3];
+        g.det = J[0] * J[3] - J[1] * J[2];
+        return g;
+    }
     if (X.pb.map_kind == MAP_SIN) return sin_map(X.pb.eps, xi, eta);
     (void)tri;
     Geo g;
@@ -849,6 +881,21 @@ static void exact(int sol, double x, double y, double* p, double* ux, double* uy
     }
 }
 
+// solution_p, solution_u, source_g at a physical point: the caller's where
+// given, else the built-in solution.
+static void pde_at(const Ctx& X, double x, double y, double* p, double* ux, double* uy, double* g) {
+    exact(X.pb.sol, x, y, p, ux, uy, g);
+    const OrcFuncs& f = X.pb.f;
+    if (f.solution_p) *p = f.solution_p(f.ctx, x, y);
+    if (f.solution_u) {
+        double u[2];
+        f.solution_u(f.ctx, x, y, u);
+        *ux = u[0];
+        *uy = u[1];
+    }
+    if (f.source_g) *g = f.source_g(f.ctx, x, y);
+}
+
 // Pulled-back data (CPU/examples/darcy_problem.cpp:37-83).  The data (source,
 // boundary pressure, boundary normal flux) are those of the K = I manufactured
 // solution whatever theta is: with a cellwise permeability the problem stays
@@ -856,7 +903,7 @@ static void exact(int sol, double x, double y, double* p, double* ux, double* uy
 static void comp_data(const Ctx& X, int cell, const Geo& G, double* ph, double* uhx, double* uhy, double* gh) {
     (void)cell;
     double p, ux, uy, g;
-    exact(X.pb.sol, G.x, G.y, &p, &ux, &uy, &g);
+    pde_at(X, G.x, G.y, &p, &ux, &uy, &g);
     *ph = p;
     // u_hat = det J * J^{-1} u
     double inv00 = G.j11 / G.det, inv01 = -G.j01 / G.det, inv10 = -G.j10 / G.det, inv11 = G.j00 / G.det;
@@ -870,8 +917,14 @@ static void kinv_hat(const Ctx& X, int cell, const Geo& G, double* T) {
     // follow the reference literally: build K_hat then invert it
     // (CPU/examples/darcy_problem.cpp:15-29,72-83; CPU/solver/DarcySolver.cpp:12-25)
     double ks = X.kappa_scale(cell);
+    // physical tensor: the caller's kappa(x, y) (times exp(theta_E)) or e^theta I
+    double K[4] = {1.0, 0.0, 0.0, 1.0};
+    if (X.pb.f.kappa) X.pb.f.kappa(X.pb.f.ctx, G.x, G.y, K);
+    for (double& v : K) v *= ks;
     double i00 = G.j11 / G.det, i01 = -G.j01 / G.det, i10 = -G.j10 / G.det, i11 = G.j00 / G.det;
-    double lxx = i00 * ks, lxy = i01 * ks, lyx = i10 * ks, lyy = i11 * ks;
+    // left = J^{-1} K, then det J * left * J^{-T} (transform_tensor)
+    double lxx = i00 * K[0] + i01 * K[2], lxy = i00 * K[1] + i01 * K[3];
+    double lyx = i10 * K[0] + i11 * K[2], lyy = i10 * K[1] + i11 * K[3];
     double kxx = G.det * (lxx * i00 + lxy * i01), kxy = G.det * (lxx * i10 + lxy * i11);
     double kyx = G.det * (lyx * i00 + lyy * i01), kyy = G.det * (lyx * i10 + lyy * i11);
     double d = kxx * kyy - kxy * kyx;
@@ -1223,7 +1276,8 @@ static void assemble(Ctx& X) {
                             }
                         }
                         double p, ux, uy, gg;
-                        exact(X.pb.sol, G.x, G.y, &p, &ux, &uy, &gg);
+                        pde_at(X, G.x, G.y, &p, &ux, &uy, &gg);
+                        if (X.pb.f.dirichlet_p) p = X.pb.f.dirichlet_p(X.pb.f.ctx, G.x, G.y);
                         mom[r] += p * Basis::edge_mono(r, x1, y1, x2, y2, qp.x, qp.y) * qp.w;
                     }
                 }
@@ -1271,7 +1325,14 @@ static void assemble(Ctx& X) {
                         Geo G = eval_map(X, c, tt, qp.x, qp.y);
                         double ph, ux, uy, gh;
                         comp_data(X, c, G, &ph, &ux, &uy, &gh);
-                        mo += (ux * nx + uy * ny) * Basis::edge_mono(d, x1, y1, x2, y2, qp.x, qp.y) * qp.w;
+                        double un = ux * nx + uy * ny;
+                        if (X.pb.f.neumann_un) {
+                            // u_hat . n_hat = det J |J^{-T} n_hat| (u . n), n the physical unit normal
+                            double mx = (G.j11 * nx - G.j10 * ny) / G.det, my = (-G.j01 * nx + G.j00 * ny) / G.det;
+                            double mn = std::sqrt(mx * mx + my * my);
+                            un = G.det * mn * X.pb.f.neumann_un(X.pb.f.ctx, G.x, G.y, mx / mn, my / mn);
+                        }
+                        mo += un * Basis::edge_mono(d, x1, y1, x2, y2, qp.x, qp.y) * qp.w;
                     }
                     int gd = d * m.ne + ge;
                     X.ub[gd] = mo;
@@ -1650,19 +1711,27 @@ int orc_edge_coeffs(int k, int nk, double xc, double yc, double h, double x1, do
 // examples 1 and 2), 1 = per-cell arc maps.  sol 0 = sin(pi x)cos(pi y),
 // 1 = sin(pi x)sin(pi y).  bc 0 = essential normal flux + mean-zero pressure
 // multiplier, 1 = Dirichlet pressure.  theta may be NULL (K = I).
-int orc_problem_new(void* mp, int k, int nq2, int nq1, int map_kind, double eps, int sol, int bc,
-                    const double* theta, void** out) {
+// funcs (may be NULL): caller-defined data, the DarcyPdeData fields and the
+// map, which replace the built-in choices member by member.
+int orc_problem_new2(void* mp, int k, int nq2, int nq1, int map_kind, double eps, int sol, int bc,
+                     const double* theta, const OrcFuncs* funcs, void** out) {
     ORC_TRY({
         Problem p;
         p.map_kind = map_kind;
         p.eps = eps;
         p.sol = sol;
         p.bc = bc;
+        if (funcs) p.f = *funcs;
         Mesh* m = (Mesh*)mp;
         if (theta) p.theta.assign(theta, theta + m->nc);
         *out = new Ctx(m, p, k, nq2, nq1);
         return 0;
     })
+}
+
+int orc_problem_new(void* mp, int k, int nq2, int nq1, int map_kind, double eps, int sol, int bc,
+                    const double* theta, void** out) {
+    return orc_problem_new2(mp, k, nq2, nq1, map_kind, eps, sol, bc, theta, nullptr, out);
 }
 
 void orc_problem_free(void* c) { delete (Ctx*)c; }

@@ -100,11 +100,55 @@ SOL_SINCOS, SOL_SINSIN = 0, 1
 BC_FLUX_MEANZERO, BC_DIRICHLET_P = 0, 1
 
 
+_KAPPA = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp)
+_SCAL = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double, ctypes.c_double)
+_VEC = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp)
+_NRM = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                        ctypes.c_double)
+_MAP = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp, _dp)
+
+
+class _OrcFuncs(ctypes.Structure):
+    _fields_ = [("ctx", ctypes.c_void_p), ("kappa", _KAPPA), ("solution_p", _SCAL), ("solution_u", _VEC),
+                ("source_g", _SCAL), ("dirichlet_p", _SCAL), ("neumann_un", _NRM), ("map", _MAP)]
+
+
+def _orc_funcs(pde):
+    """ctypes callbacks for an object with the optional attributes kappa,
+    solution_p, solution_u, source_g, dirichlet_p, neumann_un, map (Python
+    callables of physical (x, y), or (xi, eta) for map)."""
+    g = lambda name: getattr(pde, name, None)
+
+    def kappa(_, x, y, out):
+        k = np.asarray(g("kappa")(x, y), np.float64).reshape(2, 2)
+        out[0], out[1], out[2], out[3] = k[0, 0], k[0, 1], k[1, 0], k[1, 1]
+
+    def vec(_, x, y, out):
+        u = g("solution_u")(x, y)
+        out[0], out[1] = u[0], u[1]
+
+    def mp(_, xi, eta, xy, j):
+        (x, y), ((a, b), (c, d)) = g("map")(xi, eta)
+        xy[0], xy[1] = x, y
+        j[0], j[1], j[2], j[3] = a, b, c, d
+
+    def scal(name):
+        f = g(name)
+        return _SCAL(lambda _, x, y: float(f(x, y))) if f else _SCAL()
+
+    return _OrcFuncs(None, _KAPPA(kappa) if g("kappa") else _KAPPA(), scal("solution_p"),
+                     _VEC(vec) if g("solution_u") else _VEC(), scal("source_g"), scal("dirichlet_p"),
+                     _NRM(lambda _, x, y, nx, ny: float(g("neumann_un")(x, y, nx, ny))) if g("neumann_un")
+                     else _NRM(), _MAP(mp) if g("map") else _MAP())
+
+
 class Problem:
-    """Oracle discretisation: restates HdivMatrix + DarcySolver (CPU/solver/)."""
+    """Oracle discretisation: restates HdivMatrix + DarcySolver (CPU/solver/).
+    pde: optional object of callables (DarcyPdeData fields and map) replacing
+    the built-in data."""
 
     def __init__(self, mesh, k=1, nq2=9, nq1=9, map_kind=MAP_SIN, eps=0.05, sol=SOL_SINCOS,
-                 bc=BC_FLUX_MEANZERO, theta=None):
+                 bc=BC_FLUX_MEANZERO, theta=None, pde=None):
         self.mesh = mesh
         self.k = k
         h = ctypes.c_void_p()
@@ -112,8 +156,10 @@ class Problem:
         if theta is not None:
             self._theta = np.ascontiguousarray(theta, np.float64)
             th = _d(self._theta)
-        _check(lib().orc_problem_new(mesh.h, k, nq2, nq1, map_kind, ctypes.c_double(eps), sol, bc, th,
-                                     ctypes.byref(h)))
+        self._funcs = None if pde is None else _orc_funcs(pde)
+        _check(lib().orc_problem_new2(mesh.h, k, nq2, nq1, map_kind, ctypes.c_double(eps), sol, bc, th,
+                                      None if self._funcs is None else ctypes.byref(self._funcs),
+                                      ctypes.byref(h)))
         self.h = h
         self.assembled = False
 

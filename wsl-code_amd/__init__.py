@@ -18,7 +18,7 @@ from ._lib import (VemdError, lib, device_count, BC_DIRICHLET_P, BC_FLUX_MEANZER
                    SOL_SINCOS, SOL_SINSIN, FP32, FP64, MIXED, DOMAIN_ANNULUS, DOMAIN_DISK, PC_AUTO, PC_JACOBI,
                    PC_MULTIGRID)
 from .mesh import Mesh, StraightMeshReader, generate_mesh
-from .solver import DarcyProblem, DarcySolver, Solver, SolveOptions
+from .solver import DarcyProblem, DarcySolver, PdeData, Solver, SolveOptions
 
 __all__ = [
     "VemdError", "lib", "device_count", "Mesh", "StraightMeshReader", "generate_mesh", "DarcyProblem",

@@ -34,9 +34,24 @@ class MeshOpts(ctypes.Structure):
                 ("min_edge_frac", ctypes.c_double)]
 
 
+# callback types of vemd_problem_funcs
+KAPPA_FN = ctypes.CFUNCTYPE(None, c_vp, ctypes.c_double, ctypes.c_double, c_dp)
+SCALAR_FN = ctypes.CFUNCTYPE(ctypes.c_double, c_vp, ctypes.c_double, ctypes.c_double)
+VECTOR_FN = ctypes.CFUNCTYPE(None, c_vp, ctypes.c_double, ctypes.c_double, c_dp)
+NORMAL_FN = ctypes.CFUNCTYPE(ctypes.c_double, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double)
+MAP_FN = ctypes.CFUNCTYPE(None, c_vp, ctypes.c_double, ctypes.c_double, c_dp, c_dp)
+
+
+class ProblemFuncs(ctypes.Structure):
+    _fields_ = [("ctx", c_vp), ("kappa", KAPPA_FN), ("solution_p", SCALAR_FN), ("solution_u", VECTOR_FN),
+                ("source_g", SCALAR_FN), ("dirichlet_p", SCALAR_FN), ("neumann_un", NORMAL_FN), ("map", MAP_FN)]
+
+
 class ProblemDesc(ctypes.Structure):
     _fields_ = [("k", ctypes.c_int), ("nq2d", ctypes.c_int), ("nq1d", ctypes.c_int), ("map_kind", ctypes.c_int),
-                ("eps", ctypes.c_double), ("solution", ctypes.c_int), ("bc", ctypes.c_int), ("theta", c_dp)]
+                ("eps", ctypes.c_double), ("solution", ctypes.c_int), ("bc", ctypes.c_int), ("theta", c_dp),
+                ("funcs", ctypes.POINTER(ProblemFuncs))]
 
 
 class SolveOptsC(ctypes.Structure):

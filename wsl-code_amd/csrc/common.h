@@ -81,7 +81,8 @@ struct CellOps {
 
 struct Setup {
     Shape sh;
-    vemd_problem_desc desc;
+    vemd_problem_desc desc;   // desc.funcs points to funcs (or is null)
+    vemd_problem_funcs funcs{};
     int64_t nflux_total = 0, npres_total = 0, ntot = 0;
     // Per cell, reference ordering of local flux dofs; A packed lower triangle
     // (i*(i+1)/2 + j, j <= i), K = I part (the cell scale exp(-theta) applied

@@ -353,6 +353,16 @@ struct CellGeom {
             g.j10 = J[2];
             g.j11 = J[3];
         }
+        if (d.funcs && d.funcs->map) {  // the caller's map replaces the built-in one
+            double xy[2], J[4];
+            d.funcs->map(d.funcs->ctx, x, y, xy, J);
+            g.X = xy[0];
+            g.Y = xy[1];
+            g.j00 = J[0];
+            g.j01 = J[1];
+            g.j10 = J[2];
+            g.j11 = J[3];
+        }
         g.det = g.j00 * g.j11 - g.j01 * g.j10;
         return g;
     }
@@ -361,7 +371,7 @@ struct CellGeom {
 static CellGeom cell_geom(const Mesh& m, const vemd_problem_desc& d, int c) {
     CellGeom cg;
     int ne = m.npc(c), s0 = m.off[c];
-    if (d.map_kind != VEMD_MAP_ARCS || m.arcs.empty()) return cg;
+    if (d.map_kind != VEMD_MAP_ARCS || m.arcs.empty() || (d.funcs && d.funcs->map)) return cg;
     for (int j = 0; j < ne; ++j) {
         int a0 = m.nodes[s0 + j], a1 = m.nodes[s0 + (j + 1) % ne];
         auto it = m.arcs.find(Mesh::key(a0, a1));
@@ -398,6 +408,50 @@ static void exact(int sol, double x, double y, double& p, double& ux, double& uy
         uy = -M_PI * sx * cy;
     }
     g = 2 * M_PI * M_PI * p;
+}
+
+// Problem data at a physical point -- pressure, flux and source -- from the
+// caller's callbacks where given (the reference's DarcyPdeData fields
+// solution_p, solution_u, source_g), else the manufactured solution.
+static void pde_data(const vemd_problem_desc& d, double x, double y, double& p, double& ux, double& uy, double& g) {
+    exact(d.solution, x, y, p, ux, uy, g);
+    if (const vemd_problem_funcs* f = d.funcs) {
+        if (f->solution_p) p = f->solution_p(f->ctx, x, y);
+        if (f->solution_u) {
+            double u[2] = {0.0, 0.0};
+            f->solution_u(f->ctx, x, y, u);
+            ux = u[0];
+            uy = u[1];
+        }
+        if (f->source_g) g = f->source_g(f->ctx, x, y);
+    }
+}
+
+// T = K_hat^{-1} = J^T K^{-1} J / det J at a volume point (the inverse of the
+// reference's computational_kappa, det J J^{-1} K J^{-T},
+// CPU/examples/darcy_problem.cpp:15-29,72-83): K = I unless the caller gives
+// kappa, which must be symmetric positive definite.  t = [t00, t01, t11].
+static void kinv_hat(const vemd_problem_desc& d, const Geo& g, int cell, double* t) {
+    double i00 = 1.0, i01 = 0.0, i11 = 1.0;
+    if (d.funcs && d.funcs->kappa) {
+        double K[4] = {1.0, 0.0, 0.0, 1.0};
+        d.funcs->kappa(d.funcs->ctx, g.X, g.Y, K);
+        const double sc = std::fabs(K[0]) + std::fabs(K[3]);
+        const double det = K[0] * K[3] - K[1] * K[2];
+        if (!(std::fabs(K[1] - K[2]) <= 1e-12 * sc) || !(K[0] > 0.0) || !(det > 0.0) || !std::isfinite(det))
+            throw Error(VEMD_ERR_ARG, "permeability tensor not symmetric positive definite in cell " + std::to_string(cell));
+        const double k01 = 0.5 * (K[1] + K[2]);
+        const double dd = K[0] * K[3] - k01 * k01;
+        i00 = K[3] / dd;
+        i01 = -k01 / dd;
+        i11 = K[0] / dd;
+    }
+    // (K^{-1} J), then J^T (K^{-1} J) / det J
+    const double a00 = i00 * g.j00 + i01 * g.j10, a01 = i00 * g.j01 + i01 * g.j11;
+    const double a10 = i01 * g.j00 + i11 * g.j10, a11 = i01 * g.j01 + i11 * g.j11;
+    t[0] = (g.j00 * a00 + g.j10 * a10) / g.det;
+    t[1] = (g.j00 * a01 + g.j10 * a11) / g.det;
+    t[2] = (g.j01 * a01 + g.j11 * a11) / g.det;
 }
 
 // --------------------------------------------------------------- one cell
@@ -472,8 +526,9 @@ static void build_cell(const Mesh& m, const vemd_problem_desc& d, const Shape& S
     double Ti[4] = {0, 0, 0, 0};
     for (int l = 0; l < np; ++l) {
         const Geo& g = w.geo[l];
-        double t00 = (g.j00 * g.j00 + g.j10 * g.j10) / g.det, t01 = (g.j00 * g.j01 + g.j10 * g.j11) / g.det,
-               t11 = (g.j01 * g.j01 + g.j11 * g.j11) / g.det;
+        double tk[3];
+        kinv_hat(d, g, c, tk);
+        const double t00 = tk[0], t01 = tk[1], t11 = tk[2];
         double wl = w.pw[l];
         Ti[0] += t00 * wl;
         Ti[1] += t01 * wl;
@@ -583,15 +638,26 @@ static void build_cell(const Mesh& m, const vemd_problem_desc& d, const Shape& S
                 }
             }
             double mom_p[4] = {0, 0, 0, 0};
+            const vemd_problem_funcs* uf = d.funcs;
             for (size_t i = 0; i < glx.size(); ++i) {
                 Geo g = cg.eval(d, *tt, ex[i], ey[i]);
                 double pe, ux, uy, gg;
-                exact(d.solution, g.X, g.Y, pe, ux, uy, gg);
-                // u_hat = det J J^{-1} u
-                double uhx = g.j11 * ux - g.j01 * uy, uhy = -g.j10 * ux + g.j00 * uy;
+                pde_data(d, g.X, g.Y, pe, ux, uy, gg);
+                if (uf && uf->dirichlet_p) pe = uf->dirichlet_p(uf->ctx, g.X, g.Y);
+                // u_hat . n_hat with u_hat = det J J^{-1} u, i.e. det J u . (J^{-T} n_hat)
+                double uhn;
+                if (uf && uf->neumann_un) {
+                    // the physical normal is J^{-T} n_hat / |J^{-T} n_hat|
+                    const double mx = (g.j11 * nx - g.j10 * ny) / g.det, my = (-g.j01 * nx + g.j00 * ny) / g.det;
+                    const double mn = std::sqrt(mx * mx + my * my);
+                    uhn = g.det * mn * uf->neumann_un(uf->ctx, g.X, g.Y, mx / mn, my / mn);
+                } else {
+                    const double uhx = g.j11 * ux - g.j01 * uy, uhy = -g.j10 * ux + g.j00 * uy;
+                    uhn = uhx * nx + uhy * ny;
+                }
                 for (int r = 0; r <= k; ++r) {
                     double tr = ipow(et[i], r);
-                    o.ub[r * ne + j] += (uhx * nx + uhy * ny) * tr * ew[i];
+                    o.ub[r * ne + j] += uhn * tr * ew[i];
                     mom_p[r] += pe * tr * ew[i];
                 }
             }
@@ -664,7 +730,7 @@ static void build_cell(const Mesh& m, const vemd_problem_desc& d, const Shape& S
         for (int l = 0; l < np; ++l) {
             const Geo& g = w.geo[l];
             double pe, ux, uy, gg;
-            exact(d.solution, g.X, g.Y, pe, ux, uy, gg);
+            pde_data(d, g.X, g.Y, pe, ux, uy, gg);
             double mr = w.mono[l * nm + r];
             sl += mr * g.det * w.pw[l];
             sf += g.det * gg * mr * w.pw[l];
@@ -713,6 +779,10 @@ Setup* setup_prepare(const Mesh& m, const vemd_problem_desc& d) {
     S->sh = Shape(d.k);
     S->desc = d;
     S->desc.theta = nullptr;
+    if (d.funcs) {  // a copy the library owns (the caller's struct may go away)
+        S->funcs = *d.funcs;
+        S->desc.funcs = &S->funcs;
+    }
     const Shape& sh = S->sh;
     int nc = m.nc;
     auto& O = S->ofs;
@@ -897,7 +967,7 @@ void errors_host(const Solver& s, const double* sol, int nq, double* out) {
                     }
                     Geo g = cg.eval(d, t, x, y);
                     double pe, ux, uy, gg;
-                    exact(d.solution, g.X, g.Y, pe, ux, uy, gg);
+                    pde_data(d, g.X, g.Y, pe, ux, uy, gg);
                     double uhx = g.j11 * ux - g.j01 * uy, uhy = -g.j10 * ux + g.j00 * uy;
                     double dp = ph - pe, dx = vx - uhx, dy = vy - uhy;
                     ep += dp * dp * w;

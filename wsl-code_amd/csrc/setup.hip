@@ -624,6 +624,8 @@ void down(std::vector<T>& v, const T* d) {
 
 Setup* setup_build_device(const Mesh& m, const vemd_problem_desc& d, int device) {
     if (d.k > 1) throw Error(VEMD_ERR_ARG, "the device element setup supports k = 0 and k = 1");
+    if (d.funcs)
+        throw Error(VEMD_ERR_ARG, "problem callbacks run on the host: use vemd_solver_create for them");
     for (int c = 0; c < m.nc; ++c)
         if (m.npc(c) > NE)
             throw Error(VEMD_ERR_ARG, "cell " + std::to_string(c) + " has more than " + std::to_string(NE) +

@@ -5,8 +5,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (BC_DIRICHLET_P, BC_FLUX_MEANZERO, FP32, FP64, MAP_ARCS, MAP_SIN, MIXED, ProblemDesc, SOL_SINCOS,
-                   SOL_SINSIN, SolveInfo, SolveOptsC, VemdError, c_dp, c_ip, c_lp, c_vp, check, lib)
+from ._lib import (BC_DIRICHLET_P, BC_FLUX_MEANZERO, FP32, FP64, KAPPA_FN, MAP_ARCS, MAP_FN, MAP_SIN, MIXED,
+                   NORMAL_FN, SCALAR_FN, SOL_SINCOS, SOL_SINSIN, VECTOR_FN, ProblemDesc, ProblemFuncs, SolveInfo,
+                   SolveOptsC, VemdError, c_dp, c_ip, c_lp, c_vp, check, lib)
 
 
 def _d(a):
@@ -38,22 +39,112 @@ class SolveOptions:
                           self.use_graph, self.precond, self.stream or None)
 
 
+@dataclass
+class PdeData:
+    """Problem data as Python callables, the fields of the reference's
+    DarcyPdeData plus its IsoparametricMapping (CPU/examples/darcy_problem.h:
+    39-63,162-180).  Any field left None keeps the built-in choice.
+
+    kappa(x, y) -> 2x2 permeability tensor (symmetric positive definite)
+    solution_p(x, y) -> exact pressure;  solution_u(x, y) -> (ux, uy) = -K grad p
+    source_g(x, y) -> div u;  dirichlet_p(x, y) -> boundary pressure
+    neumann_un(x, y, nx, ny) -> u . n on the boundary
+    map(xi, eta) -> ((x, y), ((dx/dxi, dx/deta), (dy/dxi, dy/deta)))
+    """
+    kappa: object = None
+    solution_p: object = None
+    solution_u: object = None
+    source_g: object = None
+    dirichlet_p: object = None
+    neumann_un: object = None
+    map: object = None
+
+    def c_funcs(self):
+        """The vemd_problem_funcs struct; keep the returned object alive while
+        the solver exists (it holds the ctypes callbacks).  An exception in a
+        callable is kept (the C side then sees NaN) and re-raised by
+        raise_pending() after the library call."""
+        self._error = None
+
+        def guard(f, nan):
+            def g(*a):
+                try:
+                    return f(*a)
+                except BaseException as e:  # reported after the C call returns
+                    if self._error is None:
+                        self._error = e
+                    return nan
+            return g
+
+        def kappa(_, x, y, out):
+            k = np.asarray(self.kappa(x, y), np.float64).reshape(2, 2)
+            out[0], out[1], out[2], out[3] = k[0, 0], k[0, 1], k[1, 0], k[1, 1]
+
+        def vec(_, x, y, out):
+            u = self.solution_u(x, y)
+            out[0], out[1] = u[0], u[1]
+
+        def mp(_, xi, eta, xy, j):
+            (x, y), ((a, b), (c, d)) = self.map(xi, eta)
+            xy[0], xy[1] = x, y
+            j[0], j[1], j[2], j[3] = a, b, c, d
+
+        def fill_nan(out, n):
+            for i in range(n):
+                out[i] = float("nan")
+
+        def vguard(f, n, pos):
+            def g(*a):
+                try:
+                    f(*a)
+                except BaseException as e:
+                    if self._error is None:
+                        self._error = e
+                    for p in pos:
+                        fill_nan(a[p], n)
+            return g
+
+        def scal(f):
+            return SCALAR_FN(guard(lambda _, x, y: float(f(x, y)), float("nan"))) if f else SCALAR_FN()
+
+        nrm = guard(lambda _, x, y, nx, ny: float(self.neumann_un(x, y, nx, ny)), float("nan"))
+        return ProblemFuncs(None, KAPPA_FN(vguard(kappa, 4, [3])) if self.kappa else KAPPA_FN(),
+                            scal(self.solution_p), VECTOR_FN(vguard(vec, 2, [3])) if self.solution_u else VECTOR_FN(),
+                            scal(self.source_g), scal(self.dirichlet_p),
+                            NORMAL_FN(nrm) if self.neumann_un else NORMAL_FN(),
+                            MAP_FN(vguard(mp, 2, [3])) if self.map else MAP_FN())
+
+    def raise_pending(self):
+        e, self._error = getattr(self, "_error", None), None
+        if e is not None:
+            raise e
+
+
 class Solver:
     """Element setup (host FP64) and device solve for one mesh and problem."""
 
     def __init__(self, mesh, k=1, nq2d=9, nq1d=9, map_kind=MAP_SIN, eps=0.05, solution=SOL_SINCOS,
-                 bc=BC_FLUX_MEANZERO, theta=None, device=None, setup_device=None):
+                 bc=BC_FLUX_MEANZERO, theta=None, device=None, setup_device=None, pde=None):
         """setup_device: build the element operators on that GPU instead of
-        the host cores (k <= 1, polygons of up to 10 edges)."""
+        the host cores (k <= 1, polygons of up to 10 edges).  pde: a PdeData
+        of callables replacing the built-in problem data (host setup only)."""
         self.mesh = mesh
         self._theta = None if theta is None else _vec(theta, mesh.num_cells, "theta")
+        self._funcs = None if pde is None else pde.c_funcs()  # callbacks stay referenced by the solver
         d = ProblemDesc(k, nq2d, nq1d, map_kind, eps, solution, bc,
-                        None if self._theta is None else _d(self._theta))
+                        None if self._theta is None else _d(self._theta),
+                        None if self._funcs is None else ctypes.pointer(self._funcs))
         self.h = c_vp()
-        if setup_device is None:
-            check(lib().vemd_solver_create(mesh.h, ctypes.byref(d), ctypes.byref(self.h)))
-        else:
-            check(lib().vemd_solver_create_device(mesh.h, ctypes.byref(d), int(setup_device), ctypes.byref(self.h)))
+        self._pde = pde
+        try:
+            if setup_device is None:
+                check(lib().vemd_solver_create(mesh.h, ctypes.byref(d), ctypes.byref(self.h)))
+            else:
+                check(lib().vemd_solver_create_device(mesh.h, ctypes.byref(d), int(setup_device),
+                                                      ctypes.byref(self.h)))
+        finally:
+            if pde is not None:
+                pde.raise_pending()
         s = np.zeros(6, np.int64)
         check(lib().vemd_solver_sizes(self.h, s.ctypes.data_as(c_lp)))
         self.nflux, self.npres, self.ntot, self.nvec, self.ngroups, self.max_ne = (int(v) for v in s)
@@ -156,7 +247,11 @@ class Solver:
         and the mass int p_h det J."""
         x = _vec(x, self.ntot, "x")
         out = np.zeros(6)
-        check(lib().vemd_errors_full(self.h, _d(x), nq, _d(out)))
+        try:
+            check(lib().vemd_errors_full(self.h, _d(x), nq, _d(out)))
+        finally:
+            if self._pde is not None:
+                self._pde.raise_pending()
         return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3], div_phys=out[4], mass=out[5])
 
     def element(self, cell):
@@ -189,9 +284,12 @@ class DarcyProblem:
     Extensions: map_kind=MAP_ARCS (per-cell arc maps), solution=SOL_SINSIN,
     bc=BC_DIRICHLET_P and a per-cell log-permeability theta."""
 
-    def __init__(self, map_kind=MAP_SIN, eps=0.05, solution=SOL_SINCOS, bc=BC_FLUX_MEANZERO, theta=None):
+    def __init__(self, map_kind=MAP_SIN, eps=0.05, solution=SOL_SINCOS, bc=BC_FLUX_MEANZERO, theta=None, pde=None):
+        """pde: a PdeData of callables (the reference's DarcyPdeData fields and
+        its map) replacing the built-in data."""
         self.problem_index = 0
         self.map_kind, self.eps, self.solution, self.bc, self.theta = map_kind, eps, solution, bc, theta
+        self.pde = pde
 
     def set_problem_index(self, index):
         self.problem_index = index
@@ -212,7 +310,8 @@ class DarcySolver:
     def __init__(self, mesh_reader, problem, gauss_point_num=9, gauss_point_num_1d=9, k=1, device=0):
         mesh = getattr(mesh_reader, "mesh", mesh_reader)
         self.solver = Solver(mesh, k=k, nq2d=gauss_point_num, nq1d=gauss_point_num_1d, map_kind=problem.map_kind,
-                             eps=problem.eps, solution=problem.solution, bc=problem.bc, theta=problem.theta)
+                             eps=problem.eps, solution=problem.solution, bc=problem.bc, theta=problem.theta,
+                             pde=getattr(problem, "pde", None))
         self.device = device
         self.solution = None
         self.info = None
