@@ -512,7 +512,13 @@ def main():
     nf = s.nflux
     flux_rel = float(np.linalg.norm(xr_m[:nf] - xr_64[:nf]) / np.linalg.norm(xr_64[:nf]))
     pres_rel = float(np.linalg.norm(xr_m[nf:] - xr_64[nf:]) / np.linalg.norm(xr_64[nf:]))
-    em, e64 = s.errors(xr_m), s.errors(xr_64)
+    # error functionals of both solutions where they lie, on the device
+    torch.cuda.synchronize()
+    t0 = time.time()
+    em = s.errors_device(xm)
+    t_err = time.time() - t0
+    e64 = s.errors_device(x64)
+    log("device error functionals: %.1f ms (mixed) %s" % (1000 * t_err, em))
     cpu = None
     if not args.no_cpu_baseline:
         sizes = [n for n in (1024, 2048, 4096, 8192, 16384, 32768, 65536) if n <= args.cpu_cells]
@@ -543,7 +549,8 @@ def main():
         "mixed": infos[-1], "fp64": i64,
         "mixed_vs_fp64": {"flux_rel_l2": flux_rel, "pressure_rel_l2": pres_rel,
                           "flux_err_diff": abs(em["u_phys"] - e64["u_phys"]),
-                          "pressure_err_diff": abs(em["p_phys"] - e64["p_phys"])},
+                          "pressure_err_diff": abs(em["p_phys"] - e64["p_phys"]),
+                          "errors_on_device_ms": round(1000 * t_err, 2)},
         "apply_GBps": {"fp64": round(gbs64, 1), "fp32": round(gbs32, 1)},
         "roofline": roofline, "cpu_baseline": cpu,
     }

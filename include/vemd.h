@@ -347,6 +347,17 @@ vemd_status vemd_errors(vemd_solver s, const double* solution, int nq2d, double*
  * (GPU/src/equation_gpu.cu:385-479).  Host FP64. */
 vemd_status vemd_errors_full(vemd_solver s, const double* solution, int nq2d, double* out6);
 
+/* The six values of vemd_errors_full for a solution already in device
+ * memory: x_dev holds sizes[3] doubles in the internal layout (as
+ * vemd_solve_device writes it).  Evaluated on the GPU, one thread per cell
+ * rebuilding the cell's projector as the device element setup does, with
+ * fixed-order sums (deterministic); it runs after the work queued on the
+ * caller's HIP stream (NULL: the null stream) and returns the values.
+ * Built-in problem data, k = 0 or 1, polygons of up to 10 edges; problems
+ * with callbacks use vemd_errors_full.  Replaces error_kernel /
+ * reduce_errors (GPU/src/equation_gpu.cu:385-479). */
+vemd_status vemd_errors_device(vemd_solver s, const void* x_dev, int nq2d, double* out6, void* stream);
+
 /* Adjoint gradient of a loss L(x) with respect to theta: solves K lambda =
  * dL/dx (dldx in reference order) and returns dL/dtheta_E =
  * lambda_E^T (exp(-theta_E) A_E) x_E per cell.  No reference counterpart. */

@@ -22,6 +22,6 @@ from .solver import DarcyProblem, DarcySolver, PdeData, Solver, SolveOptions
 
 __all__ = [
     "VemdError", "lib", "device_count", "Mesh", "StraightMeshReader", "generate_mesh", "DarcyProblem",
-    "DarcySolver", "Solver", "SolveOptions", "BC_DIRICHLET_P", "BC_FLUX_MEANZERO", "MAP_ARCS", "MAP_SIN",
+    "DarcySolver", "PdeData", "Solver", "SolveOptions", "BC_DIRICHLET_P", "BC_FLUX_MEANZERO", "MAP_ARCS", "MAP_SIN",
     "SOL_SINCOS", "SOL_SINSIN", "FP32", "FP64", "MIXED", "DOMAIN_DISK", "DOMAIN_ANNULUS",
 ]

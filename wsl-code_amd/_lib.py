@@ -92,6 +92,7 @@ SIGNATURES = {
     "vemd_apply_device_split": [c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_vp],
     "vemd_apply_device_static": [c_vp, ctypes.c_int, c_vp, c_vp, c_vp],
     "vemd_edge_sides": [c_vp, c_lp, c_ip],
+    "vemd_errors_device": [c_vp, c_vp, ctypes.c_int, c_dp, c_vp],
     "vemd_solve_multi_device": [c_vp, ctypes.POINTER(SolveOptsC), ctypes.c_int, c_vp, c_vp, ctypes.POINTER(SolveInfo)],
     "vemd_apply_multi_device": [c_vp, c_vp, c_vp, c_vp, c_vp],
     "vemd_rhs": [c_vp, c_dp],

@@ -458,6 +458,13 @@ vemd_status vemd_errors_full(vemd_solver sh, const double* sol, int nq, double* 
     API_END
 }
 
+vemd_status vemd_errors_device(vemd_solver sh, const void* x_dev, int nq, double* out, void* stream) {
+    API_BEGIN
+    REQUIRE(sh && x_dev && out, "null argument");
+    dev_errors(*sh->s, x_dev, nq, stream, out);
+    API_END
+}
+
 vemd_status vemd_adjoint_theta(vemd_solver sh, const vemd_solve_opts* o, const double* sol, const double* dldx,
                                double* grad, vemd_solve_info* info) {
     API_BEGIN

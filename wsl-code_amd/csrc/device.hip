@@ -1156,7 +1156,17 @@ void dev_refresh(Solver& s) {
 }
 
 
+void dev_errors(Solver& s, const void* x_dev, int nq, void* stream, double* out6) {
+    if (!s.dev) throw Error(VEMD_ERR_STATE, "upload the solver before device error evaluation");
+    DeviceState& d = *s.dev;
+    HIPCHK(hipSetDevice(d.device));
+    ScratchUse use(d, d.stream, true, (hipStream_t)stream);
+    errors_device(s, (const double*)x_dev, nq, d.stream, out6);
+}
+
 void dev_free(Solver& s) {
+    errdev_free(s.errdev);
+    s.errdev = nullptr;
     if (!s.dev) return;
     DeviceState& d = *s.dev;
     (void)hipSetDevice(d.device);

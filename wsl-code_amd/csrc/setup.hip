@@ -14,6 +14,11 @@
 //   boundary        CPU/solver/DarcySolver.cpp:502-663
 // The per-thread matrices live in private memory; the quadrature points are
 // visited once and every volume integral is accumulated in that pass.
+//
+// The same cell function, in its error mode, evaluates the six error
+// functionals of a device-resident solution (the reference GPU port's
+// error_kernel / reduce_errors, GPU/src/equation_gpu.cu:385-479): the
+// projector is rebuilt per cell instead of being stored.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +26,7 @@
 #include <memory>
 
 #include "precond.h"
+#include "solver.h"
 #include "store.h"
 
 namespace vemd {
@@ -48,6 +54,19 @@ struct SetupOut {
     double *A, *W, *L, *f, *ru, *ub, *P, *alpha;
     char* isb;
     int* err;  // first failure: code, cell
+};
+
+// Error functionals of a solution (the error mode of the cell function): the
+// solution in the solver's internal layout on the device, where each cell
+// finds its dofs, the error quadrature rule, and six values per cell.
+struct ErrIn {
+    const double* x;           // internal-layout solution
+    const int* edge_int;       // internal number of every mesh edge
+    const int* gpos;           // store position of every cell
+    int64_t cbase, cpad;
+    int nt;
+    double ta[MAXQ2], tb[MAXQ2], tw[MAXQ2];
+    double* out;               // 6 per cell: the squared errors of errors_host and the mass
 };
 
 enum { E_JAC = 1, E_EAR, E_EDGE, E_SINGULAR, E_CONVEX, E_AREA, E_NE };
@@ -255,9 +274,11 @@ __host__ __device__ inline void dexact(int sol, double x, double y, double& p, d
     g = 2 * M_PI * M_PI * p;
 }
 
-// Element data of cell c; returns 0 or the failure code.
-template <int K>
-__host__ __device__ int setup_cell(const SetupIn& in, const SetupOut& o, int c) {
+// Element data of cell c; returns 0 or the failure code.  ERR: the same
+// projector construction, then the cell's error functionals of the solution
+// in *ei instead of the element outputs.
+template <int K, bool ERR = false>
+__host__ __device__ int setup_cell(const SetupIn& in, const SetupOut& o, int c, const ErrIn* ei = nullptr) {
     using S = Sh<K>;
     constexpr int q = S::q, p = S::p, gr = S::grad, nm = S::nm, kk = S::kk, NM = S::nmax;
     const int s0 = in.off[c], ne = in.off[c + 1] - s0;
@@ -474,6 +495,7 @@ __host__ __device__ int setup_cell(const SetupIn& in, const SetupOut& o, int c) 
                         if (cc % ne == s) v += Hes[a * kk + t2] * Hei[t2 * kk + s];
                 B[a * n + cc * ne + j] += v;
             }
+        if constexpr (ERR) continue;  // no boundary data in the error mode
         // boundary data on this edge
         double rub[kk] = {}, ubb[kk] = {};
         char isbb = 0;
@@ -517,9 +539,11 @@ __host__ __device__ int setup_cell(const SetupIn& in, const SetupOut& o, int c) 
     for (int i = edof; i < n; ++i) {
         const int gd = i - edof;
         if (gd < q - 1) W[(gd + 1) * n + i] += -1.0;
-        o.ru[v0 + i] = 0.0;  // internal dofs carry no boundary data
-        o.ub[v0 + i] = 0.0;
-        o.isb[v0 + i] = 0;
+        if constexpr (!ERR) {
+            o.ru[v0 + i] = 0.0;  // internal dofs carry no boundary data
+            o.ub[v0 + i] = 0.0;
+            o.isb[v0 + i] = 0;
+        }
     }
     // B_1 = -H* H^{-1} W, B_curl
     double Hinv[q * q];
@@ -544,6 +568,127 @@ __host__ __device__ int setup_cell(const SetupIn& in, const SetupOut& o, int c) 
     for (int a = 0; a < p; ++a)
         for (int b = 0; b < p; ++b)
             for (int i = 0; i < n; ++i) P[a * n + i] += Ginv[a * p + b] * B[b * n + i];
+    if constexpr (ERR) {
+        // errors_host on one cell: flux coefficients P u, divergence
+        // coefficients H^-1 W u (H of the error rule), then the integrands at
+        // the error rule's points of every sub-triangle
+        const ErrIn& E = *ei;
+        const int64_t gp = E.gpos[c];
+        double u[NM], coef[p], wu[q], pc[q];
+        for (int i = 0; i < n; ++i) {
+            int64_t idx;
+            double sgn = 1.0;
+            if (i < edof) {
+                const int r = i / ne, j = i % ne, ge = in.cell_edges[s0 + j];
+                idx = (int64_t)kk * E.edge_int[ge] + r;
+                if (r % 2 == 0 && in.edge_occ[ge] == 2 && in.nodes[s0 + j] == in.edge_nodes[2 * ge + 1]) sgn = -1.0;
+            } else {
+                idx = E.cbase + (int64_t)(i - edof) * E.cpad + gp;
+            }
+            u[i] = sgn * E.x[idx];
+        }
+#pragma unroll
+        for (int r = 0; r < q; ++r) pc[r] = E.x[E.cbase + (int64_t)(S::ni + r) * E.cpad + gp];
+#pragma unroll
+        for (int a = 0; a < p; ++a) {
+            double v = 0.0;
+            for (int i = 0; i < n; ++i) v += P[a * n + i] * u[i];
+            coef[a] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < q; ++r) {
+            double v = 0.0;
+            for (int i = 0; i < n; ++i) v += W[r * n + i] * u[i];
+            wu[r] = v;
+        }
+        double He[q * q] = {};
+        for (int k = 0; k < ne - 2; ++k) {
+            const double v0x = vx[tri[k][0]], v0y = vy[tri[k][0]], v1x = vx[tri[k][1]], v1y = vy[tri[k][1]],
+                         v2x = vx[tri[k][2]], v2y = vy[tri[k][2]];
+            const double J = fabs((v1x - v0x) * (v2y - v0y) - (v2x - v0x) * (v1y - v0y));
+            for (int i = 0; i < E.nt; ++i) {
+                const double x = v0x + (v1x - v0x) * E.ta[i] + (v2x - v0x) * E.tb[i];
+                const double y = v0y + (v1y - v0y) * E.ta[i] + (v2y - v0y) * E.tb[i];
+                const double xm = (x - xc) / h, ym = (y - yc) / h;
+                double mr[q];
+#pragma unroll
+                for (int r = 0; r < q; ++r) {
+                    int ea, eb;
+                    dmono(r, ea, eb);
+                    mr[r] = dpow(xm, ea) * dpow(ym, eb);
+                }
+#pragma unroll
+                for (int r = 0; r < q; ++r)
+#pragma unroll
+                    for (int t2 = 0; t2 < q; ++t2) He[r * q + t2] += mr[r] * mr[t2] * E.tw[i] * J;
+            }
+        }
+        double dcf[q];
+        {
+            // H / |E| is O(1) for the scaled monomials
+            const double sc0 = He[0] > 0 ? 1.0 / He[0] : 1.0;
+            double Hsc[q * q], Hi[q * q];
+#pragma unroll
+            for (int i = 0; i < q * q; ++i) Hsc[i] = He[i] * sc0;
+            if (!dinverse(Hsc, q, Hi)) return E_SINGULAR;
+#pragma unroll
+            for (int r = 0; r < q; ++r) {
+                double v = 0.0;
+#pragma unroll
+                for (int t2 = 0; t2 < q; ++t2) v += Hi[r * q + t2] * wu[t2] * sc0;
+                dcf[r] = v;
+            }
+        }
+        double ep = 0, eu = 0, epp = 0, eup = 0, ed = 0, ms = 0;
+        for (int k = 0; k < ne - 2; ++k) {
+            const double v0x = vx[tri[k][0]], v0y = vy[tri[k][0]], v1x = vx[tri[k][1]], v1y = vy[tri[k][1]],
+                         v2x = vx[tri[k][2]], v2y = vy[tri[k][2]];
+            const double J = fabs((v1x - v0x) * (v2y - v0y) - (v2x - v0x) * (v1y - v0y));
+            for (int i = 0; i < E.nt; ++i) {
+                const double x = v0x + (v1x - v0x) * E.ta[i] + (v2x - v0x) * E.tb[i];
+                const double y = v0y + (v1y - v0y) * E.ta[i] + (v2y - v0y) * E.tb[i];
+                const double w = E.tw[i] * J;
+                const double xm = (x - xc) / h, ym = (y - yc) / h;
+                double ph = 0, vxh = 0, vyh = 0, dv = 0;
+#pragma unroll
+                for (int r = 0; r < q; ++r) {
+                    int ea, eb;
+                    dmono(r, ea, eb);
+                    const double mv = dpow(xm, ea) * dpow(ym, eb);
+                    ph += pc[r] * mv;
+                    dv += dcf[r] * mv;
+                }
+#pragma unroll
+                for (int a = 0; a < p; ++a) {
+                    double bx, by;
+                    dhdiv<K>(a, xm, ym, h, bx, by);
+                    vxh += coef[a] * bx;
+                    vyh += coef[a] * by;
+                }
+                const DGeo g = dmap(in, vx, vy, ne, er, chord, x, y);
+                double pe, ux, uy, gg;
+                dexact(in.solution, g.X, g.Y, pe, ux, uy, gg);
+                const double uhx = g.j11 * ux - g.j01 * uy, uhy = -g.j10 * ux + g.j00 * uy;
+                const double dp = ph - pe, dx = vxh - uhx, dy = vyh - uhy;
+                ep += dp * dp * w;
+                eu += (dx * dx + dy * dy) * w;
+                epp += dp * dp * w * g.det;
+                const double fx = (g.j00 * dx + g.j01 * dy) / g.det, fy = (g.j10 * dx + g.j11 * dy) / g.det;
+                eup += (fx * fx + fy * fy) * w * g.det;
+                const double dd = dv / g.det - gg;
+                ed += dd * dd * w * g.det;
+                ms += ph * w * g.det;
+            }
+        }
+        double* eo = E.out + 6 * (int64_t)c;
+        eo[0] = ep;
+        eo[1] = eu;
+        eo[2] = epp;
+        eo[3] = eup;
+        eo[4] = ed;
+        eo[5] = ms;
+        return 0;
+    }
     // A = P^T G_T P + alpha (I - D P)^T (I - D P), lower triangle
     double T[p * NM] = {};
     for (int a = 0; a < p; ++a)
@@ -594,6 +739,44 @@ __global__ __launch_bounds__(64) void k_setup(SetupIn in, SetupOut o) {
     if (e && atomicCAS(&o.err[0], 0, e) == 0) atomicExch(&o.err[1], c);
 }
 
+template <int K>
+__global__ __launch_bounds__(64) void k_errors(SetupIn in, ErrIn ei, int* err) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= in.nc) return;
+    const int c = in.order[t];
+    const int e = setup_cell<K, true>(in, SetupOut{}, c, &ei);
+    if (e && atomicCAS(&err[0], 0, e) == 0) atomicExch(&err[1], c);
+}
+
+// Fixed-order sums of the six per-cell values: stage 1 sums 2048 cells per
+// block (8 per thread, then a shuffle tree and the four wavefronts in order),
+// stage 2 the block partials with one block.
+__device__ __forceinline__ void sum6_block(double (&v)[6], double* out) {
+    __shared__ double sh[6][4];
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] += __shfl_xor(v[k], o, 64);
+    const int w = (int)threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) out[threadIdx.x] = (sh[threadIdx.x][0] + sh[threadIdx.x][1]) + (sh[threadIdx.x][2] + sh[threadIdx.x][3]);
+}
+
+__global__ __launch_bounds__(256) void k_sum6(const double* in, int64_t n, double* out) {
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    const int64_t b0 = (int64_t)blockIdx.x * 2048;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int64_t i = b0 + threadIdx.x + 256 * u;
+        if (i < n)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] += in[6 * i + k];
+    }
+    sum6_block(v, out + 6 * (int64_t)blockIdx.x);
+}
+
 // Device buffers of one setup call, released on every path.
 struct Bufs {
     std::vector<void*> p;
@@ -622,6 +805,161 @@ void down(std::vector<T>& v, const T* d) {
 
 }  // namespace
 
+// Device copies of the mesh and numbering the error mode reads, kept per
+// solver after the first device error evaluation.
+struct ErrDev {
+    Bufs b;
+    SetupIn in{};
+    const int* edge_int = nullptr;
+    const int* gpos = nullptr;
+    double* cell6 = nullptr;
+    double* part = nullptr;
+    int* err = nullptr;
+    int device = -1;
+};
+
+void errdev_free(ErrDev* e) { delete e; }
+
+// Cells in processing order: grouped by valence so that a wavefront's
+// threads run the same loops.
+static std::vector<int> valence_order(const Mesh& m) {
+    std::vector<int> order(m.nc), cnt(65, 0), start(65, 0);
+    for (int c = 0; c < m.nc; ++c) ++cnt[std::min(m.npc(c), 64)];
+    for (int v = 1; v < 65; ++v) start[v] = start[v - 1] + cnt[v - 1];
+    for (int c = 0; c < m.nc; ++c) order[start[std::min(m.npc(c), 64)]++] = c;
+    return order;
+}
+
+// The setup's inputs that do not depend on a quadrature rule: mesh arrays on
+// the device, problem kind, arcs.
+static void setup_in_mesh(SetupIn& in, Bufs& b, const Mesh& m, const vemd_problem_desc& d) {
+    in.nc = m.nc;
+    in.map_kind = d.map_kind;
+    in.solution = d.solution;
+    in.bc = d.bc;
+    in.eps = d.eps;
+    in.arcs = d.map_kind == VEMD_MAP_ARCS && !m.arcs.empty();
+    // arc radius of every edge (the chords of the curved cells)
+    std::vector<double> er(in.arcs ? m.ne : 0, 0.0);
+    for (int e = 0; e < (int)er.size(); ++e) {
+        auto it = m.arcs.find(Mesh::key(m.edge_nodes[2 * e], m.edge_nodes[2 * e + 1]));
+        if (it != m.arcs.end()) er[e] = it->second;
+    }
+    in.xy = b.up(m.xy);
+    in.off = b.up(m.off);
+    in.nodes = b.up(m.nodes);
+    in.cell_edges = b.up(m.cell_edges);
+    in.edge_nodes = b.up(m.edge_nodes);
+    in.edge_occ = b.up(m.edge_occ);
+    in.cx = b.up(m.cx);
+    in.cy = b.up(m.cy);
+    in.diam = b.up(m.diam);
+    in.edge_r = b.up(er);
+    in.order = b.up(valence_order(m));
+}
+
+static void setup_rules(SetupIn& in, const QuadRules& qr) {
+    in.nt = (int)qr.tw.size();
+    in.ng = (int)qr.glx.size();
+    if (in.nt > MAXQ2 || in.ng > MAXQ1) throw Error(VEMD_ERR_ARG, "quadrature too large for the device setup");
+    for (int i = 0; i < in.nt; ++i) {
+        in.ta[i] = qr.ta[i];
+        in.tb[i] = qr.tb[i];
+        in.tw[i] = qr.tw[i];
+    }
+    for (int i = 0; i < in.ng; ++i) {
+        in.gx[i] = qr.glx[i];
+        in.gw[i] = qr.glw[i];
+    }
+}
+
+static void throw_cell_error(const int (&err)[2]) {
+    const std::string cell = std::to_string(err[1]);
+    switch (err[0]) {
+        case E_JAC: throw Error(VEMD_ERR_MESH, "non-positive map Jacobian in cell " + cell);
+        case E_EAR: throw Error(VEMD_ERR_MESH, "no valid ear in cell " + cell);
+        case E_EDGE: throw Error(VEMD_ERR_MESH, "degenerate edge in cell " + cell);
+        case E_SINGULAR: throw Error(VEMD_ERR_SINGULAR, "singular element matrix in cell " + cell);
+        case E_CONVEX: throw Error(VEMD_ERR_MESH, "curved boundary cell " + cell + " is not strictly convex");
+        case E_AREA: throw Error(VEMD_ERR_MESH, "cell " + cell + " has no area");
+        default:
+            throw Error(VEMD_ERR_ARG, "cell " + cell + " has more than " + std::to_string(NE) +
+                                          " edges (the device setup's limit)");
+    }
+}
+
+void errors_device(Solver& s, const double* x_dev, int nq, void* stream, double* out6) {
+    const hipStream_t st = (hipStream_t)stream;
+    const Mesh& m = *s.mesh;
+    const Setup& S = *s.setup;
+    const vemd_problem_desc& d = S.desc;
+    if (S.sh.k > 1) throw Error(VEMD_ERR_ARG, "device error functionals: k = 0 and k = 1");
+    if (d.funcs) throw Error(VEMD_ERR_ARG, "problem callbacks run on the host: use vemd_errors_full for them");
+    for (int c = 0; c < m.nc; ++c)
+        if (m.npc(c) > NE) throw Error(VEMD_ERR_ARG, "device error functionals: polygons of up to 10 edges");
+    if (!s.errdev || s.errdev->device != s.device) {
+        errdev_free(s.errdev);
+        s.errdev = nullptr;
+        std::unique_ptr<ErrDev> e(new ErrDev());
+        e->device = s.device;
+        setup_in_mesh(e->in, e->b, m, d);
+        setup_rules(e->in, quad_rules(d.nq2d, d.nq1d));
+        e->in.v_off = e->b.up(S.ofs.v_off);
+        e->edge_int = e->b.up(s.layout.edge_int);
+        e->gpos = e->b.up(s.layout.gpos_of_cell);
+        e->cell6 = e->b.alloc<double>(6 * (size_t)std::max(m.nc, 1));
+        e->part = e->b.alloc<double>(6 * (size_t)((m.nc + 2047) / 2048 + 1));
+        e->err = e->b.alloc<int>(2);
+        s.errdev = e.release();
+    }
+    ErrDev& E = *s.errdev;
+    ErrIn ei{};
+    ei.x = x_dev;
+    ei.edge_int = E.edge_int;
+    ei.gpos = E.gpos;
+    ei.cbase = s.layout.cbase;
+    ei.cpad = s.layout.cpad;
+    const QuadRules er = quad_rules(nq, 2);
+    ei.nt = (int)er.tw.size();
+    if (ei.nt > MAXQ2) throw Error(VEMD_ERR_ARG, "error quadrature too large for the device");
+    for (int i = 0; i < ei.nt; ++i) {
+        ei.ta[i] = er.ta[i];
+        ei.tb[i] = er.tb[i];
+        ei.tw[i] = er.tw[i];
+    }
+    ei.out = E.cell6;
+    HIPCHK(hipMemsetAsync(E.err, 0, 2 * sizeof(int), st));
+    if (m.nc > 0) {
+        const dim3 grid((unsigned)((m.nc + 63) / 64)), block(64);
+        if (S.sh.k == 0)
+            hipLaunchKernelGGL(k_errors<0>, grid, block, 0, st, E.in, ei, E.err);
+        else
+            hipLaunchKernelGGL(k_errors<1>, grid, block, 0, st, E.in, ei, E.err);
+        HIPCHK(hipGetLastError());
+    }
+    // fixed-order sums: cells -> blocks of 2048 -> ... -> one
+    int64_t n = m.nc;
+    const double* src = E.cell6;
+    double* dst = E.part;
+    double* spare = E.cell6;
+    do {
+        const int64_t nb = std::max<int64_t>(1, (n + 2047) / 2048);
+        hipLaunchKernelGGL(k_sum6, dim3((unsigned)nb), dim3(256), 0, st, src, n, dst);
+        HIPCHK(hipGetLastError());
+        n = nb;
+        src = dst;
+        std::swap(dst, spare);
+    } while (n > 1);
+    double t[6];
+    int err[2];
+    HIPCHK(hipMemcpyAsync(t, src, sizeof(t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(err, E.err, sizeof(err), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (err[0]) throw_cell_error(err);
+    for (int j = 0; j < 5; ++j) out6[j] = std::sqrt(t[j]);
+    out6[5] = t[5];
+}
+
 Setup* setup_build_device(const Mesh& m, const vemd_problem_desc& d, int device) {
     if (d.k > 1) throw Error(VEMD_ERR_ARG, "the device element setup supports k = 0 and k = 1");
     if (d.funcs)
@@ -635,53 +973,10 @@ Setup* setup_build_device(const Mesh& m, const vemd_problem_desc& d, int device)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(VEMD_ERR_DEVICE, "no HIP device");
     if (device < 0 || device >= ndev) throw Error(VEMD_ERR_ARG, "device out of range");
     HIPCHK(hipSetDevice(device));
-    const QuadRules qr = quad_rules(d.nq2d, d.nq1d);
     SetupIn in{};
-    in.nt = (int)qr.tw.size();
-    in.ng = (int)qr.glx.size();
-    if (in.nt > MAXQ2 || in.ng > MAXQ1) throw Error(VEMD_ERR_ARG, "quadrature too large for the device setup");
-    for (int i = 0; i < in.nt; ++i) {
-        in.ta[i] = qr.ta[i];
-        in.tb[i] = qr.tb[i];
-        in.tw[i] = qr.tw[i];
-    }
-    for (int i = 0; i < in.ng; ++i) {
-        in.gx[i] = qr.glx[i];
-        in.gw[i] = qr.glw[i];
-    }
-    in.nc = m.nc;
-    in.map_kind = d.map_kind;
-    in.solution = d.solution;
-    in.bc = d.bc;
-    in.eps = d.eps;
-    in.arcs = d.map_kind == VEMD_MAP_ARCS && !m.arcs.empty();
-    // arc radius of every edge (the chords of the curved cells)
-    std::vector<double> er(in.arcs ? m.ne : 0, 0.0);
-    for (int e = 0; e < (int)er.size(); ++e) {
-        auto it = m.arcs.find(Mesh::key(m.edge_nodes[2 * e], m.edge_nodes[2 * e + 1]));
-        if (it != m.arcs.end()) er[e] = it->second;
-    }
-    // cells grouped by valence so that a wavefront's threads run the same loops
-    std::vector<int> order(m.nc);
-    {
-        std::vector<int> cnt(65, 0);
-        for (int c = 0; c < m.nc; ++c) ++cnt[std::min(m.npc(c), 64)];
-        std::vector<int> start(65, 0);
-        for (int v = 1; v < 65; ++v) start[v] = start[v - 1] + cnt[v - 1];
-        for (int c = 0; c < m.nc; ++c) order[start[std::min(m.npc(c), 64)]++] = c;
-    }
+    setup_rules(in, quad_rules(d.nq2d, d.nq1d));
     Bufs b;
-    in.xy = b.up(m.xy);
-    in.off = b.up(m.off);
-    in.nodes = b.up(m.nodes);
-    in.cell_edges = b.up(m.cell_edges);
-    in.edge_nodes = b.up(m.edge_nodes);
-    in.edge_occ = b.up(m.edge_occ);
-    in.cx = b.up(m.cx);
-    in.cy = b.up(m.cy);
-    in.diam = b.up(m.diam);
-    in.edge_r = b.up(er);
-    in.order = b.up(order);
+    setup_in_mesh(in, b, m, d);
     in.a_off = b.up(S->ofs.a_off);
     in.w_off = b.up(S->ofs.w_off);
     in.v_off = b.up(S->ofs.v_off);
@@ -709,20 +1004,7 @@ Setup* setup_build_device(const Mesh& m, const vemd_problem_desc& d, int device)
     HIPCHK(hipDeviceSynchronize());
     int err[2] = {0, 0};
     HIPCHK(hipMemcpy(err, o.err, sizeof(err), hipMemcpyDeviceToHost));
-    if (err[0]) {
-        const std::string cell = std::to_string(err[1]);
-        switch (err[0]) {
-            case E_JAC: throw Error(VEMD_ERR_MESH, "non-positive map Jacobian in cell " + cell);
-            case E_EAR: throw Error(VEMD_ERR_MESH, "no valid ear in cell " + cell);
-            case E_EDGE: throw Error(VEMD_ERR_MESH, "degenerate edge in cell " + cell);
-            case E_SINGULAR: throw Error(VEMD_ERR_SINGULAR, "singular element matrix in cell " + cell);
-            case E_CONVEX: throw Error(VEMD_ERR_MESH, "curved boundary cell " + cell + " is not strictly convex");
-            case E_AREA: throw Error(VEMD_ERR_MESH, "cell " + cell + " has no area");
-            default:
-                throw Error(VEMD_ERR_ARG, "cell " + cell + " has more than " + std::to_string(NE) +
-                                              " edges (the device setup's limit)");
-        }
-    }
+    if (err[0]) throw_cell_error(err);
     down(S->A, o.A);
     down(S->W, o.W);
     down(S->L, o.L);

@@ -10,6 +10,7 @@
 namespace vemd {
 
 struct DeviceState;
+struct ErrDev;  // device copies for the error functionals (setup.hip)
 
 struct Solver {
     std::shared_ptr<Mesh> mesh;
@@ -27,6 +28,7 @@ struct Solver {
     double amg_lam_minv = 0.0;     // multiplier entry of M^-1 with the multigrid cycle (0: no multiplier)
     double amg_setup_s = 0.0;
     DeviceState* dev = nullptr;
+    ErrDev* errdev = nullptr;  // allocated by the first device error evaluation
     int device = -1;
     // per-instance results of the last solve (iterations, true relative residual)
     std::vector<int> inst_iterations;
@@ -62,5 +64,12 @@ void dev_solve_multi(Solver& s, const vemd_solve_opts& o, int ncol, const double
 void dev_apply_multi(Solver& s, const void* x, void* y, void* y1, void* stream);
 void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, const double* dldx_ref, double* grad,
                  vemd_solve_info* info);
+// The six error functionals of errors_host for an internal-layout solution in
+// device memory, evaluated on the device (after the caller's stream's work).
+void dev_errors(Solver& s, const void* x_dev, int nq, void* stream, double* out6);
+// setup.hip: the evaluation on a stream of the solver's device, and the release
+// of its device copies.
+void errors_device(Solver& s, const double* x_dev, int nq, void* stream, double* out6);
+void errdev_free(ErrDev* e);
 
 }  // namespace vemd

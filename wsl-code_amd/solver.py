@@ -254,6 +254,15 @@ class Solver:
                 self._pde.raise_pending()
         return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3], div_phys=out[4], mass=out[5])
 
+    def errors_device(self, x_dev, nq=9, stream=0):
+        """The six error functionals of errors() for an internal-layout FP64
+        solution in device memory (a torch tensor or a device pointer),
+        evaluated on the GPU after the work queued on `stream`."""
+        ptr = x_dev.data_ptr() if hasattr(x_dev, "data_ptr") else int(x_dev)
+        out = np.zeros(6)
+        check(lib().vemd_errors_device(self.h, c_vp(ptr), nq, _d(out), c_vp(stream or None)))
+        return dict(p_comp=out[0], u_comp=out[1], p_phys=out[2], u_phys=out[3], div_phys=out[4], mass=out[5])
+
     def element(self, cell):
         """FP64 element operator of one cell: dict with A (n x n), W (q x n), L, f, sign, gdof."""
         q = (self.k + 1) * (self.k + 2) // 2
