@@ -48,11 +48,17 @@ void Solver::to_reference(const double* in, double* ref, bool unscale_rows) cons
 }
 
 void Solver::refresh_theta_dependent() {
+    auto t0 = std::chrono::steady_clock::now();
     pack_scale(*setup, layout, scale);
     build_minv(*mesh, *setup, layout, minv);
     build_row_scale(*mesh, *setup, layout, rowscale);
     build_rhs_ref(*mesh, *setup, rhs_ref);
+    auto t1 = std::chrono::steady_clock::now();
     build_amg();
+    if (std::getenv("VEMD_VERBOSE"))
+        std::fprintf(stderr, "[vemd] theta-dependent host data %.3fs (scales, diagonal, rhs %.3fs)\n",
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
+                     std::chrono::duration<double>(t1 - t0).count());
 }
 
 void Solver::build_amg() {

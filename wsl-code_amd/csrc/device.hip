@@ -1152,7 +1152,11 @@ void dev_upload(Solver& s, int device) {
 void dev_refresh(Solver& s) {
     if (!s.dev) return;
     HIPCHK(hipSetDevice(s.dev->device));
+    auto t0 = std::chrono::steady_clock::now();
     upload_theta(s);
+    if (std::getenv("VEMD_VERBOSE"))
+        std::fprintf(stderr, "[vemd] theta-dependent device data %.3fs\n",
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
 }
 
 
