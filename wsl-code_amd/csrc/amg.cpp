@@ -95,9 +95,10 @@ Csr csr_mult(const Csr& A, const Csr& B) {
     return C;
 }
 
-Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::vector<double>& minv) {
+SchurPlan schur_plan(const Mesh& m, const Setup& S, const Layout& L) {
     const int q = L.q;
     const int64_t cpad = L.cpad, n = (int64_t)q * cpad;
+    SchurPlan P;
     // neighbour cells (self included) of every store position, sorted by position
     std::vector<std::vector<int>> nb(cpad);
     parallel_for(cpad, [&](int64_t b, int64_t e) {
@@ -117,41 +118,37 @@ Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::ve
             v.erase(std::unique(v.begin(), v.end()), v.end());
         }
     });
-    Csr A;
+    Csr& A = P.pattern;
     A.nrows = A.ncols = n;
     A.rp.assign(n + 1, 0);
     for (int64_t g = 0; g < cpad; ++g)
         for (int r = 0; r < q; ++r) A.rp[g * q + r + 1] = L.cell_of_gpos[g] < 0 ? 1 : (int64_t)q * nb[g].size();
     for (int64_t i = 0; i < n; ++i) A.rp[i + 1] += A.rp[i];
     A.ci.resize(A.rp[n]);
-    A.v.assign(A.rp[n], 0.0);
+    const int64_t nv = S.ofs.v_off[m.nc];
+    P.self_slot.assign(m.nc, 0);
+    P.nbk.assign(cpad, 1);
+    P.other_slot.assign(nv, -1);
+    P.other_i.assign(nv, -1);
+    P.tflux.assign(nv, -1);
     parallel_for(cpad, [&](int64_t b, int64_t e) {
-        std::vector<double> blk;
         for (int64_t g = b; g < e; ++g) {
             int c = L.cell_of_gpos[g];
             if (c < 0) {
-                for (int r = 0; r < q; ++r) {
-                    int64_t o = A.rp[g * q + r];
-                    A.ci[o] = (int)(g * q + r);
-                    A.v[o] = 1.0;
-                }
+                for (int r = 0; r < q; ++r) A.ci[A.rp[g * q + r]] = (int)(g * q + r);
                 continue;
             }
             const std::vector<int>& v = nb[g];
             const int nbk = (int)v.size();
-            blk.assign((size_t)nbk * q * q, 0.0);  // block b: q x q, row-major
+            P.nbk[g] = nbk;
             auto slot = [&](int gp) { return (int)(std::lower_bound(v.begin(), v.end(), gp) - v.begin()); };
+            P.self_slot[c] = slot((int)g);
             const int n1 = S.sh.nflux(m.npc(c));
             const int64_t v0 = S.ofs.v_off[c];
-            const double* Wc = &S.W[S.ofs.w_off[c]];
             for (int i = 0; i < n1; ++i) {
                 if (S.isb[v0 + i]) continue;
                 const int64_t gd = S.gdof[v0 + i];
-                const double dinv = minv[L.int_of_ref[gd]];
-                // this cell
-                double* bs = &blk[(size_t)slot((int)g) * q * q];
-                for (int r = 0; r < q; ++r)
-                    for (int s = 0; s < q; ++s) bs[r * q + s] += Wc[r * n1 + i] * Wc[s * n1 + i] * dinv;
+                P.tflux[v0 + i] = L.int_of_ref[gd];
                 // the other cell of an edge dof
                 if (i >= (S.sh.k + 1) * m.npc(c)) continue;
                 const int ed = m.cell_edges[m.off[c] + i % m.npc(c)];
@@ -166,50 +163,122 @@ Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::ve
                         break;
                     }
                 if (i2 < 0) throw Error(VEMD_ERR_INTERNAL, "edge dof not found in the neighbouring cell");
+                P.other_slot[v0 + i] = slot(L.gpos_of_cell[o]);
+                P.other_i[v0 + i] = i2;
+            }
+            for (int r = 0; r < q; ++r) {
+                int64_t o = A.rp[g * q + r];
+                for (int t = 0; t < nbk; ++t)
+                    for (int s = 0; s < q; ++s) A.ci[o++] = (int)((int64_t)v[t] * q + s);
+            }
+        }
+    });
+    return P;
+}
+
+Csr schur_values(const SchurPlan& P, const Mesh& m, const Setup& S, const Layout& L, const std::vector<double>& minv) {
+    const int q = L.q;
+    const int64_t cpad = L.cpad;
+    Csr A = P.pattern;
+    A.v.assign(A.ci.size(), 0.0);
+    parallel_for(cpad, [&](int64_t b, int64_t e) {
+        std::vector<double> blk;
+        for (int64_t g = b; g < e; ++g) {
+            int c = L.cell_of_gpos[g];
+            if (c < 0) {
+                for (int r = 0; r < q; ++r) A.v[A.rp[g * q + r]] = 1.0;
+                continue;
+            }
+            const int nbk = P.nbk[g];
+            blk.assign((size_t)nbk * q * q, 0.0);  // block b: q x q, row-major
+            const int n1 = S.sh.nflux(m.npc(c));
+            const int64_t v0 = S.ofs.v_off[c];
+            const double* Wc = &S.W[S.ofs.w_off[c]];
+            double* bs = &blk[(size_t)P.self_slot[c] * q * q];
+            for (int i = 0; i < n1; ++i) {
+                if (P.tflux[v0 + i] < 0) continue;
+                const double dinv = minv[P.tflux[v0 + i]];
+                // this cell
+                for (int r = 0; r < q; ++r)
+                    for (int s = 0; s < q; ++s) bs[r * q + s] += Wc[r * n1 + i] * Wc[s * n1 + i] * dinv;
+                // the other cell of an edge dof
+                if (P.other_slot[v0 + i] < 0) continue;
+                const int ed = m.cell_edges[m.off[c] + i % m.npc(c)];
+                const int o = m.edge_cells[2 * ed] == c ? m.edge_cells[2 * ed + 1] : m.edge_cells[2 * ed];
+                const int n2 = S.sh.nflux(m.npc(o));
+                const int i2 = P.other_i[v0 + i];
                 const double* Wo = &S.W[S.ofs.w_off[o]];
-                double* bo = &blk[(size_t)slot(L.gpos_of_cell[o]) * q * q];
+                double* bo = &blk[(size_t)P.other_slot[v0 + i] * q * q];
                 for (int r = 0; r < q; ++r)
                     for (int s = 0; s < q; ++s) bo[r * q + s] += Wc[r * n1 + i] * Wo[s * n2 + i2] * dinv;
             }
             for (int r = 0; r < q; ++r) {
                 int64_t o = A.rp[g * q + r];
                 for (int t = 0; t < nbk; ++t)
-                    for (int s = 0; s < q; ++s) {
-                        A.ci[o] = (int)((int64_t)v[t] * q + s);
-                        A.v[o] = blk[(size_t)t * q * q + r * q + s];
-                        ++o;
-                    }
+                    for (int s = 0; s < q; ++s) A.v[o++] = blk[(size_t)t * q * q + r * q + s];
             }
         }
     });
     return A;
 }
 
-// Upper estimate of the spectral radius of D^-1 A by power iteration.
-static double spectral_radius(const Csr& A, const std::vector<double>& dinv) {
+Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::vector<double>& minv) {
+    return schur_values(schur_plan(m, S, L), m, S, L, minv);
+}
+
+// Upper estimate of the spectral radius of D^-1 A by power iteration: iters
+// steps from *ev when it holds a previous estimate's vector of the right
+// length (a rebuild of the values after a theta change), else 25 from a
+// fixed pseudo-random vector; the last iterate is left in *ev.  (Fewer steps
+// from the previous vector measured up to 10% low after a change to an
+// independent theta field, beyond the 1.1 margin.)  The norms are sums over
+// 256 fixed blocks added in order, whatever the thread count.
+static double spectral_radius(const Csr& A, const std::vector<double>& dinv, std::vector<double>* ev, int iters) {
     const int64_t n = A.nrows;
     std::vector<double> x(n), y(n);
-    uint64_t s = 0x9e3779b97f4a7c15ull;
-    for (int64_t i = 0; i < n; ++i) {
-        s ^= s << 13;
-        s ^= s >> 7;
-        s ^= s << 17;
-        x[i] = 0.5 + (double)(s >> 11) * (1.0 / 9007199254740992.0);
-    }
-    double lam = 0.0;
-    for (int it = 0; it < 25; ++it) {
-        csr_spmv(A, x.data(), y.data());
-        double ny = 0.0, nx = 0.0;
+    if (ev && (int64_t)ev->size() == n) {
+        x = *ev;
+    } else {
+        iters = 25;
+        uint64_t s = 0x9e3779b97f4a7c15ull;
         for (int64_t i = 0; i < n; ++i) {
-            y[i] *= dinv[i];
-            ny += y[i] * y[i];
-            nx += x[i] * x[i];
+            s ^= s << 13;
+            s ^= s >> 7;
+            s ^= s << 17;
+            x[i] = 0.5 + (double)(s >> 11) * (1.0 / 9007199254740992.0);
+        }
+    }
+    constexpr int NB = 256;
+    std::vector<double> pny(NB), pnx(NB);
+    const int64_t bl = (n + NB - 1) / NB;
+    double lam = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        csr_spmv(A, x.data(), y.data());
+        parallel_for(NB, [&](int64_t b, int64_t e) {
+            for (int64_t k = b; k < e; ++k) {
+                double sy = 0.0, sx = 0.0;
+                for (int64_t i = k * bl; i < std::min(n, (k + 1) * bl); ++i) {
+                    y[i] *= dinv[i];
+                    sy += y[i] * y[i];
+                    sx += x[i] * x[i];
+                }
+                pny[k] = sy;
+                pnx[k] = sx;
+            }
+        });
+        double ny = 0.0, nx = 0.0;
+        for (int k = 0; k < NB; ++k) {
+            ny += pny[k];
+            nx += pnx[k];
         }
         if (!(ny > 0.0)) break;
         lam = std::sqrt(ny / nx);
-        double sc = 1.0 / std::sqrt(ny);
-        for (int64_t i = 0; i < n; ++i) x[i] = y[i] * sc;
+        const double sc = 1.0 / std::sqrt(ny);
+        parallel_for(n, [&](int64_t b, int64_t e) {
+            for (int64_t i = b; i < e; ++i) x[i] = y[i] * sc;
+        });
     }
+    if (ev) ev->swap(x);
     return 1.1 * lam;
 }
 
@@ -296,7 +365,8 @@ static void dense_inverse(std::vector<double>& a, int64_t n) {
 }
 
 Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const std::vector<double>& b00,
-              const std::vector<int>& node_order0, const AmgOptions& o, const std::vector<double>* l0, double rho) {
+              const std::vector<int>& node_order0, const AmgOptions& o, const std::vector<double>* l0, double rho,
+              const Amg* reuse) {
     Amg H;
     H.opt = o;
     std::vector<double> lvec;
@@ -316,11 +386,19 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         for (int64_t i = 0; i < n; ++i)
             for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
                 if (A.ci[p] == i && A.v[p] != 0.0) L.dinv[i] = 1.0 / A.v[p];
-        L.lmax = spectral_radius(A, L.dinv);
+        // reuse: the previous hierarchy's aggregates (its tentative
+        // prolongators) and depth; only the values are rebuilt
+        const AmgLevel* old = reuse && lev < (int)reuse->lv.size() ? &reuse->lv[lev] : nullptr;
+        if (old) L.ev = old->ev;
+        L.lmax = spectral_radius(A, L.dinv, &L.ev, 25);
         bool last = n <= o.coarse_size || lev + 1 >= o.max_levels;
         int na = 0;
         std::vector<int> agg;
-        if (!last) {
+        if (reuse) {
+            if (!old || old->A.nrows != n) throw Error(VEMD_ERR_INTERNAL, "multigrid reuse: level sizes differ");
+            last = lev + 1 == (int)reuse->lv.size();
+        }
+        if (!last && !old) {
             // node-level strength matrix from the near-null vector
             std::vector<std::vector<int>> rows_of(nn);
             for (int64_t r = 0; r < n; ++r)
@@ -366,21 +444,27 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         }
         // tentative prolongator: the near-null vector restricted to each
         // aggregate, normalised; the norms are the coarse near-null vector
-        std::vector<double> bc(na, 0.0);
-        for (int64_t r = 0; r < n; ++r)
-            if (nor[r] >= 0 && agg[nor[r]] >= 0) bc[agg[nor[r]]] += b0[r] * b0[r];
-        for (double& v : bc) v = std::sqrt(v);
+        std::vector<double> bc;
         Csr Pt;
-        Pt.nrows = n;
-        Pt.ncols = na;
-        Pt.rp.assign(n + 1, 0);
-        for (int64_t r = 0; r < n; ++r) {
-            bool on = nor[r] >= 0 && agg[nor[r]] >= 0 && b0[r] != 0.0;
-            Pt.rp[r + 1] = Pt.rp[r] + (on ? 1 : 0);
-            if (on) {
-                int a = agg[nor[r]];
-                Pt.ci.push_back(a);
-                Pt.v.push_back(b0[r] / bc[a]);
+        if (old) {
+            Pt = old->Pt;
+            na = (int)Pt.ncols;
+        } else {
+            bc.assign(na, 0.0);
+            for (int64_t r = 0; r < n; ++r)
+                if (nor[r] >= 0 && agg[nor[r]] >= 0) bc[agg[nor[r]]] += b0[r] * b0[r];
+            for (double& v : bc) v = std::sqrt(v);
+            Pt.nrows = n;
+            Pt.ncols = na;
+            Pt.rp.assign(n + 1, 0);
+            for (int64_t r = 0; r < n; ++r) {
+                bool on = nor[r] >= 0 && agg[nor[r]] >= 0 && b0[r] != 0.0;
+                Pt.rp[r + 1] = Pt.rp[r] + (on ? 1 : 0);
+                if (on) {
+                    int a = agg[nor[r]];
+                    Pt.ci.push_back(a);
+                    Pt.v.push_back(b0[r] / bc[a]);
+                }
             }
         }
         // smoothed prolongator P = (I - omega D^-1 A) Pt
@@ -417,8 +501,10 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         L.P = std::move(P);
         L.R = std::move(R);
         L.M = std::move(M);
+        L.Pt = std::move(Pt);
         H.lv.push_back(std::move(L));
         A = std::move(Ac);
+        if (old) continue;
         // the next level: one row per aggregate, ordered as created (spatially)
         nn = na;
         nor.resize(na);

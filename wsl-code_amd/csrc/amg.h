@@ -35,6 +35,8 @@ struct AmgLevel {
     double lmax = 0.0;         // upper estimate of the spectral radius of D^-1 A
     Csr P, R;                  // prolongation from the next coarser level and R = P^T (not on the coarsest)
     Csr M;                     // R A: the next level's right-hand side R (b - A x) = R b - M x (not on the coarsest)
+    Csr Pt;                    // tentative prolongator (the aggregates), kept for a rebuild of the values
+    std::vector<double> ev;    // last iterate of the spectral-radius estimate (start of the next)
 };
 
 struct Amg {
@@ -52,6 +54,19 @@ struct Amg {
 // order, exp(-theta) included); essential boundary flux dofs are left out.
 Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::vector<double>& dflux);
 
+// The same in two parts: the theta-independent sparsity and per-dof slots
+// (built once), then the values for a flux diagonal.
+struct SchurPlan {
+    Csr pattern;                      // structure of S (values empty)
+    std::vector<int> self_slot;       // per cell: its own block among its row's blocks
+    std::vector<int> nbk;             // per store position: blocks in its rows
+    std::vector<int> other_slot;      // per local flux dof: the other cell's block (-1: none)
+    std::vector<int> other_i;         // per local flux dof: its index in the other cell
+    std::vector<int64_t> tflux;       // per local flux dof: internal slot (-1: essential dof)
+};
+SchurPlan schur_plan(const Mesh& m, const Setup& S, const Layout& L);
+Csr schur_values(const SchurPlan& P, const Mesh& m, const Setup& S, const Layout& L, const std::vector<double>& dflux);
+
 // Builds the hierarchy.  Rows are grouped into nodes (node_of_row, -1 = row
 // left to the smoother), b0 is the near-null vector (one value per row) and
 // node_order the order in which aggregates are grown.  With l0 (one value per
@@ -59,9 +74,13 @@ Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::ve
 // l_c l_c^T, l_c = R ... R l0 (the restrictions of the levels), so that the
 // V-cycle is symmetric positive definite and approximates (A + rho l0 l0^T)^-1
 // (a null vector of A with a component along l0 is moved off zero).
+// With reuse, the aggregates and depth of that hierarchy (built on an
+// operator of the same sparsity) are kept and only the values are rebuilt:
+// the smoothed prolongators, Galerkin operators, spectral bounds and the
+// coarsest inverse (node_of_row, b0 and node_order are then unused).
 Amg amg_build(Csr A0, const std::vector<int>& node_of_row, int nnodes, const std::vector<double>& b0,
               const std::vector<int>& node_order, const AmgOptions& o, const std::vector<double>* l0 = nullptr,
-              double rho = 0.0);
+              double rho = 0.0, const Amg* reuse = nullptr);
 
 // Helpers exposed for tests: y = A x, C = A B, transpose.
 void csr_spmv(const Csr& A, const double* x, double* y);

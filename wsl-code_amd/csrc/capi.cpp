@@ -49,10 +49,12 @@ void Solver::to_reference(const double* in, double* ref, bool unscale_rows) cons
 
 void Solver::refresh_theta_dependent() {
     auto t0 = std::chrono::steady_clock::now();
+    if (!dplan) dplan.reset(new DiagPlan(diag_plan(*mesh, *setup, layout)));
     pack_scale(*setup, layout, scale);
-    build_minv(*mesh, *setup, layout, minv);
+    build_minv(*mesh, *setup, layout, *dplan, minv);
     build_row_scale(*mesh, *setup, layout, rowscale);
-    build_rhs_ref(*mesh, *setup, rhs_ref);
+    // without essential dofs the right-hand side has no theta-dependent lift
+    if (rhs_ref.empty() || dplan->essential) build_rhs_ref(*mesh, *setup, rhs_ref);
     auto t1 = std::chrono::steady_clock::now();
     build_amg();
     if (std::getenv("VEMD_VERBOSE"))
@@ -62,7 +64,13 @@ void Solver::refresh_theta_dependent() {
 }
 
 void Solver::build_amg() {
-    amg.reset();
+    // a rebuild after a theta change keeps the aggregates (and with them every
+    // sparsity pattern of the hierarchy) and recomputes the values only;
+    // VEMD_MG_REAGGREGATE=1 builds it afresh
+    std::unique_ptr<Amg> prev = std::move(amg);
+    const char* ra = std::getenv("VEMD_MG_REAGGREGATE");
+    if (ra && ra[0] == '1') prev.reset();
+    amg_values_only = false;
     amg_lam_minv = 0.0;
     auto t0 = std::chrono::steady_clock::now();
     // tuning overrides: smoothing degrees, Chebyshev interval ratio, strength threshold
@@ -71,7 +79,8 @@ void Solver::build_amg() {
     if (const char* e = std::getenv("VEMD_MG_RATIO")) amg_opt.cheb_ratio = std::max(2.0, std::atof(e));
     if (const char* e = std::getenv("VEMD_MG_STRENGTH")) amg_opt.strength = std::max(0.0, std::atof(e));
     const Layout& L = layout;
-    Csr S0 = schur_pressure(*mesh, *setup, L, minv);
+    if (!splan) splan.reset(new SchurPlan(schur_plan(*mesh, *setup, L)));
+    Csr S0 = schur_values(*splan, *mesh, *setup, L, minv);
     const int64_t n = S0.nrows;
     std::vector<int> node(n, -1);
     std::vector<double> b0(n, 0.0);
@@ -116,8 +125,9 @@ void Solver::build_amg() {
         amg_lam_minv = 1.0 / rho;
     }
     amg.reset(new Amg(amg_build(std::move(S0), node, (int)L.cpad, b0, L.spatial_gpos, amg_opt,
-                                l0.empty() ? nullptr : &l0, rho)));
+                                l0.empty() ? nullptr : &l0, rho, prev.get())));
     amg->bs0 = L.q;
+    amg_values_only = prev != nullptr;
     amg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (std::getenv("VEMD_VERBOSE")) {
         std::fprintf(stderr, "[vemd] multigrid setup %.2fs:", amg_setup_s);

@@ -1045,17 +1045,24 @@ static void upload_theta(Solver& s) {
     std::vector<double> bi(s.layout.nvec, 0.0);
     s.to_internal(s.rhs_ref.data(), bi.data(), true);
     HIPCHK(hipMemcpy(d.bprob, bi.data(), bi.size() * sizeof(double), hipMemcpyHostToDevice));
-    // the hierarchy depends on theta through the flux diagonal
-    delete d.amg64;
-    delete d.amg32;
-    d.amg64 = nullptr;
-    d.amg32 = nullptr;
+    // the hierarchy depends on theta through the flux diagonal; with its
+    // aggregates kept (the host rebuilt the values only) the uploaded
+    // hierarchies are refilled in place, else uploaded afresh
     destroy_graph(d.k64);
     destroy_graph(d.k32);
     multi_invalidate(d);
-    if (s.amg) {
-        d.amg64 = amg_upload<double>(*s.amg);
-        d.amg32 = amg_upload<float>(*s.amg);
+    const char* rf = std::getenv("VEMD_MG_REFILL");  // 0: always upload afresh
+    const bool refilled = !(rf && rf[0] == '0') && s.amg && s.amg_values_only && d.amg64 && d.amg32 &&
+                          amg_refresh(*d.amg64, *s.amg) && amg_refresh(*d.amg32, *s.amg);
+    if (!refilled) {
+        delete d.amg64;
+        delete d.amg32;
+        d.amg64 = nullptr;
+        d.amg32 = nullptr;
+        if (s.amg) {
+            d.amg64 = amg_upload<double>(*s.amg);
+            d.amg32 = amg_upload<float>(*s.amg);
+        }
     }
 }
 

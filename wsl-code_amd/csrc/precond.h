@@ -76,8 +76,18 @@ struct BcsrDev {
     float hscale = 1.0f;
 };
 
+// Where the values of a host CSR matrix went in a device matrix: entry p of
+// the CSR arrays sits at index at[p] of the device value array (of n values;
+// at empty: the CSR order itself).  Kept from the upload so that new values
+// of the same sparsity refill the device arrays without rebuilding them.
+struct ValMap {
+    std::vector<uint32_t> at;
+    size_t n = 0;
+};
+
 template <typename T>
 struct AmgLevelDev {
+    ValMap vA, vP, vR, vM, vRb;  // value placement of A, P, R, M, Rb
     int64_t n = 0;
     Sell<T> A, P;             // P: prolongation from the next level
     CsrDev<T> R;              // R = P^T
@@ -133,6 +143,14 @@ struct KOut {
     double *q1, *q2;
     int init;
 };
+
+// Refills the values of an uploaded hierarchy from a host hierarchy of the
+// same sparsity (the aggregates kept by amg_build's reuse): operators,
+// transfers, diagonals, Chebyshev intervals and the coarsest inverse.
+// Returns false, changing nothing that a fresh upload would not replace,
+// when the two do not match in shape or storage format.
+template <typename T>
+bool amg_refresh(AmgDev<T>& D, const Amg& H);
 
 // Enqueues one V-cycle on the stream: x = B b with b = lv[0].b, x = lv[0].x.
 // With done non-null every kernel returns at once while *done is set (work

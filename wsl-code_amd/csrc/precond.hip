@@ -548,7 +548,7 @@ template <typename T>
 // dsq: store D^-1/2 A D^-1/2 in half precision; half: store A's values
 // divided by their largest magnitude in half precision (hscale keeps it).
 static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<double>* dsq = nullptr,
-                       bool half = false) {
+                       bool half = false, ValMap* vm = nullptr) {
     Sell<T> s;
     s.bs = bs;
     s.nrows = A.nrows / bs;
@@ -583,6 +583,11 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<dou
     const double hinv = half && amax > 0.0 ? 1.0 / amax : 1.0;
     std::vector<T> val(h16 ? 0 : off[s.nslices] * bs * bs, T(0));
     std::vector<_Float16> hval(h16 ? off[s.nslices] * bs * bs : 0, (_Float16)0.0f);
+    if (vm) {
+        vm->n = (size_t)off[s.nslices] * bs * bs;
+        vm->at.assign(A.nnz(), 0);
+        if (vm->n > 0xffffffffull) throw Error(VEMD_ERR_MESH, "multigrid level too large for its value map");
+    }
     for (int64_t br = 0; br < s.nrows; ++br) {
         const int64_t sl = br >> 6, l = br & 63;
         const std::vector<int>& v = bc[br];
@@ -591,6 +596,7 @@ static Sell<T> to_sell(AmgDev<T>& D, const Csr& A, int bs, const std::vector<dou
             for (int64_t p = A.rp[br * bs + k]; p < A.rp[br * bs + k + 1]; ++p) {
                 const int64_t j = std::lower_bound(v.begin(), v.end(), A.ci[p] / bs) - v.begin();
                 const int64_t at = (off[sl] + j * 64) * bs * bs + (k * bs + A.ci[p] % bs) * 64 + l;
+                if (vm) vm->at[p] = (uint32_t)at;
                 if (dsq)
                     hval[at] = (_Float16)(A.v[p] / ((*dsq)[br * bs + k] * (*dsq)[A.ci[p]]));
                 else if (half)
@@ -635,7 +641,8 @@ static CsrDev<T> to_csr(AmgDev<T>& D, const Csr& A, bool half = false) {
 // csc[c] first; half: values divided by their largest magnitude in half
 // precision.
 template <typename T>
-static BcsrDev<T> to_bcsr(AmgDev<T>& D, const Csr& A, int bs, const std::vector<double>* csc, bool half) {
+static BcsrDev<T> to_bcsr(AmgDev<T>& D, const Csr& A, int bs, const std::vector<double>* csc, bool half,
+                          ValMap* vm = nullptr) {
     BcsrDev<T> m;
     m.nrows = A.nrows;
     m.bs = bs;
@@ -644,6 +651,7 @@ static BcsrDev<T> to_bcsr(AmgDev<T>& D, const Csr& A, int bs, const std::vector<
     std::vector<double> v;
     bc.reserve(A.nnz() / bs + A.nrows);
     v.reserve(A.nnz() + A.nrows * bs);
+    if (vm) vm->at.assign(A.nnz(), 0);
     for (int64_t i = 0; i < A.nrows; ++i) {
         int last = -1;
         for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
@@ -654,8 +662,13 @@ static BcsrDev<T> to_bcsr(AmgDev<T>& D, const Csr& A, int bs, const std::vector<
                 last = b;
             }
             v[v.size() - bs + A.ci[p] % bs] = csc ? A.v[p] / (*csc)[A.ci[p]] : A.v[p];
+            if (vm) vm->at[p] = (uint32_t)(v.size() - bs + A.ci[p] % bs);
         }
         rp[i + 1] = (int64_t)bc.size();
+    }
+    if (vm) {
+        vm->n = v.size();
+        if (vm->n > 0xffffffffull) throw Error(VEMD_ERR_MESH, "multigrid level too large for its value map");
     }
     m.rp = amg_copy(D, rp);
     m.bc = amg_copy(D, bc);
@@ -718,7 +731,7 @@ AmgDev<T>* amg_upload(const Amg& H) {
                 if (!(h.dinv[i] > 0.0) || !std::isfinite(dsq[i])) half = false;
             }
             if (half) {
-                d.A = to_sell(*D, h.A, bs, &dsq);
+                d.A = to_sell(*D, h.A, bs, &dsq, false, &d.vA);
                 Csr P = h.P, R = h.R;  // P <- D^1/2 P, R <- R D^1/2
                 for (int64_t i = 0; i < P.nrows; ++i)
                     for (int64_t p = P.rp[i]; p < P.rp[i + 1]; ++p) P.v[p] *= dsq[i];
@@ -727,13 +740,14 @@ AmgDev<T>* amg_upload(const Amg& H) {
                 // (VEMD_MG_HALF_TRANSFER=0 keeps them in FP32)
                 const char* ht = std::getenv("VEMD_MG_HALF_TRANSFER");
                 const bool htr = !(ht && ht[0] == '0') && half_safe({&P, &R});
-                d.P = to_sell(*D, P, 1, nullptr, htr);
+                d.P = to_sell(*D, P, 1, nullptr, htr, &d.vP);
                 d.R = to_csr(*D, R, htr);
+                d.vR.n = R.v.size();
                 if (fused_restriction() && rows_nonempty(h.M) && rows_nonempty(h.R)) {
                     // M = R D^1/2 (D^-1/2 A D^-1/2) = (R A) D^-1/2 on the scaled
                     // iterate; the plain R on the unscaled right-hand side
-                    d.M = to_bcsr(*D, h.M, bs, &dsq, htr && half_safe({&h.M}));
-                    d.Rb = to_bcsr(*D, h.R, bs, nullptr, htr && half_safe({&h.R}));
+                    d.M = to_bcsr(*D, h.M, bs, &dsq, htr && half_safe({&h.M}), &d.vM);
+                    d.Rb = to_bcsr(*D, h.R, bs, nullptr, htr && half_safe({&h.R}), &d.vRb);
                 }
                 if (std::getenv("VEMD_VERBOSE"))
                     std::fprintf(stderr, "[vemd] finest-level transfers of the FP32 cycle in %s\n",
@@ -743,13 +757,14 @@ AmgDev<T>* amg_upload(const Amg& H) {
                 d.sc = amg_copy(*D, sc);
                 d.dinv = nullptr;
             } else {
-                d.A = to_sell(*D, h.A, bs);
+                d.A = to_sell(*D, h.A, bs, nullptr, false, &d.vA);
                 if (l + 1 < H.lv.size()) {
-                    d.P = to_sell(*D, h.P, 1);
+                    d.P = to_sell(*D, h.P, 1, nullptr, false, &d.vP);
                     d.R = to_csr(*D, h.R);
+                    d.vR.n = h.R.v.size();
                     if (fused_restriction() && rows_nonempty(h.M) && rows_nonempty(h.R)) {
-                        d.M = to_bcsr(*D, h.M, bs, nullptr, false);
-                        d.Rb = to_bcsr(*D, h.R, bs, nullptr, false);
+                        d.M = to_bcsr(*D, h.M, bs, nullptr, false, &d.vM);
+                        d.Rb = to_bcsr(*D, h.R, bs, nullptr, false, &d.vRb);
                     }
                 }
                 std::vector<T> di(h.dinv.begin(), h.dinv.end());
@@ -775,6 +790,130 @@ AmgDev<T>* amg_upload(const Amg& H) {
         throw;
     }
     return D;
+}
+
+// New values of one device matrix: vals[p] for CSR entry p goes to index
+// vm.at[p] (vm.at empty: p) of the device array dev, in T or, with h16, in
+// half precision -- divided by the largest magnitude when normalise (hscale
+// receives it), the conversions of to_sell / to_csr / to_bcsr.
+template <typename T>
+static void refill(const ValMap& vm, const std::vector<double>& vals, bool h16, bool normalise, const void* dev,
+                   float* hscale) {
+    double amax = 0.0;
+    if (normalise)
+        for (double v : vals) amax = std::max(amax, std::fabs(v));
+    const double hinv = normalise && amax > 0.0 ? 1.0 / amax : 1.0;
+    if (hscale) *hscale = normalise && amax > 0.0 ? (float)amax : 1.0f;
+    const int64_t nnz = (int64_t)vals.size();
+    auto at = [&](int64_t p) { return vm.at.empty() ? p : (int64_t)vm.at[p]; };
+    if (h16) {
+        std::vector<_Float16> out(vm.n, (_Float16)0.0f);
+        parallel_for(nnz, [&](int64_t b, int64_t e) {
+            for (int64_t p = b; p < e; ++p) out[at(p)] = (_Float16)(vals[p] * hinv);
+        });
+        HIPCHK(hipMemcpy(const_cast<void*>(dev), out.data(), out.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    } else {
+        std::vector<T> out(vm.n, T(0));
+        parallel_for(nnz, [&](int64_t b, int64_t e) {
+            for (int64_t p = b; p < e; ++p) out[at(p)] = (T)vals[p];
+        });
+        HIPCHK(hipMemcpy(const_cast<void*>(dev), out.data(), out.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+}
+
+// Values of a CSR matrix with each entry scaled: by rs[row] and/or cs[col]
+// (multiplied), or divided by both when div.
+static std::vector<double> scaled_values(const Csr& A, const std::vector<double>* rs, const std::vector<double>* cs,
+                                         bool div) {
+    std::vector<double> v(A.v.size());
+    parallel_for(A.nrows, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i)
+            for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+                double x = A.v[p];
+                if (div) {
+                    double d = 1.0;
+                    if (rs) d = (*rs)[i];
+                    if (cs) d = rs ? d * (*cs)[A.ci[p]] : (*cs)[A.ci[p]];
+                    x = x / d;
+                } else {
+                    if (rs) x *= (*rs)[i];
+                    if (cs) x *= (*cs)[A.ci[p]];
+                }
+                v[p] = x;
+            }
+    });
+    return v;
+}
+
+template <typename T>
+bool amg_refresh(AmgDev<T>& D, const Amg& H) {
+    if (D.lv.size() != H.lv.size() || D.nc != H.ncoarse) return false;
+    // every shape and storage decision first, so that a mismatch changes nothing
+    for (size_t l = 0; l < H.lv.size(); ++l) {
+        const AmgLevel& h = H.lv[l];
+        const AmgLevelDev<T>& d = D.lv[l];
+        const bool last = l + 1 == H.lv.size();
+        if (h.A.nrows != d.n || h.A.nnz() != (int64_t)d.vA.at.size()) return false;
+        const char* he = std::getenv("VEMD_MG_HALF");
+        bool half = l == 0 && sizeof(T) == 4 && !last && !(he && he[0] == '0');
+        for (size_t i = 0; half && i < h.dinv.size(); ++i)
+            if (!(h.dinv[i] > 0.0) || !std::isfinite(1.0 / std::sqrt(h.dinv[i]))) half = false;
+        if (half != (d.sc != nullptr)) return false;
+        if (last) continue;
+        if (h.P.nnz() != (int64_t)d.vP.at.size() || h.R.nnz() != (int64_t)d.vR.n) return false;
+        const bool hasM = d.M.nrows > 0;
+        if (hasM && (h.M.nnz() != (int64_t)d.vM.at.size() || h.R.nnz() != (int64_t)d.vRb.at.size())) return false;
+        if (half) {
+            const bool htr = d.P.hval != nullptr;
+            if (htr != (d.R.hv != nullptr) || (htr && !half_safe({&h.P, &h.R}))) return false;
+            if (hasM && ((d.M.hv != nullptr) != (htr && half_safe({&h.M})) ||
+                         (d.Rb.hv != nullptr) != (htr && half_safe({&h.R}))))
+                return false;
+        }
+    }
+    for (size_t l = 0; l < H.lv.size(); ++l) {
+        const AmgLevel& h = H.lv[l];
+        AmgLevelDev<T>& d = D.lv[l];
+        const bool last = l + 1 == H.lv.size();
+        if (d.sc) {
+            std::vector<double> dsq(h.dinv.size());
+            for (size_t i = 0; i < dsq.size(); ++i) dsq[i] = 1.0 / std::sqrt(h.dinv[i]);
+            refill<T>(d.vA, scaled_values(h.A, &dsq, &dsq, true), true, false, d.A.hval, nullptr);
+            const bool htr = d.P.hval != nullptr;
+            refill<T>(d.vP, scaled_values(h.P, &dsq, nullptr, false), htr, htr,
+                      htr ? (const void*)d.P.hval : (const void*)d.P.val, &d.P.hscale);
+            refill<T>(d.vR, scaled_values(h.R, nullptr, &dsq, false), htr, htr,
+                      htr ? (const void*)d.R.hv : (const void*)d.R.v, &d.R.hscale);
+            if (d.M.nrows > 0) {
+                const bool mh = d.M.hv != nullptr, rh = d.Rb.hv != nullptr;
+                refill<T>(d.vM, scaled_values(h.M, nullptr, &dsq, true), mh, mh,
+                          mh ? (const void*)d.M.hv : (const void*)d.M.v, &d.M.hscale);
+                refill<T>(d.vRb, h.R.v, rh, rh, rh ? (const void*)d.Rb.hv : (const void*)d.Rb.v, &d.Rb.hscale);
+            }
+            std::vector<T> sc(dsq.size());
+            for (size_t i = 0; i < dsq.size(); ++i) sc[i] = (T)(1.0 / dsq[i]);
+            HIPCHK(hipMemcpy(d.sc, sc.data(), sc.size() * sizeof(T), hipMemcpyHostToDevice));
+        } else {
+            refill<T>(d.vA, h.A.v, false, false, d.A.val, nullptr);
+            if (!last) {
+                refill<T>(d.vP, h.P.v, false, false, d.P.val, nullptr);
+                refill<T>(d.vR, h.R.v, false, false, d.R.v, nullptr);
+                if (d.M.nrows > 0) {
+                    refill<T>(d.vM, h.M.v, false, false, d.M.v, nullptr);
+                    refill<T>(d.vRb, h.R.v, false, false, d.Rb.v, nullptr);
+                }
+            }
+            std::vector<T> di(h.dinv.begin(), h.dinv.end());
+            HIPCHK(hipMemcpy(d.dinv, di.data(), di.size() * sizeof(T), hipMemcpyHostToDevice));
+        }
+        const double hi = h.lmax, lo = h.lmax / H.opt.cheb_ratio;
+        d.centre = 0.5 * (hi + lo);
+        d.half = 0.5 * (hi - lo);
+        d.degree = l == 0 ? std::max(1, H.opt.degree0) : std::max(1, H.opt.degree);
+    }
+    std::vector<T> ci(H.coarse_inv.begin(), H.coarse_inv.end());
+    HIPCHK(hipMemcpy(D.cinv, ci.data(), ci.size() * sizeof(T), hipMemcpyHostToDevice));
+    return true;
 }
 
 // Vectors of one level for NC columns.
@@ -977,6 +1116,8 @@ template struct AmgDev<double>;
 template struct AmgDev<float>;
 template AmgDev<double>* amg_upload<double>(const Amg&);
 template AmgDev<float>* amg_upload<float>(const Amg&);
+template bool amg_refresh<double>(AmgDev<double>&, const Amg&);
+template bool amg_refresh<float>(AmgDev<float>&, const Amg&);
 template void amg_vcycle<double>(const AmgDev<double>&, hipStream_t, const int*, bool, const KOut<double>*);
 template void amg_vcycle<float>(const AmgDev<float>&, hipStream_t, const int*, bool, const KOut<float>*);
 template bool amg_pre_fusable<double>(const AmgDev<double>&);

@@ -20,12 +20,15 @@ struct Solver {
     std::vector<double> minv;      // preconditioner inverse, internal order
     std::vector<double> rowscale;  // internal = rowscale * reference-form rows
     std::vector<double> rhs_ref;   // reference-form right-hand side
+    std::unique_ptr<DiagPlan> dplan;  // index data of the preconditioner diagonal (theta-independent)
     // multigrid hierarchy of the pressure Schur approximation (with the
     // mean-zero multiplier: of that approximation plus a rank-one term along
     // the multiplier row, whose inverse weight preconditions the multiplier)
     std::unique_ptr<Amg> amg;
     AmgOptions amg_opt;
     double amg_lam_minv = 0.0;     // multiplier entry of M^-1 with the multigrid cycle (0: no multiplier)
+    std::unique_ptr<SchurPlan> splan;  // sparsity of the pressure Schur approximation (theta-independent)
+    bool amg_values_only = false;  // the last rebuild kept every sparsity pattern of the hierarchy
     double amg_setup_s = 0.0;
     DeviceState* dev = nullptr;
     ErrDev* errdev = nullptr;  // allocated by the first device error evaluation

@@ -337,43 +337,88 @@ void pack_scale(const Setup& S, const Layout& L, std::vector<double>& s) {
     }
 }
 
-void build_minv(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& minv) {
-    std::vector<double> d(L.nvec, 0.0);
-    const int q = L.q;
-    // flux: assembled diagonal
+DiagPlan diag_plan(const Mesh& m, const Setup& S, const Layout& L) {
+    DiagPlan P;
+    const int64_t nf = L.cbase + (int64_t)L.ni * L.cpad;
+    P.c0.assign(nf, -1);
+    P.c1.assign(nf, -1);
     for (int c = 0; c < m.nc; ++c) {
-        int n = S.sh.nflux(m.npc(c));
-        double sc = std::exp(-S.theta[c]);
-        const double* A = &S.A[S.ofs.a_off[c]];
+        const int n = S.sh.nflux(m.npc(c));
         for (int i = 0; i < n; ++i) {
-            int64_t v = S.ofs.v_off[c] + i;
-            double aii = S.isb[v] ? 1.0 : A[i * (i + 1) / 2 + i];
-            d[L.int_of_ref[S.gdof[v]]] += sc * aii;
+            const int64_t v = S.ofs.v_off[c] + i;
+            if (S.isb[v]) P.essential = true;
+            const int64_t t = L.int_of_ref[S.gdof[v]];
+            if (P.c0[t] < 0)
+                P.c0[t] = v;
+            else if (P.c1[t] < 0)
+                P.c1[t] = v;
+            else
+                throw Error(VEMD_ERR_INTERNAL, "flux dof shared by more than two cells");
         }
     }
-    // pressure: diag of W diag(A)^{-1} W^T per cell; multiplier: sum L^2 / S
-    double lam = 0.0;
-    for (int c = 0; c < m.nc; ++c) {
-        int n = S.sh.nflux(m.npc(c));
-        const double* W = &S.W[S.ofs.w_off[c]];
-        for (int r = 0; r < q; ++r) {
-            double sr = 0.0;
+    return P;
+}
+
+void build_minv(const Mesh& m, const Setup& S, const Layout& L, const DiagPlan& plan, std::vector<double>& minv) {
+    const int q = L.q;
+    const int64_t nv = S.ofs.v_off[m.nc];
+    // each local flux dof's diagonal, exp(-theta) A_ii (1 on an essential dof)
+    std::vector<double> cv(nv);
+    parallel_for(m.nc, [&](int64_t b, int64_t e) {
+        for (int64_t c = b; c < e; ++c) {
+            const int n = S.sh.nflux(m.npc((int)c));
+            const double sc = std::exp(-S.theta[c]);
+            const double* A = &S.A[S.ofs.a_off[c]];
             for (int i = 0; i < n; ++i) {
-                int64_t v = S.ofs.v_off[c] + i;
-                if (S.isb[v]) continue;
-                double w = W[r * n + i];
-                sr += w * w / d[L.int_of_ref[S.gdof[v]]];
+                const int64_t v = S.ofs.v_off[c] + i;
+                cv[v] = S.isb[v] ? 1.0 : sc * A[i * (i + 1) / 2 + i];
             }
-            if (!(sr > 0.0)) sr = 1.0;
-            d[L.int_of_ref[S.nflux_total + (int64_t)c * q + r]] = sr;
-            double l = S.L[(size_t)c * q + r];
-            lam += l * l / sr;
         }
+    });
+    // flux: the assembled diagonal, contributions added in cell order
+    std::vector<double> d(L.nvec, 0.0);
+    const int64_t nf = (int64_t)plan.c0.size();
+    parallel_for(nf, [&](int64_t b, int64_t e) {
+        for (int64_t t = b; t < e; ++t) {
+            double v = 0.0;
+            if (plan.c0[t] >= 0) v += cv[plan.c0[t]];
+            if (plan.c1[t] >= 0) v += cv[plan.c1[t]];
+            d[t] = v;
+        }
+    });
+    // pressure: diag of W diag(A)^{-1} W^T per cell; multiplier: sum L^2 / S
+    std::vector<double> lc(L.lam ? (size_t)m.nc : 0);
+    parallel_for(m.nc, [&](int64_t b, int64_t e) {
+        for (int64_t c = b; c < e; ++c) {
+            const int n = S.sh.nflux(m.npc((int)c));
+            const double* W = &S.W[S.ofs.w_off[c]];
+            double lsum = 0.0;
+            for (int r = 0; r < q; ++r) {
+                double sr = 0.0;
+                for (int i = 0; i < n; ++i) {
+                    const int64_t v = S.ofs.v_off[c] + i;
+                    if (S.isb[v]) continue;
+                    const double w = W[r * n + i];
+                    sr += w * w / d[L.int_of_ref[S.gdof[v]]];
+                }
+                if (!(sr > 0.0)) sr = 1.0;
+                d[L.int_of_ref[S.nflux_total + c * q + r]] = sr;
+                const double l = S.L[(size_t)c * q + r];
+                lsum += l * l / sr;
+            }
+            if (L.lam) lc[c] = lsum;
+        }
+    });
+    if (L.lam) {
+        double lam = 0.0;
+        for (double v : lc) lam += v;
+        d[L.lam_idx] = lam > 0 ? lam : 1.0;
     }
-    if (L.lam) d[L.lam_idx] = lam > 0 ? lam : 1.0;
     minv.assign(L.nvec, 1.0);
-    for (int64_t i = 0; i < L.nvec; ++i)
-        if (d[i] > 0.0) minv[i] = 1.0 / d[i];
+    parallel_for(L.nvec, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i)
+            if (d[i] > 0.0) minv[i] = 1.0 / d[i];
+    });
 }
 
 void build_rhs_ref(const Mesh& m, const Setup& S, std::vector<double>& b) {
@@ -385,6 +430,13 @@ void build_rhs_ref(const Mesh& m, const Setup& S, std::vector<double>& b) {
         const double* A = &S.A[S.ofs.a_off[c]];
         const double* W = &S.W[S.ofs.w_off[c]];
         int64_t v0 = S.ofs.v_off[c];
+        bool ess = false;  // a cell without essential dofs has no lift
+        for (int i = 0; i < n && !ess; ++i) ess = S.isb[v0 + i] != 0;
+        if (!ess) {
+            for (int i = 0; i < n; ++i) b[S.gdof[v0 + i]] += S.ru[v0 + i];
+            for (int r = 0; r < q; ++r) b[S.nflux_total + (int64_t)c * q + r] += S.f[(size_t)c * q + r];
+            continue;
+        }
         for (int i = 0; i < n; ++i) {
             int64_t g = S.gdof[v0 + i];
             if (S.isb[v0 + i]) {

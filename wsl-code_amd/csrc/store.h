@@ -88,10 +88,20 @@ void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<doub
 // Per-store-position cell scale exp(-theta) (padding 0).
 void pack_scale(const Setup& S, const Layout& L, std::vector<double>& s);
 
+// Theta-independent index data of the preconditioner diagonal, built once:
+// for every internal flux slot the (at most two) local flux dofs of the setup
+// (index into the per-cell dof arrays, cell order) whose diagonals sum there.
+struct DiagPlan {
+    std::vector<int64_t> c0, c1;  // -1: none
+    bool essential = false;       // some flux dof is an essential boundary dof
+};
+DiagPlan diag_plan(const Mesh& m, const Setup& S, const Layout& L);
+
 // Inverse of the block-diagonal preconditioner (internal order):
 // flux: assembled diag of exp(-theta) A; pressure: sum_i W_ri^2 / d_i;
 // multiplier: sum L_r^2 / S_r (the reference's choice, GPU/src/equation_gpu.cu:282-315).
-void build_minv(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& minv);
+// Cells in parallel; the sums run in cell order whatever the thread count.
+void build_minv(const Mesh& m, const Setup& S, const Layout& L, const DiagPlan& plan, std::vector<double>& minv);
 
 // Right-hand side in reference order and reference form: flux rows the natural
 // boundary term minus the lift, pressure rows the source minus the lift,
