@@ -1,0 +1,92 @@
+"""Second-order spaces (k = 2: three moments per edge, six pressure moments
+per cell, 18-38 local flux dofs) on the device, with the diagonal
+preconditioner of the reference (the multigrid cycle is built for k <= 1):
+
+* the reference's published k = 2 errors (CPU/results_darcy_convergence_k2.txt,
+  example 1, square meshes 4..256 cells) from device solves;
+* the FP64 and FP32 applies against the oracle's assembled operator and the
+  FP64 and mixed solves against its sparse direct solve, on the reference's
+  squares (both boundary conditions) and on a generated disk with exact arcs
+  (Dirichlet pressure, a cellwise permeability);
+* the device paths that stay k <= 1 refuse k = 2."""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+GOLD2 = [c for c in json.load(open(golden("golden_errors.json")))["cases"] if c["k"] == 2]
+
+
+@pytest.mark.parametrize("case", GOLD2, ids=lambda c: c["mesh"][:-4])
+def test_reference_k2_errors_on_device(vemd, case):
+    print("k=2 golden", case["mesh"], flush=True)
+    s = vemd.Solver(vemd.Mesh.read(golden(case["mesh"])), k=2, nq2d=case["nq2"], nq1d=case["nq1"],
+                    eps=0.05 if case["example"] == 1 else 0.0, device=0)
+    x, info = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12))
+    e = s.errors(x, case["nq2"])
+    for key in ("p_comp", "u_comp"):
+        v = case[key]
+        tol = 0.5 * 10.0 ** (np.floor(np.log10(v)) - 6) + 1e-7 * v
+        assert abs(e[key] - v) <= tol, (key, e[key], v, info)
+
+
+def _check_apply_and_solves(vemd, s, op):
+    A, _ = op.csr(False)
+    fi, _ = op.free()
+    rng = np.random.default_rng(8)
+    x = np.zeros(s.ntot)
+    x[fi] = rng.standard_normal(len(fi))
+    y_ref = A @ x
+    scale = np.abs(y_ref[fi]).max()
+    y = s.apply(x, vemd.FP64)
+    assert np.abs(y[fi] - y_ref[fi]).max() <= 1e-13 * scale
+    y32 = s.apply(x, vemd.FP32)
+    assert np.abs(y32[fi] - y_ref[fi]).max() <= 2e-5 * scale
+    xo = op.solve_direct()
+    x64, info = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12))
+    assert np.linalg.norm(x64 - xo) <= 1e-10 * np.linalg.norm(xo), info
+    xm, infom = s.solve(vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-10))
+    assert np.linalg.norm(xm - xo) <= 1e-8 * np.linalg.norm(xo), infom
+
+
+@pytest.mark.parametrize("mesh,bc", [("square_4x4.msh", 0), ("square_4x4.msh", 1), ("square_8x8.msh", 0)])
+def test_k2_square_apply_and_solves(vemd, oracle, mesh, bc):
+    print("k=2", mesh, "bc", bc, flush=True)
+    s = vemd.Solver(vemd.Mesh.read(golden(mesh)), k=2, eps=0.05, bc=bc, device=0)
+    op = oracle.Problem(oracle.Mesh.read(golden(mesh)), k=2, eps=0.05, bc=bc)
+    _check_apply_and_solves(vemd, s, op)
+
+
+def test_k2_curved_disk_with_permeability(vemd, oracle):
+    from wsl_code_amd.fields import grf_theta
+    print("k=2 disk 500 cells", flush=True)
+    m = vemd.generate_mesh(500, lloyd_iters=20, seed=91, min_edge_frac=0.05)
+    a = m.arrays()
+    th = grf_theta(a["cx"], a["cy"], sigma=0.5, ell=0.2, seed=92)
+    s = vemd.Solver(m, k=2, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P, theta=th,
+                    device=0)
+    om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+    op = oracle.Problem(om, k=2, map_kind=oracle.MAP_ARCS, sol=oracle.SOL_SINSIN, bc=oracle.BC_DIRICHLET_P, theta=th)
+    _check_apply_and_solves(vemd, s, op)
+
+
+def test_k2_refusals(vemd):
+    import ctypes
+    import torch
+    m = vemd.generate_mesh(300, lloyd_iters=10, seed=93, min_edge_frac=0.05)
+    s = vemd.Solver(m, k=2, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P, device=0)
+    b = torch.zeros(s.nvec, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(b)
+    with pytest.raises(vemd.VemdError, match="k = 0 and k = 1"):
+        s.errors_device(x)
+    with pytest.raises(vemd.VemdError):
+        vemd.Solver(m, k=2, map_kind=vemd.MAP_ARCS, bc=vemd.BC_DIRICHLET_P, setup_device=0)
+    o = vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-8).c()
+    info = (vemd._lib.SolveInfo * 1)()
+    rc = vemd.lib().vemd_solve_multi_device(s.h, ctypes.byref(o), 1, ctypes.c_void_p(b.data_ptr()),
+                                            ctypes.c_void_p(x.data_ptr()), info)
+    assert rc != 0 and b"k = 0 and k = 1" in vemd.lib().vemd_last_error()

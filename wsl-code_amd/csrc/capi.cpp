@@ -56,7 +56,9 @@ void Solver::refresh_theta_dependent() {
     // without essential dofs the right-hand side has no theta-dependent lift
     if (rhs_ref.empty() || dplan->essential) build_rhs_ref(*mesh, *setup, rhs_ref);
     auto t1 = std::chrono::steady_clock::now();
-    build_amg();
+    // the multigrid cycle is built for k = 0 and 1 (q = 1 or 3 pressure
+    // moments per cell); k = 2 keeps the diagonal preconditioner
+    if (setup->sh.k <= 1) build_amg();
     if (std::getenv("VEMD_VERBOSE"))
         std::fprintf(stderr, "[vemd] theta-dependent host data %.3fs (scales, diagonal, rhs %.3fs)\n",
                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
@@ -290,7 +292,7 @@ vemd_status vemd_solver_create(vemd_mesh mh, const vemd_problem_desc* d, vemd_so
     std::unique_ptr<Solver> s(new Solver());
     s->mesh = mh->m;
     s->setup.reset(setup_build(*s->mesh, *d));
-    if (d->k <= 1) {
+    if (d->k <= 2) {
         s->layout = layout_build(*s->mesh, *s->setup);
         s->refresh_theta_dependent();
     } else {
@@ -335,7 +337,7 @@ vemd_status vemd_solver_sizes(vemd_solver sh, int64_t* z) {
 vemd_status vemd_solver_upload(vemd_solver sh, int device) {
     API_BEGIN
     REQUIRE(sh, "null solver");
-    if (sh->s->setup->sh.k > 1) throw Error(VEMD_ERR_ARG, "the device path supports k = 0 and k = 1");
+    if (sh->s->setup->sh.k > 2) throw Error(VEMD_ERR_ARG, "the device path supports k = 0, 1 and 2");
     dev_upload(*sh->s, device);
     API_END
 }
@@ -354,7 +356,7 @@ vemd_status vemd_set_theta(vemd_solver sh, const double* theta) {
     for (int c = 0; c < s.mesh->nc; ++c)
         if (!std::isfinite(theta[c])) throw Error(VEMD_ERR_ARG, "theta must be finite");
     s.setup->theta.assign(theta, theta + s.mesh->nc);
-    if (s.setup->sh.k <= 1) {
+    if (s.setup->sh.k <= 2) {
         s.refresh_theta_dependent();
         dev_refresh(s);
     }

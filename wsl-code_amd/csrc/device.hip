@@ -357,6 +357,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <typename T, int K, int NE>
 __device__ __forceinline__ void cell_variant(int var, const ApplyArgs<T>& a, const T* __restrict__ x,
                                              const GroupDesc& g, int tau, int l, double& dot, double& lamp) {
+    if constexpr (K >= 2) {  // one variant (the tuner's variants differ only in speed)
+        (void)var;
+        cell_body<T, K, NE, Variant<0>::CH, Variant<0>::PIPE>(a, x, g, tau, l, dot, lamp);
+        return;
+    }
     switch (var) {
         case 1: cell_body<T, K, NE, Variant<1>::CH, Variant<1>::PIPE>(a, x, g, tau, l, dot, lamp); break;
         case 2: cell_body<T, K, NE, Variant<2>::CH, Variant<2>::PIPE>(a, x, g, tau, l, dot, lamp); break;
@@ -407,9 +412,12 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs<T> a) {
 
 // Entry i of the edge-dof block: the sum of the edge's two cell sides (a
 // boundary edge's second slot is the zero slot).
+// Edge of the edge-dof entry i (kk moments per edge).
+__device__ __forceinline__ int64_t edge_of(int64_t i, int kk) { return kk == 2 ? i >> 1 : (kk == 1 ? i : i / kk); }
+
 template <typename T>
 __device__ __forceinline__ T side_sum(const T* __restrict__ h, const int* __restrict__ hside, int kk, int64_t i) {
-    const int64_t e = kk == 2 ? i >> 1 : i;
+    const int64_t e = edge_of(i, kk);
     const int r = (int)(i - e * kk);
     return h[(int64_t)hside[2 * e] * kk + r] + h[(int64_t)hside[2 * e + 1] * kk + r];
 }
@@ -642,7 +650,7 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
         int rr[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t e = a.kk == 2 ? ii[u] >> 1 : ii[u];
+            const int64_t e = edge_of(ii[u], a.kk);
             rr[u] = (int)(ii[u] - e * a.kk);
             sd[u] = reinterpret_cast<const int2*>(a.hside)[e];
         }
@@ -1260,8 +1268,10 @@ static void launch_apply(Solver& s, const ApplyArgs<T>& a) {
     dim3 grid(8 * a.per_xcd), block(256);
     if (s.layout.k == 0)
         hipLaunchKernelGGL((k_apply<T, 0>), grid, block, 0, s.dev->stream, a);
-    else
+    else if (s.layout.k == 1)
         hipLaunchKernelGGL((k_apply<T, 1>), grid, block, 0, s.dev->stream, a);
+    else
+        hipLaunchKernelGGL((k_apply<T, 2>), grid, block, 0, s.dev->stream, a);
 }
 
 // Autotuner of the apply: for every valence, the variants of the cell body
@@ -1280,7 +1290,7 @@ static void autotune_apply(Solver& s) {
         tuned_ms[i] = 0.0f;
     }
     const char* env = std::getenv("VEMD_AUTOTUNE");
-    const bool on = !(env && env[0] == '0');
+    const bool on = !(env && env[0] == '0') && L.k <= 1;  // k = 2 builds one variant
     std::vector<int> rep(16, -1);  // largest group of each valence
     for (size_t gi = 0; gi < G.size(); ++gi) {
         const int ne = G[gi].ne;
@@ -1836,8 +1846,11 @@ void dev_adjoint(Solver& s, const vemd_solve_opts& o, const double* sol_ref, con
     if (L.k == 0)
         hipLaunchKernelGGL((k_adjoint<double, 0>), dim3(d.ntiles), dim3(64), 0, d.stream, a, d.x64, d.y64,
                            d.cell_of_gpos, g);
-    else
+    else if (L.k == 1)
         hipLaunchKernelGGL((k_adjoint<double, 1>), dim3(d.ntiles), dim3(64), 0, d.stream, a, d.x64, d.y64,
+                           d.cell_of_gpos, g);
+    else
+        hipLaunchKernelGGL((k_adjoint<double, 2>), dim3(d.ntiles), dim3(64), 0, d.stream, a, d.x64, d.y64,
                            d.cell_of_gpos, g);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(grad, g, s.mesh->nc * sizeof(double), hipMemcpyDeviceToHost, d.stream));

@@ -665,6 +665,7 @@ static MultiState& multi_setup(Solver& s) {
     if (d.multi) return *d.multi;
     if (s.layout.lam || s.layout.ninst != 1)
         throw Error(VEMD_ERR_ARG, "multi-right-hand-side solves need one Dirichlet-pressure mesh");
+    if (s.layout.k > 1) throw Error(VEMD_ERR_ARG, "multi-right-hand-side solves support k = 0 and k = 1");
     MultiState* m = new MultiState();
     try {
         multi_build(s, *m);
