@@ -226,13 +226,56 @@ Csr schur_pressure(const Mesh& m, const Setup& S, const Layout& L, const std::ve
     return schur_values(schur_plan(m, S, L), m, S, L, minv);
 }
 
+// Sum over a wavefront of 64 values: lane l adds the value of lane l ^ o for
+// o = 32, 16, ..., 1 (every lane ends with the same total).
+static double butterfly64(double (&v)[64]) {
+    for (int o = 32; o > 0; o >>= 1) {
+        double t[64];
+        for (int l = 0; l < 64; ++l) t[l] = v[l] + v[l ^ o];
+        std::copy(t, t + 64, v);
+    }
+    return v[0];
+}
+
+// The sums of a[i]^2 and b[i]^2 over i < n in the order of the device
+// rebuild: each group of 64 consecutive entries by the butterfly, each group
+// of 64 consecutive group sums by the butterfly, then those in order.
+static void tree_norms(int64_t n, const double* a, const double* b, double& na, double& nb) {
+    const int64_t G = (n + 63) / 64, H = (G + 63) / 64;
+    std::vector<double> ga(G), gb(G);
+    parallel_for(G, [&](int64_t lo, int64_t hi) {
+        for (int64_t g = lo; g < hi; ++g) {
+            double va[64], vb[64];
+            for (int l = 0; l < 64; ++l) {
+                const int64_t i = g * 64 + l;
+                va[l] = i < n ? a[i] * a[i] : 0.0;
+                vb[l] = i < n ? b[i] * b[i] : 0.0;
+            }
+            ga[g] = butterfly64(va);
+            gb[g] = butterfly64(vb);
+        }
+    });
+    na = 0.0;
+    nb = 0.0;
+    for (int64_t h = 0; h < H; ++h) {
+        double va[64], vb[64];
+        for (int l = 0; l < 64; ++l) {
+            const int64_t g = h * 64 + l;
+            va[l] = g < G ? ga[g] : 0.0;
+            vb[l] = g < G ? gb[g] : 0.0;
+        }
+        na += butterfly64(va);
+        nb += butterfly64(vb);
+    }
+}
+
 // Upper estimate of the spectral radius of D^-1 A by power iteration: iters
 // steps from *ev when it holds a previous estimate's vector of the right
 // length (a rebuild of the values after a theta change), else 25 from a
 // fixed pseudo-random vector; the last iterate is left in *ev.  (Fewer steps
 // from the previous vector measured up to 10% low after a change to an
-// independent theta field, beyond the 1.1 margin.)  The norms are sums over
-// 256 fixed blocks added in order, whatever the thread count.
+// independent theta field, beyond the 1.1 margin.)  The norms are summed in
+// the fixed order of tree_norms, whatever the thread count.
 static double spectral_radius(const Csr& A, const std::vector<double>& dinv, std::vector<double>* ev, int iters) {
     const int64_t n = A.nrows;
     std::vector<double> x(n), y(n);
@@ -248,29 +291,14 @@ static double spectral_radius(const Csr& A, const std::vector<double>& dinv, std
             x[i] = 0.5 + (double)(s >> 11) * (1.0 / 9007199254740992.0);
         }
     }
-    constexpr int NB = 256;
-    std::vector<double> pny(NB), pnx(NB);
-    const int64_t bl = (n + NB - 1) / NB;
     double lam = 0.0;
     for (int it = 0; it < iters; ++it) {
         csr_spmv(A, x.data(), y.data());
-        parallel_for(NB, [&](int64_t b, int64_t e) {
-            for (int64_t k = b; k < e; ++k) {
-                double sy = 0.0, sx = 0.0;
-                for (int64_t i = k * bl; i < std::min(n, (k + 1) * bl); ++i) {
-                    y[i] *= dinv[i];
-                    sy += y[i] * y[i];
-                    sx += x[i] * x[i];
-                }
-                pny[k] = sy;
-                pnx[k] = sx;
-            }
+        parallel_for(n, [&](int64_t b, int64_t e) {
+            for (int64_t i = b; i < e; ++i) y[i] *= dinv[i];
         });
-        double ny = 0.0, nx = 0.0;
-        for (int k = 0; k < NB; ++k) {
-            ny += pny[k];
-            nx += pnx[k];
-        }
+        double ny, nx;
+        tree_norms(n, y.data(), x.data(), ny, nx);
         if (!(ny > 0.0)) break;
         lam = std::sqrt(ny / nx);
         const double sc = 1.0 / std::sqrt(ny);
@@ -325,7 +353,7 @@ static int aggregate(const std::vector<std::vector<std::pair<int, double>>>& str
     return na;
 }
 
-static void dense_inverse(std::vector<double>& a, int64_t n) {
+void dense_inverse(std::vector<double>& a, int64_t n) {
     // Gauss-Jordan with partial pivoting (the coarsest operator has at most a
     // few thousand rows); the row eliminations of each pivot run in parallel.
     std::vector<double> inv(n * n, 0.0);

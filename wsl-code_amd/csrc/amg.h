@@ -82,6 +82,10 @@ Amg amg_build(Csr A0, const std::vector<int>& node_of_row, int nnodes, const std
               const std::vector<int>& node_order, const AmgOptions& o, const std::vector<double>* l0 = nullptr,
               double rho = 0.0, const Amg* reuse = nullptr);
 
+// In-place inverse of a dense row-major n x n matrix (Gauss-Jordan with
+// partial pivoting; the coarsest operator).
+void dense_inverse(std::vector<double>& a, int64_t n);
+
 // Helpers exposed for tests: y = A x, C = A B, transpose.
 void csr_spmv(const Csr& A, const double* x, double* y);
 Csr csr_mult(const Csr& A, const Csr& B);

@@ -47,18 +47,19 @@ void Solver::to_reference(const double* in, double* ref, bool unscale_rows) cons
     }
 }
 
-void Solver::refresh_theta_dependent() {
+void Solver::refresh_theta_dependent(bool host_amg) {
     auto t0 = std::chrono::steady_clock::now();
     if (!dplan) dplan.reset(new DiagPlan(diag_plan(*mesh, *setup, layout)));
     pack_scale(*setup, layout, scale);
     build_minv(*mesh, *setup, layout, *dplan, minv);
     build_row_scale(*mesh, *setup, layout, rowscale);
     // without essential dofs the right-hand side has no theta-dependent lift
-    if (rhs_ref.empty() || dplan->essential) build_rhs_ref(*mesh, *setup, rhs_ref);
+    rhs_changed = rhs_ref.empty() || dplan->essential;
+    if (rhs_changed) build_rhs_ref(*mesh, *setup, rhs_ref);
     auto t1 = std::chrono::steady_clock::now();
     // the multigrid cycle is built for k = 0 and 1 (q = 1 or 3 pressure
     // moments per cell); k = 2 keeps the diagonal preconditioner
-    if (setup->sh.k <= 1) build_amg();
+    if (setup->sh.k <= 1 && host_amg) build_amg();
     if (std::getenv("VEMD_VERBOSE"))
         std::fprintf(stderr, "[vemd] theta-dependent host data %.3fs (scales, diagonal, rhs %.3fs)\n",
                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
@@ -116,9 +117,19 @@ void Solver::build_amg() {
                 const int64_t i = g * L.q + r;
                 l0[i] = setup->L[(size_t)c * L.q + r];
                 if (r == 0) ln0 += l0[i];
-                for (int64_t p = S0.rp[i]; p < S0.rp[i + 1]; ++p)
-                    if (S0.ci[p] == i) sd += S0.v[p];
             }
+        }
+        // S0's diagonal over the live rows, in 256 blocks of rows summed in
+        // order (the order of the device rebuild)
+        const int64_t bl = (n + 255) / 256;
+        for (int k = 0; k < 256; ++k) {
+            double s = 0.0;
+            for (int64_t i = k * bl; i < std::min(n, (k + 1) * bl); ++i) {
+                if (L.cell_of_gpos[i / L.q] < 0) continue;
+                for (int64_t p = S0.rp[i]; p < S0.rp[i + 1]; ++p)
+                    if (S0.ci[p] == i) s += S0.v[p];
+            }
+            sd += s;
         }
         if (!(sd > 0.0) || !(ln0 != 0.0)) throw Error(VEMD_ERR_INTERNAL, "degenerate pressure Schur approximation");
         double c = 0.03;  // measured: 203, 250, 262 FP64 iterations at 4k, 64k, 256k disk cells
@@ -130,6 +141,7 @@ void Solver::build_amg() {
                                 l0.empty() ? nullptr : &l0, rho, prev.get())));
     amg->bs0 = L.q;
     amg_values_only = prev != nullptr;
+    amg_host_stale = false;
     amg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (std::getenv("VEMD_VERBOSE")) {
         std::fprintf(stderr, "[vemd] multigrid setup %.2fs:", amg_setup_s);
@@ -357,8 +369,10 @@ vemd_status vemd_set_theta(vemd_solver sh, const double* theta) {
         if (!std::isfinite(theta[c])) throw Error(VEMD_ERR_ARG, "theta must be finite");
     s.setup->theta.assign(theta, theta + s.mesh->nc);
     if (s.setup->sh.k <= 2) {
-        s.refresh_theta_dependent();
-        dev_refresh(s);
+        // an uploaded solver rebuilds the multigrid values on its device
+        const bool device_mg = dev_mg_rebuild_possible(s);
+        s.refresh_theta_dependent(!device_mg);
+        dev_refresh(s, device_mg);
     }
     API_END
 }

@@ -15,6 +15,7 @@
 #include <utility>
 
 #include "devstate.h"
+#include "mgbuild.h"
 
 
 namespace vemd {
@@ -1039,26 +1040,72 @@ static void* static_scratch(int device) {
     return device >= 0 && device < MAX_DEVICES ? g_scratch_ptr[device] : nullptr;
 }
 
-static void upload_theta(Solver& s) {
+bool dev_mg_rebuild_possible(const Solver& s) {
+    if (!s.dev || !s.amg || !s.splan || !s.dev->amg64 || !s.dev->amg32 || s.setup->sh.k > 1) return false;
+    for (const char* v : {"VEMD_MG_DEVICE_REBUILD", "VEMD_MG_REFILL"}) {
+        const char* e = std::getenv(v);
+        if (e && e[0] == '0') return false;
+    }
+    const char* ra = std::getenv("VEMD_MG_REAGGREGATE");
+    return !(ra && ra[0] == '1');
+}
+
+// FP32 copy of an FP64 device array.
+__global__ void k_narrow(int64_t n, const double* __restrict__ src, float* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+static void upload_theta(Solver& s, bool device_mg = false, bool rhs = true) {
     DeviceState& d = *s.dev;
     // the scales, diagonals and hierarchy below are read by queued applies,
     // the allocation-free ones included (those run on any stream, unordered)
     HIPCHK(hipDeviceSynchronize());
-    std::vector<float> f(s.scale.begin(), s.scale.end());
     HIPCHK(hipMemcpy(d.o64.scale, s.scale.data(), s.scale.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d.o32.scale, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
-    std::vector<float> m(s.minv.begin(), s.minv.end());
     HIPCHK(hipMemcpy(d.o64.minv, s.minv.data(), s.minv.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d.o32.minv, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
-    std::vector<double> bi(s.layout.nvec, 0.0);
-    s.to_internal(s.rhs_ref.data(), bi.data(), true);
-    HIPCHK(hipMemcpy(d.bprob, bi.data(), bi.size() * sizeof(double), hipMemcpyHostToDevice));
+    const int64_t ns = (int64_t)s.scale.size(), nm = (int64_t)s.minv.size();
+    hipLaunchKernelGGL(k_narrow, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, d.stream, ns,
+                       (const double*)d.o64.scale, d.o32.scale);
+    hipLaunchKernelGGL(k_narrow, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, d.stream, nm,
+                       (const double*)d.o64.minv, d.o32.minv);
+    HIPCHK(hipGetLastError());
+    if (rhs) {
+        std::vector<double> bi(s.layout.nvec, 0.0);
+        s.to_internal(s.rhs_ref.data(), bi.data(), true);
+        HIPCHK(hipMemcpy(d.bprob, bi.data(), bi.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipStreamSynchronize(d.stream));
     // the hierarchy depends on theta through the flux diagonal; with its
     // aggregates kept (the host rebuilt the values only) the uploaded
     // hierarchies are refilled in place, else uploaded afresh
     destroy_graph(d.k64);
     destroy_graph(d.k32);
     multi_invalidate(d);
+    if (device_mg) {
+        // the hierarchy's values rebuilt on the device from the flux diagonal
+        // just uploaded; VEMD_MG_VERIFY=1 also rebuilds them on the host and
+        // requires every value to be bitwise equal
+        if (!d.mgb) d.mgb = mg_build_create(s, d.cell_of_gpos);
+        if (mg_build_run(*d.mgb, s, d.o64.minv, *d.amg64, *d.amg32, d.stream)) {
+            const char* vf = std::getenv("VEMD_MG_VERIFY");
+            if (vf && vf[0] == '1') {
+                s.build_amg();
+                double mr = 0.0;
+                const int64_t nd = mg_build_compare(*d.mgb, s, &mr);
+                if (std::getenv("VEMD_VERBOSE"))
+                    std::fprintf(stderr, "[vemd] device multigrid rebuild against the host build: %lld values differ\n",
+                                 (long long)nd);
+                if (nd != 0)
+                    throw Error(VEMD_ERR_INTERNAL, "device multigrid rebuild differs from the host build in " +
+                                                       std::to_string(nd) + " values (largest relative difference " +
+                                                       std::to_string(mr) + ")");
+            }
+            return;
+        }
+        s.build_amg();  // values the device path cannot store as uploaded: host rebuild
+    }
+    // a host hierarchy whose values were last rebuilt on the device
+    if (s.amg && s.amg_host_stale) s.build_amg();
     const char* rf = std::getenv("VEMD_MG_REFILL");  // 0: always upload afresh
     const bool refilled = !(rf && rf[0] == '0') && s.amg && s.amg_values_only && d.amg64 && d.amg32 &&
                           amg_refresh(*d.amg64, *s.amg) && amg_refresh(*d.amg32, *s.amg);
@@ -1164,11 +1211,11 @@ void dev_upload(Solver& s, int device) {
     autotune_apply<float>(s);
 }
 
-void dev_refresh(Solver& s) {
+void dev_refresh(Solver& s, bool device_mg) {
     if (!s.dev) return;
     HIPCHK(hipSetDevice(s.dev->device));
     auto t0 = std::chrono::steady_clock::now();
-    upload_theta(s);
+    upload_theta(s, device_mg, s.rhs_changed);
     if (std::getenv("VEMD_VERBOSE"))
         std::fprintf(stderr, "[vemd] theta-dependent device data %.3fs\n",
                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
@@ -1194,6 +1241,7 @@ void dev_free(Solver& s) {
     multi_free(d);
     delete d.amg64;
     delete d.amg32;
+    delete d.mgb;
     for (void* p : d.allocs) (void)hipFree(p);
     if (d.st_host) (void)hipHostFree(d.st_host);
     if (d.hsum) (void)hipHostFree(d.hsum);

@@ -14,6 +14,7 @@
 namespace vemd {
 
 struct MultiState;
+struct MgBuild;
 
 template <typename T>
 struct DevOps {
@@ -85,6 +86,7 @@ struct DeviceState {
     float* h32 = nullptr;
     AmgDev<double>* amg64 = nullptr;  // multigrid pressure preconditioner (when built)
     AmgDev<float>* amg32 = nullptr;
+    MgBuild* mgb = nullptr;  // index data of the device rebuild of the multigrid values (made on first use)
     double* q1 = nullptr;  // per-chunk partials of the pressure-block dots (zero outside it)
     double* q2 = nullptr;
     std::vector<void*> allocs;

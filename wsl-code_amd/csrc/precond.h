@@ -83,6 +83,7 @@ struct BcsrDev {
 struct ValMap {
     std::vector<uint32_t> at;
     size_t n = 0;
+    uint32_t* dev = nullptr;  // device copy of at (made by a device refill; owned by the cycle)
 };
 
 template <typename T>

@@ -20,6 +20,7 @@ struct Solver {
     std::vector<double> minv;      // preconditioner inverse, internal order
     std::vector<double> rowscale;  // internal = rowscale * reference-form rows
     std::vector<double> rhs_ref;   // reference-form right-hand side
+    bool rhs_changed = true;       // the last theta refresh rebuilt rhs_ref
     std::unique_ptr<DiagPlan> dplan;  // index data of the preconditioner diagonal (theta-independent)
     // multigrid hierarchy of the pressure Schur approximation (with the
     // mean-zero multiplier: of that approximation plus a rank-one term along
@@ -29,6 +30,10 @@ struct Solver {
     double amg_lam_minv = 0.0;     // multiplier entry of M^-1 with the multigrid cycle (0: no multiplier)
     std::unique_ptr<SchurPlan> splan;  // sparsity of the pressure Schur approximation (theta-independent)
     bool amg_values_only = false;  // the last rebuild kept every sparsity pattern of the hierarchy
+    // the hierarchy's values were last rebuilt on the device: the host copy
+    // keeps its patterns, aggregates, spectral bounds and coarsest inverse, but
+    // not its matrix values (rebuilt on the host before another upload)
+    bool amg_host_stale = false;
     double amg_setup_s = 0.0;
     DeviceState* dev = nullptr;
     ErrDev* errdev = nullptr;  // allocated by the first device error evaluation
@@ -41,13 +46,18 @@ struct Solver {
     // host-side conversions between reference order and internal order
     void to_internal(const double* ref, double* in, bool scale_rows) const;
     void to_reference(const double* in, double* ref, bool unscale_rows) const;
-    void refresh_theta_dependent();
+    // host_amg false: the multigrid values are left to the device rebuild
+    void refresh_theta_dependent(bool host_amg = true);
     void build_amg();
 };
 
 int device_count();
 void dev_upload(Solver& s, int device);
-void dev_refresh(Solver& s);  // cell scales and preconditioner after a theta change
+// Cell scales and preconditioner after a theta change; with device_mg the
+// multigrid values are rebuilt on the device (on the host if that fails).
+void dev_refresh(Solver& s, bool device_mg = false);
+// Whether a theta change can rebuild the multigrid values on the device.
+bool dev_mg_rebuild_possible(const Solver& s);
 void dev_free(Solver& s);
 // Apply variant chosen by the autotuner for valences 3..10 (FP64, FP32 pairs; -1: none).
 int dev_tuning(const Solver& s, int* out16);
