@@ -1,13 +1,17 @@
 """Second-order spaces (k = 2: three moments per edge, six pressure moments
-per cell, 18-38 local flux dofs) on the device, with the diagonal
-preconditioner of the reference (the multigrid cycle is built for k <= 1):
+per cell, 18-38 local flux dofs) on the device, preconditioned by the
+multigrid cycle (6 x 6 blocks on the finest level) and by the reference's
+diagonal:
 
 * the reference's published k = 2 errors (CPU/results_darcy_convergence_k2.txt,
   example 1, square meshes 4..256 cells) from device solves;
 * the FP64 and FP32 applies against the oracle's assembled operator and the
-  FP64 and mixed solves against its sparse direct solve, on the reference's
-  squares (both boundary conditions) and on a generated disk with exact arcs
-  (Dirichlet pressure, a cellwise permeability);
+  FP64 and mixed solves, with either preconditioner, against its sparse
+  direct solve, on the reference's squares (both boundary conditions) and on
+  a generated disk with exact arcs (Dirichlet pressure, a cellwise
+  permeability);
+* multigrid iterations on disks of 2k and 8k cells: far fewer than with the
+  diagonal and growing little with the mesh;
 * the device paths that stay k <= 1 refuse k = 2."""
 import json
 
@@ -47,10 +51,11 @@ def _check_apply_and_solves(vemd, s, op):
     y32 = s.apply(x, vemd.FP32)
     assert np.abs(y32[fi] - y_ref[fi]).max() <= 2e-5 * scale
     xo = op.solve_direct()
-    x64, info = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12))
-    assert np.linalg.norm(x64 - xo) <= 1e-10 * np.linalg.norm(xo), info
-    xm, infom = s.solve(vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-10))
-    assert np.linalg.norm(xm - xo) <= 1e-8 * np.linalg.norm(xo), infom
+    for pc in (vemd.PC_MULTIGRID, vemd.PC_JACOBI):
+        x64, info = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-12, precond=pc))
+        assert np.linalg.norm(x64 - xo) <= 1e-10 * np.linalg.norm(xo), (pc, info)
+        xm, infom = s.solve(vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-10, precond=pc))
+        assert np.linalg.norm(xm - xo) <= 1e-8 * np.linalg.norm(xo), (pc, infom)
 
 
 @pytest.mark.parametrize("mesh,bc", [("square_4x4.msh", 0), ("square_4x4.msh", 1), ("square_8x8.msh", 0)])
@@ -72,6 +77,23 @@ def test_k2_curved_disk_with_permeability(vemd, oracle):
     om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
     op = oracle.Problem(om, k=2, map_kind=oracle.MAP_ARCS, sol=oracle.SOL_SINSIN, bc=oracle.BC_DIRICHLET_P, theta=th)
     _check_apply_and_solves(vemd, s, op)
+
+
+def test_k2_multigrid_iterations(vemd):
+    its = {}
+    for n in (2000, 8000):
+        print("k=2 iterations, disk", n, flush=True)
+        m = vemd.generate_mesh(n, lloyd_iters=20, seed=95, min_edge_frac=0.05)
+        s = vemd.Solver(m, k=2, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINSIN, bc=vemd.BC_DIRICHLET_P,
+                        theta=np.random.default_rng(96).standard_normal(m.num_cells), device=0)
+        for pc in (vemd.PC_MULTIGRID, vemd.PC_JACOBI):
+            _, info = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-8, precond=pc))
+            assert info["rel_residual"] <= 1e-8
+            its[n, pc] = info["iterations"]
+    print(its)
+    mg, jac = vemd.PC_MULTIGRID, vemd.PC_JACOBI
+    assert its[8000, mg] <= 1.5 * its[2000, mg], its
+    assert its[8000, mg] <= 0.25 * its[8000, jac], its
 
 
 def test_k2_refusals(vemd):

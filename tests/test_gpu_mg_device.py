@@ -10,8 +10,8 @@ order and no fused multiply-adds.  Checks:
 * VEMD_MG_VERIFY=1 rebuilds every value on the host as well and requires each
   one (operators, diagonals, power-iteration vectors, spectral bounds, P, R,
   R A, the rank-one weight, every entry of the coarsest inverse) to be
-  bitwise equal: k = 0 and 1, Dirichlet pressure on a curved annulus and the
-  reference's mean-zero problem on a reference square, three successive
+  bitwise equal: k = 0, 1 and 2, Dirichlet pressure on a curved annulus and
+  the reference's mean-zero problem on a reference square, three successive
   theta fields each;
 * solves after a device rebuild equal, bitwise, the solves of a twin solver
   whose hierarchy was rebuilt on the host (VEMD_MG_DEVICE_REBUILD=0), FP64
@@ -31,7 +31,7 @@ def _annulus(vemd, n=3000, seed=41):
                               min_edge_frac=0.05)
 
 
-CASES = [("annulus", 1, 1), ("annulus", 0, 1), ("square", 1, 0), ("square", 0, 0)]
+CASES = [("annulus", 1, 1), ("annulus", 0, 1), ("square", 1, 0), ("square", 0, 0), ("annulus", 2, 1), ("square", 2, 0)]
 
 
 def _case(vemd, name, k, bc):
@@ -58,7 +58,10 @@ def test_device_rebuild_is_bitwise_the_host_build(vemd, monkeypatch, name, k, bc
     assert info["rel_residual"] <= 1e-10
 
 
-@pytest.mark.parametrize("name,k,bc", CASES[:3], ids=[f"{c[0]}-k{c[1]}-bc{c[2]}" for c in CASES[:3]])
+SOLVE_CASES = CASES[:3] + CASES[4:5]
+
+
+@pytest.mark.parametrize("name,k,bc", SOLVE_CASES, ids=[f"{c[0]}-k{c[1]}-bc{c[2]}" for c in SOLVE_CASES])
 def test_solves_after_device_and_host_rebuilds_are_identical(vemd, monkeypatch, name, k, bc):
     print("device vs host rebuild, solves:", name, "k", k, "bc", bc, flush=True)
     m, kw = _case(vemd, name, k, bc)

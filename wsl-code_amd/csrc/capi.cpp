@@ -57,9 +57,8 @@ void Solver::refresh_theta_dependent(bool host_amg) {
     rhs_changed = rhs_ref.empty() || dplan->essential;
     if (rhs_changed) build_rhs_ref(*mesh, *setup, rhs_ref);
     auto t1 = std::chrono::steady_clock::now();
-    // the multigrid cycle is built for k = 0 and 1 (q = 1 or 3 pressure
-    // moments per cell); k = 2 keeps the diagonal preconditioner
-    if (setup->sh.k <= 1 && host_amg) build_amg();
+    // the multigrid cycle: k = 0, 1 and 2 (q = 1, 3 or 6 pressure moments per cell)
+    if (setup->sh.k <= 2 && host_amg) build_amg();
     if (std::getenv("VEMD_VERBOSE"))
         std::fprintf(stderr, "[vemd] theta-dependent host data %.3fs (scales, diagonal, rhs %.3fs)\n",
                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),

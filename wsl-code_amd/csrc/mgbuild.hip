@@ -688,8 +688,9 @@ bool mg_build_run(MgBuild& B, Solver& s, const double* minv_dev, AmgDev<double>&
     // level 0: the Schur approximation for the new flux diagonal
     {
         MgLevelBuild& b = B.lv[0];
-        auto schur = B.q == 3 ? k_schur<3> : k_schur<1>;
-        if (B.q != 1 && B.q != 3) throw Error(VEMD_ERR_INTERNAL, "device multigrid rebuild: 1 or 3 pressure moments");
+        auto schur = B.q == 6 ? k_schur<6> : B.q == 3 ? k_schur<3> : k_schur<1>;
+        if (B.q != 1 && B.q != 3 && B.q != 6)
+            throw Error(VEMD_ERR_INTERNAL, "device multigrid rebuild: 1, 3 or 6 pressure moments");
         hipLaunchKernelGGL(schur, grid1(B.cpad), dim3(256), 0, st, B.cpad, B.cell_of_gpos, (const int64_t*)b.A.rp,
                            b.Av, (const int*)B.nbk, (const int*)B.self_slot, (const int*)B.n1, (const int64_t*)B.v0,
                            (const int64_t*)B.w0, (const int64_t*)B.tflux, (const int*)B.oslot, (const int64_t*)B.ow,
@@ -708,8 +709,8 @@ bool mg_build_run(MgBuild& B, Solver& s, const double* minv_dev, AmgDev<double>&
         AmgLevelDev<double>& d64 = D64.lv[l];
         fill(n, b.A.rp, b.A.ci, b.Av, map_dev(D64, d64.vA), 0, nullptr, nullptr, 1.0, (double*)d64.A.val, st);
         const int bs = d64.A.bs;
-        if (bs != 1 && bs != 3) throw Error(VEMD_ERR_INTERNAL, "device multigrid rebuild: block size");
-        auto pspmv = bs == 3 ? k_pspmv<3> : k_pspmv<1>;
+        if (bs != 1 && bs != 3 && bs != 6) throw Error(VEMD_ERR_INTERNAL, "device multigrid rebuild: block size");
+        auto pspmv = bs == 6 ? k_pspmv<6> : bs == 3 ? k_pspmv<3> : k_pspmv<1>;
         // spectral bound: 25 power-iteration steps from the previous iterate
         hipLaunchKernelGGL(k_preset, dim3(1), dim3(1), 0, st, sc);
         const int64_t G = (n + 63) / 64;

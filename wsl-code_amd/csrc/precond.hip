@@ -1061,6 +1061,9 @@ static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const
     switch (L.A.bs) {
         case 1: return level_cycle<T, 1, NC>(D, l, st, ms, pre_done, ko);
         case 3: return level_cycle<T, 3, NC>(D, l, st, ms, pre_done, ko);
+        case 6:  // k = 2 (single column only: the multi-column path is k <= 1)
+            if constexpr (NC == 1) return level_cycle<T, 6, NC>(D, l, st, ms, pre_done, ko);
+            [[fallthrough]];
         default: throw Error(VEMD_ERR_INTERNAL, "unsupported multigrid block size");
     }
 }
