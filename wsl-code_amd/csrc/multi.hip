@@ -133,29 +133,36 @@ struct MrC {
 };
 
 // Loads of one cell (gp < 0: none): its packed operator (float4 per lane,
-// two rounds), its scale and the lane's row slot (lanes < M).
+// two rounds), its scale and, for the lanes of edge rows, the raw edge index
+// (the slot of other rows needs no load).  Every lane loads (indices clamped
+// into the cell's data) and nothing loaded is used here: the slot is formed
+// when the cell is staged (mr_slot), so that a wait for these loads does not
+// also wait for the loads of the cells behind them.
 template <int K, int NE>
 __device__ __forceinline__ void mr_loads(const MrArgs& a, const MrGroup& g, int64_t gp, int l, float4& o0, float4& o1,
-                                          uint32_t& slot, float& s) {
+                                          uint32_t& raw, float& s) {
     using C = MrC<K, NE>;
-    const float4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-    o0 = o1 = z;
-    slot = 0;
-    s = 0.0f;
-    if (gp < 0) return;
-    const int64_t cg = gp - g.gpos0;
+    // no cell (gp < 0): the group's first cell is loaded instead and never
+    // multiplied, so that every step issues the same loads (the waits for a
+    // cell's loads then count exactly the loads issued after them)
+    const int64_t cg = gp < 0 ? 0 : gp - g.gpos0;
     const float4* src = reinterpret_cast<const float4*>(a.op + g.gop) + cg * C::NV4;
-    if (l < C::NV4) o0 = src[l];
-    if (C::NV4 > 64 && l + 64 < C::NV4) o1 = src[l + 64];
-    s = a.scale[gp];
-    if (l < C::NED) {
-        const uint32_t id = a.idx[g.i_off + ((cg >> 6) * NE + l % NE) * 64 + (cg & 63)];
-        slot = (id & SIDE1) | ((id & 0x7fffffffu) + (uint32_t)(l / NE));
-    } else if (l < C::N) {
-        slot = (uint32_t)(a.cbase + (l - C::NED) * a.cpad + gp);
-    } else if (l < C::M) {
-        slot = (uint32_t)(a.cbase + (C::NI + l - C::N) * a.cpad + gp);
-    }
+    o0 = src[l < C::NV4 ? l : C::NV4 - 1];
+    o1 = C::NV4 > 64 ? src[l + 64 < C::NV4 ? l + 64 : C::NV4 - 1] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    s = a.scale[g.gpos0 + cg];
+    const int le = l < C::NED ? l : 0;
+    raw = a.idx[g.i_off + ((cg >> 6) * NE + le % NE) * 64 + (cg & 63)];
+}
+
+// The row slot of lane l (< M) of the cell at store position gp from the raw
+// edge index of mr_loads.
+template <int K, int NE>
+__device__ __forceinline__ uint32_t mr_slot(const MrArgs& a, int64_t gp, int l, uint32_t raw) {
+    using C = MrC<K, NE>;
+    if (l < C::NED) return (raw & SIDE1) | ((raw & 0x7fffffffu) + (uint32_t)(l / NE));
+    if (l < C::N) return (uint32_t)(a.cbase + (l - C::NED) * a.cpad + gp);
+    if (l < C::M) return (uint32_t)(a.cbase + (C::NI + l - C::N) * a.cpad + gp);
+    return 0;
 }
 
 // The cells of one group at store positions [p0, p1) (all real), pipelined as
@@ -167,6 +174,8 @@ __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_
                                         float* lkx, double& dot) {
     using C = MrC<K, NE>;
     const int hi = l >> 5, j = l & 31;
+    float* const Y0 = a.Y;   // (the per-row choice between them then selects a value,
+    float* const Y1 = a.Y1;  // not the address of an argument)
     // this lane's entry of the cell block at step kk: staged operator offset,
     // and whether it belongs to s A (else -W or zero)
     // two 16-bit offsets per register (the operator has at most 348 entries)
@@ -189,24 +198,23 @@ __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_
         }
         off2[kk >> 1] |= (uint32_t)o << (16 * (kk & 1));
     }
-    auto stage = [&](const float4& o0, const float4& o1, uint32_t slot, int b, float (&B)[C::KS])
+    auto stage = [&](const float4& o0, const float4& o1, uint32_t raw, int64_t gp, int b, float (&B)[C::KS])
         __attribute__((always_inline)) {
         float4* d = reinterpret_cast<float4*>(lop[b]);
         if (l < C::NV4) d[l] = o0;
         if (C::NV4 > 64 && l + 64 < C::NV4) d[l + 64] = o1;
         if (l == 0) lop[b][C::ZERO] = 0.0f;
-        if (l < C::M) lsl[b][l] = slot;
+        if (l < C::M) lsl[b][l] = mr_slot<K, NE>(a, gp < 0 ? (int64_t)g.gpos0 : gp, l, raw);
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         __builtin_amdgcn_wave_barrier();
+        // every lane gathers: a row the block does not have (the odd row
+        // count's last step, upper half-wave) reads row 0 instead, which
+        // meets only zero entries of the staged block (x is finite)
 #pragma unroll
         for (int kk = 0; kk < C::KS; ++kk) {
             const int c = 2 * kk + hi;
-            if (c < C::M) {
-                const uint32_t sl = lsl[b][c];
-                B[kk] = Xj[(int64_t)(sl & 0x7fffffffu) * NR + (l & 31)];
-            } else {
-                B[kk] = 0.0f;  // the odd row count's last step, upper half-wave
-            }
+            const uint32_t sl = lsl[b][c < C::M ? c : 0];
+            B[kk] = Xj[(int64_t)(sl & 0x7fffffffu) * NR + (l & 31)];
         }
     };
     auto multiply = [&](int b, float s, const float (&B)[C::KS]) __attribute__((always_inline)) {
@@ -219,7 +227,7 @@ __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_
         }
         if (j < C::M) {
             const uint32_t sl = lsl[b][j];
-            float* __restrict__ dst = (sl & SIDE1) ? a.Y1 : a.Y;
+            float* __restrict__ dst = (sl & SIDE1) ? Y1 : Y0;
             float4* row = reinterpret_cast<float4*>(dst + (int64_t)(sl & 0x7fffffffu) * NR);
 #pragma unroll
             for (int q = 0; q < 4; ++q) row[2 * q + hi] = float4{acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
@@ -247,7 +255,7 @@ __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_
     // copied (a copy would wait for the loads still filling them).
     struct Set {
         float4 o0, o1;
-        uint32_t sl;
+        uint32_t sl;  // raw edge index (edge-row lanes)
         float s;
         int64_t gp;
     };
@@ -260,9 +268,9 @@ __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_
     auto nxt = [&](int64_t gp) { return gp >= 0 && gp + 1 < p1 ? gp + 1 : (int64_t)-1; };
     load(S0, p0);
     load(S1, nxt(p0));
-    stage(S0.o0, S0.o1, S0.sl, 0, B0);
+    stage(S0.o0, S0.o1, S0.sl, S0.gp, 0, B0);
     load(S2, nxt(S1.gp));
-    if (S1.gp >= 0) stage(S1.o0, S1.o1, S1.sl, 1, B1);
+    stage(S1.o0, S1.o1, S1.sl, S1.gp, 1, B1);  // (no cell: staged and never multiplied)
     // step: multiply cell c (set c, buffer bc), stage cell n2 (two behind it)
     // into its buffer with its rows gathered, and load the cell after n2 into
     // c's registers
@@ -270,7 +278,7 @@ __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_
         __attribute__((always_inline)) {
         const float sc = c.s;
         load(c, nxt(n2.gp));
-        if (n2.gp >= 0) stage(n2.o0, n2.o1, n2.sl, bn2, Bn2);
+        stage(n2.o0, n2.o1, n2.sl, n2.gp, bn2, Bn2);
         multiply(bc, sc, Bc);
         __builtin_amdgcn_wave_barrier();  // buffer bc is free for the cell loaded now
     };
