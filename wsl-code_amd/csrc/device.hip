@@ -710,6 +710,131 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
     block_pair(d1, d2, a.p1 + blockIdx.x, a.p2 + blockIdx.x);
 }
 
+// The same sweep in FP32 for two moments per edge (k = 1), by pairs of
+// entries: thread t of a chunk takes entries 2p, 2p + 1 for p = t + 256 u,
+// so every vector access is an 8-byte load or store per lane (the FP32 sweep
+// with one entry per lane issues twice the instructions per byte of the FP64
+// one), and in the edge chunks a pair is one edge's two moments: one side
+// table entry and one 8-byte gather from each cell side.  Chunk starts are
+// even; a chunk of odd length ends in a half pair.
+__global__ __launch_bounds__(256) void k_update_pairs(UpdArgs<float> a) {
+    constexpr int U = CHUNK / 512;
+    const VecChunk ch = a.chunks[blockIdx.x];
+    const MState* st = a.st + ch.inst;
+    if (st->done) return;
+    const int it = st->itn;
+    float* __restrict__ r2 = a.R[it & 1];
+    float* __restrict__ r1 = a.R[(it + 1) & 1];
+    float* __restrict__ zo = a.Z[(it + 1) & 1];
+    float* __restrict__ wn = a.WB[(it + 1) & 1];
+    const float* __restrict__ wp = a.WB[it & 1];
+    const float caz = (float)st->c_az, cr1 = (float)st->c_r1, cr2 = (float)st->c_r2;
+    const bool pend = st->pending;
+    const float uv = (float)st->u_v, uw1 = (float)st->u_w1, uw2 = (float)st->u_w2, ux = (float)st->u_x;
+    const bool cyc = a.amg && ch.pslot;
+    const int pr = ch.pslot - 1;
+    const int64_t end = ch.start + ch.len;
+    int64_t ii[U];
+    bool ok0[U], ok1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t i = ch.start + 2 * ((int64_t)threadIdx.x + 256 * u);
+        ok0[u] = i < end;
+        ok1[u] = i + 1 < end;
+        ii[u] = ok1[u] ? i : ch.start;  // whole pairs in bounds; a half pair is done entry by entry below
+    }
+    auto ld2 = [](const float* p, int64_t i) { return *reinterpret_cast<const float2*>(p + i); };
+    float2 az[U], r1v[U], r2v[U];
+    if (ch.start < a.edof) {  // a chunk of edge dofs: pair = the edge's two moments
+        int2 sd[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) sd[u] = reinterpret_cast<const int2*>(a.hside)[ii[u] >> 1];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float2 h0 = ld2(a.H, (int64_t)sd[u].x * 2), h1 = ld2(a.H, (int64_t)sd[u].y * 2);
+            az[u] = float2{h0.x + h1.x, h0.y + h1.y};
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) az[u] = ld2(a.Az, ii[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        r1v[u] = ld2(r1, ii[u]);
+        r2v[u] = ld2(r2, ii[u]);
+    }
+    float2 zv[U], wnv[U], wpv[U], xv[U], mv[U];
+    if (pend) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            zv[u] = ld2(zo, ii[u]);
+            wnv[u] = ld2(wn, ii[u]);
+            wpv[u] = ld2(wp, ii[u]);
+            xv[u] = ld2(a.x, ii[u]);
+        }
+    }
+    float sv[U][2];
+    if (!cyc) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) mv[u] = ld2(a.minv, ii[u]);
+    } else if (a.bd0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            sv[u][0] = a.bsc[cyc_index(a, ii[u], pr)];
+            sv[u][1] = a.bsc[cyc_index(a, ii[u] + 1, pr)];
+        }
+    }
+    double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!ok0[u]) continue;
+        if (!ok1[u]) {
+            // the chunk's last entry without a partner (only the final chunk
+            // of multiplier and trash slots, never an edge or pressure chunk)
+            const int64_t i = ch.start + 2 * ((int64_t)threadIdx.x + 256 * u);
+            const float y = caz * a.Az[i] - cr1 * r1[i] - cr2 * r2[i];
+            if (pend) {
+                const float w = uv * zo[i] - uw1 * wn[i] - uw2 * wp[i];
+                wn[i] = w;
+                a.x[i] += ux * w;
+            }
+            r1[i] = y;
+            const float zn = a.minv[i] * y;
+            zo[i] = zn;
+            d1 += (double)y * (double)zn;
+            d2 += (double)zn * (double)r2[i];
+            continue;
+        }
+        const int64_t i = ii[u];
+        const float y0 = caz * az[u].x - cr1 * r1v[u].x - cr2 * r2v[u].x;
+        const float y1 = caz * az[u].y - cr1 * r1v[u].y - cr2 * r2v[u].y;
+        if (pend) {
+            const float w0 = uv * zv[u].x - uw1 * wnv[u].x - uw2 * wpv[u].x;
+            const float w1 = uv * zv[u].y - uw1 * wnv[u].y - uw2 * wpv[u].y;
+            *reinterpret_cast<float2*>(wn + i) = float2{w0, w1};
+            *reinterpret_cast<float2*>(a.x + i) = float2{xv[u].x + ux * w0, xv[u].y + ux * w1};
+        }
+        *reinterpret_cast<float2*>(r1 + i) = float2{y0, y1};
+        if (cyc) {
+            const int64_t j0 = cyc_index(a, i, pr), j1 = cyc_index(a, i + 1, pr);
+            a.bp[j0] = y0;  // preconditioned by the multigrid cycle, which also forms the dots
+            a.bp[j1] = y1;
+            if (a.bd0) {
+                a.bd0[j0] = a.c0 * sv[u][0] * y0;
+                a.bd0[j1] = a.c0 * sv[u][1] * y1;
+            }
+        } else {
+            const float z0 = mv[u].x * y0, z1 = mv[u].y * y1;
+            *reinterpret_cast<float2*>(zo + i) = float2{z0, z1};
+            d1 += (double)y0 * (double)z0;
+            d2 += (double)z0 * (double)r2v[u].x;
+            d1 += (double)y1 * (double)z1;
+            d2 += (double)z1 * (double)r2v[u].y;
+        }
+    }
+    block_pair(d1, d2, a.p1 + blockIdx.x, a.p2 + blockIdx.x);
+}
+
 // Pressure chunks after the multigrid cycle: z_p = cycle output, with the
 // partials of y.z and z.r2 (init: z of the right-hand side and b.z).
 template <typename T>
@@ -1590,6 +1715,12 @@ static KOut<T> kout(Solver& s, DevKrylov<T>& k, bool init) {
     return o;
 }
 
+// The FP32 sweep by entry pairs (k = 1) unless VEMD_PAIR_SWEEP=0.
+static bool pair_sweep() {
+    const char* e = std::getenv("VEMD_PAIR_SWEEP");
+    return !(e && e[0] == '0');
+}
+
 // One MINRES iteration of every unfinished instance: apply, alfa,
 // Lanczos/precondition/update, [pressure cycle], Givens.
 template <typename T>
@@ -1601,7 +1732,14 @@ static void enqueue_iteration(Solver& s, DevKrylov<T>& k, bool amg) {
     hipLaunchKernelGGL((k_finA<T>), dim3(d.ninst), dim3(FIN), 0, d.stream, d.st, d.pdot, d.plam, d.inst_tile0, k.Az,
                        k.Z[0], k.Z[1], L.lam ? L.lam_idx : (int64_t)-1);
     UpdArgs<T> u = upd_args(s, k, amg);
-    hipLaunchKernelGGL((k_update<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
+    if constexpr (std::is_same<T, float>::value) {
+        if (L.k == 1 && pair_sweep())
+            hipLaunchKernelGGL(k_update_pairs, dim3(d.nchunks), dim3(256), 0, d.stream, u);
+        else
+            hipLaunchKernelGGL((k_update<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
+    } else {
+        hipLaunchKernelGGL((k_update<T>), dim3(d.nchunks), dim3(256), 0, d.stream, u);
+    }
     const int* q0 = d.inst_chunk0;
     if (amg) {
         const AmgDev<T>& M = *amg_of<T>(d);
