@@ -277,3 +277,34 @@ def test_vector_range_limit_fails_loudly(vemd, monkeypatch):
     assert vemd.Solver(m, k=1).nvec * 8 == nbytes  # exactly at the limit
     monkeypatch.setenv("VEMD_MAX_VECTOR_BYTES", str(1 << 40))  # the variable cannot raise the limit
     assert vemd.Solver(m, k=1).nvec * 8 == nbytes
+
+
+def test_meanzero_batch_has_one_multiplier_per_instance(vemd):
+    """A batch under the reference's condition (essential normal flux and a
+    mean-zero pressure multiplier) carries one multiplier per instance, the
+    last unknowns in instance order: the sizes, the right-hand side (zero
+    multiplier rows), the split into instance vectors and the reference <->
+    internal maps agree with the instances solved alone."""
+    import ctypes
+    parts = [vemd.Mesh.read(golden("square_2x2.msh")), vemd.Mesh.read(golden("square_4x4.msh")),
+             vemd.Mesh.read(golden("square_2x2.msh"))]
+    u = vemd.Mesh.concat(parts)
+    kw = dict(k=1, bc=vemd.BC_FLUX_MEANZERO)
+    sb = vemd.Solver(u, **kw)
+    singles = [vemd.Solver(m, **kw) for m in parts]
+    assert sb.ntot == sb.nflux + sb.npres + len(parts)
+    assert sb.ntot == sum(s.ntot for s in singles)
+    for bi, s in zip(sb.instance_vectors(sb.rhs()), singles):
+        assert np.array_equal(bi, s.rhs())
+        assert bi[-1] == 0.0
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(sb.ntot)
+    dp = ctypes.POINTER(ctypes.c_double)
+    xi = np.zeros(sb.nvec)
+    xr = np.zeros(sb.ntot)
+    vemd._lib.check(vemd.lib().vemd_to_internal(sb.h, x.ctypes.data_as(dp), xi.ctypes.data_as(dp), 0))
+    vemd._lib.check(vemd.lib().vemd_to_reference(sb.h, xi.ctypes.data_as(dp), xr.ctypes.data_as(dp), 0))
+    assert np.array_equal(x, xr)
+    # the multipliers sit in distinct internal slots
+    lam = xi[np.isin(xi, x[-len(parts):])]
+    assert len(lam) == len(parts)

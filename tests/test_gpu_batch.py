@@ -2,7 +2,8 @@
 one operator store and one captured Krylov loop, each with its own
 recurrences, stopping test and refinement.  Every instance must agree with the
 oracle's direct solve of that instance's mesh (FP64 to 1e-10; mixed to 1e-5
-with error functionals within 1e-9) and with a solve of that instance alone."""
+with error functionals within 1e-9) and with a solve of that instance alone,
+under Dirichlet pressure and under the reference's mean-zero condition."""
 import numpy as np
 import pytest
 
@@ -95,3 +96,40 @@ def test_batch_of_one_equals_plain_solve(vemd, batch):
     x2, i2 = s.solve(vemd.SolveOptions(precision=vemd.FP64, rtol=1e-10))
     assert i1["iterations"] == i2["iterations"] and np.array_equal(x1, x2)  # deterministic
     assert s.batch_info()[0]["iterations"] == i2["iterations"]
+
+
+@pytest.mark.parametrize("pc", ["multigrid", "jacobi"])
+def test_meanzero_batch_matches_oracle(vemd, oracle, pc):
+    """The reference's own condition in a batch (essential normal flux and one
+    mean-zero pressure multiplier per instance; the multigrid cycle then
+    carries one rank-one term per instance): every instance of a batch of
+    reference squares and curved disks and annuli, each with its own theta,
+    equals the oracle's direct solve of that instance (FP64 to 1e-10, mixed to
+    1e-7), multiplier included."""
+    from conftest import golden
+    print("mean-zero batch,", pc, flush=True)
+    parts = [vemd.Mesh.read(golden("square_8x8.msh")),
+             vemd.generate_mesh(1500, domain=vemd.DOMAIN_DISK, lloyd_iters=20, seed=71, min_edge_frac=0.05),
+             vemd.Mesh.read(golden("square_16x16.msh")),
+             vemd.generate_mesh(2500, domain=vemd.DOMAIN_ANNULUS, r_inner=0.5, lloyd_iters=20, seed=72,
+                                min_edge_frac=0.05)]
+    rng = np.random.default_rng(73)
+    thetas = [rng.standard_normal(m.num_cells) for m in parts]
+    kw = dict(k=1, map_kind=vemd.MAP_ARCS, solution=vemd.SOL_SINCOS, bc=vemd.BC_FLUX_MEANZERO)
+    sb = vemd.Solver(vemd.Mesh.concat(parts), theta=np.concatenate(thetas), device=0, **kw)
+    ref = []
+    for m, th in zip(parts, thetas):
+        a = m.arrays()
+        om = oracle.Mesh.from_arrays(a["xy"], a["cell_off"], a["cell_nodes"], a["arc_nodes"], a["arc_radius"])
+        ref.append(oracle.Problem(om, k=1, map_kind=oracle.MAP_ARCS, sol=oracle.SOL_SINCOS,
+                                  bc=oracle.BC_FLUX_MEANZERO, theta=th).solve_direct())
+    p = vemd.PC_MULTIGRID if pc == "multigrid" else vemd.PC_JACOBI
+    for prec, rtol, tol in ((vemd.FP64, 1e-12, 1e-10), (vemd.MIXED, 1e-10, 1e-7)):
+        xb, info = sb.solve(vemd.SolveOptions(precision=prec, rtol=rtol, precond=p))
+        per = sb.batch_info()
+        print("  precision", prec, info, [r["iterations"] for r in per], flush=True)
+        assert all(r["rel_residual"] <= rtol for r in per), per
+        for xi, xo in zip(sb.instance_vectors(xb), ref):
+            assert len(xi) == len(xo)
+            assert np.linalg.norm(xi - xo) <= tol * np.linalg.norm(xo)
+            assert abs(xi[-1] - xo[-1]) <= tol * np.linalg.norm(xo)

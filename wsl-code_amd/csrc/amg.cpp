@@ -394,14 +394,16 @@ void dense_inverse(std::vector<double>& a, int64_t n) {
 
 Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const std::vector<double>& b00,
               const std::vector<int>& node_order0, const AmgOptions& o, const std::vector<double>* l0, double rho,
-              const Amg* reuse) {
+              const Amg* reuse, const std::vector<int>* row_inst) {
     Amg H;
     H.opt = o;
     std::vector<double> lvec;
+    std::vector<int> rinst;  // instance of each row of the current level (empty: one rank-one term)
     if (l0) {
         if ((int64_t)l0->size() != A.nrows) throw Error(VEMD_ERR_INTERNAL, "rank-one term of the wrong length");
         lvec = *l0;
         H.rho = rho;
+        if (row_inst) rinst = *row_inst;
     }
     std::vector<int> nor = node_of_row0, order = node_order0;
     std::vector<double> b0 = b00;
@@ -525,6 +527,17 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
             csr_spmv(R, lvec.data(), lc.data());
             lvec.swap(lc);
         }
+        if (!rinst.empty()) {
+            // an aggregate's instance: that of its rows (the tentative prolongator)
+            std::vector<int> ci(Pt.ncols, -1);
+            for (int64_t r = 0; r < Pt.nrows; ++r)
+                for (int64_t p = Pt.rp[r]; p < Pt.rp[r + 1]; ++p) {
+                    int& t = ci[Pt.ci[p]];
+                    if (t >= 0 && t != rinst[r]) throw Error(VEMD_ERR_INTERNAL, "an aggregate joins two instances");
+                    t = rinst[r];
+                }
+            rinst.swap(ci);
+        }
         L.A = std::move(A);
         L.P = std::move(P);
         L.R = std::move(R);
@@ -551,7 +564,9 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         for (int64_t p = C.A.rp[i]; p < C.A.rp[i + 1]; ++p) H.coarse_inv[i * H.ncoarse + C.A.ci[p]] = C.A.v[p];
     if (!lvec.empty())
         for (int64_t i = 0; i < H.ncoarse; ++i)
-            for (int64_t j = 0; j < H.ncoarse; ++j) H.coarse_inv[i * H.ncoarse + j] += rho * lvec[i] * lvec[j];
+            for (int64_t j = 0; j < H.ncoarse; ++j)
+                if (rinst.empty() || (rinst[i] >= 0 && rinst[i] == rinst[j]))
+                    H.coarse_inv[i * H.ncoarse + j] += rho * lvec[i] * lvec[j];
     dense_inverse(H.coarse_inv, H.ncoarse);
     return H;
 }

@@ -78,9 +78,12 @@ Csr schur_values(const SchurPlan& P, const Mesh& m, const Setup& S, const Layout
 // operator of the same sparsity) are kept and only the values are rebuilt:
 // the smoothed prolongators, Galerkin operators, spectral bounds and the
 // coarsest inverse (node_of_row, b0 and node_order are then unused).
+// With row_inst (one instance per row of A0, -1 for none; aggregates never
+// join rows of different instances) the rank-one term is one per instance:
+// rho l_c l_c^T is added only where both coarse rows belong to one instance.
 Amg amg_build(Csr A0, const std::vector<int>& node_of_row, int nnodes, const std::vector<double>& b0,
               const std::vector<int>& node_order, const AmgOptions& o, const std::vector<double>* l0 = nullptr,
-              double rho = 0.0, const Amg* reuse = nullptr);
+              double rho = 0.0, const Amg* reuse = nullptr, const std::vector<int>* row_inst = nullptr);
 
 // In-place inverse of a dense row-major n x n matrix (Gauss-Jordan with
 // partial pivoting; the coarsest operator).

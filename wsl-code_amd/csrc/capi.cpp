@@ -73,7 +73,7 @@ void Solver::build_amg() {
     const char* ra = std::getenv("VEMD_MG_REAGGREGATE");
     if (ra && ra[0] == '1') prev.reset();
     amg_values_only = false;
-    amg_lam_minv = 0.0;
+    amg_lam_minv.clear();
     auto t0 = std::chrono::steady_clock::now();
     // tuning overrides: smoothing degrees, Chebyshev interval ratio, strength threshold
     if (const char* e = std::getenv("VEMD_MG_DEGREE0")) amg_opt.degree0 = std::max(1, std::atoi(e));
@@ -102,42 +102,74 @@ void Solver::build_amg() {
     // a fraction c of S0's mean diagonal: then the coarse correction, not the
     // smoothers, sets the cycle's response to n0 (as the exact inverse does),
     // on any mesh size, and the coarsest matrix stays well conditioned.
+    // A batch has one multiplier per instance and S0 one null vector per
+    // instance; each gets its own rank-one term (rho_i from the instance's own
+    // rows; aggregates never cross instances, so the terms stay in the
+    // instances' blocks of the coarsest matrix).
     std::vector<double> l0;
+    std::vector<int> row_inst;
     double rho = 0.0;
     if (setup->desc.bc == VEMD_BC_FLUX_MEANZERO) {
+        const int ni = L.ninst;
         l0.assign(n, 0.0);
-        double sd = 0.0, ln0 = 0.0;
-        int64_t nlive = 0;
-        for (int64_t g = 0; g < L.cpad; ++g) {
-            const int c = L.cell_of_gpos[g];
-            if (c < 0) continue;
-            ++nlive;
-            for (int r = 0; r < L.q; ++r) {
-                const int64_t i = g * L.q + r;
-                l0[i] = setup->L[(size_t)c * L.q + r];
-                if (r == 0) ln0 += l0[i];
+        std::vector<double> sd(ni, 0.0), ln0(ni, 0.0);
+        std::vector<int64_t> nlive(ni, 0);
+        for (int i = 0; i < ni; ++i)
+            for (int64_t g = L.inst_gpos0[i]; g < L.inst_gpos0[i + 1]; ++g) {
+                const int c = L.cell_of_gpos[g];
+                if (c < 0) continue;
+                ++nlive[i];
+                for (int r = 0; r < L.q; ++r) {
+                    const int64_t row = g * L.q + r;
+                    l0[row] = setup->L[(size_t)c * L.q + r];
+                    if (r == 0) ln0[i] += l0[row];
+                }
+            }
+        // S0's diagonal over each instance's live rows, in 256 blocks of rows
+        // summed in order (one instance: all rows, the order of the device
+        // rebuild)
+        for (int i = 0; i < ni; ++i) {
+            const int64_t r0 = ni == 1 ? 0 : (int64_t)L.q * L.inst_gpos0[i];
+            const int64_t r1 = ni == 1 ? n : (int64_t)L.q * L.inst_gpos0[i + 1];
+            const int64_t bl = (r1 - r0 + 255) / 256;
+            for (int k = 0; k < 256; ++k) {
+                double sk = 0.0;
+                for (int64_t row = r0 + k * bl; row < std::min(r1, r0 + (k + 1) * bl); ++row) {
+                    if (L.cell_of_gpos[row / L.q] < 0) continue;
+                    for (int64_t p = S0.rp[row]; p < S0.rp[row + 1]; ++p)
+                        if (S0.ci[p] == row) sk += S0.v[p];
+                }
+                sd[i] += sk;
             }
         }
-        // S0's diagonal over the live rows, in 256 blocks of rows summed in
-        // order (the order of the device rebuild)
-        const int64_t bl = (n + 255) / 256;
-        for (int k = 0; k < 256; ++k) {
-            double s = 0.0;
-            for (int64_t i = k * bl; i < std::min(n, (k + 1) * bl); ++i) {
-                if (L.cell_of_gpos[i / L.q] < 0) continue;
-                for (int64_t p = S0.rp[i]; p < S0.rp[i + 1]; ++p)
-                    if (S0.ci[p] == i) s += S0.v[p];
-            }
-            sd += s;
-        }
-        if (!(sd > 0.0) || !(ln0 != 0.0)) throw Error(VEMD_ERR_INTERNAL, "degenerate pressure Schur approximation");
         double c = 0.03;  // measured: 203, 250, 262 FP64 iterations at 4k, 64k, 256k disk cells
         if (const char* e = std::getenv("VEMD_MG_RHO")) c = std::max(1e-12, std::atof(e));
-        rho = c * (sd / ((double)nlive * L.q)) * (double)nlive / (ln0 * ln0);
-        amg_lam_minv = 1.0 / rho;
+        std::vector<double> rhoi(ni);
+        amg_lam_minv.assign(ni, 0.0);
+        for (int i = 0; i < ni; ++i) {
+            if (!(sd[i] > 0.0) || !(ln0[i] != 0.0))
+                throw Error(VEMD_ERR_INTERNAL, "degenerate pressure Schur approximation");
+            rhoi[i] = c * (sd[i] / ((double)nlive[i] * L.q)) * (double)nlive[i] / (ln0[i] * ln0[i]);
+            amg_lam_minv[i] = 1.0 / rhoi[i];
+        }
+        if (ni == 1) {
+            rho = rhoi[0];
+        } else {
+            // rho_i folded into the instance's rows of l0 (rho_i l l^T = (sqrt(rho_i) l)(sqrt(rho_i) l)^T)
+            rho = 1.0;
+            row_inst.assign(n, -1);
+            for (int i = 0; i < ni; ++i) {
+                const double f = std::sqrt(rhoi[i]);
+                for (int64_t row = (int64_t)L.q * L.inst_gpos0[i]; row < (int64_t)L.q * L.inst_gpos0[i + 1]; ++row) {
+                    l0[row] *= f;
+                    row_inst[row] = i;
+                }
+            }
+        }
     }
     amg.reset(new Amg(amg_build(std::move(S0), node, (int)L.cpad, b0, L.spatial_gpos, amg_opt,
-                                l0.empty() ? nullptr : &l0, rho, prev.get())));
+                                l0.empty() ? nullptr : &l0, rho, prev.get(),
+                                row_inst.empty() ? nullptr : &row_inst)));
     amg->bs0 = L.q;
     amg_values_only = prev != nullptr;
     amg_host_stale = false;

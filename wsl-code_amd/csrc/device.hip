@@ -319,7 +319,7 @@ __device__ __forceinline__ void cell_body(const ApplyArgs<T>& a, const T* __rest
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] *= s;
     if (a.lam_idx >= 0) {
-        const T xl = x[a.lam_idx];
+        const T xl = x[a.lam_idx + g.inst];  // the instance's multiplier
 #pragma unroll
         for (int r = 0; r < Q; ++r) {
             T lr = a.L[g.l_off + r * (int64_t)g.npad + lane];
@@ -448,11 +448,13 @@ __device__ double block_sum_fixed(const double* p, int n) {
     return r;
 }
 
-// Multiplier row of a plain apply: y_lambda = sum of per-tile partials.
+// Multiplier rows of a plain apply, one block per instance: y_lambda of
+// instance i = the sum of the per-tile partials of its tiles.
 template <typename T>
-__global__ void k_lam_row(const double* plam, int ntiles, T* y, int64_t lam_idx) {
-    double s = block_sum_fixed(plam, ntiles);
-    if (threadIdx.x == 0) y[lam_idx] = (T)s;
+__global__ void k_lam_row(const double* plam, const int* inst_tile0, T* y, int64_t lam_idx) {
+    const int t0 = inst_tile0[blockIdx.x];
+    double s = block_sum_fixed(plam + t0, inst_tile0[blockIdx.x + 1] - t0);
+    if (threadIdx.x == 0) y[lam_idx + blockIdx.x] = (T)s;
 }
 
 // Sums of the partials src[k][b..e) (a null array sums to zero) in a fixed
@@ -548,10 +550,10 @@ __global__ void k_finA(MState* sts, const double* pdot, const double* plam, cons
     block_sums(pdot, lam_idx >= 0 ? plam : nullptr, inst_tile0[blockIdx.x], inst_tile0[blockIdx.x + 1], zaz, yl);
     if (done) return;
     if (threadIdx.x == 0) {
-        if (lam_idx >= 0) {
+        if (lam_idx >= 0) {  // the instance's multiplier row
             const T* z = (sv.itn & 1) ? Z1 : Z0;
-            Az[lam_idx] = (T)yl;
-            zaz += (double)z[lam_idx] * yl;
+            Az[lam_idx + blockIdx.x] = (T)yl;
+            zaz += (double)z[lam_idx + blockIdx.x] * yl;
         }
         double beta = sv.beta;
         double alfa = zaz / (beta * beta) - (sv.itn > 0 ? sv.rho / sv.oldb : 0.0);
@@ -716,7 +718,8 @@ __global__ __launch_bounds__(256) void k_update(UpdArgs<T> a) {
 // with one entry per lane issues twice the instructions per byte of the FP64
 // one), and in the edge chunks a pair is one edge's two moments: one side
 // table entry and one 8-byte gather from each cell side.  Chunk starts are
-// even; a chunk of odd length ends in a half pair.
+// even except the one-entry chunk of an instance's multiplier; a chunk of odd
+// length ends in a half pair.
 __global__ __launch_bounds__(256) void k_update_pairs(UpdArgs<float> a) {
     constexpr int U = CHUNK / 512;
     const VecChunk ch = a.chunks[blockIdx.x];
@@ -741,7 +744,11 @@ __global__ __launch_bounds__(256) void k_update_pairs(UpdArgs<float> a) {
         const int64_t i = ch.start + 2 * ((int64_t)threadIdx.x + 256 * u);
         ok0[u] = i < end;
         ok1[u] = i + 1 < end;
-        ii[u] = ok1[u] ? i : ch.start;  // whole pairs in bounds; a half pair is done entry by entry below
+        // lanes without a whole pair load the chunk's first pair (unused);
+        // a one-entry chunk (an instance's multiplier: never an edge or
+        // pressure chunk, whose indexing needs an entry of the chunk) loads
+        // the vectors' first pair.  A half pair is done entry by entry below.
+        ii[u] = ok1[u] ? i : (ch.len >= 2 ? ch.start : 0);
     }
     auto ld2 = [](const float* p, int64_t i) { return *reinterpret_cast<const float2*>(p + i); };
     float2 az[U], r1v[U], r2v[U];
@@ -789,8 +796,8 @@ __global__ __launch_bounds__(256) void k_update_pairs(UpdArgs<float> a) {
     for (int u = 0; u < U; ++u) {
         if (!ok0[u]) continue;
         if (!ok1[u]) {
-            // the chunk's last entry without a partner (only the final chunk
-            // of multiplier and trash slots, never an edge or pressure chunk)
+            // the chunk's last entry without a partner (a multiplier chunk,
+            // never an edge or pressure chunk)
             const int64_t i = ch.start + 2 * ((int64_t)threadIdx.x + 256 * u);
             const float y = caz * a.Az[i] - cr1 * r1[i] - cr2 * r2[i];
             if (pend) {
@@ -1167,6 +1174,7 @@ static void* static_scratch(int device) {
 
 bool dev_mg_rebuild_possible(const Solver& s) {
     if (!s.dev || !s.amg || !s.splan || !s.dev->amg64 || !s.dev->amg32 || s.setup->sh.k > 2) return false;
+    if (s.layout.lam && s.layout.ninst > 1) return false;  // per-instance rank-one terms: host rebuild
     for (const char* v : {"VEMD_MG_DEVICE_REBUILD", "VEMD_MG_REFILL"}) {
         const char* e = std::getenv(v);
         if (e && e[0] == '0') return false;
@@ -1541,7 +1549,8 @@ static void apply_plain(Solver& s, const T* x, T* y, T* h_out = nullptr) {
         hipLaunchKernelGGL((k_combine<T>), dim3(d.nblk), dim3(256), 0, d.stream, y, (const T*)h, (const int*)d.hside,
                            L.k + 1, L.cbase, L.trash_idx);
     if (s.layout.lam)
-        hipLaunchKernelGGL((k_lam_row<T>), dim3(1), dim3(256), 0, d.stream, d.plam, d.ntiles, y, s.layout.lam_idx);
+        hipLaunchKernelGGL((k_lam_row<T>), dim3(d.ninst), dim3(256), 0, d.stream, (const double*)d.plam,
+                           (const int*)d.inst_tile0, y, s.layout.lam_idx);
     HIPCHK(hipGetLastError());
 }
 
@@ -1595,7 +1604,8 @@ static void apply_static(Solver& s, const T* x, T* y, T* h, double* plam) {
     hipLaunchKernelGGL((k_combine<T>), dim3(d.nblk), dim3(256), 0, d.stream, y, (const T*)h, (const int*)d.hside, kk,
                        L.cbase, L.trash_idx);
     if (L.lam)
-        hipLaunchKernelGGL((k_lam_row<T>), dim3(1), dim3(256), 0, d.stream, (const double*)plam, d.ntiles, y,
+        hipLaunchKernelGGL((k_lam_row<T>), dim3(d.ninst), dim3(256), 0, d.stream, (const double*)plam,
+                           (const int*)d.inst_tile0, y,
                            L.lam_idx);
     HIPCHK(hipGetLastError());
 }
@@ -1892,11 +1902,18 @@ static void solve_device(Solver& s, const vemd_solve_opts& o, vemd_solve_info* i
         throw Error(VEMD_ERR_ARG, "no multigrid hierarchy on this solver");
     const bool amg = o.precond != VEMD_PC_JACOBI && d.amg64 != nullptr;
     if (s.layout.lam) {
-        // the multiplier row's preconditioner depends on the pressure block's
-        const double lm = amg ? s.amg_lam_minv : s.minv[s.layout.lam_idx];
-        const float lf = (float)lm;
-        HIPCHK(hipMemcpyAsync(d.o64.minv + s.layout.lam_idx, &lm, sizeof(double), hipMemcpyHostToDevice, d.stream));
-        HIPCHK(hipMemcpyAsync(d.o32.minv + s.layout.lam_idx, &lf, sizeof(float), hipMemcpyHostToDevice, d.stream));
+        // the multiplier rows' preconditioner depends on the pressure block's
+        const int nl = s.layout.ninst;
+        std::vector<double> lm(nl);
+        std::vector<float> lf(nl);
+        for (int i = 0; i < nl; ++i) {
+            lm[i] = amg ? s.amg_lam_minv[i] : s.minv[s.layout.lam_idx + i];
+            lf[i] = (float)lm[i];
+        }
+        HIPCHK(hipMemcpyAsync(d.o64.minv + s.layout.lam_idx, lm.data(), nl * sizeof(double), hipMemcpyHostToDevice,
+                              d.stream));
+        HIPCHK(hipMemcpyAsync(d.o32.minv + s.layout.lam_idx, lf.data(), nl * sizeof(float), hipMemcpyHostToDevice,
+                              d.stream));
         HIPCHK(hipStreamSynchronize(d.stream));  // the sources are on this stack frame
     }
     auto worst = [&]() { return *std::max_element(rel.begin(), rel.end()); };

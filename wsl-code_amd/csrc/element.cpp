@@ -841,7 +841,8 @@ void setup_finish(const Mesh& m, Setup* S) {
     }
     S->nflux_total = (int64_t)m.ne * (d.k + 1) + (int64_t)nc * sh.ni;
     S->npres_total = (int64_t)nc * sh.q;
-    S->ntot = S->nflux_total + S->npres_total + (d.bc == VEMD_BC_FLUX_MEANZERO ? 1 : 0);
+    // one mean-zero multiplier per batch instance (the last unknowns, in instance order)
+    S->ntot = S->nflux_total + S->npres_total + (d.bc == VEMD_BC_FLUX_MEANZERO ? m.ninst() : 0);
 }
 
 Setup* setup_build(const Mesh& m, const vemd_problem_desc& d) {

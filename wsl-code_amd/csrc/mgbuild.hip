@@ -803,7 +803,7 @@ bool mg_build_run(MgBuild& B, Solver& s, const double* minv_dev, AmgDev<double>&
     H.coarse_inv = cinv;
     B.rho = rho;
     B.cinv = cinv;
-    s.amg_lam_minv = B.meanzero ? 1.0 / rho : 0.0;
+    s.amg_lam_minv.assign(B.meanzero ? 1 : 0, B.meanzero ? 1.0 / rho : 0.0);
     s.amg_host_stale = true;
     // refill both cycles
     refill_level_values(D64, B, H, nullptr, hst, st);

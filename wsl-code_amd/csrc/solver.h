@@ -27,7 +27,7 @@ struct Solver {
     // the multiplier row, whose inverse weight preconditions the multiplier)
     std::unique_ptr<Amg> amg;
     AmgOptions amg_opt;
-    double amg_lam_minv = 0.0;     // multiplier entry of M^-1 with the multigrid cycle (0: no multiplier)
+    std::vector<double> amg_lam_minv;  // per instance: multiplier entry of M^-1 with the multigrid cycle (empty: no multiplier)
     std::unique_ptr<SchurPlan> splan;  // sparsity of the pressure Schur approximation (theta-independent)
     bool amg_values_only = false;  // the last rebuild kept every sparsity pattern of the hierarchy
     // the hierarchy's values were last rebuilt on the device: the host copy

@@ -163,12 +163,13 @@ Layout layout_build(const Mesh& m, const Setup& S) {
     L.cbase = (int64_t)kk * m.ne;
     int nslot = L.ni + L.q;
     // trash slots of padding lanes (k + 1 entries, aligned like an edge's
-    // moments since cbase and cpad are even), then the multiplier
+    // moments since cbase and cpad are even), then the multipliers (one per
+    // instance: lam_idx + i)
     L.trash_idx = L.cbase + (int64_t)nslot * L.cpad;
     L.lam_idx = L.lam ? L.trash_idx + kk : -1;
-    L.nvec = L.trash_idx + kk + (L.lam ? 1 : 0);
-    // Krylov sweep chunks, instance-major
-    if (L.ninst > 1 && L.lam) throw Error(VEMD_ERR_ARG, "a batch needs the Dirichlet-pressure condition");
+    L.nvec = L.trash_idx + kk + (L.lam ? L.ninst : 0);
+    // Krylov sweep chunks, instance-major (an instance's multiplier is a
+    // chunk of its own, so that it follows that instance's recurrence)
     L.chunks.clear();
     L.pchunks.clear();
     L.inst_chunk0.assign(L.ninst + 1, 0);
@@ -183,8 +184,20 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         for (int sl = 0; sl < nslot; ++sl)
             add(L.cbase + (int64_t)sl * L.cpad + L.inst_gpos0[i], L.cbase + (int64_t)sl * L.cpad + L.inst_gpos0[i + 1], i,
                 sl >= L.ni ? 1 + sl - L.ni : 0);
-        if (i + 1 == L.ninst) add(L.cbase + (int64_t)nslot * L.cpad, L.nvec, i, 0);  // multiplier and trash slot
+        if (L.lam) add(L.lam_idx + i, L.lam_idx + i + 1, i, 0);
+        if (i + 1 == L.ninst) add(L.trash_idx, L.trash_idx + kk, i, 0);  // trash slot
         L.inst_chunk0[i + 1] = (int)L.chunks.size();
+    }
+    // what the FP32 sweep by entry pairs (k = 1) assumes of the chunks: edge
+    // and pressure chunks start at an even entry and hold whole pairs (it
+    // reads an edge's two moments and a chunk's entries two at a time); only
+    // a multiplier has a chunk of one entry
+    for (const VecChunk& ch : L.chunks) {
+        if (L.k != 1) break;
+        const bool edge = ch.start < L.cbase, pres = ch.pslot != 0;
+        const bool ok = ch.len >= 1 && (ch.len == 1 ? !edge && !pres : (ch.start % 2 == 0)) &&
+                        (!(edge || pres) || ch.len % 2 == 0);
+        if (!ok) throw Error(VEMD_ERR_INTERNAL, "Krylov sweep chunk of an unexpected shape");
     }
     // reference -> internal map
     L.int_of_ref.assign(S.ntot, -1);
@@ -195,7 +208,8 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         for (int i = 0; i < L.ni; ++i) L.int_of_ref[S.nflux_total - (int64_t)nc * L.ni + (int64_t)c * L.ni + i] = L.cbase + i * L.cpad + gp;
         for (int r = 0; r < L.q; ++r) L.int_of_ref[S.nflux_total + (int64_t)c * L.q + r] = L.cbase + (L.ni + r) * L.cpad + gp;
     }
-    if (L.lam) L.int_of_ref[S.ntot - 1] = L.lam_idx;
+    if (L.lam)
+        for (int i = 0; i < L.ninst; ++i) L.int_of_ref[S.ntot - L.ninst + i] = L.lam_idx + i;
     // packed edge-dof indices (side0: the edges' first sides, above)
     // per-cell stream lengths in 16-byte vectors: packed A, then the rows
     // 1..q-1 of W at the edge columns, each padded
@@ -410,9 +424,14 @@ void build_minv(const Mesh& m, const Setup& S, const Layout& L, const DiagPlan& 
         }
     });
     if (L.lam) {
-        double lam = 0.0;
-        for (double v : lc) lam += v;
-        d[L.lam_idx] = lam > 0 ? lam : 1.0;
+        // per instance: its cells in order
+        for (int i = 0; i < L.ninst; ++i) {
+            const int c0 = m.inst_cell0.empty() ? 0 : m.inst_cell0[i];
+            const int c1 = m.inst_cell0.empty() ? m.nc : m.inst_cell0[i + 1];
+            double lam = 0.0;
+            for (int c = c0; c < c1; ++c) lam += lc[c];
+            d[L.lam_idx + i] = lam > 0 ? lam : 1.0;
+        }
     }
     minv.assign(L.nvec, 1.0);
     parallel_for(L.nvec, [&](int64_t b, int64_t e) {
@@ -467,7 +486,8 @@ void build_row_scale(const Mesh& m, const Setup& S, const Layout& L, std::vector
         }
         for (int r = 0; r < L.q; ++r) rs[L.int_of_ref[S.nflux_total + (int64_t)c * L.q + r]] = -1.0;
     }
-    if (L.lam) rs[L.lam_idx] = -1.0;
+    if (L.lam)
+        for (int i = 0; i < L.ninst; ++i) rs[L.lam_idx + i] = -1.0;
 }
 
 }  // namespace vemd

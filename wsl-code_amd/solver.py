@@ -204,9 +204,11 @@ class Solver:
     def instance_vectors(self, x):
         """Splits a reference-order vector of a batch into the instances'
         own reference-order vectors (edge moments r * E + e, internal flux
-        dofs, pressure moments; edges and cells of instance i follow those
-        of instances < i)."""
+        dofs, pressure moments and, with the mean-zero condition, the
+        multiplier; edges and cells of instance i follow those of instances
+        < i, and the multipliers close the vector in instance order)."""
         parts = self.mesh.parts
+        nlam = len(parts) if self.ntot - self.nflux - self.npres == len(parts) else 0
         kk = self.k + 1
         q = (self.k + 1) * (self.k + 2) // 2
         ni = (q - 1) + (q - kk)
@@ -219,6 +221,8 @@ class Solver:
             v = [x[r * E + e0:r * E + e0 + e] for r in range(kk)]
             v.append(x[kk * E + ni * c0:kk * E + ni * (c0 + c)])
             v.append(x[nflux + q * c0:nflux + q * (c0 + c)])
+            if nlam:
+                v.append(x[self.ntot - nlam + len(out):self.ntot - nlam + len(out) + 1])
             out.append(np.concatenate(v))
             e0 += e
             c0 += c
