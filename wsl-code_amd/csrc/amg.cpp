@@ -394,9 +394,12 @@ void dense_inverse(std::vector<double>& a, int64_t n) {
 
 Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const std::vector<double>& b00,
               const std::vector<int>& node_order0, const AmgOptions& o, const std::vector<double>* l0, double rho,
-              const Amg* reuse, const std::vector<int>* row_inst) {
+              const Amg* reuse, const std::vector<int>* row_inst, const std::vector<std::vector<double>>* nv) {
     Amg H;
     H.opt = o;
+    const int K0 = reuse ? reuse->nvec0 : 1 + (nv ? (int)nv->size() : 0);  // vectors of the first coarsening
+    H.nvec0 = K0;
+    int cur_bs = 1;  // rows per node of the current level (level 0: the caller's)
     std::vector<double> lvec;
     std::vector<int> rinst;  // instance of each row of the current level (empty: one rank-one term)
     if (l0) {
@@ -411,6 +414,7 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
     double eps = o.strength;
     for (int lev = 0;; ++lev) {
         AmgLevel L;
+        L.bs = cur_bs;
         const int64_t n = A.nrows;
         L.dinv.assign(n, 1.0);
         for (int64_t i = 0; i < n; ++i)
@@ -476,9 +480,59 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         // aggregate, normalised; the norms are the coarse near-null vector
         std::vector<double> bc;
         Csr Pt;
+        int K = 1;  // coarse rows per aggregate
         if (old) {
             Pt = old->Pt;
+            K = lev == 0 ? K0 : 1;
             na = (int)Pt.ncols;
+        } else if (lev == 0 && K0 > 1) {
+            // several near-null vectors: each aggregate's rows times the
+            // vectors, Gram-Schmidt in order (the vectors are independent on
+            // any aggregate of whole cells); the first vector's coarse image
+            // (its norm on the aggregate, in the aggregate's first row) is
+            // the next level's near-null vector
+            K = K0;
+            std::vector<std::vector<int64_t>> arows(na);
+            for (int64_t r = 0; r < n; ++r)
+                if (nor[r] >= 0 && agg[nor[r]] >= 0) arows[agg[nor[r]]].push_back(r);
+            std::vector<std::vector<std::pair<int, double>>> prow(n);
+            bc.assign((size_t)na * K, 0.0);
+            for (int a = 0; a < na; ++a) {
+                const std::vector<int64_t>& rr = arows[a];
+                const int m = (int)rr.size();
+                std::vector<std::vector<double>> V(K, std::vector<double>(m));
+                for (int i = 0; i < m; ++i) {
+                    V[0][i] = b0[rr[i]];
+                    for (int k = 1; k < K; ++k) V[k][i] = (*nv)[k - 1][rr[i]];
+                }
+                for (int k = 0; k < K; ++k) {
+                    for (int j = 0; j < k; ++j) {
+                        double d = 0.0;
+                        for (int i = 0; i < m; ++i) d += V[k][i] * V[j][i];
+                        for (int i = 0; i < m; ++i) V[k][i] -= d * V[j][i];
+                    }
+                    double nn = 0.0;
+                    for (int i = 0; i < m; ++i) nn += V[k][i] * V[k][i];
+                    nn = std::sqrt(nn);
+                    if (!(nn > 0.0)) throw Error(VEMD_ERR_INTERNAL, "dependent near-null vectors on an aggregate");
+                    for (int i = 0; i < m; ++i) V[k][i] /= nn;
+                    if (k == 0) bc[(size_t)a * K] = nn;
+                }
+                for (int i = 0; i < m; ++i)
+                    for (int k = 0; k < K; ++k)
+                        if (V[k][i] != 0.0) prow[rr[i]].push_back({a * K + k, V[k][i]});
+            }
+            Pt.nrows = n;
+            Pt.ncols = (int64_t)na * K;
+            Pt.rp.assign(n + 1, 0);
+            for (int64_t r = 0; r < n; ++r) {
+                Pt.rp[r + 1] = Pt.rp[r] + (int64_t)prow[r].size();
+                for (const auto& e : prow[r]) {
+                    Pt.ci.push_back(e.first);
+                    Pt.v.push_back(e.second);
+                }
+            }
+            na = na * K;
         } else {
             bc.assign(na, 0.0);
             for (int64_t r = 0; r < n; ++r)
@@ -545,12 +599,14 @@ Amg amg_build(Csr A, const std::vector<int>& node_of_row0, int nnodes0, const st
         L.Pt = std::move(Pt);
         H.lv.push_back(std::move(L));
         A = std::move(Ac);
+        cur_bs = K;
         if (old) continue;
-        // the next level: one row per aggregate, ordered as created (spatially)
-        nn = na;
+        // the next level: K rows per aggregate (one node), ordered as created (spatially)
+        nn = na / K;
         nor.resize(na);
-        std::iota(nor.begin(), nor.end(), 0);
-        order = nor;
+        for (int c = 0; c < na; ++c) nor[c] = c / K;
+        order.resize(nn);
+        std::iota(order.begin(), order.end(), 0);
         b0 = bc;
         eps *= 0.5;
     }

@@ -36,6 +36,7 @@ struct AmgLevel {
     Csr P, R;                  // prolongation from the next coarser level and R = P^T (not on the coarsest)
     Csr M;                     // R A: the next level's right-hand side R (b - A x) = R b - M x (not on the coarsest)
     Csr Pt;                    // tentative prolongator (the aggregates), kept for a rebuild of the values
+    int bs = 1;                // rows per node of this level (level 0: set by the caller)
     std::vector<double> ev;    // last iterate of the spectral-radius estimate (start of the next)
 };
 
@@ -46,6 +47,7 @@ struct Amg {
     std::vector<double> coarse_inv;   // dense inverse of the coarsest operator, row-major
     int64_t ncoarse = 0;
     double rho = 0.0;                 // weight of the rank-one coarse term (0: none)
+    int nvec0 = 1;                    // near-null vectors of the first coarsening (level 1 has that many rows per node)
 };
 
 // Pressure Schur approximation on the cells' pressure moments, cell-major:
@@ -81,9 +83,14 @@ Csr schur_values(const SchurPlan& P, const Mesh& m, const Setup& S, const Layout
 // With row_inst (one instance per row of A0, -1 for none; aggregates never
 // join rows of different instances) the rank-one term is one per instance:
 // rho l_c l_c^T is added only where both coarse rows belong to one instance.
+// With nv (further near-null vectors, each one value per row of A0) the
+// first coarsening carries b0 and nv: per aggregate the vectors restricted
+// to its rows, orthonormalised in order, so that level 1 has 1 + nv.size()
+// rows per aggregate; the later coarsenings carry b0's coarse image only.
 Amg amg_build(Csr A0, const std::vector<int>& node_of_row, int nnodes, const std::vector<double>& b0,
               const std::vector<int>& node_order, const AmgOptions& o, const std::vector<double>* l0 = nullptr,
-              double rho = 0.0, const Amg* reuse = nullptr, const std::vector<int>* row_inst = nullptr);
+              double rho = 0.0, const Amg* reuse = nullptr, const std::vector<int>* row_inst = nullptr,
+              const std::vector<std::vector<double>>* nv = nullptr);
 
 // In-place inverse of a dense row-major n x n matrix (Gauss-Jordan with
 // partial pivoting; the coarsest operator).

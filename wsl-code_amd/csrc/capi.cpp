@@ -167,9 +167,27 @@ void Solver::build_amg() {
             }
         }
     }
+    // VEMD_MG_LINEAR=1: the first coarsening also carries the linear pressure
+    // fields x and y (cell value x_c, linear moment h; the pressure
+    // coefficients are those of the scaled monomials (x - x_c) / h), so that
+    // level 1 has three rows per aggregate
+    std::vector<std::vector<double>> nv;
+    if (const char* e = std::getenv("VEMD_MG_LINEAR"))
+        if (e[0] == '1' && L.q >= 3) {
+            nv.assign(2, std::vector<double>(n, 0.0));
+            for (int64_t g = 0; g < L.cpad; ++g) {
+                const int c = L.cell_of_gpos[g];
+                if (c < 0) continue;
+                const double h = mesh->diam[c];
+                nv[0][g * L.q] = mesh->cx[c];
+                nv[0][g * L.q + 1] = h;
+                nv[1][g * L.q] = mesh->cy[c];
+                nv[1][g * L.q + 2] = h;
+            }
+        }
     amg.reset(new Amg(amg_build(std::move(S0), node, (int)L.cpad, b0, L.spatial_gpos, amg_opt,
                                 l0.empty() ? nullptr : &l0, rho, prev.get(),
-                                row_inst.empty() ? nullptr : &row_inst)));
+                                row_inst.empty() ? nullptr : &row_inst, nv.empty() ? nullptr : &nv)));
     amg->bs0 = L.q;
     amg_values_only = prev != nullptr;
     amg_host_stale = false;

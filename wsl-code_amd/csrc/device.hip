@@ -1175,6 +1175,7 @@ static void* static_scratch(int device) {
 bool dev_mg_rebuild_possible(const Solver& s) {
     if (!s.dev || !s.amg || !s.splan || !s.dev->amg64 || !s.dev->amg32 || s.setup->sh.k > 2) return false;
     if (s.layout.lam && s.layout.ninst > 1) return false;  // per-instance rank-one terms: host rebuild
+    if (s.amg->nvec0 > 1) return false;  // tentative prolongator rows with several entries: host rebuild
     for (const char* v : {"VEMD_MG_DEVICE_REBUILD", "VEMD_MG_REFILL"}) {
         const char* e = std::getenv(v);
         if (e && e[0] == '0') return false;

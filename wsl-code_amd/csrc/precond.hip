@@ -719,7 +719,7 @@ AmgDev<T>* amg_upload(const Amg& H) {
             const AmgLevel& h = H.lv[l];
             AmgLevelDev<T>& d = D->lv[l];
             d.n = h.A.nrows;
-            const int bs = l == 0 ? H.bs0 : 1;
+            const int bs = l == 0 ? H.bs0 : h.bs;
             if (h.A.nrows % bs) throw Error(VEMD_ERR_INTERNAL, "finest multigrid level is not blocked");
             // FP32 cycles run the finest level scaled to a unit diagonal with
             // its operator in half precision; VEMD_MG_HALF=0 keeps it in FP32
