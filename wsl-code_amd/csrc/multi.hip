@@ -101,6 +101,7 @@ struct MrArgs {
     double* pw;               // per-wave, per-column partials of x.Kx (nullptr: not needed)
     const MState* st;         // per column
     const int* all_done;
+    int off32;                // every row of X lies within 4 GiB of its buffer's start (32-bit gather offsets)
 };
 
 // The MFMA element apply on 32 interleaved columns: y = [[s A, -W^T], [-W,
@@ -118,7 +119,7 @@ struct MrArgs {
 // rows fills m of the tile's 32 rows (a hexagon at k = 1: 18).
 // ---------------------------------------------------------------------------
 
-constexpr int MR_LDX = 33;  // row pitch of the per-wave x.Kx transpose tile (bank-conflict free)
+constexpr int MR_LDX = 36;  // row pitch of the per-wave K x tile (16-byte aligned rows)
 #ifndef MR_WAVES
 #define MR_WAVES 4  // occupancy bound of k_apply_mr (waves per SIMD)
 #endif
@@ -167,8 +168,11 @@ __device__ __forceinline__ uint32_t mr_slot(const MrArgs& a, int64_t gp, int l, 
 
 // The cells of one group at store positions [p0, p1) (all real), pipelined as
 // in k_apply_mr: while cell i is multiplied the rows of cell i + 1 are
-// gathered and the operator of cell i + 2 is loaded.
-template <int K, int NE>
+// gathered and the operator of cell i + 2 is loaded.  U: Xj is the same
+// buffer for the whole wave and every row lies within 4 GiB of it, so a
+// gather is a wave-uniform base plus a 32-bit lane offset (no 64-bit address
+// arithmetic and no 64-bit address registers per gather).
+template <int K, int NE, bool U>
 __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_t p0, int64_t p1, int l,
                                         const float* __restrict__ Xj, float (*lop)[OPMAX], uint32_t (*lsl)[32],
                                         float* lkx, double& dot) {
@@ -214,7 +218,11 @@ __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_
         for (int kk = 0; kk < C::KS; ++kk) {
             const int c = 2 * kk + hi;
             const uint32_t sl = lsl[b][c < C::M ? c : 0];
-            B[kk] = Xj[(int64_t)(sl & 0x7fffffffu) * NR + (l & 31)];
+            if constexpr (U)
+                B[kk] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(Xj) +
+                                                        (((sl & 0x7fffffffu) << 7) | (uint32_t)((l & 31) << 2)));
+            else
+                B[kk] = Xj[(int64_t)(sl & 0x7fffffffu) * NR + (l & 31)];
         }
     };
     auto multiply = [&](int b, float s, const float (&B)[C::KS]) __attribute__((always_inline)) {
@@ -225,28 +233,37 @@ __device__ __forceinline__ void mr_run(const MrArgs& a, const MrGroup& g, int64_
             const float av = (isA >> kk) & 1u ? s * v : -v;
             acc = __builtin_amdgcn_mfma_f32_32x32x2f32(B[kk], av, acc, 0, 0, 0);
         }
+        // K x goes through LDS as a row-major tile (row j: its 32 columns).
+        // Read back eight lanes to a row, each store instruction then writes
+        // eight whole 128-byte rows (stored straight from the MFMA layout,
+        // one instruction would touch 32 rows with 32 bytes each); and for
+        // x.Kx the lanes that hold x read K x from the tile.
         if (j < C::M) {
-            const uint32_t sl = lsl[b][j];
-            float* __restrict__ dst = (sl & SIDE1) ? Y1 : Y0;
-            float4* row = reinterpret_cast<float4*>(dst + (int64_t)(sl & 0x7fffffffu) * NR);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) row[2 * q + hi] = float4{acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+            for (int q = 0; q < 4; ++q)
+                *reinterpret_cast<float4*>(&lkx[j * MR_LDX + 8 * q + 4 * hi]) =
+                    float4{acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int t = 0; t < (C::M + 7) / 8; ++t) {
+            const int r = 8 * t + (l >> 3);
+            if (r < C::M) {
+                const uint32_t sl = lsl[b][r];
+                float* __restrict__ dst = (sl & SIDE1) ? Y1 : Y0;
+                *reinterpret_cast<float4*>(dst + (int64_t)(sl & 0x7fffffffu) * NR + 4 * (l & 7)) =
+                    *reinterpret_cast<const float4*>(&lkx[r * MR_LDX + 4 * (l & 7)]);
+            }
         }
         if (a.pw) {
-            // x.Kx: (K x) goes through LDS to the lanes that hold x
-            if (j < C::M) {
-#pragma unroll
-                for (int r = 0; r < 16; ++r) lkx[j * MR_LDX + 8 * (r >> 2) + 4 * hi + (r & 3)] = acc[r];
-            }
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int kk = 0; kk < C::KS; ++kk) {
                 const int c = 2 * kk + hi;
                 if (c < C::M) dot += (double)B[kk] * (double)lkx[c * MR_LDX + (l & 31)];
             }
-            __builtin_amdgcn_wave_barrier();
         }
+        __builtin_amdgcn_wave_barrier();
     };
     // Three cells in flight: while cell c is multiplied, the rows of c + 1 and
     // c + 2 are being gathered and the operator of c + 3 is being loaded.
@@ -301,19 +318,37 @@ __global__ __launch_bounds__(256, MR_WAVES) void k_apply_mr(MrArgs a, MrGroup g,
     if (a.all_done && *a.all_done) return;
     __shared__ __attribute__((aligned(16))) float lop[4][3][OPMAX];
     __shared__ uint32_t lsl[4][3][32];
-    __shared__ float lkx[4][32 * MR_LDX];
+    __shared__ __attribute__((aligned(16))) float lkx[4][32 * MR_LDX];
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int wave = blockIdx.x * 4 + w;
     const int l = threadIdx.x & 63;
     const int col = l & 31;
-    const float* Xj = a.st ? a.X[a.st[col].itn & 1] : a.X[0];
+    // Each column reads the buffer of its iteration parity.  The columns still
+    // iterating share one parity (a refinement pass starts every column at
+    // iteration 0 and each iteration advances all of them); a finished
+    // column's products are never read, so it may read that buffer too.
+    int par = 0;
+    bool mixed = false;
+    if (a.st) {
+        const MState& sc = a.st[col];
+        const bool act = !sc.done;
+        const uint64_t odd = __ballot(act && (sc.itn & 1)), even = __ballot(act && !(sc.itn & 1));
+        par = odd ? 1 : 0;
+        mixed = odd && even;
+    }
     // rows a cell does not have are not stored; the padding row reads zero
     if (zero_trash && wave == 0 && l < NR) a.Y[a.trash * NR + l] = 0.0f;
     double dot = 0.0;
     const int64_t end = (int64_t)g.gpos0 + g.count;
     const int64_t p0 = (int64_t)g.gpos0 + (int64_t)wave * cpw;
     const int64_t p1 = p0 + cpw < end ? p0 + cpw : end;
-    if (p0 < p1) mr_run<K, NE>(a, g, p0, p1, l, Xj, lop[w], lsl[w], lkx[w], dot);
+    if (p0 < p1) {
+        if (a.off32 && !mixed)
+            mr_run<K, NE, true>(a, g, p0, p1, l, a.X[__builtin_amdgcn_readfirstlane(par)], lop[w], lsl[w], lkx[w], dot);
+        else
+            mr_run<K, NE, false>(a, g, p0, p1, l, a.st ? a.X[a.st[col].itn & 1] : a.X[0], lop[w], lsl[w], lkx[w],
+                                 dot);
+    }
     if (a.pw) {
         dot += __shfl_xor(dot, 32, 64);
         if (l < 32) a.pw[(int64_t)(wave0 + wave) * NR + col] = dot;
@@ -810,6 +845,7 @@ static MrArgs mr_args(Solver& s, MultiState& m, const float* x0, const float* x1
     a.pw = solve ? m.pw : nullptr;
     a.st = solve ? m.st : nullptr;
     a.all_done = solve ? m.all_done : nullptr;
+    a.off32 = (uint64_t)s.layout.nvec * NR * sizeof(float) <= (uint64_t(1) << 32) ? 1 : 0;
     return a;
 }
 
