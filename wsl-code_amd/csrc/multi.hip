@@ -461,26 +461,67 @@ __global__ __launch_bounds__(256) void k_update_mr(MrUpd a) {
         const bool pend = st->pending;
         const float uv = (float)st->u_v, uw1 = (float)st->u_w1, uw2 = (float)st->u_w2, ux = (float)st->u_x;
         const bool cyc = a.amg && ch.pslot;
+        const bool edge = ch.start < a.edof;  // a chunk lies wholly inside or outside the edge dofs
         const int pr = ch.pslot - 1;
-        for (int64_t row = ch.start + r0; row < ch.start + ch.len; row += 8) {
-            const int64_t e = row * NR + j;
-            float r2i = r2[e];
-            float az = a.Az[e];
-            if (row < a.edof) az += a.Az1[e];
-            float y = caz * az - cr1 * r1[e] - cr2 * r2i;
-            if (pend) {
-                float w = uv * zo[e] - uw1 * wn[e] - uw2 * wp[e];
-                wn[e] = w;
-                a.x[e] += ux * w;
+        const int64_t end = ch.start + ch.len;
+        float* __restrict__ x = a.x;
+        const float* __restrict__ Az = a.Az;
+        const float* __restrict__ Az1 = a.Az1;
+        // UR rows per thread and trip, every load issued before the first use
+        // (rows past the chunk read its first row and store nothing)
+        constexpr int UR = 4;
+        for (int64_t row0 = ch.start + r0; row0 < end; row0 += 8 * UR) {
+            int64_t e[UR];
+            bool ok[UR];
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                const int64_t row = row0 + 8 * u;
+                ok[u] = row < end;
+                e[u] = (ok[u] ? row : row0) * NR + j;
             }
-            r1[e] = y;
-            if (cyc) {
-                a.bp[((row - a.pb - (int64_t)pr * a.cpad) * a.q + pr) * NR + j] = y;
-            } else {
-                float zn = a.minv[row] * y;
-                zo[e] = zn;
-                d1 += (double)y * (double)zn;
-                d2 += (double)zn * (double)r2i;
+            float r2v[UR], azv[UR], r1v[UR], zv[UR], wnv[UR], wpv[UR], xv[UR], mv[UR];
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                r2v[u] = r2[e[u]];
+                azv[u] = Az[e[u]];
+                r1v[u] = r1[e[u]];
+            }
+            if (edge) {
+#pragma unroll
+                for (int u = 0; u < UR; ++u) azv[u] += Az1[e[u]];
+            }
+            if (pend) {
+#pragma unroll
+                for (int u = 0; u < UR; ++u) {
+                    zv[u] = zo[e[u]];
+                    wnv[u] = wn[e[u]];
+                    wpv[u] = wp[e[u]];
+                    xv[u] = x[e[u]];
+                }
+            }
+            if (!cyc) {
+#pragma unroll
+                for (int u = 0; u < UR; ++u) mv[u] = a.minv[e[u] / NR];
+            }
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                if (!ok[u]) continue;
+                const float y = caz * azv[u] - cr1 * r1v[u] - cr2 * r2v[u];
+                if (pend) {
+                    const float w = uv * zv[u] - uw1 * wnv[u] - uw2 * wpv[u];
+                    wn[e[u]] = w;
+                    x[e[u]] = xv[u] + ux * w;
+                }
+                r1[e[u]] = y;
+                if (cyc) {
+                    const int64_t row = row0 + 8 * u;
+                    a.bp[((row - a.pb - (int64_t)pr * a.cpad) * a.q + pr) * NR + j] = y;
+                } else {
+                    const float zn = mv[u] * y;
+                    zo[e[u]] = zn;
+                    d1 += (double)y * (double)zn;
+                    d2 += (double)zn * (double)r2v[u];
+                }
             }
         }
     }
@@ -499,13 +540,35 @@ __global__ __launch_bounds__(256) void k_pdot_mr(MrUpd a, const int* pchunks, co
         float* __restrict__ zo = init ? a.Z[0] : a.Z[(it + 1) & 1];
         const float* __restrict__ r2 = a.R[it & 1];
         const int pr = ch.pslot - 1;
-        for (int64_t row = ch.start + r0; row < ch.start + ch.len; row += 8) {
-            const int64_t e = row * NR + j;
-            const int64_t k = ((row - a.pb - (int64_t)pr * a.cpad) * a.q + pr) * NR + j;
-            const float z = xp[k];
-            zo[e] = z;
-            d1 += (double)a.bp[k] * (double)z;
-            if (!init) d2 += (double)z * (double)r2[e];
+        const int64_t end = ch.start + ch.len;
+        const float* __restrict__ bp = a.bp;
+        // UR rows per thread and trip, every load issued before the first use
+        constexpr int UR = 4;
+        for (int64_t row0 = ch.start + r0; row0 < end; row0 += 8 * UR) {
+            int64_t e[UR], k[UR];
+            bool ok[UR];
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                const int64_t row0u = row0 + 8 * u;
+                ok[u] = row0u < end;
+                const int64_t row = ok[u] ? row0u : row0;
+                e[u] = row * NR + j;
+                k[u] = ((row - a.pb - (int64_t)pr * a.cpad) * a.q + pr) * NR + j;
+            }
+            float zv[UR], bv[UR], rv[UR];
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                zv[u] = xp[k[u]];
+                bv[u] = bp[k[u]];
+                rv[u] = init ? 0.0f : r2[e[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                if (!ok[u]) continue;
+                zo[e[u]] = zv[u];
+                d1 += (double)bv[u] * (double)zv[u];
+                if (!init) d2 += (double)zv[u] * (double)rv[u];
+            }
         }
     }
     col_pair(d1, d2, q1 + (int64_t)c * NR, q2 + (int64_t)c * NR);
