@@ -221,7 +221,7 @@ typedef struct {
                            flux block and one smoothed-aggregation V-cycle on the pressure Schur
                            approximation S = W diag(A)^-1 W^T (with the mean-zero multiplier, on
                            S + rho L L^T, the multiplier row then preconditioned by 1 / rho);
-                           VEMD_PC_AUTO (0): multigrid where built (k = 0 and 1), else Jacobi */
+                           VEMD_PC_AUTO (0): multigrid where built (k = 0, 1, 2), else Jacobi */
     void* stream;       /* the caller's HIP stream (hipStream_t; NULL = the null stream).  The
                            device-resident solves (vemd_solve_device, vemd_solve_multi_device)
                            wait for work queued on it before they read b_dev, and work queued

@@ -353,7 +353,7 @@ vemd_status vemd_solver_create(vemd_mesh mh, const vemd_problem_desc* d, vemd_so
     std::unique_ptr<Solver> s(new Solver());
     s->mesh = mh->m;
     s->setup.reset(setup_build(*s->mesh, *d));
-    if (d->k <= 2) {
+    if (d->k <= 3) {
         s->layout = layout_build(*s->mesh, *s->setup);
         s->refresh_theta_dependent();
     } else {
@@ -398,7 +398,7 @@ vemd_status vemd_solver_sizes(vemd_solver sh, int64_t* z) {
 vemd_status vemd_solver_upload(vemd_solver sh, int device) {
     API_BEGIN
     REQUIRE(sh, "null solver");
-    if (sh->s->setup->sh.k > 2) throw Error(VEMD_ERR_ARG, "the device path supports k = 0, 1 and 2");
+    if (sh->s->setup->sh.k > 3) throw Error(VEMD_ERR_ARG, "the device path supports k = 0 to 3");
     dev_upload(*sh->s, device);
     API_END
 }
@@ -417,7 +417,7 @@ vemd_status vemd_set_theta(vemd_solver sh, const double* theta) {
     for (int c = 0; c < s.mesh->nc; ++c)
         if (!std::isfinite(theta[c])) throw Error(VEMD_ERR_ARG, "theta must be finite");
     s.setup->theta.assign(theta, theta + s.mesh->nc);
-    if (s.setup->sh.k <= 2) {
+    if (s.setup->sh.k <= 3) {
         // an uploaded solver rebuilds the multigrid values on its device
         const bool device_mg = dev_mg_rebuild_possible(s);
         s.refresh_theta_dependent(!device_mg);

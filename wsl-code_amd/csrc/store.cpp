@@ -50,7 +50,7 @@ Layout layout_build(const Mesh& m, const Setup& S) {
         if (m.npc(c) > MAX_NE)
             throw Error(VEMD_ERR_MESH, "cell " + std::to_string(c) + " has more than 10 edges (device limit)");
     }
-    if (L.k > 2) throw Error(VEMD_ERR_ARG, "the device path supports k = 0, 1 and 2");
+    if (L.k > 3) throw Error(VEMD_ERR_ARG, "the device path supports k = 0 to 3");
     // batch instance of every cell
     L.ninst = m.ninst();
     std::vector<int> inst(nc, 0);
