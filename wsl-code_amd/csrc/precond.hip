@@ -146,12 +146,101 @@ __device__ __forceinline__ T sell_row_lanes(const int* __restrict__ col, const V
     return acc;
 }
 
+// The 32-column kernels give each block row one half-wave (its 32 columns),
+// so every block value would be loaded by 32 lanes at once and one load
+// instruction would fetch two values: the values cost far more load
+// instructions (address work) than the gathers.  A 256-thread block of such a
+// kernel covers 8 block rows of one slice; it stages their values (as T) and
+// block columns in LDS with one load per value, and the lanes read them from
+// there (broadcast reads).  Slices wider than SW entries are not staged.
+template <typename T, int B>
+struct Stage32 {
+    static constexpr int SW = B == 1 ? 64 : 16;
+    T v[SW][B * B][8];
+    int c[SW][8];
+};
+
+template <typename T, int B, int NC, typename S>
+__device__ __forceinline__ const S* stage_rows(const Sell<T>& A, S& st) {
+    const int64_t br0 = (int64_t)blockIdx.x * (256 / NC);  // 8 block rows, within one slice of 64
+    if (br0 >= A.nrows) return nullptr;                      // (uniform over the block)
+    const int64_t sl = br0 >> 6;
+    const int w = A.width[sl];
+    if (w > S::SW) return nullptr;
+    const int l0 = (int)(br0 & 63);
+    const int64_t o = A.off[sl];
+    // rows past nrows are slice padding: zero values, column 0, in bounds
+    for (int e = threadIdx.x; e < w * B * B * 8; e += 256) {
+        const int r8 = e & 7, rc = (e >> 3) % (B * B), k = (e >> 3) / (B * B);
+        const int64_t at = (o + (int64_t)k * 64) * (B * B) + (int64_t)rc * 64 + l0 + r8;
+        st.v[k][rc][r8] = A.hval ? (T)A.hval[at] : A.val[at];
+    }
+    for (int e = threadIdx.x; e < w * 8; e += 256) st.c[e >> 3][e & 7] = A.col[o + (int64_t)(e >> 3) * 64 + l0 + (e & 7)];
+    __syncthreads();
+    return &st;
+}
+
+// sell_acc over a staged row (r8 of the block's 8), in the same entry order.
+template <typename T, int B, bool HY, bool HC>
+__device__ __forceinline__ void sell_acc_staged(const Stage32<T, B>& S, int w, int r8, int j, const T* __restrict__ x,
+                                                const T* __restrict__ y, const T* __restrict__ cs, T (&out)[B]) {
+    constexpr int NC = MCOLS;
+    constexpr int U = B == 1 ? 8 : 2;
+#pragma unroll
+    for (int r = 0; r < B; ++r) out[r] = T(0);
+    int k = 0;
+    for (; k + U <= w; k += U) {
+        int64_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = (int64_t)S.c[k + u][r8] * B;
+        T xc[U][B];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int cc = 0; cc < B; ++cc) {
+                const int64_t xi = (c[u] + cc) * NC + j;
+                T xv = x[xi];
+                if constexpr (HY) xv += y[xi];
+                if constexpr (HC) xv *= cs[c[u] + cc];
+                xc[u][cc] = xv;
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int cc = 0; cc < B; ++cc) out[r] += S.v[k + u][r * B + cc][r8] * xc[u][cc];
+    }
+    for (; k < w; ++k) {
+        const int64_t c = (int64_t)S.c[k][r8] * B;
+        T xc[B];
+#pragma unroll
+        for (int cc = 0; cc < B; ++cc) {
+            const int64_t xi = (c + cc) * NC + j;
+            T xv = x[xi];
+            if constexpr (HY) xv += y[xi];
+            if constexpr (HC) xv *= cs[c + cc];
+            xc[cc] = xv;
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int cc = 0; cc < B; ++cc) out[r] += S.v[k][r * B + cc][r8] * xc[cc];
+    }
+}
+
 // UG: gathers in flight per lane for scalar rows (0: the default of the level
 // operators; the prolongation's short rows pass 4).
 template <typename T, int B, int NC, int UG, bool HY, bool HC>
 __device__ __forceinline__ void sell_row_impl(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x,
                                               T (&out)[B], const T* __restrict__ y, const T* __restrict__ ds,
-                                              int sub) {
+                                              int sub, const Stage32<T, B>* stg) {
+    if constexpr (NC == MCOLS) {
+        if (stg) {
+            sell_acc_staged<T, B, HY, HC>(*stg, A.width[brow >> 6], (int)(brow & 7), j, x, y, ds, out);
+            return;
+        }
+    }
     if constexpr (UG == 0 && B == 1 && NC == 1) {
         if (lanes_per_row<B, NC>(A) > 1) {
             const int64_t s = brow >> 6;
@@ -184,17 +273,30 @@ __device__ __forceinline__ void sell_row_impl(const Sell<T>& A, int64_t brow, in
 template <typename T, int B, int NC, int UG = 0>
 __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, const T* __restrict__ x, T (&out)[B],
                                          const T* __restrict__ y = nullptr, const T* __restrict__ ds = nullptr,
-                                         int sub = 0) {
+                                         int sub = 0, const Stage32<T, B>* stg = nullptr) {
     if (y) {
         if (ds)
-            sell_row_impl<T, B, NC, UG, true, true>(A, brow, j, x, out, y, ds, sub);
+            sell_row_impl<T, B, NC, UG, true, true>(A, brow, j, x, out, y, ds, sub, stg);
         else
-            sell_row_impl<T, B, NC, UG, true, false>(A, brow, j, x, out, y, ds, sub);
+            sell_row_impl<T, B, NC, UG, true, false>(A, brow, j, x, out, y, ds, sub, stg);
     } else {
         if (ds)
-            sell_row_impl<T, B, NC, UG, false, true>(A, brow, j, x, out, y, ds, sub);
+            sell_row_impl<T, B, NC, UG, false, true>(A, brow, j, x, out, y, ds, sub, stg);
         else
-            sell_row_impl<T, B, NC, UG, false, false>(A, brow, j, x, out, y, ds, sub);
+            sell_row_impl<T, B, NC, UG, false, false>(A, brow, j, x, out, y, ds, sub, stg);
+    }
+}
+
+// The staged block rows of a 32-column block-row kernel (null: not staged,
+// or a single-column kernel).  Every thread of the block calls it before any
+// returns (it holds a barrier).
+template <typename T, int B, int NC>
+__device__ __forceinline__ const Stage32<T, B>* stage_block(const Sell<T>& A) {
+    if constexpr (NC == MCOLS) {
+        __shared__ Stage32<T, B> st;
+        return stage_rows<T, B, NC>(A, st);
+    } else {
+        return nullptr;
     }
 }
 
@@ -213,6 +315,7 @@ __device__ __forceinline__ void sell_row(const Sell<T>& A, int64_t brow, int j, 
     const int64_t bi = t / (NC * LPR);                                    \
     const int j = (int)((t / LPR) % NC);                                  \
     const int sub = (int)(t % LPR);                                       \
+    const Stage32<T, B>* stg = stage_block<T, B, NC>(A);                  \
     if (bi >= A.nrows) return;
 
 // Diagonal entry of D^-1 (null: unit diagonal) and the right-hand side of a
@@ -248,7 +351,7 @@ __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, co
                                             int last, const T* xs, const T* b0, const T* es, const int* done) {
     AMG_BROW
     T ad[B];
-    sell_row<T, B, NC>(A, bi, j, din, ad, nullptr, nullptr, sub);
+    sell_row<T, B, NC>(A, bi, j, din, ad, nullptr, nullptr, sub, stg);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
@@ -274,7 +377,7 @@ __global__ __launch_bounds__(256) void k_step0(Sell<T> A, const T* b, const T* d
                                              T c2, int last, const int* done) {
     AMG_BROW
     T ad[B];
-    sell_row<T, B, NC>(A, bi, j, b, ad, nullptr, dinv, sub);
+    sell_row<T, B, NC>(A, bi, j, b, ad, nullptr, dinv, sub, stg);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
@@ -298,7 +401,7 @@ __global__ __launch_bounds__(256) void k_resid(Sell<T> A, const T* b, const T* e
                                              const int* done) {
     AMG_BROW
     T ax[B];
-    sell_row<T, B, NC>(A, bi, j, x, ax, d, nullptr, sub);
+    sell_row<T, B, NC>(A, bi, j, x, ax, d, nullptr, sub, stg);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
@@ -312,7 +415,7 @@ __global__ __launch_bounds__(256) void k_post0(Sell<T> A, const T* b, const T* e
                                              T* d, T c, const int* done) {
     AMG_BROW
     T ax[B];
-    sell_row<T, B, NC>(A, bi, j, x, ax, nullptr, nullptr, sub);
+    sell_row<T, B, NC>(A, bi, j, x, ax, nullptr, nullptr, sub, stg);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t i = (bi * B + k) * NC + j;
