@@ -114,9 +114,9 @@ struct MrArgs {
 // v_mfma_f32_32x32x2f32 with the gathered rows as the A operand (lane l: row
 // 2 kk + l / 32 of column l % 32) and the cell block as the B operand (lane
 // l: entry (l % 32, 2 kk + l / 32)), so D = (block X)^T: lane l holds cell
-// row l % 32 for the 16 columns 8 g + 4 (l / 32) + e (g, e < 4), and its
-// results go out as four 16-byte stores to one row address.  A block of m
-// rows fills m of the tile's 32 rows (a hexagon at k = 1: 18).
+// row l % 32 for the 16 columns 8 g + 4 (l / 32) + e (g, e < 4); the results
+// pass through a per-wave LDS tile and leave as whole 128-byte rows.  A block
+// of m rows fills m of the tile's 32 rows (a hexagon at k = 1: 18).
 // ---------------------------------------------------------------------------
 
 constexpr int MR_LDX = 36;  // row pitch of the per-wave K x tile (16-byte aligned rows)
@@ -425,6 +425,9 @@ struct MrUpd {
     int64_t pb, cpad;
     int q;
     float* bp;   // multigrid input, [cell-major pressure row][NR]
+    float* bd0 = nullptr;        // non-null: also the cycle's first update bd0 = c0 * bsc * bp
+    const float* bsc = nullptr;  // per pressure row
+    float c0 = 0.0f;
 };
 
 // Column partials of a block of 8 row groups x NR columns.
@@ -515,7 +518,9 @@ __global__ __launch_bounds__(256) void k_update_mr(MrUpd a) {
                 r1[e[u]] = y;
                 if (cyc) {
                     const int64_t row = row0 + 8 * u;
-                    a.bp[((row - a.pb - (int64_t)pr * a.cpad) * a.q + pr) * NR + j] = y;
+                    const int64_t prow = (row - a.pb - (int64_t)pr * a.cpad) * a.q + pr;
+                    a.bp[prow * NR + j] = y;
+                    if (a.bd0) a.bd0[prow * NR + j] = a.c0 * a.bsc[prow] * y;
                 } else {
                     const float zn = mv[u] * y;
                     zo[e[u]] = zn;
@@ -942,8 +947,14 @@ static MrUpd mr_upd(Solver& s, MultiState& m, bool amg) {
             d.chunks,         m.p1,           m.p2,           m.st, amg ? 1 : 0, 0,      L.cpad,        L.q,
             nullptr};
     if (amg) {
+        const AmgDev<float>& M = *d.amg32;
         u.pb = L.cbase + (int64_t)L.ni * L.cpad;
-        u.bp = d.amg32->lv[0].mb;
+        u.bp = M.lv[0].mb;
+        if (amg_pre_fusable(M)) {  // the cycle's first update formed by the Krylov sweep
+            u.bd0 = M.lv[0].md0;
+            u.bsc = M.lv[0].sc ? M.lv[0].sc : M.lv[0].dinv;
+            u.c0 = (float)(1.0 / M.lv[0].centre);
+        }
     }
     return u;
 }
@@ -956,7 +967,7 @@ static void enqueue_iteration_mr(Solver& s, MultiState& m, bool amg) {
     MrUpd u = mr_upd(s, m, amg);
     hipLaunchKernelGGL(k_update_mr, dim3(d.nchunks), dim3(256), 0, d.stream, u);
     if (amg) {
-        amg_vcycle_multi(*d.amg32, d.stream, m.all_done);
+        amg_vcycle_multi(*d.amg32, d.stream, m.all_done, u.bd0 != nullptr);
         hipLaunchKernelGGL(k_pdot_mr, dim3(d.npchunks), dim3(256), 0, d.stream, u, (const int*)d.pchunks,
                            (const float*)d.amg32->lv[0].mx, m.q1, m.q2, 0);
     }

@@ -171,9 +171,10 @@ bool amg_kout_fusable(const AmgDev<T>& D);
 
 // The same cycle applied to MCOLS interleaved columns (entry (row, j) at
 // row * MCOLS + j): b = lv[0].mb, x = lv[0].mx.  amg_multi_alloc first.
+// pre_done as for amg_vcycle, with the first update in lv[0].md0.
 template <typename T>
 void amg_multi_alloc(AmgDev<T>& D);
 template <typename T>
-void amg_vcycle_multi(const AmgDev<T>& D, hipStream_t stream, const int* done);
+void amg_vcycle_multi(const AmgDev<T>& D, hipStream_t stream, const int* done, bool pre_done = false);
 
 }  // namespace vemd

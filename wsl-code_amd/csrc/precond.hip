@@ -1212,9 +1212,10 @@ void amg_multi_alloc(AmgDev<T>& D) {
 }
 
 template <typename T>
-void amg_vcycle_multi(const AmgDev<T>& D, hipStream_t st, const int* ms) {
+void amg_vcycle_multi(const AmgDev<T>& D, hipStream_t st, const int* ms, bool pre_done) {
     if (!D.multi) throw Error(VEMD_ERR_STATE, "multi-column cycle buffers not allocated");
-    vcycle_level<T, MCOLS>(D, 0, st, ms);
+    if (pre_done && !amg_pre_fusable(D)) throw Error(VEMD_ERR_INTERNAL, "multigrid cycle cannot take a formed start");
+    vcycle_level<T, MCOLS>(D, 0, st, ms, pre_done);
     HIPCHK(hipGetLastError());
 }
 
@@ -1232,7 +1233,7 @@ template bool amg_kout_fusable<double>(const AmgDev<double>&);
 template bool amg_kout_fusable<float>(const AmgDev<float>&);
 template void amg_multi_alloc<double>(AmgDev<double>&);
 template void amg_multi_alloc<float>(AmgDev<float>&);
-template void amg_vcycle_multi<double>(const AmgDev<double>&, hipStream_t, const int*);
-template void amg_vcycle_multi<float>(const AmgDev<float>&, hipStream_t, const int*);
+template void amg_vcycle_multi<double>(const AmgDev<double>&, hipStream_t, const int*, bool);
+template void amg_vcycle_multi<float>(const AmgDev<float>&, hipStream_t, const int*, bool);
 
 }  // namespace vemd
