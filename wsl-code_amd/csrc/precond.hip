@@ -318,6 +318,21 @@ __device__ __forceinline__ const Stage32<T, B>* stage_block(const Sell<T>& A) {
     const Stage32<T, B>* stg = stage_block<T, B, NC>(A);                  \
     if (bi >= A.nrows) return;
 
+// Rows of finished instances of a batched solve, left alone by the finest
+// level's block-row and prolongation kernels after the first cycle of a
+// refinement pass (nothing reads them: the instances' Krylov output is not
+// written either, and the hierarchy has no coupling between instances):
+// instance of every 64-block-row tile and the instances' Krylov states; bs
+// rows per block row.  ti null: every row is computed.
+struct RowSkip {
+    const int* ti = nullptr;
+    const MState* st = nullptr;
+    int bs = 1;
+};
+__device__ __forceinline__ bool skipped(const RowSkip& k, int64_t brow) {
+    return k.ti && k.st[k.ti[brow >> 6]].done;
+}
+
 // Diagonal entry of D^-1 (null: unit diagonal) and the right-hand side of a
 // scaled level (es non-null: es * b).
 template <typename T>
@@ -348,8 +363,10 @@ __global__ __launch_bounds__(256) void k_pre0(int64_t n, const T* b, const T* es
 // (times es on a scaled level) and d = din formed by the caller.
 template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, const T* din, T* dout, T* x, T c1, T c2,
-                                            int last, const T* xs, const T* b0, const T* es, const int* done) {
+                                            int last, const T* xs, const T* b0, const T* es, const int* done,
+                                            RowSkip sk) {
     AMG_BROW
+    if (skipped(sk, bi)) return;
     T ad[B];
     sell_row<T, B, NC>(A, bi, j, din, ad, nullptr, nullptr, sub, stg);
 #pragma unroll
@@ -374,8 +391,9 @@ __global__ __launch_bounds__(256) void k_step(Sell<T> A, T* r, const T* dinv, co
 // r = b - A d0, d' = c1 d0 + c2 D^-1 r, x = d0 (last: x = d0 + d' only).
 template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_step0(Sell<T> A, const T* b, const T* dinv, T* r, T* dout, T* x, T c0, T c1,
-                                             T c2, int last, const int* done) {
+                                             T c2, int last, const int* done, RowSkip sk) {
     AMG_BROW
+    if (skipped(sk, bi)) return;
     T ad[B];
     sell_row<T, B, NC>(A, bi, j, b, ad, nullptr, dinv, sub, stg);
 #pragma unroll
@@ -412,8 +430,9 @@ __global__ __launch_bounds__(256) void k_resid(Sell<T> A, const T* b, const T* e
 // First post-smoothing step: r = b - A x, d = D^-1 r / centre (x unchanged).
 template <typename T, int B, int NC>
 __global__ __launch_bounds__(256) void k_post0(Sell<T> A, const T* b, const T* es, const T* x, const T* dinv, T* r,
-                                             T* d, T c, const int* done) {
+                                             T* d, T c, const int* done, RowSkip sk) {
     AMG_BROW
+    if (skipped(sk, bi)) return;
     T ax[B];
     sell_row<T, B, NC>(A, bi, j, x, ax, nullptr, nullptr, sub, stg);
 #pragma unroll
@@ -573,9 +592,11 @@ __global__ __launch_bounds__(256) void k_step_z(Sell<T> A, const T* r, const T* 
 // level's and dc the coarse level's pending Chebyshev update (null where the
 // iterate is complete).
 template <typename T, int NC>
-__global__ __launch_bounds__(256) void k_prol(Sell<T> P, const T* xc, const T* dc, T* x, const T* d, const int* done) {
+__global__ __launch_bounds__(256) void k_prol(Sell<T> P, const T* xc, const T* dc, T* x, const T* d, const int* done,
+                                            RowSkip sk) {
     const int64_t n = P.nrows;
     AMG_ROW
+    if (skipped(sk, row / sk.bs)) return;
     T px[1];
     sell_row<T, 1, NC, 4>(P, row, (int)(i % NC), xc, px, dc);
     px[0] *= (T)P.hscale;  // 1 unless the values are stored scaled in half precision
@@ -1041,7 +1062,8 @@ static dim3 grid_of(int64_t threads) { return dim3((unsigned)((threads + 255) / 
 // step and no fused start the update stays pending in din.
 template <typename T, int B, int NC>
 static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStream_t st, const int* ms, T*& din,
-                         bool fused, const T* xs, bool from_b = false, const KOut<T>* ko = nullptr) {
+                         bool fused, const T* xs, bool from_b = false, const KOut<T>* ko = nullptr,
+                         RowSkip sk = RowSkip{}) {
     const double sigma = L.centre / L.half;
     double rho = 1.0 / sigma;
     T* dout = din == v.d0 ? v.d1 : v.d0;
@@ -1056,14 +1078,14 @@ static void smooth_steps(const AmgLevelDev<T>& L, const LevelVecs<T>& v, hipStre
             hipLaunchKernelGGL((k_step0<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A)), dim3(256), 0, st,
                                L.A, (const T*)v.b,
                                (const T*)L.dinv, v.r, din, v.x, (T)(1.0 / L.centre), (T)(rn * rho),
-                               (T)(2.0 * rn / L.half), last, ms);
+                               (T)(2.0 * rn / L.half), last, ms, sk);
         } else {
             const bool b0 = from_b && k == 1;
             hipLaunchKernelGGL((k_step<T, B, NC>), grid_of(L.A.nrows * NC * lanes_per_row<B, NC>(L.A)), dim3(256), 0, st,
                                L.A, v.r,
                                (const T*)L.dinv, (const T*)din, dout, v.x, (T)(rn * rho), (T)(2.0 * rn / L.half),
                                last, last ? xs : nullptr, b0 ? (const T*)v.b : nullptr, b0 ? (const T*)L.sc : nullptr,
-                               ms);
+                               ms, sk);
             std::swap(din, dout);
         }
         if (last) din = nullptr;
@@ -1128,11 +1150,15 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
     // (gathers of B consecutive entries) are cheaper with it materialised
     const bool fused = B == 1 && L.degree >= 2 && !L.sc;
     const T* sc = L.sc;
+    // finished instances of a batch: their finest-level rows are left alone
+    // after a pass's first cycle
+    RowSkip sk;
+    if (NC == 1 && l == 0 && ko && ko->tile_inst && !ko->init) sk = RowSkip{ko->tile_inst, ko->st, B};
     // pre_done: the caller has written d0 = b / centre (D^-1 or sc applied)
     if (!fused && !pre_done)
         hipLaunchKernelGGL((k_pre0<T, NC>), g, b, 0, st, L.n, (const T*)v.b, sc, (const T*)L.dinv, v.r, v.d0, v.x, c0,
                            ms);
-    smooth_steps<T, B, NC>(L, v, st, ms, din, fused, (const T*)nullptr, pre_done);
+    smooth_steps<T, B, NC>(L, v, st, ms, din, fused, (const T*)nullptr, pre_done, nullptr, sk);
     if (NC == 1 && L.M.nrows > 0) {
         launch_rres<T, B>(L, (const T*)v.b, (const T*)v.x, (const T*)din, vc.b, st, ms);
     } else {
@@ -1141,11 +1167,11 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
         hipLaunchKernelGGL((k_restrict<T, NC>), gr, b, 0, st, L.R, (const T*)v.r, vc.b, ms);
     }
     const T* dc = vcycle_level<T, NC>(D, l + 1, st, ms);
-    hipLaunchKernelGGL((k_prol<T, NC>), g, b, 0, st, L.P, (const T*)vc.x, dc, v.x, (const T*)din, ms);
+    hipLaunchKernelGGL((k_prol<T, NC>), g, b, 0, st, L.P, (const T*)vc.x, dc, v.x, (const T*)din, ms, sk);
     din = v.d0;
     hipLaunchKernelGGL((k_post0<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)L.dinv,
-                       v.r, v.d0, c0, ms);
-    smooth_steps<T, B, NC>(L, v, st, ms, din, false, sc, false, ko);
+                       v.r, v.d0, c0, ms, sk);
+    smooth_steps<T, B, NC>(L, v, st, ms, din, false, sc, false, ko, sk);
     if (l > 0 || !din) return din;
     hipLaunchKernelGGL((k_addto<T, NC>), g, b, 0, st, L.n, v.x, (const T*)din, sc, ms);
     return nullptr;
