@@ -52,7 +52,8 @@ void Solver::refresh_theta_dependent(bool host_amg) {
     if (!dplan) dplan.reset(new DiagPlan(diag_plan(*mesh, *setup, layout)));
     pack_scale(*setup, layout, scale);
     build_minv(*mesh, *setup, layout, *dplan, minv);
-    build_row_scale(*mesh, *setup, layout, rowscale);
+    // the row scales depend on theta only through essential flux rows
+    if (rowscale.empty() || dplan->essential) build_row_scale(*mesh, *setup, layout, rowscale);
     // without essential dofs the right-hand side has no theta-dependent lift
     rhs_changed = rhs_ref.empty() || dplan->essential;
     if (rhs_changed) build_rhs_ref(*mesh, *setup, rhs_ref);

@@ -345,10 +345,12 @@ void pack_store(const Mesh& m, const Setup& S, const Layout& L, std::vector<doub
 
 void pack_scale(const Setup& S, const Layout& L, std::vector<double>& s) {
     s.assign(L.cpad, 0.0);
-    for (int64_t gp = 0; gp < L.cpad; ++gp) {
-        int c = L.cell_of_gpos[gp];
-        if (c >= 0) s[gp] = std::exp(-S.theta[c]);
-    }
+    parallel_for(L.cpad, [&](int64_t b, int64_t e) {
+        for (int64_t gp = b; gp < e; ++gp) {
+            const int c = L.cell_of_gpos[gp];
+            if (c >= 0) s[gp] = std::exp(-S.theta[c]);
+        }
+    });
 }
 
 DiagPlan diag_plan(const Mesh& m, const Setup& S, const Layout& L) {
@@ -478,14 +480,18 @@ void build_rhs_ref(const Mesh& m, const Setup& S, std::vector<double>& b) {
 
 void build_row_scale(const Mesh& m, const Setup& S, const Layout& L, std::vector<double>& rs) {
     rs.assign(L.nvec, 1.0);
-    for (int c = 0; c < m.nc; ++c) {
-        int n = S.sh.nflux(m.npc(c));
-        for (int i = 0; i < n; ++i) {
-            int64_t v = S.ofs.v_off[c] + i;
-            if (S.isb[v]) rs[L.int_of_ref[S.gdof[v]]] = std::exp(-S.theta[c]);
+    // a cell writes its own pressure rows and its essential flux rows (a
+    // boundary edge has one cell): no two cells write one entry
+    parallel_for(m.nc, [&](int64_t b, int64_t e) {
+        for (int64_t c = b; c < e; ++c) {
+            const int n = S.sh.nflux(m.npc((int)c));
+            for (int i = 0; i < n; ++i) {
+                const int64_t v = S.ofs.v_off[c] + i;
+                if (S.isb[v]) rs[L.int_of_ref[S.gdof[v]]] = std::exp(-S.theta[c]);
+            }
+            for (int r = 0; r < L.q; ++r) rs[L.int_of_ref[S.nflux_total + c * L.q + r]] = -1.0;
         }
-        for (int r = 0; r < L.q; ++r) rs[L.int_of_ref[S.nflux_total + (int64_t)c * L.q + r]] = -1.0;
-    }
+    });
     if (L.lam)
         for (int i = 0; i < L.ninst; ++i) rs[L.lam_idx + i] = -1.0;
 }
