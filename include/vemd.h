@@ -175,7 +175,7 @@ typedef struct {
 vemd_status vemd_solver_create(vemd_mesh m, const vemd_problem_desc* desc, vemd_solver* out);
 
 /* The same solver with the element operators built on a GPU (one thread per
- * cell, FP64; k = 0 or 1, polygons of up to 10 edges) and copied back into
+ * cell, FP64; k = 0, 1 or 2, polygons of up to 10 edges) and copied back into
  * the host arrays; agrees with vemd_solver_create to rounding.  Replaces the
  * same constructor and the reference GPU port's per-element integration from
  * host-packed quadrature samples (GPU/src/hdiv_gpu.cu:45-135,

@@ -242,11 +242,11 @@ def test_solve_without_device_fails_loudly(vemd):
 
 
 def test_device_setup_limits_checked_on_the_host(vemd):
-    """The GPU element setup rejects k > 1 and polygons of more than 10 edges
+    """The GPU element setup rejects k > 2 and polygons of more than 10 edges
     before touching a device, and without a device it fails loudly."""
     m = vemd.Mesh.read(golden("square_2x2.msh"))
     with pytest.raises(vemd.VemdError) as ei:
-        vemd.Solver(m, k=2, setup_device=0)
+        vemd.Solver(m, k=3, nq2d=12, setup_device=0)
     assert ei.value.status == "ERR_ARG"
     # one regular 11-gon
     t = np.linspace(0.0, 2.0 * np.pi, 12)[:-1]

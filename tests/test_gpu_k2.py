@@ -105,8 +105,6 @@ def test_k2_refusals(vemd):
     x = torch.zeros_like(b)
     with pytest.raises(vemd.VemdError, match="k = 0 and k = 1"):
         s.errors_device(x)
-    with pytest.raises(vemd.VemdError):
-        vemd.Solver(m, k=2, map_kind=vemd.MAP_ARCS, bc=vemd.BC_DIRICHLET_P, setup_device=0)
     o = vemd.SolveOptions(precision=vemd.MIXED, rtol=1e-8).c()
     info = (vemd._lib.SolveInfo * 1)()
     rc = vemd.lib().vemd_solve_multi_device(s.h, ctypes.byref(o), 1, ctypes.c_void_p(b.data_ptr()),

@@ -1,8 +1,8 @@
 """Element setup on the GPU (vemd_solver_create_device) against the oracle
 and the host setup: every cell's signed A_E, W_E, L_E, f_E, the assembled
 right-hand side (boundary data included) and the projector (through the flux
-error), on straight, sinusoidally mapped and curved-arc meshes, k = 0 and 1,
-both boundary conditions.  The device threads follow the host arithmetic step
+error), on straight, sinusoidally mapped and curved-arc meshes, k = 0, 1 and
+2, both boundary conditions.  The device threads follow the host arithmetic step
 by step, so the two agree to the rounding of fused multiply-adds and the math
 library: relative 1e-11 per cell; the oracle computes the same quantities
 in its own order, and the device values meet it to 1e-11 too."""
@@ -22,6 +22,9 @@ CASES = [
     ("annulus2500", 1, "arcs", 0.0, "sincos", "flux"),
     ("rosette", 1, "sin", 0.05, "sincos", "flux"),  # a 10-gon: the largest polygon of the GPU setup
     ("rosette", 0, "arcs", 0.0, "sinsin", "dirichlet"),
+    ("square_4x4.msh", 2, "sin", 0.05, "sincos", "flux"),
+    ("disk1500", 2, "arcs", 0.0, "sinsin", "dirichlet"),
+    ("rosette", 2, "arcs", 0.0, "sinsin", "dirichlet"),
 ]
 IDS = ["%s-k%d-%s-%s" % (c[0], c[1], c[2], c[5]) for c in CASES]
 
@@ -120,7 +123,7 @@ def test_device_setup_solves_like_host(vemd):
     assert np.linalg.norm(xh - xd) <= 1e-9 * np.linalg.norm(xh)
 
 
-def test_device_setup_rejects_k2(vemd):
+def test_device_setup_rejects_k3(vemd):
     m = vemd.Mesh.read(golden("square_2x2.msh"))
-    with pytest.raises(RuntimeError, match="k = 0 and k = 1"):
-        vemd.Solver(m, k=2, setup_device=0)
+    with pytest.raises(RuntimeError, match="k = 0, 1 and 2"):
+        vemd.Solver(m, k=3, nq2d=12, setup_device=0)

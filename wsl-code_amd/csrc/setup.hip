@@ -961,7 +961,7 @@ void errors_device(Solver& s, const double* x_dev, int nq, void* stream, double*
 }
 
 Setup* setup_build_device(const Mesh& m, const vemd_problem_desc& d, int device) {
-    if (d.k > 1) throw Error(VEMD_ERR_ARG, "the device element setup supports k = 0 and k = 1");
+    if (d.k > 2) throw Error(VEMD_ERR_ARG, "the device element setup supports k = 0, 1 and 2");
     if (d.funcs)
         throw Error(VEMD_ERR_ARG, "problem callbacks run on the host: use vemd_solver_create for them");
     for (int c = 0; c < m.nc; ++c)
@@ -997,8 +997,10 @@ Setup* setup_build_device(const Mesh& m, const vemd_problem_desc& d, int device)
     if (m.nc > 0) {
         if (d.k == 0)
             hipLaunchKernelGGL(k_setup<0>, grid, block, 0, 0, in, o);
-        else
+        else if (d.k == 1)
             hipLaunchKernelGGL(k_setup<1>, grid, block, 0, 0, in, o);
+        else
+            hipLaunchKernelGGL(k_setup<2>, grid, block, 0, 0, in, o);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipDeviceSynchronize());
