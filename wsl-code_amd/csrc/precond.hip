@@ -488,10 +488,13 @@ __global__ __launch_bounds__(256) void k_restrict(CsrDev<T> R, const T* v, T* y,
 template <typename T, int B, int LPR, int U, bool RH, bool MH, bool HY>
 __global__ __launch_bounds__(256) void k_rres(BcsrDev<T> R, BcsrDev<T> M, const T* __restrict__ b,
                                             const T* __restrict__ x, const T* __restrict__ d, T* __restrict__ y,
-                                            const int* done) {
+                                            const int* done, RowSkip sk) {
     if (done && *done) return;
     const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) / LPR;
     const int lane = threadIdx.x % LPR;
+    // a coarse row of a finished instance (an aggregate lies in one instance:
+    // its first fine block row names it); the row's LPR lanes leave together
+    if (sk.ti && row < M.nrows && skipped(sk, (int64_t)M.bc[M.rp[row]])) return;
     T acc = T(0);
     if (row < M.nrows) {
         T rb = T(0), mx = T(0);
@@ -1109,11 +1112,11 @@ static const T* vcycle_level(const AmgDev<T>& D, size_t l, hipStream_t st, const
 // loads; scalar levels: 32 lanes, 4 entries each.
 template <typename T, int B, int LPR, int U>
 static void launch_rres_(const AmgLevelDev<T>& L, const T* b, const T* x, const T* d, T* y, hipStream_t st,
-                         const int* ms) {
+                         const int* ms, RowSkip sk) {
     const dim3 g = grid_of(L.M.nrows * LPR), bl(256);
     const bool rh = L.Rb.hv != nullptr, mh = L.M.hv != nullptr;
 #define VEMD_RRES(RH, MH, HY) \
-    hipLaunchKernelGGL((k_rres<T, B, LPR, U, RH, MH, HY>), g, bl, 0, st, L.Rb, L.M, b, x, d, y, ms)
+    hipLaunchKernelGGL((k_rres<T, B, LPR, U, RH, MH, HY>), g, bl, 0, st, L.Rb, L.M, b, x, d, y, ms, sk)
     if (d) {
         if (rh && mh) VEMD_RRES(true, true, true);
         else if (rh) VEMD_RRES(true, false, true);
@@ -1132,8 +1135,8 @@ static void launch_rres_(const AmgLevelDev<T>& L, const T* b, const T* x, const 
 // 1M cells against 16 and 64 lanes and 1, 4 or 8 entries).
 template <typename T, int B>
 static void launch_rres(const AmgLevelDev<T>& L, const T* b, const T* x, const T* d, T* y, hipStream_t st,
-                        const int* ms) {
-    launch_rres_<T, B, 32, 2>(L, b, x, d, y, st, ms);
+                        const int* ms, RowSkip sk) {
+    launch_rres_<T, B, 32, 2>(L, b, x, d, y, st, ms, sk);
 }
 
 template <typename T, int B, int NC>
@@ -1160,7 +1163,7 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
                            ms);
     smooth_steps<T, B, NC>(L, v, st, ms, din, fused, (const T*)nullptr, pre_done, nullptr, sk);
     if (NC == 1 && L.M.nrows > 0) {
-        launch_rres<T, B>(L, (const T*)v.b, (const T*)v.x, (const T*)din, vc.b, st, ms);
+        launch_rres<T, B>(L, (const T*)v.b, (const T*)v.x, (const T*)din, vc.b, st, ms, sk);
     } else {
         hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)din,
                            v.r, ms);
