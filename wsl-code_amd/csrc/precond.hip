@@ -550,6 +550,68 @@ __global__ __launch_bounds__(256) void k_rres(BcsrDev<T> R, BcsrDev<T> M, const 
     if (row < M.nrows && lane == 0) y[row] = acc;
 }
 
+// The same residual-restriction on MCOLS interleaved columns: a thread per
+// (coarse row, column), y = Rb b - M (x + d) over the row's entries in order
+// (the residual b - A x is never formed, and M = R A gathers far fewer fine
+// rows than A's rows followed by R's would).
+template <typename T, int B, bool RH, bool MH, bool HY>
+__global__ __launch_bounds__(256) void k_rres_mc(BcsrDev<T> R, BcsrDev<T> M, const T* __restrict__ b,
+                                               const T* __restrict__ x, const T* __restrict__ d, T* __restrict__ y,
+                                               const int* done) {
+    if (done && *done) return;
+    constexpr int NC = MCOLS;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t row = t / NC;
+    const int j = (int)(t % NC);
+    if (row >= M.nrows) return;
+    T rb = T(0), mx = T(0);
+    const int64_t r1 = R.rp[row + 1], m1 = M.rp[row + 1];
+#pragma unroll 4
+    for (int64_t p = R.rp[row]; p < r1; ++p) {
+        const int64_t c = (int64_t)R.bc[p] * B;
+#pragma unroll
+        for (int cc = 0; cc < B; ++cc) {
+            const T rv = RH ? (T)R.hv[p * B + cc] : R.v[p * B + cc];
+            rb += rv * b[(c + cc) * NC + j];
+        }
+    }
+#pragma unroll 4
+    for (int64_t e = M.rp[row]; e < m1; ++e) {
+        const int64_t c = (int64_t)M.bc[e] * B;
+#pragma unroll
+        for (int cc = 0; cc < B; ++cc) {
+            T v = x[(c + cc) * NC + j];
+            if constexpr (HY) v += d[(c + cc) * NC + j];
+            const T mv = MH ? (T)M.hv[e * B + cc] : M.v[e * B + cc];
+            mx += mv * v;
+        }
+    }
+    if constexpr (RH) rb *= (T)R.hscale;
+    if constexpr (MH) mx *= (T)M.hscale;
+    y[row * NC + j] = rb - mx;
+}
+
+template <typename T, int B>
+static void launch_rres_mc(const AmgLevelDev<T>& L, const T* b, const T* x, const T* d, T* y, hipStream_t st,
+                           const int* ms) {
+    const dim3 g((unsigned)((L.M.nrows * MCOLS + 255) / 256)), bl(256);
+    const bool rh = L.Rb.hv != nullptr, mh = L.M.hv != nullptr;
+#define VEMD_RRES_MC(RH, MH, HY) \
+    hipLaunchKernelGGL((k_rres_mc<T, B, RH, MH, HY>), g, bl, 0, st, L.Rb, L.M, b, x, d, y, ms)
+    if (d) {
+        if (rh && mh) VEMD_RRES_MC(true, true, true);
+        else if (rh) VEMD_RRES_MC(true, false, true);
+        else if (mh) VEMD_RRES_MC(false, true, true);
+        else VEMD_RRES_MC(false, false, true);
+    } else {
+        if (rh && mh) VEMD_RRES_MC(true, true, false);
+        else if (rh) VEMD_RRES_MC(true, false, false);
+        else if (mh) VEMD_RRES_MC(false, true, false);
+        else VEMD_RRES_MC(false, false, false);
+    }
+#undef VEMD_RRES_MC
+}
+
 // Last Chebyshev step of the finest level (single column, one lane per block
 // row, so a wavefront is one 64-position tile) with the Krylov output of
 // KOut: x + d + d' (times xs on a scaled level) goes to the pressure block
@@ -1164,6 +1226,8 @@ static const T* level_cycle(const AmgDev<T>& D, size_t l, hipStream_t st, const 
     smooth_steps<T, B, NC>(L, v, st, ms, din, fused, (const T*)nullptr, pre_done, nullptr, sk);
     if (NC == 1 && L.M.nrows > 0) {
         launch_rres<T, B>(L, (const T*)v.b, (const T*)v.x, (const T*)din, vc.b, st, ms, sk);
+    } else if (NC == MCOLS && L.M.nrows > 0) {
+        launch_rres_mc<T, B>(L, (const T*)v.b, (const T*)v.x, (const T*)din, vc.b, st, ms);
     } else {
         hipLaunchKernelGGL((k_resid<T, B, NC>), gb, b, 0, st, L.A, (const T*)v.b, sc, (const T*)v.x, (const T*)din,
                            v.r, ms);
