@@ -185,7 +185,7 @@ template <typename T, int B, bool HY, bool HC>
 __device__ __forceinline__ void sell_acc_staged(const Stage32<T, B>& S, int w, int r8, int j, const T* __restrict__ x,
                                                 const T* __restrict__ y, const T* __restrict__ cs, T (&out)[B]) {
     constexpr int NC = MCOLS;
-    constexpr int U = B == 1 ? 8 : 2;
+    constexpr int U = B == 1 ? 8 : 4;
 #pragma unroll
     for (int r = 0; r < B; ++r) out[r] = T(0);
     int k = 0;
